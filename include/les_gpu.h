@@ -1,0 +1,224 @@
+/*
+ * les_gpu.h -- C-ABI of the MI355X boundary-layer LES timestep.
+ *
+ * Drop-in boundary for the per-step routines of the GAbelois/OverlappingBLs
+ * solver (the SBL/code Fortran files).  Every entry point mirrors one reference routine;
+ * the citation on each declaration names the routine it replaces.  The
+ * Fortran side binds these with ISO_C_BINDING (see the les_gpu_mod module
+ * shipped with the package).
+ *
+ * Conventions
+ *  - Grid: nxt x nyt horizontal points, z nodes 0..nzt.  Physical uvp levels
+ *    are 1..nzt-1 (z = (k-1/2)dz), w levels are 1..nzt (z = (k-1)dz); level 0
+ *    is the ghost below the wall.  One context holds the whole domain, so the
+ *    reference's z-slab ghosts are ordinary neighbouring levels.
+ *  - A 3-D field is (nzt+1) planes of nyt rows of nxt reals, x fastest.
+ *    Host arrays passed to les_set_field/les_get_field may carry a row
+ *    stride `ld` >= nxt (ld = ldx = nxt+2 for the reference's arrays); the
+ *    padding columns nxt+1..ldx are not represented on the device.
+ *  - All reals are FP64 at the boundary.  les_set_precision selects the
+ *    internal arithmetic of the step (LES_FP64 or LES_MIXED).
+ *  - Every call returns LES_OK (0) or a negative status.  The reference
+ *    prints and stops on the same conditions; its Fortran wrapper turns a
+ *    nonzero status into write + stop.
+ *  - One host thread per context; calls are synchronous unless stated.
+ */
+#ifndef LES_GPU_H
+#define LES_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define LES_OK          0
+#define LES_E_BADARG   -1   /* invalid argument / unsupported option       */
+#define LES_E_HIP      -2   /* HIP runtime error                           */
+#define LES_E_CFL      -3   /* CFL > 0.6 (901_debug.f90:123-128)            */
+#define LES_E_NOMEM    -4   /* device allocation failed                    */
+#define LES_E_NODEV    -5   /* no GPU visible                              */
+
+/* ---- precision --------------------------------------------------------- */
+#define LES_FP64   0
+#define LES_MIXED  1
+
+/* ---- option codes (namelist strings of 001_param.f90) -------------------- */
+#define LES_WALL_ROUGH   0   /* walltype = 'rough'  */
+#define LES_WALL_SMOOTH  1   /* walltype = 'smooth' */
+#define LES_BC_WALL        0 /* lbc_mom / ubc_mom = 'wall'        */
+#define LES_BC_STRESS_FREE 1 /* lbc_mom / ubc_mom = 'stress free' */
+
+/*
+ * Run parameters, already nondimensionalised as init_nondimensional leaves
+ * them (102_initial.f90:176-256): lengths / z_i, dt * u_scale / z_i,
+ * coriol = f z_i / u_scale, ug = ug_dim / u_scale.  Dimensional inputs that
+ * the reference rescales inside the step (wt_s, dTdz_top, nu_molec) stay
+ * dimensional, as in the namelist.
+ */
+typedef struct les_params {
+    int32_t nxt, nyt, nzt;     /* domain_param; nzt counts the top ghost    */
+    int32_t model;             /* flow_param model; only 5 (Lagrangian Sdep)*/
+    int32_t cs_count;          /* time_param cs_count                       */
+    int32_t dyn_init;          /* flow_param DYN_init                       */
+    int32_t inilag;            /* flow_param inilag                         */
+    int32_t restart_vel;       /* flow_param restart_vel                    */
+    int32_t sgs;               /* flow_param sgs                            */
+    int32_t molec;             /* flow_param molec                          */
+    int32_t walltype;          /* LES_WALL_*                                */
+    int32_t lbc_mom, ubc_mom;  /* LES_BC_*                                  */
+    int32_t ubc;               /* flow_param ubc (1: sponge on)             */
+    int32_t damping_method;    /* flow_param damping_method (only 1)        */
+    int32_t coriolis_forcing;  /* flow_param coriolis_forcing               */
+    int32_t use_mean_p_force;  /* flow_param use_mean_p_force               */
+    int32_t ocean_flag;        /* flow_param ocean_flag (only 0)            */
+    int32_t theta_flag;        /* temperature_param theta_flag              */
+    int32_t restart_theta;     /* temperature_param restart_theta           */
+    int32_t theta_init_time;   /* temperature_param theta_init_time         */
+    int32_t passive_scalar;    /* 001_param.f90:512 compile-time switch     */
+    int32_t lbc;               /* temperature_param lbc (only 1: flux)      */
+    int32_t pad0;
+    double lx_tot, ly_tot, lz_tot;   /* / z_i                               */
+    double dt;                       /* nondimensional                      */
+    double z_i, u_scale, T_scale;    /* scales                              */
+    double nu_molec;                 /* m^2/s                               */
+    double zo, d0;                   /* roughness, displacement / z_i       */
+    double Pr;                       /* turbulent Prandtl number            */
+    double wt_s;                     /* surface kinematic heat flux, K m/s  */
+    double dTdz_top;                 /* K/m                                 */
+    double coriol, ug, vg;           /* nondimensional Coriolis forcing     */
+    double mean_p_force;             /* nondimensional                      */
+    double lz_npz;                   /* lz*npz of step_scalar (= lz_tot)     */
+} les_params;
+
+/* ---- field identifiers ----------------------------------------------------
+ * 3-D fields hold nzt+1 planes.  The plane fields (LES_F_*_SFC) hold one
+ * plane.  The names follow the reference's module variables.            */
+enum les_field {
+    LES_F_U = 0, LES_F_V, LES_F_W, LES_F_THETA,            /* sim_param     */
+    LES_F_RHSX, LES_F_RHSY, LES_F_RHSZ, LES_F_RHST,         /* RHS (this step)*/
+    LES_F_RHSX_F, LES_F_RHSY_F, LES_F_RHSZ_F, LES_F_RHST_F, /* RHS history   */
+    LES_F_DUDX, LES_F_DUDY, LES_F_DUDZ,
+    LES_F_DVDX, LES_F_DVDY, LES_F_DVDZ,
+    LES_F_DWDX, LES_F_DWDY, LES_F_DWDZ,
+    LES_F_TXX, LES_F_TXY, LES_F_TXZ, LES_F_TYY, LES_F_TYZ, LES_F_TZZ,
+    LES_F_DIVTX, LES_F_DIVTY, LES_F_DIVTZ,
+    LES_F_P, LES_F_DPDX, LES_F_DPDY, LES_F_DPDZ,
+    LES_F_CS_OPT2, LES_F_F_LM, LES_F_F_MM, LES_F_F_QN, LES_F_F_NN,   /* sgsmodule */
+    LES_F_NU_T, LES_F_U_LAG, LES_F_V_LAG, LES_F_W_LAG,
+    LES_F_S11, LES_F_S12, LES_F_S13, LES_F_S22, LES_F_S23, LES_F_S33, /* w-node strain */
+    LES_F_CX, LES_F_CY, LES_F_CZ,                       /* convec output       */
+    LES_F_DTDX, LES_F_DTDY, LES_F_DTDZ, LES_F_BETA_SCAL, LES_F_SGS_T3,
+    LES_F_WORK0, LES_F_WORK1, LES_F_WORK2,              /* caller scratch      */
+    LES_N_FIELDS3D,
+    /* one-plane surface fields (bottombc module) */
+    LES_F_USTAR_SFC = 100, LES_F_PSI_M_SFC, LES_F_PHI_M_SFC,
+    LES_F_PSI_H_SFC, LES_F_PHI_H_SFC,
+    LES_F_END
+};
+
+typedef struct les_ctx les_ctx;
+
+/* ---- lifecycle (102_initial.f90:4-61 init) -------------------------------- */
+/* Allocates the device state for one domain and precomputes wavenumbers,
+ * test-filter kernels and the pressure TDMA factorisation
+ * (002_fft.f90:121-189, 006_test_filter.f90:41-77, 109_press_stag_array.f90:4-103). */
+int les_create(const les_params* prm, les_ctx** out);
+int les_destroy(les_ctx* ctx);
+int les_set_precision(les_ctx* ctx, int precision);
+/* Stokes-drift profiles ust(0:nzt), vst(0:nzt) (014_stokes_drift.f90:29-115). */
+int les_set_stokes(les_ctx* ctx, const double* ust, const double* vst);
+/* Sponge profile sponge(0:nzt) (007_topbc.f90:15-85); computed by les_create
+ * for damping_method 1, overridable here. */
+int les_set_sponge(les_ctx* ctx, const double* sponge);
+
+/* ---- field transfer -------------------------------------------------------- */
+/* Copy levels lev0..lev0+nlev-1 of a field between host and device.  The
+ * host block is nlev planes of nyt rows with row stride ld (doubles). */
+int les_set_field(les_ctx* ctx, int field, const double* host, int ld, int lev0, int nlev);
+int les_get_field(les_ctx* ctx, int field, double* host, int ld, int lev0, int nlev);
+/* Scalar state carried across steps (lagrange_Sdep's saved init flags). */
+int les_set_sgs_init_flags(les_ctx* ctx, int f_lm_mm_init, int f_qn_nn_init);
+int les_get_sgs_init_flags(les_ctx* ctx, int* f_lm_mm_init, int* f_qn_nn_init);
+
+/* ---- derivative and transform routines (fields on the device) ------------ */
+/* derivatives::ddx, 004_derivatives.f90:18-41.  All levels 0..nzt. */
+int les_ddx(les_ctx* ctx, int dfdx, int f);
+/* derivatives::ddy, 004_derivatives.f90:45-81. */
+int les_ddy(les_ctx* ctx, int dfdy, int f);
+/* derivatives::ddz_uv, 004_derivatives.f90:85-116 (global-domain form). */
+int les_ddz_uv(les_ctx* ctx, int dfdz, int f);
+/* derivatives::ddz_w, 004_derivatives.f90:155-183. */
+int les_ddz_w(les_ctx* ctx, int dfdz, int f);
+/* filter_data, 006_test_filter.f90:82-105: removes the x and y Nyquist modes. */
+int les_filter_data(les_ctx* ctx, int f);
+/* test_filter on one plane, 006_test_filter.f90:19-37; which = 2 (G_test,
+ * width 2 delta) or 4 (G_test_test, width 4 delta). */
+int les_test_filter(les_ctx* ctx, int f, int level, int which);
+
+/* ---- one-plane host-array transforms (drop-ins for the reference's calls) -- */
+/* fftn_mpi, 203_fftn_mpi.f90:4-35: plane (ld x nyt) -> half spectrum
+ * (nxt/2+1) x nyt complex (interleaved re,im), Nyquist row/column zeroed,
+ * unnormalised. */
+int les_fftn(les_ctx* ctx, const double* in, int ld, double* out_c);
+/* ifftn_mpi, 203_fftn_mpi.f90:39-67 (unnormalised). */
+int les_ifftn(les_ctx* ctx, const double* in_c, double* out, int ld);
+/* zero_padding, 108_01_padd.f90:73-132: (ld x nyt) -> (3nxt/2 x 3nyt/2). */
+int les_zero_padding(les_ctx* ctx, const double* var, int ld, double* var_big);
+/* zero_unpadding, 108_01_padd.f90:136-197: (3nxt/2 x 3nyt/2) -> (ld x nyt). */
+int les_zero_unpadding(les_ctx* ctx, const double* var_big, double* var, int ld);
+
+/* ---- physics routines (fields on the device) ------------------------------- */
+/* obukhov, 010_scalars.f90:675-856 (land branch). */
+int les_obukhov(les_ctx* ctx, int jt);
+/* wallstress_bottom / wallstress_top, 104_wallstress.f90:98-260. */
+int les_wallstress_bottom(les_ctx* ctx);
+int les_wallstress_top(les_ctx* ctx);
+/* sgs_stag, 106_sgs_stag.f90:5-318, including lagrange_Sdep
+ * (106_03_lagrange_Sdep.f90:6-362) and interpolag_Sdep
+ * (106_06_interpolag_Sdep.f90:4-336) on update steps. */
+int les_sgs_stag(les_ctx* ctx, int jt);
+/* lagrange_Sdep alone, 106_03_lagrange_Sdep.f90:6-362 (with its
+ * interpolag_Sdep); reads the w-node strain LES_F_S11..LES_F_S33. */
+int les_lagrange_sdep(les_ctx* ctx, int jt);
+/* interpolag_Sdep alone, 106_06_interpolag_Sdep.f90:4-336: moves F_LM, F_MM,
+ * F_QN, F_NN to their upstream positions and zeroes u_lag, v_lag, w_lag. */
+int les_interpolag_sdep(les_ctx* ctx);
+/* divstress_uv, 107_divstress.f90:4-44 / divstress_w, 107_divstress.f90:48-104. */
+int les_divstress_uv(les_ctx* ctx, int divt, int tx, int ty, int tz);
+int les_divstress_w(les_ctx* ctx, int divt, int tx, int ty, int tz);
+/* convec, 108_convec.f90:11-202 -> LES_F_CX, LES_F_CY, LES_F_CZ. */
+int les_convec(les_ctx* ctx);
+/* theta_all_in_one, 010_scalars.f90:111-200 (with scalar_RHS_calc,
+ * calc_beta, step_scalar). */
+int les_theta_all_in_one(les_ctx* ctx, int jt);
+/* press_stag_array, 109_press_stag_array.f90:107-319, then dpdz
+ * (103_flosol.f90:174-175). */
+int les_press_stag_array(les_ctx* ctx, int jt);
+/* project, 110_project.f90:4-62, followed by data_exchange ghost rules
+ * (202_parallel.f90:722-733). */
+int les_project(les_ctx* ctx);
+
+/* ---- whole step --------------------------------------------------------- */
+/* flosol, 103_flosol.f90:4-290, minus file I/O and the ke/dudt diagnostics.
+ * Returns LES_E_CFL if check_cfl would abort (the state is then undefined,
+ * as the reference stops the program). */
+int les_step(les_ctx* ctx, int jt);
+/* nsteps consecutive flosol steps jt0..jt0+nsteps-1, enqueued without host
+ * synchronisation; CFL is checked once at the end for every step.
+ * *failed_jt receives the first step whose CFL exceeded 0.6 (or 0). */
+int les_run(les_ctx* ctx, int jt0, int nsteps, int* failed_jt);
+/* Last CFL value computed by newstep/check_cfl (901_debug.f90:66-143). */
+int les_get_cfl(les_ctx* ctx, double* cfl);
+
+/* ---- timing support ------------------------------------------------------ */
+/* HIP stream the context enqueues on (hipStream_t as void*). */
+void* les_stream(les_ctx* ctx);
+/* Synchronise the context's stream. */
+int les_sync(les_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LES_GPU_H */

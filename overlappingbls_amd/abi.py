@@ -1,0 +1,302 @@
+"""ctypes mirror of include/les_gpu.h and a thin object wrapper over it.
+
+The same wrapper drives any library that exports the les_gpu.h entry points
+under a symbol prefix: the HIP library exports them as ``les_*``.  Method
+names follow the reference routines they replace (SBL/code/*.f90), so a test
+reads like the reference's call sequence in flosol (103_flosol.f90:4-290).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import numpy as np
+
+LES_OK = 0
+LES_E_BADARG = -1
+LES_E_HIP = -2
+LES_E_CFL = -3
+LES_E_NOMEM = -4
+LES_E_NODEV = -5
+
+LES_FP64 = 0
+LES_MIXED = 1
+
+WALL_ROUGH, WALL_SMOOTH = 0, 1
+BC_WALL, BC_STRESS_FREE = 0, 1
+
+_FIELDS3D = [
+    "U", "V", "W", "THETA",
+    "RHSX", "RHSY", "RHSZ", "RHST",
+    "RHSX_F", "RHSY_F", "RHSZ_F", "RHST_F",
+    "DUDX", "DUDY", "DUDZ", "DVDX", "DVDY", "DVDZ", "DWDX", "DWDY", "DWDZ",
+    "TXX", "TXY", "TXZ", "TYY", "TYZ", "TZZ",
+    "DIVTX", "DIVTY", "DIVTZ",
+    "P", "DPDX", "DPDY", "DPDZ",
+    "CS_OPT2", "F_LM", "F_MM", "F_QN", "F_NN",
+    "NU_T", "U_LAG", "V_LAG", "W_LAG",
+    "S11", "S12", "S13", "S22", "S23", "S33",
+    "CX", "CY", "CZ",
+    "DTDX", "DTDY", "DTDZ", "BETA_SCAL", "SGS_T3",
+    "WORK0", "WORK1", "WORK2",
+]
+FIELD = {name: i for i, name in enumerate(_FIELDS3D)}
+N_FIELDS3D = len(_FIELDS3D)
+SURFACE = {"USTAR_SFC": 100, "PSI_M_SFC": 101, "PHI_M_SFC": 102, "PSI_H_SFC": 103, "PHI_H_SFC": 104}
+FIELD.update(SURFACE)
+
+
+class LesParams(C.Structure):
+    """les_params of include/les_gpu.h (field order and types must match)."""
+
+    _fields_ = [
+        ("nxt", C.c_int32), ("nyt", C.c_int32), ("nzt", C.c_int32),
+        ("model", C.c_int32), ("cs_count", C.c_int32), ("dyn_init", C.c_int32),
+        ("inilag", C.c_int32), ("restart_vel", C.c_int32), ("sgs", C.c_int32),
+        ("molec", C.c_int32), ("walltype", C.c_int32),
+        ("lbc_mom", C.c_int32), ("ubc_mom", C.c_int32),
+        ("ubc", C.c_int32), ("damping_method", C.c_int32),
+        ("coriolis_forcing", C.c_int32), ("use_mean_p_force", C.c_int32),
+        ("ocean_flag", C.c_int32), ("theta_flag", C.c_int32),
+        ("restart_theta", C.c_int32), ("theta_init_time", C.c_int32),
+        ("passive_scalar", C.c_int32), ("lbc", C.c_int32), ("pad0", C.c_int32),
+        ("lx_tot", C.c_double), ("ly_tot", C.c_double), ("lz_tot", C.c_double),
+        ("dt", C.c_double),
+        ("z_i", C.c_double), ("u_scale", C.c_double), ("T_scale", C.c_double),
+        ("nu_molec", C.c_double), ("zo", C.c_double), ("d0", C.c_double),
+        ("Pr", C.c_double), ("wt_s", C.c_double), ("dTdz_top", C.c_double),
+        ("coriol", C.c_double), ("ug", C.c_double), ("vg", C.c_double),
+        ("mean_p_force", C.c_double), ("lz_npz", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_PD = C.POINTER(C.c_double)
+
+
+def _pd(a: np.ndarray):
+    return a.ctypes.data_as(_PD)
+
+
+class LesError(RuntimeError):
+    def __init__(self, fn, status):
+        super().__init__(f"{fn} returned status {status}")
+        self.status = status
+
+
+def bind(lib: C.CDLL, prefix: str):
+    """Declare argument types of every les_gpu.h entry point under prefix."""
+    P = C.c_void_p
+    i = C.c_int
+    sig = {
+        "create": [C.POINTER(LesParams), C.POINTER(C.c_void_p)],
+        "destroy": [P],
+        "set_precision": [P, i],
+        "set_stokes": [P, _PD, _PD],
+        "set_sponge": [P, _PD],
+        "set_field": [P, i, _PD, i, i, i],
+        "get_field": [P, i, _PD, i, i, i],
+        "set_sgs_init_flags": [P, i, i],
+        "get_sgs_init_flags": [P, C.POINTER(i), C.POINTER(i)],
+        "ddx": [P, i, i], "ddy": [P, i, i], "ddz_uv": [P, i, i], "ddz_w": [P, i, i],
+        "filter_data": [P, i],
+        "test_filter": [P, i, i, i],
+        "fftn": [P, _PD, i, _PD],
+        "ifftn": [P, _PD, _PD, i],
+        "zero_padding": [P, _PD, i, _PD],
+        "zero_unpadding": [P, _PD, _PD, i],
+        "obukhov": [P, i],
+        "wallstress_bottom": [P], "wallstress_top": [P],
+        "sgs_stag": [P, i],
+        "lagrange_sdep": [P, i],
+        "interpolag_sdep": [P],
+        "divstress_uv": [P, i, i, i, i], "divstress_w": [P, i, i, i, i],
+        "convec": [P],
+        "theta_all_in_one": [P, i],
+        "press_stag_array": [P, i],
+        "project": [P],
+        "step": [P, i],
+        "run": [P, i, i, C.POINTER(i)],
+        "get_cfl": [P, C.POINTER(C.c_double)],
+        "sync": [P],
+    }
+    fns = {}
+    for name, argtypes in sig.items():
+        f = getattr(lib, prefix + name, None)
+        if f is None:
+            continue
+        f.argtypes = argtypes
+        f.restype = C.c_int
+        fns[name] = f
+    return fns
+
+
+class LesContext:
+    """One domain's state behind a les_gpu.h-style library.
+
+    Arrays at the boundary are numpy float64, C order (levels, nyt, nxt),
+    i.e. the reference's (ldx, nynpy, 0:nz) column-major layout without the
+    two padding columns.
+    """
+
+    def __init__(self, fns: dict, prm: LesParams, label: str):
+        self._f = fns
+        self.label = label
+        self.prm = prm
+        h = C.c_void_p()
+        self._call("create", C.byref(prm), C.byref(h))
+        self._h = h
+        self.nx, self.ny, self.nz = prm.nxt, prm.nyt, prm.nzt
+        self.nlev = self.nz + 1
+
+    def _call(self, name, *args):
+        st = self._f[name](*args)
+        if st != LES_OK:
+            raise LesError(f"{self.label}:{name}", st)
+        return st
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._f["destroy"](self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state transfer -------------------------------------------------
+    def set(self, name, arr, lev0=0):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if name in SURFACE:
+            a = a.reshape(1, self.ny, self.nx)
+            self._call("set_field", self._h, FIELD[name], _pd(a), self.nx, 0, 1)
+            return
+        a = a.reshape(-1, self.ny, self.nx)
+        self._call("set_field", self._h, FIELD[name], _pd(a), self.nx, lev0, a.shape[0])
+
+    def get(self, name, lev0=0, nlev=None):
+        if name in SURFACE:
+            out = np.empty((self.ny, self.nx))
+            self._call("get_field", self._h, FIELD[name], _pd(out), self.nx, 0, 1)
+            return out
+        if nlev is None:
+            nlev = self.nlev - lev0
+        out = np.empty((nlev, self.ny, self.nx))
+        self._call("get_field", self._h, FIELD[name], _pd(out), self.nx, lev0, nlev)
+        return out
+
+    def set_stokes(self, ust, vst):
+        u = np.ascontiguousarray(ust, dtype=np.float64)
+        v = np.ascontiguousarray(vst, dtype=np.float64)
+        self._call("set_stokes", self._h, _pd(u), _pd(v))
+
+    def set_precision(self, p):
+        self._call("set_precision", self._h, p)
+
+    def sgs_init_flags(self):
+        a, b = C.c_int(), C.c_int()
+        self._call("get_sgs_init_flags", self._h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def set_sgs_init_flags(self, a, b):
+        self._call("set_sgs_init_flags", self._h, int(a), int(b))
+
+    def cfl(self):
+        x = C.c_double()
+        self._call("get_cfl", self._h, C.byref(x))
+        return x.value
+
+    # ---- routines -------------------------------------------------------
+    def ddx(self, dst, src):
+        self._call("ddx", self._h, FIELD[dst], FIELD[src])
+
+    def ddy(self, dst, src):
+        self._call("ddy", self._h, FIELD[dst], FIELD[src])
+
+    def ddz_uv(self, dst, src):
+        self._call("ddz_uv", self._h, FIELD[dst], FIELD[src])
+
+    def ddz_w(self, dst, src):
+        self._call("ddz_w", self._h, FIELD[dst], FIELD[src])
+
+    def filter_data(self, f):
+        self._call("filter_data", self._h, FIELD[f])
+
+    def test_filter(self, f, level, which=2):
+        self._call("test_filter", self._h, FIELD[f], int(level), int(which))
+
+    def fftn(self, plane):
+        a = np.ascontiguousarray(plane, dtype=np.float64)
+        out = np.zeros((self.ny, self.nx // 2 + 1), dtype=np.complex128)
+        self._call("fftn", self._h, _pd(a), a.shape[-1], out.ctypes.data_as(_PD))
+        return out
+
+    def ifftn(self, spec):
+        s = np.ascontiguousarray(spec, dtype=np.complex128)
+        out = np.zeros((self.ny, self.nx))
+        self._call("ifftn", self._h, s.ctypes.data_as(_PD), _pd(out), self.nx)
+        return out
+
+    def zero_padding(self, plane):
+        a = np.ascontiguousarray(plane, dtype=np.float64)
+        out = np.zeros((3 * self.ny // 2, 3 * self.nx // 2))
+        self._call("zero_padding", self._h, _pd(a), a.shape[-1], _pd(out))
+        return out
+
+    def zero_unpadding(self, big):
+        a = np.ascontiguousarray(big, dtype=np.float64)
+        out = np.zeros((self.ny, self.nx))
+        self._call("zero_unpadding", self._h, _pd(a), _pd(out), self.nx)
+        return out
+
+    def obukhov(self, jt):
+        self._call("obukhov", self._h, int(jt))
+
+    def wallstress_bottom(self):
+        self._call("wallstress_bottom", self._h)
+
+    def wallstress_top(self):
+        self._call("wallstress_top", self._h)
+
+    def sgs_stag(self, jt):
+        self._call("sgs_stag", self._h, int(jt))
+
+    def lagrange_sdep(self, jt):
+        self._call("lagrange_sdep", self._h, int(jt))
+
+    def interpolag_sdep(self):
+        self._call("interpolag_sdep", self._h)
+
+    def divstress_uv(self, divt, tx, ty, tz):
+        self._call("divstress_uv", self._h, FIELD[divt], FIELD[tx], FIELD[ty], FIELD[tz])
+
+    def divstress_w(self, divt, tx, ty, tz):
+        self._call("divstress_w", self._h, FIELD[divt], FIELD[tx], FIELD[ty], FIELD[tz])
+
+    def convec(self):
+        self._call("convec", self._h)
+
+    def theta_all_in_one(self, jt):
+        self._call("theta_all_in_one", self._h, int(jt))
+
+    def press_stag_array(self, jt):
+        self._call("press_stag_array", self._h, int(jt))
+
+    def project(self):
+        self._call("project", self._h)
+
+    def step(self, jt):
+        """flosol (103_flosol.f90:4-290); raises LesError(status=LES_E_CFL)."""
+        self._call("step", self._h, int(jt))
+
+    def run(self, jt0, nsteps):
+        failed = C.c_int(0)
+        st = self._f["run"](self._h, int(jt0), int(nsteps), C.byref(failed))
+        if st != LES_OK:
+            raise LesError(f"{self.label}:run (step {failed.value})", st)
+
+    def sync(self):
+        if "sync" in self._f:
+            self._call("sync", self._h)

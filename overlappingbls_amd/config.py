@@ -1,0 +1,225 @@
+"""Run configuration: namelist-equivalent inputs, nondimensionalisation and
+seeded synthetic initial fields for the boundary-layer LES step.
+
+The reference reads Fortran namelists (001_param.f90) and nondimensionalises
+them in init_nondimensional (102_initial.f90:176-256).  `Namelist` holds the
+same inputs, dimensional where the namelist is dimensional, and
+`Namelist.params()` returns the les_params block the C-ABI takes.
+
+Synthetic fields come from a counter-based generator keyed by
+(seed, field id, i, j, k), so a field does not depend on how the domain is
+split, and the same seed gives the same field on every machine.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from .abi import LesParams, WALL_ROUGH, WALL_SMOOTH, BC_WALL, BC_STRESS_FREE
+
+BOGUS = -1234567890.0  # 001_param.f90:40
+ZO1 = 0.01             # 008_bottombc.f90:18, roughness in metres
+DO1 = 0.0              # 008_bottombc.f90:19
+
+
+@dataclasses.dataclass
+class Namelist:
+    """Inputs of the namelists in 001_param.f90 that the step reads."""
+
+    nxt: int = 64
+    nyt: int = 64
+    nzt: int = 65                       # z nodes incl. the top ghost
+    lx_tot: float = 200.0 * math.pi     # metres (domain_param)
+    ly_tot: float = 200.0 * math.pi
+    lz_tot: float = 100.0
+    z_i: float = 100.0
+    dt: float = 2e-4 * 100.0            # seconds; nondimensional dt * z_i / u_scale
+    u_scale: float = 1.0
+    cs_count: int = 5
+    dyn_init: int = 5
+    inilag: bool = True
+    restart_vel: bool = False
+    walltype: str = "rough"
+    lbc_mom: str = "wall"
+    ubc_mom: str = "stress free"
+    ubc: int = 0
+    damping_method: int = 1
+    coriolis_forcing: bool = False
+    freq_coriolis: float = 1e-4
+    ug_dim: float = 0.0
+    vg_dim: float = 0.0
+    use_mean_p_force: bool = False
+    mean_p_force: float = 0.0
+    nu_molec: float = 1.14e-5
+    molec: bool = False
+    Pr: float = 0.4
+    theta_flag: bool = True
+    restart_theta: bool = False
+    theta_init_time: int = 1
+    passive_scalar: bool = False
+    T_scale: float = 300.0
+    wt_s: float = 0.0                   # K m/s
+    dTdz_top: float = 0.0               # K/m
+    npz: int = 2                        # only lz*npz of step_scalar depends on it
+
+    def params(self) -> LesParams:
+        p = LesParams()
+        p.nxt, p.nyt, p.nzt = self.nxt, self.nyt, self.nzt
+        p.model = 5
+        p.cs_count, p.dyn_init = self.cs_count, self.dyn_init
+        p.inilag, p.restart_vel = int(self.inilag), int(self.restart_vel)
+        p.sgs, p.molec = 1, int(self.molec)
+        p.walltype = {"rough": WALL_ROUGH, "smooth": WALL_SMOOTH}[self.walltype]
+        bc = {"wall": BC_WALL, "stress free": BC_STRESS_FREE}
+        p.lbc_mom, p.ubc_mom = bc[self.lbc_mom], bc[self.ubc_mom]
+        p.ubc, p.damping_method = self.ubc, self.damping_method
+        p.coriolis_forcing = int(self.coriolis_forcing)
+        p.use_mean_p_force = int(self.use_mean_p_force)
+        p.ocean_flag = 0
+        p.theta_flag, p.restart_theta = int(self.theta_flag), int(self.restart_theta)
+        p.theta_init_time, p.passive_scalar, p.lbc = self.theta_init_time, int(self.passive_scalar), 1
+        # init_nondimensional, 102_initial.f90:186-225
+        z_i, us = self.z_i, self.u_scale
+        p.lx_tot, p.ly_tot, p.lz_tot = self.lx_tot / z_i, self.ly_tot / z_i, self.lz_tot / z_i
+        p.dt = self.dt * us / z_i
+        p.z_i, p.u_scale, p.T_scale = z_i, us, self.T_scale
+        p.nu_molec = self.nu_molec
+        p.zo, p.d0 = ZO1 / z_i, DO1 / z_i          # patches, 008_bottombc.f90:119-120
+        p.Pr, p.wt_s, p.dTdz_top = self.Pr, self.wt_s, self.dTdz_top
+        if self.coriolis_forcing:
+            p.coriol = self.freq_coriolis * z_i / us
+            p.ug, p.vg = self.ug_dim / us, self.vg_dim / us
+        if self.use_mean_p_force:
+            p.mean_p_force = (1.0 / p.lz_tot) if self.mean_p_force == 0.0 else self.mean_p_force / us**2 * z_i
+        lz = self.lz_tot / self.npz / z_i          # 102_initial.f90:142, 186
+        p.lz_npz = lz * self.npz
+        return p
+
+
+def workload(name: str) -> Namelist:
+    """Named workloads.  sbl64/sbl128/sbl256 are the 64^3, 128^3 and 256^3
+    surface boundary layers (domain 2pi x 2pi x 1 in z_i units, z0 = 1e-4);
+    tiny* are small cases for tests."""
+    base = dict(lx_tot=200.0 * math.pi, ly_tot=200.0 * math.pi, lz_tot=100.0, z_i=100.0,
+                dt=2e-4 * 100.0, u_scale=1.0, dTdz_top=0.01 * 300.0 / 100.0)
+    if name == "sbl64":
+        return Namelist(nxt=64, nyt=64, nzt=65, **base)
+    if name == "sbl128":
+        return Namelist(nxt=128, nyt=128, nzt=129, wt_s=0.01 * 300.0, passive_scalar=True,
+                        coriolis_forcing=True, freq_coriolis=1e-4, ug_dim=1.0, vg_dim=0.0, **base)
+    if name == "sbl256":
+        return Namelist(nxt=256, nyt=256, nzt=257, wt_s=-0.005 * 300.0, **base)
+    if name == "tiny":
+        return Namelist(nxt=16, nyt=16, nzt=9, cs_count=2, dyn_init=2, **base)
+    if name == "tiny_ragged":
+        return Namelist(nxt=24, nyt=12, nzt=7, cs_count=2, dyn_init=2, wt_s=0.01 * 300.0,
+                        coriolis_forcing=True, ug_dim=1.0, vg_dim=0.2, ubc=1, **base)
+    if name == "small":
+        return Namelist(nxt=32, nyt=32, nzt=17, cs_count=3, dyn_init=3, wt_s=-0.005 * 300.0, **base)
+    raise KeyError(name)
+
+
+# --------------------------------------------------------------------------
+# counter-based generator (splitmix64 finaliser over a keyed counter)
+# --------------------------------------------------------------------------
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _mix(x: np.ndarray) -> np.ndarray:
+    x = x.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        x = x ^ (x >> np.uint64(31))
+    return x
+
+
+def uniform(seed: int, field_id: int, shape) -> np.ndarray:
+    """U(-1, 1) keyed by (seed, field_id, global index); shape (nz, ny, nx)."""
+    nz, ny, nx = shape
+    k, j, i = np.meshgrid(np.arange(nz, dtype=np.uint64), np.arange(ny, dtype=np.uint64),
+                          np.arange(nx, dtype=np.uint64), indexing="ij")
+    with np.errstate(over="ignore"):
+        key = (np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15)
+               + np.uint64(field_id) * np.uint64(0xD1B54A32D192ED03))
+        ctr = key + ((k * np.uint64(1 << 20) + j) * np.uint64(1 << 20) + i)
+    bits = _mix(_mix(ctr)) >> np.uint64(11)
+    return (bits.astype(np.float64) * (1.0 / 9007199254740992.0)) * 2.0 - 1.0
+
+
+def gaussian(seed: int, field_id: int, shape) -> np.ndarray:
+    u1 = 0.5 * (uniform(seed, 2 * field_id + 1000, shape) + 1.0)
+    u2 = 0.5 * (uniform(seed, 2 * field_id + 1001, shape) + 1.0)
+    u1 = np.maximum(u1, 1e-300)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def initial_fields(nl: Namelist, seed: int = 1234) -> dict:
+    """Seeded synthetic initial state, levels 0..nzt (ghost levels BOGUS as
+    data_exchange leaves them, 202_parallel.f90:722-733).
+
+    u = (1/0.4) ln(z/z0) (1 + 0.1 U(-1,1)) for z < 0.5, v = w = 0 plus the
+    same-amplitude noise on v and w below z = 0.5; theta = 1 + gamma max(0,
+    z - 0.3) (+ N(0, 0.05) noise for a passive scalar).
+    """
+    p = nl.params()
+    nx, ny, nz = p.nxt, p.nyt, p.nzt
+    dz = p.lz_tot / (nz - 1)
+    shape = (nz + 1, ny, nx)
+    k = np.arange(nz + 1, dtype=np.float64)
+    zuvp = (k - 0.5) * dz
+    zw = (k - 1.0) * dz
+    z0 = p.zo
+    prof = np.where(zuvp > 0, (1.0 / 0.4) * np.log(np.maximum(zuvp, 1e-300) / z0), 0.0)
+    low = (zuvp < 0.5).astype(np.float64)[:, None, None]
+    loww = (zw < 0.5).astype(np.float64)[:, None, None]
+    u = prof[:, None, None] * (1.0 + 0.1 * low * uniform(seed, 1, shape))
+    v = 0.1 * prof[:, None, None] * low * uniform(seed, 2, shape)
+    w = 0.1 * np.maximum(prof, 0.0)[:, None, None] * loww * uniform(seed, 3, shape) * 0.1
+    gamma = p.dTdz_top / p.T_scale * p.z_i
+    theta = 1.0 + gamma * np.maximum(0.0, zuvp - 0.3)[:, None, None] + 0.0 * u
+    if nl.passive_scalar:
+        theta = theta + 0.05 * gaussian(seed, 4, shape)
+    w[1] = 0.0      # wall (110_project.f90:58-60)
+    w[nz] = 0.0     # top (110_project.f90:55)
+    for f in (u, v, w, theta):
+        f[0] = BOGUS
+    u[nz] = BOGUS
+    v[nz] = BOGUS
+    theta[nz] = BOGUS
+    zeros = np.zeros(shape)
+    return {
+        "U": u, "V": v, "W": w, "THETA": theta,
+        "RHSX": zeros.copy(), "RHSY": zeros.copy(), "RHSZ": zeros.copy(), "RHST": zeros.copy(),
+        "CS_OPT2": zeros.copy(), "F_LM": zeros.copy(), "F_MM": zeros.copy(),
+        "F_QN": zeros.copy(), "F_NN": zeros.copy(),
+        "U_LAG": zeros.copy(), "V_LAG": zeros.copy(), "W_LAG": zeros.copy(),
+        "SGS_T3": zeros.copy(),
+    }
+
+
+def load_state(ctx, fields: dict, filter_velocity: bool = True):
+    """Upload an initial state and apply init's filter_data(u, v, w)
+    (102_initial.f90:55-57)."""
+    for name, arr in fields.items():
+        ctx.set(name, arr)
+    if filter_velocity:
+        for f in ("U", "V", "W"):
+            ctx.filter_data(f)
+
+
+STATE_FIELDS = ("U", "V", "W", "THETA", "RHSX", "RHSY", "RHSZ", "RHST",
+                "CS_OPT2", "F_LM", "F_MM", "F_QN", "F_NN", "U_LAG", "V_LAG", "W_LAG", "SGS_T3")
+
+
+def physical_levels(name: str, nz: int):
+    """Levels of a state field that carry physical values after a step."""
+    if name == "W":
+        return slice(1, nz + 1)
+    if name in ("CS_OPT2", "F_LM", "F_MM", "F_QN", "F_NN"):
+        return slice(1, nz + 1)
+    if name == "SGS_T3":
+        return slice(1, nz + 1)
+    return slice(1, nz)

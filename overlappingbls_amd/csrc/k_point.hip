@@ -1,0 +1,1019 @@
+// Pointwise, stencil and reduction kernels of the timestep.  Each kernel
+// restates one loop nest of the reference in global-domain form (levels
+// 0..nzt; the bottom z-slab's rules at levels 0/1, the top z-slab's at
+// nzt-1/nzt).  Streaming kernels are HBM-bound: one read of each input and
+// one write of each output per point, grid-stride over (level, plane).
+#include "les_internal.hpp"
+
+namespace les {
+
+// Kernel-side view of the run constants.
+struct KP {
+    long plane, plane2;
+    int nx, ny, nz;
+    double dx, dy, dz, dt;
+    double delta;       // filter_size*(dx dy dz)^(1/3), 106_sgs_stag.f90:33
+    double nu;          // molecular viscosity (nondim) if molec
+};
+
+static KP kp(const les_ctx* c) {
+    KP k;
+    k.plane = (long)c->plane;
+    k.plane2 = (long)c->plane2;
+    k.nx = c->nx; k.ny = c->ny; k.nz = c->nz;
+    k.dx = c->dx; k.dy = c->dy; k.dz = c->dz; k.dt = c->prm.dt;
+    k.delta = 1.0 * pow(c->dx * c->dy * c->dz, 1.0 / 3.0);
+    k.nu = c->prm.molec ? (c->prm.nu_molec / (c->prm.u_scale * c->prm.z_i)) : 0.0;
+    return k;
+}
+
+static inline dim3 grid_for(long n) {
+    long b = (n + 255) / 256;
+    if (b > 65535L * 16) b = 65535L * 16;
+    return dim3((unsigned)b);
+}
+
+#define GRID_LOOP(i, n) for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long)gridDim.x * blockDim.x)
+
+__device__ __forceinline__ double smag(double s11, double s12, double s13, double s22, double s23, double s33) {
+    return sqrt(2.0 * (s11 * s11 + s22 * s22 + s33 * s33 + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
+}
+
+// ---- fills and reductions ------------------------------------------------
+__global__ void k_fill(double* f, long n, double v) {
+    GRID_LOOP(i, n) f[i] = v;
+}
+
+void fill_levels(hipStream_t s, double* f, size_t plane, int lev0, int nlev, double v) {
+    const long n = (long)plane * nlev;
+    hipLaunchKernelGGL(k_fill, grid_for(n), dim3(256), 0, s, f + (size_t)lev0 * plane, n, v);
+}
+
+// one block per level: out[l] = sum(plane l) / nn, deterministic tree order
+__global__ __launch_bounds__(256) void k_level_means(const double* __restrict__ f, long plane, double nn,
+                                                      double* __restrict__ out) {
+    __shared__ double sh[256];
+    const double* p = f + (long)blockIdx.x * plane;
+    double s = 0.0;
+    for (long i = threadIdx.x; i < plane; i += 256) s += p[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0] / nn;
+}
+
+void level_means(hipStream_t s, const double* f, size_t plane, int lev0, int nlev, double nn, double* out) {
+    hipLaunchKernelGGL(k_level_means, dim3(nlev), dim3(256), 0, s, f + (size_t)lev0 * plane, (long)plane, nn, out);
+}
+
+// obukhov sums at the wall level: [0] sum sqrt(u^2+v^2), [1] sum psi_m,
+// [2] sum sgs_t3(1), [3] sum theta(1)
+__global__ __launch_bounds__(256) void k_obukhov_sums(const double* __restrict__ u1, const double* __restrict__ v1,
+                                                       const double* __restrict__ psi, const double* __restrict__ t3,
+                                                       const double* __restrict__ th1, long plane,
+                                                       double* __restrict__ out) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    const int which = blockIdx.x;
+    for (long i = threadIdx.x; i < plane; i += 256) {
+        if (which == 0) s += sqrt(u1[i] * u1[i] + v1[i] * v1[i]);
+        else if (which == 1) s += psi[i];
+        else if (which == 2) s += t3 ? t3[i] : 0.0;
+        else s += th1 ? th1[i] : 0.0;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[which] = sh[0];
+}
+
+// neutral / theta-off branch, 010_scalars.f90:718-738
+__global__ void k_obukhov_neutral(double* ustar, double* phi_m, double* psi_h, double* phi_h,
+                                  const double* __restrict__ sums, long plane, double nn, int smooth, double zo,
+                                  double dz, double nu_nd) {
+    const double su = sums[0], sp = sums[1];
+    GRID_LOOP(i, plane) {
+        phi_m[i] = 1.0; psi_h[i] = 0.0; phi_h[i] = 1.0;
+        if (smooth) ustar[i] = su / nn / (log(0.5 * dz / nu_nd) / VONK + B_SMOOTH);
+        else ustar[i] = su / nn * VONK / (log(0.5 * dz / zo) - sp / nn);
+    }
+}
+
+// theta-active branch with homogeneous surface, 010_scalars.f90:758-843
+__global__ void k_obukhov_active(double* ustar, double* psi_m, double* phi_m, double* psi_h, double* phi_h,
+                                 const double* __restrict__ sums, long plane, double nn, double zo_, double dz,
+                                 double g_) {
+    const double wt_ = sums[2] / nn;
+    const double U_tmp = sums[0] / nn;
+    const double psi_m_tmp = sums[1] / nn;
+    const double theta_ = sums[3] / nn;
+    const double ustar_ = U_tmp * VONK / (log(0.5 * dz / zo_) - psi_m_tmp);
+    const double L = -(ustar_ * ustar_ * ustar_) / (VONK * g_ / theta_ * wt_);
+    double pm, ph, fm, fh;
+    if (L < 0.0 && wt_ != 0.0) {
+        const double x = pow(1.0 - 16.0 * dz / 2.0 / L, 0.25);
+        pm = 2.0 * log((1.0 + x) / 2.0) + log((1.0 + x * x) / 2.0) - 2.0 * atan(x) + PI / 2.0;
+        ph = 2.0 * log((1.0 + x * x) / 2.0);
+        fm = 1.0 / x;
+        fh = 1.0 / (x * x);
+    } else if (L > 0.0 && wt_ != 0.0) {
+        const double zeta = 0.5 * dz / L;
+        fm = 1.0 + 4.8 * zeta;
+        fh = 1.0 + 7.8 * zeta;
+        pm = -1.0 * 4.8 * zeta;
+        ph = -1.0 * 7.8 * zeta;
+    } else {
+        pm = 0.0; ph = 0.0; fm = 1.0; fh = 1.0;
+    }
+    GRID_LOOP(i, plane) {
+        ustar[i] = ustar_;
+        psi_m[i] = pm; psi_h[i] = ph; phi_m[i] = fm; phi_h[i] = fh;
+    }
+}
+
+void obukhov_dev(les_ctx* c, int jt) {
+    const les_params& P = c->prm;
+    const long np = (long)c->plane;
+    const double nn = (double)(c->nx * c->ny);
+    const double* u1 = c->f3[LES_F_U] + np;
+    const double* v1 = c->f3[LES_F_V] + np;
+    hipStream_t s = c->stream;
+    if (!P.theta_flag || jt < P.theta_init_time) {
+        if (!P.restart_theta) fill_levels(s, c->psi_m, c->plane, 0, 1, 0.0);
+        hipLaunchKernelGGL(k_obukhov_sums, dim3(2), dim3(256), 0, s, u1, v1, c->psi_m, (const double*)nullptr,
+                           (const double*)nullptr, np, c->red);
+        hipLaunchKernelGGL(k_obukhov_neutral, grid_for(np), dim3(256), 0, s, c->ustar, c->phi_m, c->psi_h,
+                           c->phi_h, (const double*)c->red, np, nn, (int)(P.walltype == LES_WALL_SMOOTH), P.zo,
+                           c->dz, P.nu_molec / P.u_scale / P.z_i);
+        return;
+    }
+    double* t3 = c->f3[LES_F_SGS_T3] + np;
+    if (jt == P.theta_init_time && !P.restart_theta)
+        fill_levels(s, t3, c->plane, 0, 1, P.wt_s / P.u_scale / P.T_scale);
+    hipLaunchKernelGGL(k_obukhov_sums, dim3(4), dim3(256), 0, s, u1, v1, c->psi_m, (const double*)t3,
+                       (const double*)(c->f3[LES_F_THETA] + np), np, c->red);
+    // zo_ = exp(mean(log(zo))) of a homogeneous zo, summed in order
+    double sl = 0.0;
+    const double lz = log(P.zo);
+    for (long i = 0; i < np; ++i) sl += lz;
+    const double zo_ = exp(sl / nn);
+    const double g_ = GRAV / (P.u_scale * P.u_scale / P.z_i);
+    hipLaunchKernelGGL(k_obukhov_active, grid_for(np), dim3(256), 0, s, c->ustar, c->psi_m, c->phi_m, c->psi_h,
+                       c->phi_h, (const double*)c->red, np, nn, zo_, c->dz, g_);
+}
+
+// ---- wall stress, 104_wallstress.f90:98-260 ------------------------------
+__global__ void k_wall(double* txz, double* tyz, double* dudz, double* dvdz, double* ustar,
+                       const double* __restrict__ u, const double* __restrict__ v,
+                       const double* __restrict__ u1, const double* __restrict__ v1,
+                       const double* __restrict__ psi_m, const double* __restrict__ phi_m, long plane, int top,
+                       int smooth, double dz, double d0, double zo, double nu_nd) {
+    GRID_LOOP(i, plane) {
+        double denom;
+        if (smooth) denom = log(0.5 * dz / nu_nd) / VONK + B_SMOOTH;
+        else if (top) denom = log((0.5 * dz - d0) / zo) / VONK;
+        else denom = (log((0.5 * dz - d0) / zo) - psi_m[i]) / VONK;
+        const double u_avg = sqrt(u1[i] * u1[i] + v1[i] * v1[i]);
+        const double us = u_avg / denom;
+        const double cst = -(us * us) / u_avg;
+        if (!top) {
+            ustar[i] = us;
+            txz[i] = cst * u1[i];
+            tyz[i] = cst * v1[i];
+            const double du = us / ((0.5 * dz - d0) * VONK) * u[i] / u_avg * phi_m[i];
+            const double dv = us / ((0.5 * dz - d0) * VONK) * v[i] / u_avg * phi_m[i];
+            dudz[i] = (u[i] == 0.0) ? 0.0 : du;
+            dvdz[i] = (v[i] == 0.0) ? 0.0 : dv;
+        } else {
+            txz[i] = -cst * u1[i];
+            tyz[i] = -cst * v1[i];
+            const double du = -us / (0.5 * dz * VONK) * u[i] / u_avg;
+            const double dv = -us / (0.5 * dz * VONK) * v[i] / u_avg;
+            dudz[i] = (u[i] == 0.0) ? 0.0 : du;
+            dvdz[i] = (v[i] == 0.0) ? 0.0 : dv;
+        }
+    }
+}
+
+void wallstress_dev(les_ctx* c, int top, const double* u1, const double* v1) {
+    const les_params& P = c->prm;
+    const long np = (long)c->plane;
+    const int lev = top ? c->nz : 1;
+    const int ulev = top ? c->nz - 1 : 1;
+    double* txz = c->f3[LES_F_TXZ] + lev * np;
+    double* tyz = c->f3[LES_F_TYZ] + lev * np;
+    double* dudz = c->f3[LES_F_DUDZ] + lev * np;
+    double* dvdz = c->f3[LES_F_DVDZ] + lev * np;
+    const int sf = top ? (P.ubc_mom == LES_BC_STRESS_FREE) : (P.lbc_mom == LES_BC_STRESS_FREE);
+    if (sf) {
+        fill_levels(c->stream, txz, c->plane, 0, 1, 0.0);
+        fill_levels(c->stream, tyz, c->plane, 0, 1, 0.0);
+        fill_levels(c->stream, dudz, c->plane, 0, 1, 0.0);
+        fill_levels(c->stream, dvdz, c->plane, 0, 1, 0.0);
+        return;
+    }
+    hipLaunchKernelGGL(k_wall, grid_for(np), dim3(256), 0, c->stream, txz, tyz, dudz, dvdz, c->ustar,
+                       (const double*)(c->f3[LES_F_U] + ulev * np), (const double*)(c->f3[LES_F_V] + ulev * np),
+                       u1, v1, (const double*)c->psi_m, (const double*)c->phi_m, np, top,
+                       (int)(P.walltype == LES_WALL_SMOOTH), c->dz, P.d0, P.zo, P.nu_molec / P.u_scale / P.z_i);
+}
+
+// ---- sgs_stag, 106_sgs_stag.f90 ---------------------------------------------
+struct Grad {
+    const double *ux, *uy, *uz, *vx, *vy, *vz, *wx, *wy, *wz;
+};
+struct Strain {
+    double *s11, *s12, *s13, *s22, *s23, *s33;
+};
+
+// S_ij on w-nodes, uvp nodes at the wall (:47-68) and the top (:83-104)
+__global__ void k_strain(Grad g, Strain S, KP p) {
+    const long n = p.plane * p.nz;  // levels 1..nz
+    GRID_LOOP(t, n) {
+        const long i = p.plane + t;
+        const int k = (int)(i / p.plane);
+        double ux, uy, uz, vx, vy, vz, wx, wy, wz;
+        if (k == 1) {
+            ux = g.ux[i]; uy = g.uy[i]; uz = g.uz[i]; vx = g.vx[i]; vy = g.vy[i]; vz = g.vz[i];
+            wx = 0.5 * (g.wx[i] + g.wx[i + p.plane]);
+            wy = 0.5 * (g.wy[i] + g.wy[i + p.plane]);
+            wz = g.wz[i];
+        } else if (k == p.nz) {
+            const long m = i - p.plane;
+            ux = g.ux[m]; uy = g.uy[m]; uz = g.uz[i]; vx = g.vx[m]; vy = g.vy[m]; vz = g.vz[i];
+            wx = 0.5 * (g.wx[m] + g.wx[i]);
+            wy = 0.5 * (g.wy[m] + g.wy[i]);
+            wz = g.wz[m];
+        } else {
+            const long m = i - p.plane;
+            ux = 0.5 * (g.ux[i] + g.ux[m]); uy = 0.5 * (g.uy[i] + g.uy[m]); uz = g.uz[i];
+            vx = 0.5 * (g.vx[i] + g.vx[m]); vy = 0.5 * (g.vy[i] + g.vy[m]); vz = g.vz[i];
+            wx = g.wx[i]; wy = g.wy[i]; wz = 0.5 * (g.wz[i] + g.wz[m]);
+        }
+        S.s11[i] = ux; S.s12[i] = 0.5 * (uy + vx); S.s13[i] = 0.5 * (uz + wx);
+        S.s22[i] = vy; S.s23[i] = 0.5 * (vz + wy); S.s33[i] = wz;
+    }
+}
+
+static Grad grads(les_ctx* c) {
+    return {c->f3[LES_F_DUDX], c->f3[LES_F_DUDY], c->f3[LES_F_DUDZ], c->f3[LES_F_DVDX], c->f3[LES_F_DVDY],
+            c->f3[LES_F_DVDZ], c->f3[LES_F_DWDX], c->f3[LES_F_DWDY], c->f3[LES_F_DWDZ]};
+}
+static Strain strains(les_ctx* c) {
+    return {c->f3[LES_F_S11], c->f3[LES_F_S12], c->f3[LES_F_S13], c->f3[LES_F_S22], c->f3[LES_F_S23],
+            c->f3[LES_F_S33]};
+}
+
+void strain_dev(les_ctx* c) {
+    KP p = kp(c);
+    hipLaunchKernelGGL(k_strain, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, grads(c), strains(c), p);
+}
+
+__global__ void k_lag_acc(double* ul, double* vl, double* wl, const double* __restrict__ u,
+                          const double* __restrict__ v, const double* __restrict__ w, long n) {
+    GRID_LOOP(i, n) {
+        ul[i] = ul[i] + u[i];
+        vl[i] = vl[i] + v[i];
+        wl[i] = wl[i] + w[i];
+    }
+}
+
+void lag_accumulate_dev(les_ctx* c) {
+    const long n = (long)(c->plane * c->nlev);
+    hipLaunchKernelGGL(k_lag_acc, grid_for(n), dim3(256), 0, c->stream, c->f3[LES_F_U_LAG], c->f3[LES_F_V_LAG],
+                       c->f3[LES_F_W_LAG], (const double*)c->f3[LES_F_U], (const double*)c->f3[LES_F_V],
+                       (const double*)c->f3[LES_F_W], n);
+}
+
+void cs_init_dev(les_ctx* c) { fill_levels(c->stream, c->f3[LES_F_CS_OPT2], c->plane, 1, c->nz, 0.03); }
+
+// Nu_t (:206-216) and tau_ij (:219-315)
+__global__ void k_stress(Strain S, const double* __restrict__ Cs, double* Nu, double* txx, double* txy,
+                         double* txz, double* tyy, double* tyz, double* tzz, KP p) {
+    // pass 1 is Nu_t; it is computed per point of the levels this point reads
+    const long n = p.plane * (p.nz + 1);
+    const double l2 = p.delta * p.delta;
+    GRID_LOOP(i, n) {
+        const int k = (int)(i / p.plane);
+        if (k == 0) {
+            txx[i] = BOGUS; txy[i] = BOGUS; txz[i] = BOGUS; tyy[i] = BOGUS; tyz[i] = BOGUS; tzz[i] = BOGUS;
+            continue;
+        }
+        const double nu_k = smag(S.s11[i], S.s12[i], S.s13[i], S.s22[i], S.s23[i], S.s33[i]) * Cs[i] * l2;
+        Nu[i] = nu_k;
+        if (k == p.nz) {
+            txx[i] = BOGUS; txy[i] = BOGUS; tyy[i] = BOGUS; tzz[i] = BOGUS;
+            continue;  // txz, tyz(nz) from wallstress_top
+        }
+        if (k == 1) {
+            const double f = -2.0 * (nu_k + p.nu);
+            txx[i] = f * S.s11[i]; txy[i] = f * S.s12[i]; tyy[i] = f * S.s22[i]; tzz[i] = f * S.s33[i];
+            continue;  // txz, tyz(1) from wallstress_bottom
+        }
+        if (k == p.nz - 1) {
+            const long t = i + p.plane;
+            const double nu_t = smag(S.s11[t], S.s12[t], S.s13[t], S.s22[t], S.s23[t], S.s33[t]) * Cs[t] * l2;
+            const double f = -2.0 * (nu_t + p.nu);
+            txx[i] = f * S.s11[t]; txy[i] = f * S.s12[t]; tyy[i] = f * S.s22[t]; tzz[i] = f * S.s33[t];
+        } else {
+            const long t = i + p.plane;
+            const double nu_p = smag(S.s11[t], S.s12[t], S.s13[t], S.s22[t], S.s23[t], S.s33[t]) * Cs[t] * l2;
+            const double kk = 0.5 * (nu_k + nu_p);
+            const double f = -2.0 * (kk + p.nu);
+            txx[i] = f * 0.5 * (S.s11[i] + S.s11[t]);
+            txy[i] = f * 0.5 * (S.s12[i] + S.s12[t]);
+            tyy[i] = f * 0.5 * (S.s22[i] + S.s22[t]);
+            tzz[i] = f * 0.5 * (S.s33[i] + S.s33[t]);
+        }
+        txz[i] = -2.0 * (nu_k + p.nu) * S.s13[i];
+        tyz[i] = -2.0 * (nu_k + p.nu) * S.s23[i];
+    }
+}
+
+void nu_t_stress_dev(les_ctx* c) {
+    KP p = kp(c);
+    hipLaunchKernelGGL(k_stress, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, strains(c),
+                       (const double*)c->f3[LES_F_CS_OPT2], c->f3[LES_F_NU_T], c->f3[LES_F_TXX],
+                       c->f3[LES_F_TXY], c->f3[LES_F_TXZ], c->f3[LES_F_TYY], c->f3[LES_F_TYZ], c->f3[LES_F_TZZ], p);
+}
+
+// ---- interpolag_Sdep, 106_06_interpolag_Sdep.f90:4-336 -------------------
+__device__ __forceinline__ double ulag_at(const double* __restrict__ ul, const double* __restrict__ st, long i,
+                                          int k, long plane, int nz, double csc) {
+    if (k == 1) return ul[i] / csc + st[1];                                 // :181-186
+    if (k == nz) return ul[i - plane] + st[nz - 1];                         // :195-199 (raw sum)
+    const double a = 0.5 * (ul[i] / csc + ul[i - plane] / csc);            // :207-213
+    return a + 0.5 * (st[k] + st[k - 1]);
+}
+
+__device__ __forceinline__ double wlag_at(const double* __restrict__ wl, long i, int k, long plane, int nz,
+                                          double csc) {
+    if (k == 1) return 0.25 * (wl[i + plane] / csc);                        // :221
+    if (k == nz) return 0.25 * (wl[i - plane] / csc);                       // :225
+    return wl[i] / csc;
+}
+
+__global__ void k_interpolag(const double* __restrict__ ul, const double* __restrict__ vl,
+                             const double* __restrict__ wl, const double* __restrict__ ust,
+                             const double* __restrict__ vst, const double* __restrict__ o0,
+                             const double* __restrict__ o1, const double* __restrict__ o2,
+                             const double* __restrict__ o3, double* f0, double* f1, double* f2, double* f3, KP p,
+                             double csc) {
+    const long n = p.plane * p.nz;
+    GRID_LOOP(t, n) {
+        const long i = p.plane + t;
+        const int k = (int)(i / p.plane);
+        const long q = i - (long)k * p.plane;
+        const int jy = (int)(q / p.nx), jx = (int)(q - (long)jy * p.nx);
+        const double xu = ulag_at(ul, ust, i, k, p.plane, p.nz, csc);
+        const double yv = ulag_at(vl, vst, i, k, p.plane, p.nz, csc);
+        const double xp = -xu * p.dt * csc / p.dx;
+        const double yp = -yv * p.dt * csc / p.dy;
+        double zp;
+        if (k <= 2) {                                                        // :239-257
+            const long i2 = q + 2 * p.plane;
+            double z2 = -wlag_at(wl, i2, 2, p.plane, p.nz, csc) * p.dt * csc / p.dz;
+            z2 = z2 + fmin(z2, 0.0);
+            z2 = fmin(1.0, z2);
+            z2 = fmax(-1.0, z2);
+            zp = z2;
+        } else {
+            zp = -wlag_at(wl, i, k, p.plane, p.nz, csc) * p.dt * csc / p.dz;
+        }
+        const int ax = (copysign(1.0, xp) > 0.0) ? 1 : -1;
+        const int ay = (copysign(1.0, yp) > 0.0) ? 1 : -1;
+        const int az = (copysign(1.0, zp) > 0.0) ? 1 : -1;
+        const int x2 = (jx + ax + p.nx) % p.nx, y2 = (jy + ay + p.ny) % p.ny;
+        int k2 = k + az;
+        k2 = k2 < 1 ? 1 : (k2 > p.nz ? p.nz : k2);
+        const double cx = fabs(xp), cy = fabs(yp), cz = fabs(zp);
+        const double fx = 1.0 - cx, fy = 1.0 - cy, fz = 1.0 - cz;
+        const long b00 = (long)k * p.plane, b10 = (long)k2 * p.plane;
+        const long r0 = (long)jy * p.nx, r1 = (long)y2 * p.nx;
+        const double* src[4] = {o0, o1, o2, o3};
+        double* dst[4] = {f0, f1, f2, f3};
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const double* A = src[m];
+            dst[m][i] = fx * fy * (A[b00 + r0 + jx] * fz + A[b10 + r0 + jx] * cz) +
+                        fx * cy * (A[b00 + r1 + jx] * fz + A[b10 + r1 + jx] * cz) +
+                        cx * fy * (A[b00 + r0 + x2] * fz + A[b10 + r0 + x2] * cz) +
+                        cx * cy * (A[b00 + r1 + x2] * fz + A[b10 + r1 + x2] * cz);
+        }
+    }
+}
+
+void interpolag_dev(les_ctx* c) {
+    KP p = kp(c);
+    const size_t bytes = sizeof(double) * c->plane * c->nlev;
+    const int ids[4] = {LES_F_F_LM, LES_F_F_MM, LES_F_F_QN, LES_F_F_NN};
+    for (int m = 0; m < 4; ++m)
+        hipMemcpyAsync(c->fold[m], c->f3[ids[m]], bytes, hipMemcpyDeviceToDevice, c->stream);
+    hipLaunchKernelGGL(k_interpolag, grid_for(p.plane * p.nz), dim3(256), 0, c->stream,
+                       (const double*)c->f3[LES_F_U_LAG], (const double*)c->f3[LES_F_V_LAG],
+                       (const double*)c->f3[LES_F_W_LAG], (const double*)c->ust, (const double*)c->vst,
+                       (const double*)c->fold[0], (const double*)c->fold[1], (const double*)c->fold[2],
+                       (const double*)c->fold[3], c->f3[ids[0]], c->f3[ids[1]], c->f3[ids[2]], c->f3[ids[3]], p,
+                       (double)c->prm.cs_count);
+    const long n = (long)(c->plane * c->nlev);                              // :330-332
+    hipLaunchKernelGGL(k_fill, grid_for(n), dim3(256), 0, c->stream, c->f3[LES_F_U_LAG], n, 0.0);
+    hipLaunchKernelGGL(k_fill, grid_for(n), dim3(256), 0, c->stream, c->f3[LES_F_V_LAG], n, 0.0);
+    hipLaunchKernelGGL(k_fill, grid_for(n), dim3(256), 0, c->stream, c->f3[LES_F_W_LAG], n, 0.0);
+}
+
+// ---- lagrange_Sdep, 106_03_lagrange_Sdep.f90:70-331 -------------------------
+// buf21 layout: field f (0..20) of level jz (1..nz) at ((f * nz) + jz-1) * plane.
+// f: 0 u_bar, 1 v_bar, 2 w_bar, 3..8 products p11 p12 p13 p22 p23 p33,
+//    9..14 S_ij, 15..20 |S| S_ij  (ij order 11 12 13 22 23 33)
+__global__ void k_lag_prep(const double* __restrict__ u, const double* __restrict__ v,
+                           const double* __restrict__ w, Strain S, double* __restrict__ buf, KP p) {
+    const long n = p.plane * p.nz;
+    GRID_LOOP(t, n) {
+        const long i = p.plane + t;
+        const int k = (int)(i / p.plane);
+        double ub, vb, wb;
+        if (k == 1) { ub = u[i]; vb = v[i]; wb = 0.25 * w[i + p.plane]; }
+        else if (k == p.nz) { ub = u[i - p.plane]; vb = v[i - p.plane]; wb = 0.25 * w[i - p.plane]; }
+        else { ub = 0.5 * (u[i] + u[i - p.plane]); vb = 0.5 * (v[i] + v[i - p.plane]); wb = w[i]; }
+        const double s[6] = {S.s11[i], S.s12[i], S.s13[i], S.s22[i], S.s23[i], S.s33[i]};
+        const double sm = smag(s[0], s[1], s[2], s[3], s[4], s[5]);
+        const long fs = p.plane * p.nz;
+        double* o = buf + t;
+        o[0 * fs] = ub; o[1 * fs] = vb; o[2 * fs] = wb;
+        o[3 * fs] = ub * ub; o[4 * fs] = ub * vb; o[5 * fs] = ub * wb;
+        o[6 * fs] = vb * vb; o[7 * fs] = vb * wb; o[8 * fs] = wb * wb;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            o[(9 + m) * fs] = s[m];
+            o[(15 + m) * fs] = sm * s[m];
+        }
+    }
+}
+
+void lagrange_prep_dev(les_ctx* c, double* buf21) {
+    KP p = kp(c);
+    hipLaunchKernelGGL(k_lag_prep, grid_for(p.plane * p.nz), dim3(256), 0, c->stream,
+                       (const double*)c->f3[LES_F_U], (const double*)c->f3[LES_F_V], (const double*)c->f3[LES_F_W],
+                       strains(c), buf21, p);
+}
+
+__device__ __forceinline__ void lag_relax(double& F1, double& F2, double A, double B, double opftdelta,
+                                          double lagran_dt) {
+    const double CLIP = (double)1.e-24f;
+    double Tn = fmax(F1 * F2, CLIP);
+    Tn = opftdelta * pow(Tn, -1.0 / 8.0);
+    Tn = fmax(CLIP, Tn);
+    const double dumfac = lagran_dt / Tn;
+    const double epsi = dumfac / (1.0 + dumfac);
+    F1 = epsi * A + (1.0 - epsi) * F1;
+    F2 = epsi * B + (1.0 - epsi) * F2;
+    F1 = fmax(CLIP, F1);
+}
+
+__global__ void k_lag_contract(const double* __restrict__ b2, const double* __restrict__ b4, double* FLM,
+                               double* FMM, double* FQN, double* FNN, double* Cs, KP p, int init_lm, int init_qn,
+                               double lagran_dt, double bexp) {
+    const long n = p.plane * p.nz;
+    const long fs = n;
+    const double CLIP = (double)1.e-24f;
+    const double cst = 2.0 * (p.delta * p.delta);
+    const double opftdelta = OPFTIME * p.delta;
+    GRID_LOOP(t, n) {
+        const long i = p.plane + t;
+        const int k = (int)(i / p.plane);
+        const double* a = b2 + t;
+        const double* h = b4 + t;
+        const double ub = a[0], vb = a[fs], wb = a[2 * fs];
+        const double uh = h[0], vh = h[fs], wh = h[2 * fs];
+        const double l11 = a[3 * fs] - ub * ub, l12 = a[4 * fs] - ub * vb, l13 = a[5 * fs] - ub * wb;
+        const double l22 = a[6 * fs] - vb * vb, l23 = a[7 * fs] - vb * wb, l33 = a[8 * fs] - wb * wb;
+        const double q11 = h[3 * fs] - uh * uh, q12 = h[4 * fs] - uh * vh, q13 = h[5 * fs] - uh * wh;
+        const double q22 = h[6 * fs] - vh * vh, q23 = h[7 * fs] - vh * wh, q33 = h[8 * fs] - wh * wh;
+        double sb[6], sh[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) { sb[m] = a[(9 + m) * fs]; sh[m] = h[(9 + m) * fs]; }
+        const double S_bar = smag(sb[0], sb[1], sb[2], sb[3], sb[4], sb[5]);
+        const double S_hat = smag(sh[0], sh[1], sh[2], sh[3], sh[4], sh[5]);
+        double M[6], N[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            M[m] = cst * (a[(15 + m) * fs] - 4.0 * S_bar * sb[m]);
+            N[m] = cst * (h[(15 + m) * fs] - 16.0 * S_hat * sh[m]);
+        }
+        const double LM = l11 * M[0] + l22 * M[3] + l33 * M[5] + 2.0 * (l12 * M[1] + l13 * M[2] + l23 * M[4]);
+        const double MM = M[0] * M[0] + M[3] * M[3] + M[5] * M[5] + 2.0 * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]);
+        const double QN = q11 * N[0] + q22 * N[3] + q33 * N[5] + 2.0 * (q12 * N[1] + q13 * N[2] + q23 * N[4]);
+        const double NN = N[0] * N[0] + N[3] * N[3] + N[5] * N[5] + 2.0 * (N[1] * N[1] + N[2] * N[2] + N[4] * N[4]);
+        double flm = FLM[i], fmm = FMM[i], fqn = FQN[i], fnn = FNN[i];
+        if (init_lm) { fmm = MM; flm = 0.03 * MM; }
+        lag_relax(flm, fmm, LM, MM, opftdelta, lagran_dt);
+        const double cs2d = fmax(CLIP, flm / fmm);
+        if (init_qn) { fnn = NN; fqn = 0.03 * NN; }
+        lag_relax(fqn, fnn, QN, NN, opftdelta, lagran_dt);
+        const double cs4d = fmax(CLIP, fqn / fnn);
+        double beta = pow(cs4d / cs2d, bexp);
+        if (k == p.nz) beta = 1.0;
+        const double betaclip = fmax(beta, 1.0 / (2.0 * 4.0));
+        FLM[i] = flm; FMM[i] = fmm; FQN[i] = fqn; FNN[i] = fnn;
+        Cs[i] = fmax(CLIP, cs2d / betaclip);
+    }
+}
+
+void lagrange_contract_dev(les_ctx* c, const double* b2, const double* b4, const double* raw, int init_lm,
+                           int init_qn) {
+    (void)raw;
+    KP p = kp(c);
+    const double bexp = log(2.0) / (log(4.0) - log(2.0));
+    hipLaunchKernelGGL(k_lag_contract, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, b2, b4,
+                       c->f3[LES_F_F_LM], c->f3[LES_F_F_MM], c->f3[LES_F_F_QN], c->f3[LES_F_F_NN],
+                       c->f3[LES_F_CS_OPT2], p, init_lm, init_qn, c->prm.dt * (double)c->prm.cs_count, bexp);
+}
+
+// ---- vertical differences, 004_derivatives.f90:85-183 --------------------
+__global__ void k_ddz_uv(double* d, const double* __restrict__ f, KP p) {
+    const long n = p.plane * (p.nz + 1);
+    const double cst = 1.0 / p.dz;
+    GRID_LOOP(i, n) {
+        const int k = (int)(i / p.plane);
+        if (k == 0) d[i] = BOGUS;
+        else if (k >= 2 && k <= p.nz - 1) d[i] = cst * (f[i] - f[i - p.plane]);
+    }
+}
+
+__global__ void k_ddz_w(double* d, const double* __restrict__ f, KP p) {
+    const long n = p.plane * (p.nz + 1);
+    const double cst = 1.0 / p.dz;
+    GRID_LOOP(i, n) {
+        const int k = (int)(i / p.plane);
+        if (k == 0) d[i] = BOGUS;
+        else if (k == p.nz) d[i] = 0.0;
+        else d[i] = cst * (f[i + p.plane] - f[i]);
+    }
+}
+
+void ddz_uv_dev(les_ctx* c, double* d, const double* f) {
+    KP p = kp(c);
+    hipLaunchKernelGGL(k_ddz_uv, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, d, f, p);
+}
+void ddz_w_dev(les_ctx* c, double* d, const double* f) {
+    KP p = kp(c);
+    hipLaunchKernelGGL(k_ddz_w, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, d, f, p);
+}
+
+// ---- divstress, 107_divstress.f90:36-102 ---------------------------------------
+// a = d/dx tx, b = d/dy ty, f = tz (FD taken here: ddz_w for uv, ddz_uv for w)
+__global__ void k_divstress(double* out, const double* __restrict__ a, const double* __restrict__ b,
+                            const double* __restrict__ f, KP p, int wver) {
+    const long n = p.plane * (p.nz + 1);
+    const double cst = 1.0 / p.dz;
+    GRID_LOOP(i, n) {
+        const int k = (int)(i / p.plane);
+        if (!wver) {
+            if (k == 0 || k == p.nz) out[i] = BOGUS;
+            else out[i] = a[i] + b[i] + cst * (f[i + p.plane] - f[i]);
+        } else {
+            if (k == 0) out[i] = BOGUS;
+            else if (k == 1 || k == p.nz) out[i] = a[i] + b[i];
+            else out[i] = a[i] + b[i] + cst * (f[i] - f[i - p.plane]);
+        }
+    }
+}
+
+void divstress_combine_dev(les_ctx* c, double* out, const double* a, const double* b, const double* d, int wver) {
+    KP p = kp(c);
+    hipLaunchKernelGGL(k_divstress, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, out, a, b, d, p, wver);
+}
+
+// ---- convec, 108_convec.f90:44-69 and cross_product :103-202 -------------------
+__global__ void k_vort(double* v1, double* v2, double* v3, Grad g, KP p) {
+    const long n = p.plane * p.nz;
+    GRID_LOOP(t, n) {
+        const long i = p.plane + t;
+        const int k = (int)(i / p.plane);
+        if (k == 1) {
+            v1[i] = 0.5 * (g.wy[i] + g.wy[i + p.plane]) - g.vz[i];
+            v2[i] = g.uz[i] - 0.5 * (g.wx[i] + g.wx[i + p.plane]);
+        } else if (k == p.nz) {
+            v1[i] = 0.5 * (g.wy[i] + g.wy[i - p.plane]) - g.vz[i];
+            v2[i] = g.uz[i] - 0.5 * (g.wx[i] + g.wx[i - p.plane]);
+        } else {
+            v1[i] = g.wy[i] - g.vz[i];
+            v2[i] = g.uz[i] - g.wx[i];
+        }
+        v3[i] = g.vx[i] - g.uy[i];
+    }
+}
+
+void vorticity_dev(les_ctx* c, double* v1, double* v2, double* v3) {
+    KP p = kp(c);
+    hipLaunchKernelGGL(k_vort, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, v1, v2, v3, grads(c), p);
+}
+
+__global__ void k_cross(const double* __restrict__ ub, const double* __restrict__ vb,
+                        const double* __restrict__ wb, const double* __restrict__ b1,
+                        const double* __restrict__ b2, const double* __restrict__ b3,
+                        const double* __restrict__ ust, const double* __restrict__ vst, double* cxb, double* cyb,
+                        double* czb, KP p) {
+    const long P2 = p.plane2;
+    const long n = P2 * (p.nz - 1);  // levels 1..nz-1
+    GRID_LOOP(t, n) {
+        const long i = P2 + t;
+        const int k = (int)(i / P2);
+        if (k == 1) {
+            cxb[i] = (vb[i] + vst[1]) * (-b3[i]) + 0.5 * wb[i + P2] * b2[i + P2];
+            cyb[i] = (ub[i] + ust[1]) * (b3[i]) + 0.5 * wb[i + P2] * (-b1[i + P2]);
+            czb[i] = 0.0;
+        } else {
+            cxb[i] = (vb[i] + vst[k]) * (-b3[i]) + 0.5 * (wb[i + P2] * (b2[i + P2]) + wb[i] * (b2[i]));
+            cyb[i] = (ub[i] + ust[k]) * (b3[i]) + 0.5 * (wb[i + P2] * (-b1[i + P2]) + wb[i] * (-b1[i]));
+            czb[i] = 0.5 * ((ub[i] + ub[i - P2] + ust[k] + ust[k - 1]) * (-b2[i]) +
+                            (vb[i] + vb[i - P2] + vst[k] + vst[k - 1]) * (b1[i]));
+        }
+    }
+}
+
+void cross_product_dev(les_ctx* c, const double* b1, const double* b2, const double* b3, double* cxb,
+                       double* cyb, double* czb) {
+    KP p = kp(c);
+    hipLaunchKernelGGL(k_cross, grid_for(p.plane2 * (p.nz - 1)), dim3(256), 0, c->stream,
+                       (const double*)c->u_big, (const double*)c->v_big, (const double*)c->w_big, b1, b2, b3,
+                       (const double*)c->ust, (const double*)c->vst, cxb, cyb, czb, p);
+}
+
+// ---- scalar transport, 010_scalars.f90:111-671 ----------------------------------
+__global__ void k_theta_prep(double* dTdz, double* rhsf, const double* __restrict__ rhs, KP p, double top) {
+    const long n = p.plane * (p.nz + 1);
+    GRID_LOOP(i, n) {
+        rhsf[i] = rhs[i];
+        if (i >= p.plane * p.nz) dTdz[i] = top;
+    }
+}
+
+void theta_prep_dev(les_ctx* c) {
+    KP p = kp(c);
+    const les_params& P = c->prm;
+    hipLaunchKernelGGL(k_theta_prep, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, c->f3[LES_F_DTDZ],
+                       c->f3[LES_F_RHST_F], (const double*)c->f3[LES_F_RHST], p, P.dTdz_top / P.T_scale * P.z_i);
+}
+
+// dealiased advection on the 3/2 grid, levels 1..nz-1 (:466-484)
+__global__ void k_scalar_adv(const double* __restrict__ ub, const double* __restrict__ vb,
+                             const double* __restrict__ wb, const double* __restrict__ sx,
+                             const double* __restrict__ sy, const double* __restrict__ sz,
+                             const double* __restrict__ ust, const double* __restrict__ vst, double* rm, KP p) {
+    const long P2 = p.plane2;
+    const long n = P2 * (p.nz - 1);
+    GRID_LOOP(t, n) {
+        const long i = P2 + t;
+        const int k = (int)(i / P2);
+        if (k == 1)
+            rm[i] = (ub[i] + ust[1]) * sx[i] + (vb[i] + vst[1]) * sy[i] + (0.5 * wb[i + P2]) * sz[i + P2];
+        else
+            rm[i] = (ub[i] + ust[k]) * sx[i] + (vb[i] + vst[k]) * sy[i] + (wb[i] * sz[i] + wb[i + P2] * sz[i + P2]) / 2.0;
+    }
+}
+
+void scalar_adv_dev(les_ctx* c, const double* dsx, const double* dsy, const double* dsz, double* rm) {
+    KP p = kp(c);
+    hipLaunchKernelGGL(k_scalar_adv, grid_for(p.plane2 * (p.nz - 1)), dim3(256), 0, c->stream,
+                       (const double*)c->u_big, (const double*)c->v_big, (const double*)c->w_big, dsx, dsy, dsz,
+                       (const double*)c->ust, (const double*)c->vst, rm, p);
+}
+
+// SGS flux components: comp 0 x (:513-541), 1 y (:552-569), 2 z (:578-594)
+__global__ void k_scalar_flux(double* temp, double* sgs_t3, const double* __restrict__ Nu,
+                              const double* __restrict__ ds, KP p, int comp, double iPr, double surf_flux) {
+    const long n = p.plane * (p.nz + 1);
+    GRID_LOOP(i, n) {
+        const int k = (int)(i / p.plane);
+        double t = 0.0;
+        if (comp < 2) {
+            if (k == 1) t = iPr * Nu[i] * ds[i];
+            else if (k == p.nz - 1) t = iPr * Nu[i + p.plane] * ds[i];
+            else if (k >= 2 && k <= p.nz - 2) t = iPr * 0.5 * (Nu[i] + Nu[i + p.plane]) * ds[i];
+        } else {
+            if (k == 1) { t = -1.0 * surf_flux; sgs_t3[i] = surf_flux; }
+            else if (k >= 2) { t = iPr * Nu[i] * ds[i]; sgs_t3[i] = -1.0 * t; }
+        }
+        temp[i] = t;
+    }
+}
+
+void scalar_flux_dev(les_ctx* c, double* temp, int comp) {
+    KP p = kp(c);
+    const les_params& P = c->prm;
+    const double* ds = c->f3[comp == 0 ? LES_F_DTDX : comp == 1 ? LES_F_DTDY : LES_F_DTDZ];
+    hipLaunchKernelGGL(k_scalar_flux, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, temp,
+                       c->f3[LES_F_SGS_T3], (const double*)c->f3[LES_F_NU_T], ds, p, comp, 1.0 / P.Pr,
+                       P.wt_s / P.T_scale / P.u_scale);
+}
+
+// mode 0: RHS = -RHS + d (first flux), 1: RHS = RHS + d; levels 1..nz-1
+__global__ void k_scalar_accum(double* rhs, const double* __restrict__ d, KP p, int mode) {
+    const long n = p.plane * (p.nz - 1);
+    GRID_LOOP(t, n) {
+        const long i = p.plane + t;
+        rhs[i] = mode == 0 ? (-1.0 * rhs[i] + d[i]) : (rhs[i] + d[i]);
+    }
+}
+
+void scalar_accum_dev(les_ctx* c, const double* dtemp, int mode) {
+    KP p = kp(c);
+    hipLaunchKernelGGL(k_scalar_accum, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream,
+                       c->f3[LES_F_RHST], dtemp, p, mode);
+}
+
+// calc_beta, land branch (:214-258); means[k] = plane mean of theta(k)
+__global__ void k_calc_beta(double* beta, const double* __restrict__ th, const double* __restrict__ bar, KP p,
+                            double g_hat) {
+    const long n = p.plane * (p.nz + 1);
+    GRID_LOOP(i, n) {
+        const int k = (int)(i / p.plane);
+        double b = 0.0;
+        if (k >= 2 && k <= p.nz - 1) {
+            const double above = (th[i] - bar[k]) / bar[k];
+            const double below = (th[i - p.plane] - bar[k - 1]) / bar[k - 1];
+            b = g_hat * (above + below) / 2.0;
+        }
+        beta[i] = b;
+    }
+}
+
+void calc_beta_dev(les_ctx* c, const double* means) {
+    KP p = kp(c);
+    const les_params& P = c->prm;
+    hipLaunchKernelGGL(k_calc_beta, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream,
+                       c->f3[LES_F_BETA_SCAL], (const double*)c->f3[LES_F_THETA], means, p,
+                       GRAV * (P.z_i / (P.u_scale * P.u_scale)));
+}
+
+// step_scalar (:624-671): AB2 on 1..nz-1, top boundary row
+__global__ void k_step_scalar(double* th, const double* __restrict__ rhs, const double* __restrict__ rhsf, KP p,
+                              int above1, double topgrad) {
+    const long n = p.plane * p.nz;  // levels 1..nz
+    GRID_LOOP(t, n) {
+        const long i = p.plane + t;
+        const int k = (int)(i / p.plane);
+        if (k <= p.nz - 1) th[i] = th[i] + p.dt * (1.5 * rhs[i] - 0.5 * rhsf[i]);
+    }
+}
+
+__global__ void k_scalar_top(double* th, KP p, int above1, double topgrad) {
+    GRID_LOOP(q, p.plane) {
+        const long i = (long)p.nz * p.plane + q;
+        th[i] = above1 ? th[i - p.plane] + topgrad : th[i - p.plane];
+    }
+}
+
+void step_scalar_dev(les_ctx* c) {
+    KP p = kp(c);
+    const les_params& P = c->prm;
+    const double topgrad = P.dTdz_top / P.T_scale * P.z_i * c->dz;
+    hipLaunchKernelGGL(k_step_scalar, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, c->f3[LES_F_THETA],
+                       (const double*)c->f3[LES_F_RHST], (const double*)c->f3[LES_F_RHST_F], p,
+                       (int)(P.lz_npz > 1.0), topgrad);
+    hipLaunchKernelGGL(k_scalar_top, grid_for(p.plane), dim3(256), 0, c->stream, c->f3[LES_F_THETA], p,
+                       (int)(P.lz_npz > 1.0), topgrad);
+}
+
+// ---- pressure, 109_press_stag_array.f90:158-199 -------------------------------
+__global__ void k_press_prep(double* rx, double* ry, double* rz, const double* __restrict__ u,
+                             const double* __restrict__ v, const double* __restrict__ w,
+                             const double* __restrict__ divtz, KP p, double cst) {
+    const long n = p.plane * (p.nz + 1);
+    const double tadv1 = 1.5;
+    GRID_LOOP(i, n) {
+        const int k = (int)(i / p.plane);
+        if (k == 0) {
+            rz[i] = cst * divtz[i + p.plane];
+            rx[i] = 0.0; ry[i] = 0.0;
+        } else if (k == p.nz) {
+            rz[i] = cst * divtz[i];
+            rx[i] = 0.0; ry[i] = 0.0;
+        } else {
+            rx[i] = cst / tadv1 * (u[i] / p.dt);
+            ry[i] = cst / tadv1 * (v[i] / p.dt);
+            rz[i] = cst / tadv1 * (w[i] / p.dt);
+        }
+    }
+}
+
+void press_rhs_prep_dev(les_ctx* c, double* rx, double* ry, double* rz) {
+    KP p = kp(c);
+    hipLaunchKernelGGL(k_press_prep, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, rx, ry, rz,
+                       (const double*)c->f3[LES_F_U], (const double*)c->f3[LES_F_V], (const double*)c->f3[LES_F_W],
+                       (const double*)c->f3[LES_F_DIVTZ], p, 1.0 / (double)(c->nx * c->ny));
+}
+
+__global__ void k_dpdz(double* dpdz, const double* __restrict__ pr, KP p) {
+    const long n = p.plane * (p.nz - 1);
+    GRID_LOOP(t, n) {
+        const long i = p.plane + t;
+        dpdz[i] = (pr[i] - pr[i - p.plane]) / p.dz;
+    }
+}
+
+void dpdz_dev(les_ctx* c) {
+    KP p = kp(c);
+    hipLaunchKernelGGL(k_dpdz, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, c->f3[LES_F_DPDZ],
+                       (const double*)c->f3[LES_F_P], p);
+}
+
+// ---- RHS assembly and AB2, 103_flosol.f90:115-162, 319-389 -----------------
+struct RhsArgs {
+    double *rx, *ry, *rz, *rxf, *ryf, *rzf;
+    const double *cx, *cy, *cz, *dx, *dy, *dz, *beta, *u, *v, *w, *sponge;
+};
+
+__global__ void k_rhs(RhsArgs a, KP p, int buoy, int cori, double coriol, double ug, double vg, int damp,
+                      int euler) {
+    const long n = p.plane * (p.nz - 1);
+    GRID_LOOP(t, n) {
+        const long i = p.plane + t;
+        const int k = (int)(i / p.plane);
+        double rx = -a.cx[i] - a.dx[i];
+        double ry = -a.cy[i] - a.dy[i];
+        double rz = -a.cz[i] - a.dz[i];
+        if (buoy) rz = rz + a.beta[i];
+        if (cori) {
+            rx = rx + coriol * (a.v[i] - vg);
+            ry = ry + coriol * (ug - a.u[i]);
+        }
+        if (damp) rz = rz - a.sponge[k] * a.w[i];
+        a.rx[i] = rx; a.ry[i] = ry; a.rz[i] = rz;
+        if (euler) { a.rxf[i] = rx; a.ryf[i] = ry; a.rzf[i] = rz; }
+    }
+}
+
+void rhs_assembly_dev(les_ctx* c, int jt) {
+    KP p = kp(c);
+    const les_params& P = c->prm;
+    RhsArgs a;
+    a.rx = c->f3[LES_F_RHSX]; a.ry = c->f3[LES_F_RHSY]; a.rz = c->f3[LES_F_RHSZ];
+    a.rxf = c->f3[LES_F_RHSX_F]; a.ryf = c->f3[LES_F_RHSY_F]; a.rzf = c->f3[LES_F_RHSZ_F];
+    a.cx = c->f3[LES_F_CX]; a.cy = c->f3[LES_F_CY]; a.cz = c->f3[LES_F_CZ];
+    a.dx = c->f3[LES_F_DIVTX]; a.dy = c->f3[LES_F_DIVTY]; a.dz = c->f3[LES_F_DIVTZ];
+    a.beta = c->f3[LES_F_BETA_SCAL];
+    a.u = c->f3[LES_F_U]; a.v = c->f3[LES_F_V]; a.w = c->f3[LES_F_W]; a.sponge = c->sponge;
+    const int damp = (P.ubc_mom == LES_BC_STRESS_FREE && P.ubc == 1 && P.damping_method == 1);
+    hipLaunchKernelGGL(k_rhs, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, a, p,
+                       (int)(P.theta_flag && !P.passive_scalar), P.coriolis_forcing, P.coriol, P.ug, P.vg, damp,
+                       (int)(jt == 1 && !P.restart_vel));
+}
+
+__global__ void k_ab2(double* u, double* v, double* w, const double* __restrict__ rx,
+                      const double* __restrict__ ry, const double* __restrict__ rz, const double* __restrict__ rxf,
+                      const double* __restrict__ ryf, const double* __restrict__ rzf, KP p, double force) {
+    const long n = p.plane * p.nz;  // levels 0..nz-1
+    const double tadv1 = 1.5, tadv2 = 1.0 - tadv1;
+    GRID_LOOP(i, n) {
+        if (i < p.plane) {
+            u[i] = BOGUS; v[i] = BOGUS; w[i] = BOGUS;
+            continue;
+        }
+        u[i] = u[i] + p.dt * (tadv1 * rx[i] + tadv2 * rxf[i] + 0.0 + force);
+        v[i] = v[i] + p.dt * (tadv1 * ry[i] + tadv2 * ryf[i] + 0.0);
+        w[i] = w[i] + p.dt * (tadv1 * rz[i] + tadv2 * rzf[i] + 0.0);
+    }
+}
+
+void ab2_dev(les_ctx* c) {
+    KP p = kp(c);
+    const les_params& P = c->prm;
+    hipLaunchKernelGGL(k_ab2, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, c->f3[LES_F_U], c->f3[LES_F_V],
+                       c->f3[LES_F_W], (const double*)c->f3[LES_F_RHSX], (const double*)c->f3[LES_F_RHSY],
+                       (const double*)c->f3[LES_F_RHSZ], (const double*)c->f3[LES_F_RHSX_F],
+                       (const double*)c->f3[LES_F_RHSY_F], (const double*)c->f3[LES_F_RHSZ_F], p,
+                       P.use_mean_p_force ? P.mean_p_force : 0.0);
+}
+
+__global__ void k_rhs_gradp(double* rx, double* ry, double* rz, const double* __restrict__ px,
+                            const double* __restrict__ py, const double* __restrict__ pz, KP p) {
+    const long n = p.plane * (p.nz - 1);
+    GRID_LOOP(t, n) {
+        const long i = p.plane + t;
+        rx[i] = rx[i] - px[i];
+        ry[i] = ry[i] - py[i];
+        rz[i] = rz[i] - pz[i];
+    }
+}
+
+void rhs_minus_gradp_dev(les_ctx* c) {
+    KP p = kp(c);
+    hipLaunchKernelGGL(k_rhs_gradp, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, c->f3[LES_F_RHSX],
+                       c->f3[LES_F_RHSY], c->f3[LES_F_RHSZ], (const double*)c->f3[LES_F_DPDX],
+                       (const double*)c->f3[LES_F_DPDY], (const double*)c->f3[LES_F_DPDZ], p);
+}
+
+// project (110_project.f90:16-40) and the boundary rows that follow
+__global__ void k_project(double* u, double* v, double* w, const double* __restrict__ px,
+                          const double* __restrict__ py, const double* __restrict__ pz, KP p) {
+    const long n = p.plane * (p.nz - 1);
+    const double tadv1 = 1.5;
+    GRID_LOOP(t, n) {
+        const long i = p.plane + t;
+        const int k = (int)(i / p.plane);
+        double r = -tadv1 * px[i];
+        u[i] = u[i] + p.dt * r;
+        r = -tadv1 * py[i];
+        v[i] = v[i] + p.dt * r;
+        if (k >= 2) {
+            r = -tadv1 * pz[i];
+            w[i] = w[i] + p.dt * r;
+        }
+    }
+}
+
+__global__ void k_project_bc(double* u, double* v, double* w, double* th, KP p, int sf, int theta) {
+    GRID_LOOP(q, p.plane) {
+        const long top = (long)p.nz * p.plane + q;
+        if (sf) {
+            u[top - p.plane] = u[top - 2 * p.plane];
+            v[top - p.plane] = v[top - 2 * p.plane];
+        }
+        w[top] = 0.0;
+        w[p.plane + q] = 0.0;
+        u[q] = BOGUS; v[q] = BOGUS; w[q] = BOGUS;   // data_exchange, 202_parallel.f90:722-733
+        u[top] = BOGUS; v[top] = BOGUS;
+        if (theta) { th[q] = BOGUS; th[top] = BOGUS; }
+    }
+}
+
+void project_dev(les_ctx* c) {
+    KP p = kp(c);
+    const les_params& P = c->prm;
+    hipLaunchKernelGGL(k_project, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, c->f3[LES_F_U],
+                       c->f3[LES_F_V], c->f3[LES_F_W], (const double*)c->f3[LES_F_DPDX],
+                       (const double*)c->f3[LES_F_DPDY], (const double*)c->f3[LES_F_DPDZ], p);
+    hipLaunchKernelGGL(k_project_bc, grid_for(p.plane), dim3(256), 0, c->stream, c->f3[LES_F_U], c->f3[LES_F_V],
+                       c->f3[LES_F_W], c->f3[LES_F_THETA], p, (int)(P.ubc_mom == LES_BC_STRESS_FREE),
+                       P.theta_flag);
+}
+
+// ---- check_cfl, 901_debug.f90:85-99 ----------------------------------------------
+// Two-stage max over levels 1..nz-1 of u, v, w; the final block forms
+// u_max dt/dx + v_max dt/dy + w_max dt/dz.
+constexpr int MAXB = 512;
+
+__global__ __launch_bounds__(256) void k_max3_part(const double* __restrict__ u, const double* __restrict__ v,
+                                                   const double* __restrict__ w, long lo, long hi,
+                                                   double* __restrict__ part) {
+    __shared__ double sh[3][256];
+    double m[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (long i = lo + (long)blockIdx.x * 256 + threadIdx.x; i < hi; i += (long)gridDim.x * 256) {
+        m[0] = fmax(m[0], u[i]);
+        m[1] = fmax(m[1], v[i]);
+        m[2] = fmax(m[2], w[i]);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) sh[a][threadIdx.x] = m[a];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) sh[a][threadIdx.x] = fmax(sh[a][threadIdx.x], sh[a][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) part[a * MAXB + blockIdx.x] = sh[a][0];
+}
+
+__global__ __launch_bounds__(256) void k_max3_final(const double* __restrict__ part, int nb, double* out,
+                                                    double cu, double cv, double cw) {
+    __shared__ double sh[3][256];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double m = -INFINITY;
+        for (int b = threadIdx.x; b < nb; b += 256) m = fmax(m, part[a * MAXB + b]);
+        sh[a][threadIdx.x] = m;
+    }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) sh[a][threadIdx.x] = fmax(sh[a][threadIdx.x], sh[a][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0][0] * cu + sh[1][0] * cv + sh[2][0] * cw;
+}
+
+void max3_dev(les_ctx* c, double* out) {
+    const long lo = (long)c->plane, hi = (long)c->plane * c->nz;
+    long nb = (hi - lo + 255) / 256;
+    if (nb > MAXB) nb = MAXB;
+    const double dt = c->prm.dt;
+    double* part = c->red + 64;  // red holds >= 64 + 3*MAXB doubles
+    hipLaunchKernelGGL(k_max3_part, dim3((unsigned)nb), dim3(256), 0, c->stream, (const double*)c->f3[LES_F_U],
+                       (const double*)c->f3[LES_F_V], (const double*)c->f3[LES_F_W], lo, hi, part);
+    hipLaunchKernelGGL(k_max3_final, dim3(1), dim3(256), 0, c->stream, (const double*)part, (int)nb, out,
+                       dt / c->dx, dt / c->dy, dt / c->dz);
+}
+
+}  // namespace les

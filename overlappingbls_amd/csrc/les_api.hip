@@ -1,0 +1,717 @@
+// C-ABI of the MI355X boundary-layer LES step (include/les_gpu.h): context
+// lifecycle, field transfer, per-routine entry points and the flosol
+// orchestration (103_flosol.f90:4-290).  All work is enqueued on one HIP
+// stream per context; entry points synchronise before returning.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <utility>
+#include <vector>
+#include "les_internal.hpp"
+
+using namespace les;
+
+namespace {
+
+int good_size(int n) {
+    if (n < 8 || n % 4) return 0;
+    while (n % 2 == 0) n /= 2;
+    while (n % 3 == 0) n /= 3;
+    return n == 1;
+}
+
+template <typename T>
+hipError_t dalloc(les_ctx* c, T** p, size_t count) {
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, sizeof(T) * (count ? count : 1));
+    if (e != hipSuccess) return e;
+    c->allocs.push_back(v);
+    *p = static_cast<T*>(v);
+    return hipSuccess;
+}
+
+hipError_t make_twiddle(les_ctx* c, Twiddle& t, int n) {
+    t.n = n;
+    t.plan = make_plan(n);
+    std::vector<cx<double>> hd(n);
+    std::vector<cx<float>> hf(n);
+    for (int m = 0; m < n; ++m) {
+        long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+        hd[m] = {(double)cosl(a), -(double)sinl(a)};
+        hf[m] = {(float)cosl(a), -(float)sinl(a)};
+    }
+    hipError_t e = dalloc(c, &t.d, n);
+    if (e != hipSuccess) return e;
+    e = dalloc(c, &t.f, n);
+    if (e != hipSuccess) return e;
+    e = hipMemcpy(t.d, hd.data(), sizeof(cx<double>) * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    return hipMemcpy(t.f, hf.data(), sizeof(cx<float>) * n, hipMemcpyHostToDevice);
+}
+
+// Wavenumbers, Nyquist mask and cutoff filters: init_wavenumber
+// (002_fft.f90:121-189) and init_test_filter (006_test_filter.f90:41-77).
+hipError_t init_spectral(les_ctx* c) {
+    const int lhx = c->lhx, ny = c->ny;
+    const size_t ns = c->ns;
+    const double lx = c->prm.lx_tot, ly = c->prm.ly_tot;
+    std::vector<double> kx2d(ns), ky2d(ns), k2(ns), G2(ns), G4(ns);
+    std::vector<unsigned char> nn(ns, 1);
+    for (int j = 0; j < ny; ++j) {
+        const int m = ((j + ny / 2) % ny) - ny / 2;
+        for (int i = 0; i < lhx; ++i) {
+            const size_t q = (size_t)j * lhx + i;
+            kx2d[q] = (i == lhx - 1) ? 0.0 : (double)i * (2.0 * PI / lx);
+            ky2d[q] = (double)m * (2.0 * PI / ly);
+        }
+    }
+    for (int j = 0; j < ny; ++j) {
+        const size_t q = (size_t)j * lhx + lhx - 1;
+        ky2d[q] = 0.0;
+        nn[q] = 0;
+    }
+    for (int i = 0; i < lhx; ++i) {
+        const size_t q = (size_t)(ny / 2) * lhx + i;
+        kx2d[q] = 0.0;
+        ky2d[q] = 0.0;
+        nn[q] = 0;
+    }
+    for (size_t q = 0; q < ns; ++q) k2[q] = kx2d[q] * kx2d[q] + ky2d[q] * ky2d[q];
+    const double alphas[2] = {2.0, 4.0};
+    std::vector<double>* Gs[2] = {&G2, &G4};
+    for (int a = 0; a < 2; ++a) {
+        const double delta = alphas[a] * sqrt(c->dx * c->dy);
+        const double kc2 = (PI / delta) * (PI / delta);
+        for (size_t q = 0; q < ns; ++q) (*Gs[a])[q] = (k2[q] >= kc2) ? 0.0 : 1.0 / (double)(c->nx * c->ny);
+    }
+    // TDMA coefficients with the wall row at level 1 and the top row at
+    // nzt-1, factorised (109_press_stag_array.f90:4-103)
+    const int nz = c->nz;
+    const double dz = c->dz;
+    std::vector<double> cb(ns * nz, 0.0), cc(ns * nz, 0.0);
+    for (size_t q = 0; q < ns; ++q) {
+        if (!nn[q]) continue;
+        double cprev = 0.0;
+        for (int k = 1; k <= nz - 1; ++k) {
+            double a = 0.0, b = 0.0, cv = 0.0;
+            if (k2[q] > 0) {
+                if (k == 1) { a = 0.0; b = -(k2[q] + 1.0 / (dz * dz)); cv = 1.0 / (dz * dz); }
+                else if (k == nz - 1) { a = 1.0 / (dz * dz); b = -(k2[q] + 1.0 / (dz * dz)); cv = 0.0; }
+                else { a = 1.0 / (dz * dz); b = -(k2[q] + 2.0 / (dz * dz)); cv = 1.0 / (dz * dz); }
+            } else {
+                if (k == 1) { a = 0.0; b = 1.0 / dz; }
+                else { a = -1.0 / dz; b = 1.0 / dz; }
+            }
+            const double bp = b - a * cprev;
+            const double cp = cv / bp;
+            cb[(size_t)k * ns + q] = bp;
+            cc[(size_t)k * ns + q] = cp;
+            cprev = cp;
+        }
+    }
+    hipError_t e;
+    if ((e = dalloc(c, &c->kx2d, ns)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->ky2d, ns)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->k2, ns)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->Gtest, ns)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->Gtest2, ns)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->notnyq, ns)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->coef_b, ns * nz)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->coef_c, ns * nz)) != hipSuccess) return e;
+    hipMemcpy(c->kx2d, kx2d.data(), ns * 8, hipMemcpyHostToDevice);
+    hipMemcpy(c->ky2d, ky2d.data(), ns * 8, hipMemcpyHostToDevice);
+    hipMemcpy(c->k2, k2.data(), ns * 8, hipMemcpyHostToDevice);
+    hipMemcpy(c->Gtest, G2.data(), ns * 8, hipMemcpyHostToDevice);
+    hipMemcpy(c->Gtest2, G4.data(), ns * 8, hipMemcpyHostToDevice);
+    hipMemcpy(c->notnyq, nn.data(), ns, hipMemcpyHostToDevice);
+    hipMemcpy(c->coef_b, cb.data(), ns * nz * 8, hipMemcpyHostToDevice);
+    return hipMemcpy(c->coef_c, cc.data(), ns * nz * 8, hipMemcpyHostToDevice);
+}
+
+std::vector<double> sponge_profile(const les_ctx* c) {
+    // setsponge, damping_method 1 (007_topbc.f90:26-36), global level index
+    std::vector<double> sp(c->nlev, 0.0);
+    if (c->prm.ubc != 1) return sp;
+    const int nzt = c->nz;
+    const double factor = 9.0 / (double)(nzt - 3 * nzt / 4 + 1);
+    const double top = c->prm.z_i / (50.0 * c->prm.u_scale);
+    for (int k = 1; k <= nzt - 1; ++k)
+        if (k > 3 * nzt / 4 + 1) sp[k] = top * pow(5.0, (double)(k - nzt) * factor);
+    return sp;
+}
+
+double* F(les_ctx* c, int id) { return c->f3[id]; }
+
+double* surf(les_ctx* c, int id) {
+    switch (id) {
+        case LES_F_USTAR_SFC: return c->ustar;
+        case LES_F_PSI_M_SFC: return c->psi_m;
+        case LES_F_PHI_M_SFC: return c->phi_m;
+        case LES_F_PSI_H_SFC: return c->psi_h;
+        case LES_F_PHI_H_SFC: return c->phi_h;
+        default: return nullptr;
+    }
+}
+
+int finish(les_ctx* c) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return LES_E_HIP;
+    e = hipGetLastError();
+    return e == hipSuccess ? LES_OK : LES_E_HIP;
+}
+
+bool valid_field(int f) { return f >= 0 && f < LES_N_FIELDS3D; }
+
+// ---- transform building blocks on device fields -------------------------------
+double kfx(const les_ctx* c) { return 2.0 * PI / c->prm.lx_tot; }
+double kfy(const les_ctx* c) { return 2.0 * PI / c->prm.ly_tot; }
+
+void ddx_f(les_ctx* c, double* d, const double* f, int lev0, int nlev) {
+    xline_real<double>(c->stream, f + (size_t)lev0 * c->plane, d + (size_t)lev0 * c->plane, (long)nlev * c->ny,
+                       c->nx, c->nx, c->tw_x, c->tw_x, OP_DERIV, kfx(c));
+}
+void ddy_f(les_ctx* c, double* d, const double* f, int lev0, int nlev) {
+    yline_real<double>(c->stream, f + (size_t)lev0 * c->plane, d + (size_t)lev0 * c->plane, nlev, c->nx, c->nx,
+                       c->nx, (long)c->plane, (long)c->plane, c->tw_y, c->tw_y, OP_DERIV, kfy(c));
+}
+void filter_f(les_ctx* c, double* f, int lev0, int nlev) {
+    double* p = f + (size_t)lev0 * c->plane;
+    xline_real<double>(c->stream, p, p, (long)nlev * c->ny, c->nx, c->nx, c->tw_x, c->tw_x, OP_NYQ, 0.0);
+    yline_real<double>(c->stream, p, p, nlev, c->nx, c->nx, c->nx, (long)c->plane, (long)c->plane, c->tw_y, c->tw_y,
+                       OP_NYQ, 0.0);
+}
+// zero_padding of nlev planes: src (nx x ny planes) -> dst (nx2 x ny2 planes)
+void pad_f(les_ctx* c, const double* src, double* dst, int nlev) {
+    double* tmp = c->work3[6];  // >= nlev * ny * nx2
+    xline_real<double>(c->stream, src, tmp, (long)nlev * c->ny, c->nx, c->nx2, c->tw_x, c->tw_x2, OP_PAD, 0.0);
+    yline_real<double>(c->stream, tmp, dst, nlev, c->nx2, c->nx2, c->nx2, (long)c->ny * c->nx2, (long)c->plane2,
+                       c->tw_y, c->tw_y2, OP_PAD, 0.0);
+}
+// zero_unpadding of nlev planes: src (nx2 x ny2) -> dst (nx x ny)
+void unpad_f(les_ctx* c, const double* src, double* dst, int nlev) {
+    double* tmp = c->work3[6];  // >= nlev * ny2 * nx
+    xline_real<double>(c->stream, src, tmp, (long)nlev * c->ny2, c->nx2, c->nx, c->tw_x2, c->tw_x, OP_UNPAD, 0.0);
+    yline_real<double>(c->stream, tmp, dst, nlev, c->nx, c->nx, c->nx, (long)c->ny2 * c->nx, (long)c->plane,
+                       c->tw_y2, c->tw_y, OP_UNPAD, 0.0);
+}
+// test_filter of nplanes consecutive planes in place
+void test_filter_f(les_ctx* c, double* p, int nplanes, const double* G) {
+    cx<double>* sp = c->spec3[2];  // >= nplanes planes of ns
+    x_r2c<double>(c->stream, p, sp, (long)nplanes * c->ny, c->nx, c->lhx, c->tw_x, 1.0);
+    yline_spec<double>(c->stream, sp, sp, nplanes, c->lhx, (long)c->ns, c->tw_y, SP_FILT, G);
+    x_c2r<double>(c->stream, sp, p, (long)nplanes * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
+}
+
+// ---- routines ---------------------------------------------------------------
+void wallstress(les_ctx* c, int top) {
+    const les_params& P = c->prm;
+    const int sf = top ? (P.ubc_mom == LES_BC_STRESS_FREE) : (P.lbc_mom == LES_BC_STRESS_FREE);
+    if (sf) {
+        wallstress_dev(c, top, nullptr, nullptr);
+        return;
+    }
+    const int lev = top ? c->nz - 1 : 1;
+    double* w = c->work3[7];  // two planes: u1, v1
+    hipMemcpyAsync(w, F(c, LES_F_U) + (size_t)lev * c->plane, c->plane * 8, hipMemcpyDeviceToDevice, c->stream);
+    hipMemcpyAsync(w + c->plane, F(c, LES_F_V) + (size_t)lev * c->plane, c->plane * 8, hipMemcpyDeviceToDevice,
+                   c->stream);
+    test_filter_f(c, w, 2, c->Gtest);
+    wallstress_dev(c, top, w, w + c->plane);
+}
+
+void lagrange(les_ctx* c, int jt) {
+    const les_params& P = c->prm;
+    const int nz = c->nz;
+    const long np21 = 21L * nz;  // planes
+    interpolag_dev(c);
+    double* raw = c->lag_buf;
+    double* b2 = raw + (size_t)np21 * c->plane;
+    double* b4 = b2 + (size_t)np21 * c->plane;
+    cx<double>* spf = c->lag_spec;
+    cx<double>* spt = spf + (size_t)np21 * c->ns;
+    lagrange_prep_dev(c, raw);
+    x_r2c<double>(c->stream, raw, spf, np21 * c->ny, c->nx, c->lhx, c->tw_x, 1.0);
+    yline_spec<double>(c->stream, spf, spf, (int)np21, c->lhx, (long)c->ns, c->tw_y, SP_FWD, nullptr);
+    yline_spec<double>(c->stream, spf, spt, (int)np21, c->lhx, (long)c->ns, c->tw_y, SP_MULINV, c->Gtest);
+    x_c2r<double>(c->stream, spt, b2, np21 * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
+    yline_spec<double>(c->stream, spf, spt, (int)np21, c->lhx, (long)c->ns, c->tw_y, SP_MULINV, c->Gtest2);
+    x_c2r<double>(c->stream, spt, b4, np21 * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
+    const int init_lm = P.inilag && !c->f_lm_mm_init && (jt == P.cs_count || jt == P.dyn_init);
+    const int init_qn = P.inilag && !c->f_qn_nn_init && (jt == P.cs_count || jt == P.dyn_init);
+    lagrange_contract_dev(c, b2, b4, raw, init_lm, init_qn);
+    if (init_lm) c->f_lm_mm_init = 1;
+    if (init_qn) c->f_qn_nn_init = 1;
+}
+
+void sgs_stag(les_ctx* c, int jt) {
+    const les_params& P = c->prm;
+    strain_dev(c);
+    lag_accumulate_dev(c);
+    if (jt == 1 && P.inilag) cs_init_dev(c);
+    else if ((jt >= P.dyn_init || P.restart_vel) && (jt % P.cs_count) == 0) lagrange(c, jt);
+    nu_t_stress_dev(c);
+}
+
+void divstress(les_ctx* c, double* out, const double* tx, const double* ty, const double* tz, int wver) {
+    double* a = c->work3[0];
+    double* b = c->work3[1];
+    ddx_f(c, a, tx, 0, (int)c->nlev);
+    ddy_f(c, b, ty, 0, (int)c->nlev);
+    divstress_combine_dev(c, out, a, b, tz, wver);
+}
+
+void convec(les_ctx* c) {
+    const int nz = c->nz;
+    const size_t np = c->plane, np2 = c->plane2;
+    pad_f(c, F(c, LES_F_U), c->u_big, (int)c->nlev);
+    pad_f(c, F(c, LES_F_V), c->v_big, (int)c->nlev);
+    pad_f(c, F(c, LES_F_W), c->w_big, (int)c->nlev);
+    double *v1 = c->work3[2], *v2 = c->work3[3], *v3 = c->work3[4];
+    vorticity_dev(c, v1, v2, v3);
+    pad_f(c, v1 + np, c->big3[0] + np2, nz);
+    pad_f(c, v2 + np, c->big3[1] + np2, nz);
+    pad_f(c, v3 + np, c->big3[2] + np2, nz);
+    cross_product_dev(c, c->big3[0], c->big3[1], c->big3[2], c->big3[3], c->big3[4], c->big3[5]);
+    unpad_f(c, c->big3[3] + np2, F(c, LES_F_CX) + np, nz - 1);
+    unpad_f(c, c->big3[4] + np2, F(c, LES_F_CY) + np, nz - 1);
+    unpad_f(c, c->big3[5] + np2, F(c, LES_F_CZ) + np, nz - 1);
+    fill_levels(c->stream, F(c, LES_F_CX), np, 0, 1, BOGUS);
+    fill_levels(c->stream, F(c, LES_F_CY), np, 0, 1, BOGUS);
+    fill_levels(c->stream, F(c, LES_F_CZ), np, 0, 1, BOGUS);
+    fill_levels(c->stream, F(c, LES_F_CX), np, nz, 1, BOGUS);
+    fill_levels(c->stream, F(c, LES_F_CY), np, nz, 1, BOGUS);
+}
+
+void theta_all_in_one(les_ctx* c, int jt) {
+    const les_params& P = c->prm;
+    const int nz = c->nz;
+    const size_t np = c->plane, np2 = c->plane2;
+    if (!(P.theta_flag && jt >= P.theta_init_time)) {
+        fill_levels(c->stream, F(c, LES_F_BETA_SCAL), np, 0, (int)c->nlev, 0.0);
+        return;
+    }
+    double* th = F(c, LES_F_THETA);
+    filter_f(c, th, 0, (int)c->nlev);
+    ddx_f(c, F(c, LES_F_DTDX), th, 0, (int)c->nlev);
+    ddy_f(c, F(c, LES_F_DTDY), th, 0, (int)c->nlev);
+    ddz_uv_dev(c, F(c, LES_F_DTDZ), th);
+    theta_prep_dev(c);
+    pad_f(c, F(c, LES_F_DTDX) + np, c->big3[0] + np2, nz - 1);
+    pad_f(c, F(c, LES_F_DTDY) + np, c->big3[1] + np2, nz - 1);
+    pad_f(c, F(c, LES_F_DTDZ) + 2 * np, c->big3[2] + 2 * np2, nz - 1);
+    scalar_adv_dev(c, c->big3[0], c->big3[1], c->big3[2], c->big3[3]);
+    unpad_f(c, c->big3[3] + np2, F(c, LES_F_RHST) + np, nz - 1);
+    double* temp = c->work3[0];
+    double* dtemp = c->work3[1];
+    scalar_flux_dev(c, temp, 0);
+    ddx_f(c, dtemp, temp, 0, (int)c->nlev);
+    scalar_accum_dev(c, dtemp, 0);
+    scalar_flux_dev(c, temp, 1);
+    ddy_f(c, dtemp, temp, 0, (int)c->nlev);
+    scalar_accum_dev(c, dtemp, 1);
+    scalar_flux_dev(c, temp, 2);
+    ddz_w_dev(c, dtemp, temp);
+    scalar_accum_dev(c, dtemp, 1);
+    level_means(c->stream, th, np, 0, nz, (double)(c->nx * c->ny), c->red);
+    calc_beta_dev(c, c->red);
+    if (jt == P.theta_init_time && !P.restart_theta)
+        hipMemcpyAsync(F(c, LES_F_RHST_F), F(c, LES_F_RHST), np * c->nlev * 8, hipMemcpyDeviceToDevice, c->stream);
+    step_scalar_dev(c);
+}
+
+void press_stag_array(les_ctx* c) {
+    const int nz = c->nz;
+    const size_t np = c->plane, ns = c->ns;
+    double *rx = c->work3[0], *ry = c->work3[1], *rz = c->work3[2];
+    press_rhs_prep_dev(c, rx, ry, rz);
+    cx<double>*Hx = c->spec3[0], *Hy = c->spec3[1], *Hz = c->spec3[2];
+    const long rows = (long)(nz + 1) * c->ny;
+    x_r2c<double>(c->stream, rx, Hx, rows, c->nx, c->lhx, c->tw_x, 1.0);
+    x_r2c<double>(c->stream, ry, Hy, rows, c->nx, c->lhx, c->tw_x, 1.0);
+    x_r2c<double>(c->stream, rz, Hz, rows, c->nx, c->lhx, c->tw_x, 1.0);
+    yline_spec<double>(c->stream, Hx, Hx, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
+    yline_spec<double>(c->stream, Hy, Hy, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
+    yline_spec<double>(c->stream, Hz, Hz, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
+    cx<double>* ph = Hx;  // each lane overwrites only the entries it has read
+    tdma_press(c, Hx, Hy, Hz, ph);
+    yline_spec<double>(c->stream, ph, ph, nz, c->lhx, (long)ns, c->tw_y, SP_INV, nullptr);
+    x_c2r<double>(c->stream, ph, F(c, LES_F_P), (long)nz * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
+    x_c2r<double>(c->stream, ph + ns, F(c, LES_F_DPDX) + np, (long)(nz - 1) * c->ny, c->lhx, c->nx, c->tw_x, 1,
+                  kfx(c));
+    ddy_f(c, F(c, LES_F_DPDY), F(c, LES_F_P), 1, nz - 1);
+    dpdz_dev(c);
+}
+
+void step_body(les_ctx* c, int jt) {
+    // newstep: RHS_f = RHS (103_flosol.f90:311-313) as a pointer exchange
+    std::swap(c->f3[LES_F_RHSX], c->f3[LES_F_RHSX_F]);
+    std::swap(c->f3[LES_F_RHSY], c->f3[LES_F_RHSY_F]);
+    std::swap(c->f3[LES_F_RHSZ], c->f3[LES_F_RHSZ_F]);
+    obukhov_dev(c, jt);
+    const int nl = (int)c->nlev;
+    ddx_f(c, F(c, LES_F_DUDX), F(c, LES_F_U), 0, nl);
+    ddy_f(c, F(c, LES_F_DUDY), F(c, LES_F_U), 0, nl);
+    ddx_f(c, F(c, LES_F_DVDX), F(c, LES_F_V), 0, nl);
+    ddy_f(c, F(c, LES_F_DVDY), F(c, LES_F_V), 0, nl);
+    ddx_f(c, F(c, LES_F_DWDX), F(c, LES_F_W), 0, nl);
+    ddy_f(c, F(c, LES_F_DWDY), F(c, LES_F_W), 0, nl);
+    ddz_uv_dev(c, F(c, LES_F_DUDZ), F(c, LES_F_U));
+    ddz_uv_dev(c, F(c, LES_F_DVDZ), F(c, LES_F_V));
+    ddz_w_dev(c, F(c, LES_F_DWDZ), F(c, LES_F_W));
+    wallstress(c, 0);
+    wallstress(c, 1);
+    sgs_stag(c, jt);
+    divstress(c, F(c, LES_F_DIVTX), F(c, LES_F_TXX), F(c, LES_F_TXY), F(c, LES_F_TXZ), 0);
+    divstress(c, F(c, LES_F_DIVTY), F(c, LES_F_TXY), F(c, LES_F_TYY), F(c, LES_F_TYZ), 0);
+    divstress(c, F(c, LES_F_DIVTZ), F(c, LES_F_TXZ), F(c, LES_F_TYZ), F(c, LES_F_TZZ), 1);
+    convec(c);
+    theta_all_in_one(c, jt);
+    rhs_assembly_dev(c, jt);
+    ab2_dev(c);
+    press_stag_array(c);
+    rhs_minus_gradp_dev(c);
+    project_dev(c);
+    filter_f(c, F(c, LES_F_U), 0, nl);
+    filter_f(c, F(c, LES_F_V), 0, nl);
+    filter_f(c, F(c, LES_F_W), 0, nl);
+}
+
+}  // namespace
+
+// ============================================================================
+// C-ABI
+// ============================================================================
+extern "C" {
+
+int les_create(const les_params* prm, les_ctx** out) {
+    if (!prm || !out) return LES_E_BADARG;
+    *out = nullptr;
+    if (!good_size(prm->nxt) || !good_size(prm->nyt) || prm->nzt < 4) return LES_E_BADARG;
+    if (prm->model != 5 || prm->ocean_flag || !prm->sgs || prm->lbc != 1 || prm->cs_count < 1) return LES_E_BADARG;
+    if (prm->ubc == 1 && prm->damping_method != 1) return LES_E_BADARG;
+    if (3 * prm->nxt / 2 > 3072 || 3 * prm->nyt / 2 > 3072) return LES_E_BADARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return LES_E_NODEV;
+    les_ctx* c = new les_ctx();
+    c->prm = *prm;
+    c->nx = prm->nxt; c->ny = prm->nyt; c->nz = prm->nzt;
+    c->nx2 = 3 * c->nx / 2; c->ny2 = 3 * c->ny / 2;
+    c->lhx = c->nx / 2 + 1; c->lhy = c->ny / 2 + 1;
+    c->plane = (size_t)c->nx * c->ny;
+    c->plane2 = (size_t)c->nx2 * c->ny2;
+    c->nlev = (size_t)c->nz + 1;
+    c->ns = (size_t)c->lhx * c->ny;
+    c->dx = prm->lx_tot / c->nx;
+    c->dy = prm->ly_tot / c->ny;
+    c->dz = prm->lz_tot / (c->nz - 1);
+    c->precision = LES_FP64;
+    hipGetDevice(&c->dev);
+    if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return LES_E_HIP; }
+    const size_t F3 = c->plane * c->nlev;
+    hipError_t e = hipSuccess;
+    for (int f = 0; f < LES_N_FIELDS3D && e == hipSuccess; ++f) e = dalloc(c, &c->f3[f], F3);
+    double** surfs[5] = {&c->ustar, &c->psi_m, &c->phi_m, &c->psi_h, &c->phi_h};
+    for (int s = 0; s < 5 && e == hipSuccess; ++s) e = dalloc(c, surfs[s], c->plane);
+    const size_t B3 = c->plane2 * c->nlev;
+    if (e == hipSuccess) e = dalloc(c, &c->u_big, B3);
+    if (e == hipSuccess) e = dalloc(c, &c->v_big, B3);
+    if (e == hipSuccess) e = dalloc(c, &c->w_big, B3);
+    for (int i = 0; i < 6 && e == hipSuccess; ++i) e = dalloc(c, &c->big3[i], B3);
+    for (int i = 0; i < 8 && e == hipSuccess; ++i) e = dalloc(c, &c->work3[i], i == 6 ? (F3 * 3) / 2 + c->plane2 : F3);
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = dalloc(c, &c->spec3[i], c->ns * c->nlev);
+    const size_t np21 = (size_t)21 * c->nz;
+    if (e == hipSuccess) e = dalloc(c, &c->lag_buf, 3 * np21 * c->plane);
+    if (e == hipSuccess) e = dalloc(c, &c->lag_spec, 2 * np21 * c->ns);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = dalloc(c, &c->fold[i], F3);
+    if (e == hipSuccess) e = dalloc(c, &c->scal, 64);
+    if (e == hipSuccess) e = dalloc(c, &c->red, 64 + 3 * 512 + c->nlev);
+    c->cfl_cap = 4096;
+    if (e == hipSuccess) e = dalloc(c, &c->cfl_dev, (size_t)c->cfl_cap);
+    if (e == hipSuccess) e = dalloc(c, &c->ust, c->nlev);
+    if (e == hipSuccess) e = dalloc(c, &c->vst, c->nlev);
+    if (e == hipSuccess) e = dalloc(c, &c->sponge, c->nlev);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->cfl_host, sizeof(double) * c->cfl_cap, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        les_destroy(c);
+        return LES_E_NOMEM;
+    }
+    // twiddles, wavenumbers, filters, TDMA factorisation
+    if (make_twiddle(c, c->tw_x, c->nx) != hipSuccess || make_twiddle(c, c->tw_y, c->ny) != hipSuccess ||
+        make_twiddle(c, c->tw_x2, c->nx2) != hipSuccess || make_twiddle(c, c->tw_y2, c->ny2) != hipSuccess ||
+        init_spectral(c) != hipSuccess) {
+        les_destroy(c);
+        return LES_E_HIP;
+    }
+    for (int f = 0; f < LES_N_FIELDS3D; ++f) hipMemsetAsync(c->f3[f], 0, F3 * 8, c->stream);
+    hipMemsetAsync(c->ust, 0, c->nlev * 8, c->stream);
+    hipMemsetAsync(c->vst, 0, c->nlev * 8, c->stream);
+    hipMemsetAsync(c->psi_m, 0, c->plane * 8, c->stream);
+    hipMemsetAsync(c->psi_h, 0, c->plane * 8, c->stream);
+    hipMemsetAsync(c->ustar, 0, c->plane * 8, c->stream);
+    fill_levels(c->stream, c->phi_m, c->plane, 0, 1, 1.0);
+    fill_levels(c->stream, c->phi_h, c->plane, 0, 1, 1.0);
+    std::vector<double> sp = sponge_profile(c);
+    hipMemcpyAsync(c->sponge, sp.data(), c->nlev * 8, hipMemcpyHostToDevice, c->stream);
+    if (finish(c) != LES_OK) {
+        les_destroy(c);
+        return LES_E_HIP;
+    }
+    *out = c;
+    return LES_OK;
+}
+
+int les_destroy(les_ctx* c) {
+    if (!c) return LES_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) hipFree(p);
+    if (c->cfl_host) hipHostFree(c->cfl_host);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return LES_OK;
+}
+
+int les_set_precision(les_ctx* c, int precision) {
+    if (!c) return LES_E_BADARG;
+    if (precision != LES_FP64) return LES_E_BADARG;
+    c->precision = precision;
+    return LES_OK;
+}
+
+int les_set_stokes(les_ctx* c, const double* ust, const double* vst) {
+    if (!c || !ust || !vst) return LES_E_BADARG;
+    hipMemcpyAsync(c->ust, ust, c->nlev * 8, hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(c->vst, vst, c->nlev * 8, hipMemcpyHostToDevice, c->stream);
+    return finish(c);
+}
+
+int les_set_sponge(les_ctx* c, const double* sp) {
+    if (!c || !sp) return LES_E_BADARG;
+    hipMemcpyAsync(c->sponge, sp, c->nlev * 8, hipMemcpyHostToDevice, c->stream);
+    return finish(c);
+}
+
+int les_set_field(les_ctx* c, int id, const double* h, int ld, int lev0, int nlev) {
+    if (!c || !h || ld < c->nx) return LES_E_BADARG;
+    double* d;
+    if (valid_field(id)) {
+        if (lev0 < 0 || nlev < 0 || lev0 + nlev > (int)c->nlev) return LES_E_BADARG;
+        d = c->f3[id] + (size_t)lev0 * c->plane;
+    } else {
+        d = surf(c, id);
+        if (!d || lev0 != 0 || nlev != 1) return LES_E_BADARG;
+    }
+    hipMemcpy2DAsync(d, (size_t)c->nx * 8, h, (size_t)ld * 8, (size_t)c->nx * 8, (size_t)nlev * c->ny,
+                     hipMemcpyHostToDevice, c->stream);
+    return finish(c);
+}
+
+int les_get_field(les_ctx* c, int id, double* h, int ld, int lev0, int nlev) {
+    if (!c || !h || ld < c->nx) return LES_E_BADARG;
+    const double* d;
+    if (valid_field(id)) {
+        if (lev0 < 0 || nlev < 0 || lev0 + nlev > (int)c->nlev) return LES_E_BADARG;
+        d = c->f3[id] + (size_t)lev0 * c->plane;
+    } else {
+        d = surf(c, id);
+        if (!d || lev0 != 0 || nlev != 1) return LES_E_BADARG;
+    }
+    hipMemcpy2DAsync(h, (size_t)ld * 8, d, (size_t)c->nx * 8, (size_t)c->nx * 8, (size_t)nlev * c->ny,
+                     hipMemcpyDeviceToHost, c->stream);
+    return finish(c);
+}
+
+int les_set_sgs_init_flags(les_ctx* c, int a, int b) {
+    if (!c) return LES_E_BADARG;
+    c->f_lm_mm_init = a;
+    c->f_qn_nn_init = b;
+    return LES_OK;
+}
+
+int les_get_sgs_init_flags(les_ctx* c, int* a, int* b) {
+    if (!c || !a || !b) return LES_E_BADARG;
+    *a = c->f_lm_mm_init;
+    *b = c->f_qn_nn_init;
+    return LES_OK;
+}
+
+int les_ddx(les_ctx* c, int d, int f) {
+    if (!c || !valid_field(d) || !valid_field(f)) return LES_E_BADARG;
+    ddx_f(c, F(c, d), F(c, f), 0, (int)c->nlev);
+    return finish(c);
+}
+int les_ddy(les_ctx* c, int d, int f) {
+    if (!c || !valid_field(d) || !valid_field(f)) return LES_E_BADARG;
+    ddy_f(c, F(c, d), F(c, f), 0, (int)c->nlev);
+    return finish(c);
+}
+int les_ddz_uv(les_ctx* c, int d, int f) {
+    if (!c || !valid_field(d) || !valid_field(f) || d == f) return LES_E_BADARG;
+    ddz_uv_dev(c, F(c, d), F(c, f));
+    return finish(c);
+}
+int les_ddz_w(les_ctx* c, int d, int f) {
+    if (!c || !valid_field(d) || !valid_field(f) || d == f) return LES_E_BADARG;
+    ddz_w_dev(c, F(c, d), F(c, f));
+    return finish(c);
+}
+int les_filter_data(les_ctx* c, int f) {
+    if (!c || !valid_field(f)) return LES_E_BADARG;
+    filter_f(c, F(c, f), 0, (int)c->nlev);
+    return finish(c);
+}
+int les_test_filter(les_ctx* c, int f, int level, int which) {
+    if (!c || !valid_field(f) || level < 0 || level > c->nz || (which != 2 && which != 4)) return LES_E_BADARG;
+    test_filter_f(c, F(c, f) + (size_t)level * c->plane, 1, which == 2 ? c->Gtest : c->Gtest2);
+    return finish(c);
+}
+
+int les_fftn(les_ctx* c, const double* in, int ld, double* out_c) {
+    if (!c || !in || !out_c || ld < c->nx) return LES_E_BADARG;
+    double* w = c->work3[7];
+    hipMemcpy2DAsync(w, (size_t)c->nx * 8, in, (size_t)ld * 8, (size_t)c->nx * 8, c->ny, hipMemcpyHostToDevice,
+                     c->stream);
+    cx<double>* sp = c->spec3[2];
+    x_r2c<double>(c->stream, w, sp, c->ny, c->nx, c->lhx, c->tw_x, 1.0);
+    yline_spec<double>(c->stream, sp, sp, 1, c->lhx, (long)c->ns, c->tw_y, SP_FWD, nullptr);
+    hipMemcpyAsync(out_c, sp, c->ns * 16, hipMemcpyDeviceToHost, c->stream);
+    return finish(c);
+}
+
+int les_ifftn(les_ctx* c, const double* in_c, double* out, int ld) {
+    if (!c || !in_c || !out || ld < c->nx) return LES_E_BADARG;
+    cx<double>* sp = c->spec3[2];
+    hipMemcpyAsync(sp, in_c, c->ns * 16, hipMemcpyHostToDevice, c->stream);
+    yline_spec<double>(c->stream, sp, sp, 1, c->lhx, (long)c->ns, c->tw_y, SP_INV, nullptr);
+    double* w = c->work3[7];
+    x_c2r<double>(c->stream, sp, w, c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
+    hipMemcpy2DAsync(out, (size_t)ld * 8, w, (size_t)c->nx * 8, (size_t)c->nx * 8, c->ny, hipMemcpyDeviceToHost,
+                     c->stream);
+    return finish(c);
+}
+
+int les_zero_padding(les_ctx* c, const double* var, int ld, double* big) {
+    if (!c || !var || !big || ld < c->nx) return LES_E_BADARG;
+    double* w = c->work3[7];
+    hipMemcpy2DAsync(w, (size_t)c->nx * 8, var, (size_t)ld * 8, (size_t)c->nx * 8, c->ny, hipMemcpyHostToDevice,
+                     c->stream);
+    pad_f(c, w, c->big3[0], 1);
+    hipMemcpyAsync(big, c->big3[0], c->plane2 * 8, hipMemcpyDeviceToHost, c->stream);
+    return finish(c);
+}
+
+int les_zero_unpadding(les_ctx* c, const double* big, double* var, int ld) {
+    if (!c || !var || !big || ld < c->nx) return LES_E_BADARG;
+    hipMemcpyAsync(c->big3[0], big, c->plane2 * 8, hipMemcpyHostToDevice, c->stream);
+    double* w = c->work3[7];
+    unpad_f(c, c->big3[0], w, 1);
+    hipMemcpy2DAsync(var, (size_t)ld * 8, w, (size_t)c->nx * 8, (size_t)c->nx * 8, c->ny, hipMemcpyDeviceToHost,
+                     c->stream);
+    return finish(c);
+}
+
+int les_obukhov(les_ctx* c, int jt) {
+    if (!c) return LES_E_BADARG;
+    obukhov_dev(c, jt);
+    return finish(c);
+}
+int les_wallstress_bottom(les_ctx* c) {
+    if (!c) return LES_E_BADARG;
+    wallstress(c, 0);
+    return finish(c);
+}
+int les_wallstress_top(les_ctx* c) {
+    if (!c) return LES_E_BADARG;
+    wallstress(c, 1);
+    return finish(c);
+}
+int les_sgs_stag(les_ctx* c, int jt) {
+    if (!c) return LES_E_BADARG;
+    sgs_stag(c, jt);
+    return finish(c);
+}
+int les_lagrange_sdep(les_ctx* c, int jt) {
+    if (!c) return LES_E_BADARG;
+    lagrange(c, jt);
+    return finish(c);
+}
+int les_interpolag_sdep(les_ctx* c) {
+    if (!c) return LES_E_BADARG;
+    interpolag_dev(c);
+    return finish(c);
+}
+int les_divstress_uv(les_ctx* c, int divt, int tx, int ty, int tz) {
+    if (!c || !valid_field(divt) || !valid_field(tx) || !valid_field(ty) || !valid_field(tz)) return LES_E_BADARG;
+    divstress(c, F(c, divt), F(c, tx), F(c, ty), F(c, tz), 0);
+    return finish(c);
+}
+int les_divstress_w(les_ctx* c, int divt, int tx, int ty, int tz) {
+    if (!c || !valid_field(divt) || !valid_field(tx) || !valid_field(ty) || !valid_field(tz)) return LES_E_BADARG;
+    divstress(c, F(c, divt), F(c, tx), F(c, ty), F(c, tz), 1);
+    return finish(c);
+}
+int les_convec(les_ctx* c) {
+    if (!c) return LES_E_BADARG;
+    convec(c);
+    return finish(c);
+}
+int les_theta_all_in_one(les_ctx* c, int jt) {
+    if (!c) return LES_E_BADARG;
+    theta_all_in_one(c, jt);
+    return finish(c);
+}
+int les_press_stag_array(les_ctx* c, int jt) {
+    (void)jt;
+    if (!c) return LES_E_BADARG;
+    press_stag_array(c);
+    return finish(c);
+}
+int les_project(les_ctx* c) {
+    if (!c) return LES_E_BADARG;
+    project_dev(c);
+    return finish(c);
+}
+
+int les_step(les_ctx* c, int jt) {
+    int failed = 0;
+    return les_run(c, jt, 1, &failed);
+}
+
+int les_run(les_ctx* c, int jt0, int nsteps, int* failed_jt) {
+    if (!c || nsteps < 0 || !failed_jt) return LES_E_BADARG;
+    *failed_jt = 0;
+    int done = 0;
+    while (done < nsteps) {
+        const int chunk = (nsteps - done) < c->cfl_cap ? (nsteps - done) : c->cfl_cap;
+        for (int s = 0; s < chunk; ++s) {
+            max3_dev(c, c->cfl_dev + s);  // check_cfl of newstep (901_debug.f90:85-99)
+            step_body(c, jt0 + done + s);
+        }
+        hipMemcpyAsync(c->cfl_host, c->cfl_dev, sizeof(double) * chunk, hipMemcpyDeviceToHost, c->stream);
+        const int st = finish(c);
+        if (st != LES_OK) return st;
+        for (int s = 0; s < chunk; ++s) {
+            c->cfl = c->cfl_host[s];
+            if (c->cfl > 0.6) {   // CFL_max = 0.6 (901_debug.f90:80, :123)
+                *failed_jt = jt0 + done + s;
+                return LES_E_CFL;
+            }
+        }
+        done += chunk;
+    }
+    return LES_OK;
+}
+
+int les_get_cfl(les_ctx* c, double* cfl) {
+    if (!c || !cfl) return LES_E_BADARG;
+    *cfl = c->cfl;
+    return LES_OK;
+}
+
+void* les_stream(les_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int les_sync(les_ctx* c) {
+    if (!c) return LES_E_BADARG;
+    return finish(c);
+}
+
+}  // extern "C"

@@ -1,0 +1,148 @@
+// Internal declarations of the MI355X LES timestep library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+#include "../../include/les_gpu.h"
+#include "fft_lds.hpp"
+
+namespace les {
+
+constexpr double BOGUS = -1234567890.0;  // 001_param.f90:40
+constexpr double VONK = 0.4;             // 001_param.f90:32
+constexpr double GRAV = 9.81;            // 001_param.f90:33
+constexpr double PI = 3.1415926535897932384626433;
+constexpr double B_SMOOTH = 5.2;         // 008_bottombc.f90:25
+constexpr double OPFTIME = 1.5;          // 005_sgs.f90:12
+
+// Spectral multipliers applied between the forward and inverse transform of
+// a line kernel.
+enum LineOp : int {
+    OP_DERIV = 0,   // i k / n     (ddx / ddy)
+    OP_NYQ = 1,     // 1/n, Nyquist zeroed (filter_data)
+    OP_PAD = 2,     // n -> 3n/2, |m| < n/2 kept, 1/n (zero_padding)
+    OP_UNPAD = 3,   // 3n/2 -> n, |m| < n/2 kept, 1/(3n/2) (zero_unpadding)
+};
+
+enum SpecOp : int {
+    SP_FWD = 0,     // forward c2c along y, ky Nyquist row zeroed (fftn_mpi)
+    SP_INV = 1,     // backward c2c along y (ifftn_mpi)
+    SP_FILT = 2,    // forward, Nyquist row zeroed, * G, backward (test_filter)
+    SP_MULINV = 3,  // * G, backward
+};
+
+// Twiddle tables for one transform length, both precisions.
+struct Twiddle {
+    int n = 0;
+    FftPlan plan{};
+    cx<double>* d = nullptr;
+    cx<float>* f = nullptr;
+};
+
+struct Scalars;  // device-side per-step scalars (obukhov)
+
+}  // namespace les
+
+struct les_ctx {
+    les_params prm;
+    int nx, ny, nz, nx2, ny2, lhx, lhy;
+    size_t plane, plane2, nlev, ns;  // ns = lhx*ny half-spectrum plane
+    double dx, dy, dz;
+    int precision;
+    int dev;
+    hipStream_t stream;
+    // device fields (FP64), nlev planes each
+    double* f3[LES_N_FIELDS3D];
+    double *ustar, *psi_m, *phi_m, *psi_h, *phi_h;  // one plane each
+    double *u_big, *v_big, *w_big;                  // nlev planes of nx2*ny2
+    // wavenumber tables (half spectrum, 002_fft.f90:143-167)
+    double *kx2d, *ky2d, *k2, *Gtest, *Gtest2;
+    unsigned char* notnyq;
+    double *coef_b, *coef_c;  // factorised TDMA, nz planes of ns
+    double *ust, *vst, *sponge;
+    les::Twiddle tw_x, tw_y, tw_x2, tw_y2;
+    // work memory
+    double* work3[8];          // 8 extra nlev real fields
+    double* big3[6];           // 6 padded fields (nlev planes of plane2)
+    les::cx<double>* spec3[3]; // 3 half-spectrum fields (nlev planes of ns)
+    double* lag_buf;           // lagrange_Sdep: 21 fields x 3 (raw, 2d, 4d)
+    les::cx<double>* lag_spec; // 21 spectra
+    double* fold[4];           // interpolag copies of F_*
+    double* scal;              // device scalars
+    double* red;               // reduction scratch (nlev * 8)
+    double* cfl_dev;           // CFL per step of les_run
+    double* cfl_host;          // pinned
+    int cfl_cap;
+    int f_lm_mm_init, f_qn_nn_init;
+    double cfl;
+    std::vector<void*> allocs;
+};
+
+namespace les {
+
+// ---- line transforms (k_lines.hip) ---------------------------------------
+// Real x-lines: nrows rows, row r at in + r*ld_in (length nin) -> out + r*ld_out.
+template <typename T>
+void xline_real(hipStream_t s, const T* in, T* out, long nrows, int ld_in, int ld_out,
+                const Twiddle& tin, const Twiddle& tout, LineOp op, double kfac);
+// Real y-lines on nplanes planes: (nin rows x ncols) -> (nout rows x ncols).
+template <typename T>
+void yline_real(hipStream_t s, const T* in, T* out, int nplanes, int ncols, int ld_in, int ld_out,
+                long ps_in, long ps_out, const Twiddle& tin, const Twiddle& tout, LineOp op, double kfac);
+// x r2c (fftn_mpi first half): nrows real rows (length n) -> half-spectrum rows (lhx).
+template <typename T>
+void x_r2c(hipStream_t s, const T* in, cx<T>* out, long nrows, int ld_in, int ldc,
+           const Twiddle& tw, double scale);
+// x c2r (ifftn_mpi second half); deriv != 0 applies i*kx (kx Nyquist = 0) first.
+template <typename T>
+void x_c2r(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int ld_out,
+           const Twiddle& tw, int deriv, double kfac);
+// complex y-lines over nplanes half-spectrum planes (lhx columns, ny rows).
+template <typename T>
+void yline_spec(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lhx, long ps,
+                const Twiddle& tw, SpecOp op, const double* G);
+
+// ---- TDMA (k_tdma.hip) --------------------------------------------------
+void tdma_press(les_ctx* c, const cx<double>* Hx, const cx<double>* Hy, const cx<double>* Hz,
+                cx<double>* ph);
+
+// ---- pointwise / reductions (k_point.hip) -------------------------------
+void fill_levels(hipStream_t s, double* f, size_t plane, int lev0, int nlev, double v);
+void level_means(hipStream_t s, const double* f, size_t plane, int lev0, int nlev, double inv_n, double* out);
+void obukhov_dev(les_ctx* c, int jt);
+void wallstress_dev(les_ctx* c, int top, const double* u1, const double* v1);
+void strain_dev(les_ctx* c);
+void lag_accumulate_dev(les_ctx* c);
+void nu_t_stress_dev(les_ctx* c);
+void cs_init_dev(les_ctx* c);
+void interpolag_dev(les_ctx* c);
+void lagrange_prep_dev(les_ctx* c, double* buf21);
+void lagrange_contract_dev(les_ctx* c, const double* b2, const double* b4, const double* raw, int init_lm, int init_qn);
+void ddz_uv_dev(les_ctx* c, double* d, const double* f);
+void ddz_w_dev(les_ctx* c, double* d, const double* f);
+void divstress_combine_dev(les_ctx* c, double* out, const double* a, const double* b, const double* d, int w_version);
+void vorticity_dev(les_ctx* c, double* v1, double* v2, double* v3);
+void cross_product_dev(les_ctx* c, const double* b1, const double* b2, const double* b3,
+                       double* cx, double* cy, double* cz);
+void theta_prep_dev(les_ctx* c);
+void scalar_adv_dev(les_ctx* c, const double* dsx, const double* dsy, const double* dsz, double* rm);
+void scalar_flux_dev(les_ctx* c, double* temp, int comp);
+void scalar_accum_dev(les_ctx* c, const double* dtemp, int mode);
+void calc_beta_dev(les_ctx* c, const double* means);
+void step_scalar_dev(les_ctx* c);
+void press_rhs_prep_dev(les_ctx* c, double* rx, double* ry, double* rz);
+void dpdz_dev(les_ctx* c);
+void rhs_assembly_dev(les_ctx* c, int jt);
+void ab2_dev(les_ctx* c);
+void rhs_minus_gradp_dev(les_ctx* c);
+void project_dev(les_ctx* c);
+void max3_dev(les_ctx* c, double* out);
+
+}  // namespace les
+
+#define LES_HIP_CHECK(expr)                                  \
+    do {                                                     \
+        hipError_t e_ = (expr);                              \
+        if (e_ != hipSuccess) return LES_E_HIP;              \
+    } while (0)
