@@ -213,6 +213,18 @@ int les_run(les_ctx* ctx, int jt0, int nsteps, int* failed_jt);
 int les_get_cfl(les_ctx* ctx, double* cfl);
 
 /* ---- timing support ------------------------------------------------------ */
+/* Kernel families timed by les_time_kernel (no reference counterpart: the
+ * reference has no timers; these back the roofline figures). */
+#define LES_K_TDMA    0   /* batched Thomas solve of press_stag_array        */
+#define LES_K_XDERIV  1   /* ddx of one field (x-line FFT pass)              */
+#define LES_K_YDERIV  2   /* ddy of one field (y-line FFT pass)              */
+#define LES_K_PAD     3   /* zero_padding of one field (x then y pass)       */
+#define LES_K_COPY    4   /* plain device copy of one field (calibration)    */
+/* Average device time (ms) of `reps` launches of one kernel family on the
+ * context's stream, bracketed by HIP events, and the algorithmic HBM bytes
+ * one launch must move.  Operates on scratch fields only. */
+int les_time_kernel(les_ctx* ctx, int kernel, int reps, double* ms, double* bytes);
+
 /* HIP stream the context enqueues on (hipStream_t as void*). */
 void* les_stream(les_ctx* ctx);
 /* Synchronise the context's stream. */

@@ -17,6 +17,8 @@ LES_E_CFL = -3
 LES_E_NOMEM = -4
 LES_E_NODEV = -5
 
+K_TDMA, K_XDERIV, K_YDERIV, K_PAD, K_COPY = 0, 1, 2, 3, 4
+
 LES_FP64 = 0
 LES_MIXED = 1
 
@@ -119,6 +121,7 @@ def bind(lib: C.CDLL, prefix: str):
         "run": [P, i, i, C.POINTER(i)],
         "get_cfl": [P, C.POINTER(C.c_double)],
         "sync": [P],
+        "time_kernel": [P, i, i, C.POINTER(C.c_double), C.POINTER(C.c_double)],
     }
     fns = {}
     for name, argtypes in sig.items():
@@ -296,6 +299,12 @@ class LesContext:
         st = self._f["run"](self._h, int(jt0), int(nsteps), C.byref(failed))
         if st != LES_OK:
             raise LesError(f"{self.label}:run (step {failed.value})", st)
+
+    def time_kernel(self, kernel, reps=20):
+        """(ms per launch, algorithmic bytes per launch) of one kernel family."""
+        ms, nb = C.c_double(), C.c_double()
+        self._call("time_kernel", self._h, int(kernel), int(reps), C.byref(ms), C.byref(nb))
+        return ms.value, nb.value
 
     def sync(self):
         if "sync" in self._f:

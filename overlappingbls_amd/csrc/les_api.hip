@@ -707,6 +707,41 @@ int les_get_cfl(les_ctx* c, double* cfl) {
     return LES_OK;
 }
 
+int les_time_kernel(les_ctx* c, int kernel, int reps, double* ms, double* bytes) {
+    if (!c || !ms || !bytes || reps < 1 || kernel < 0 || kernel > 4) return LES_E_BADARG;
+    const size_t F3 = c->plane * c->nlev;
+    const int nz = c->nz;
+    double* a = c->work3[0];
+    double* b = c->work3[1];
+    auto launch = [&]() {
+        switch (kernel) {
+            case LES_K_TDMA: tdma_press(c, c->spec3[0], c->spec3[1], c->spec3[2], c->spec3[0]); break;
+            case LES_K_XDERIV: ddx_f(c, b, a, 0, (int)c->nlev); break;
+            case LES_K_YDERIV: ddy_f(c, b, a, 0, (int)c->nlev); break;
+            case LES_K_PAD: pad_f(c, a, c->big3[0], (int)c->nlev); break;
+            default: hipMemcpyAsync(b, a, F3 * 8, hipMemcpyDeviceToDevice, c->stream); break;
+        }
+    };
+    switch (kernel) {
+        case LES_K_TDMA: *bytes = (double)c->ns * 16.0 * (double)(2 * (nz - 1) + (nz + 1) + nz); break;
+        case LES_K_PAD: *bytes = (double)c->nlev * 8.0 * ((double)c->plane + (double)c->plane2); break;
+        default: *bytes = 2.0 * (double)F3 * 8.0; break;
+    }
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return LES_E_HIP;
+    launch();  // warm
+    hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps; ++r) launch();
+    hipEventRecord(e1, c->stream);
+    const int st = finish(c);
+    float t = 0.f;
+    hipEventElapsedTime(&t, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *ms = (double)t / reps;
+    return st;
+}
+
 void* les_stream(les_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 int les_sync(les_ctx* c) {

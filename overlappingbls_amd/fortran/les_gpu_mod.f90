@@ -1,0 +1,218 @@
+!-----------------------------------------------------------------------
+! les_gpu_mod: ISO_C_BINDING interface to the MI355X LES timestep library
+! (include/les_gpu.h).  The reference's Fortran driver keeps its namelists
+! and call sequence; the per-step routines reach the GPU through these
+! interfaces.  Array arguments are the reference's own contiguous arrays,
+! passed by address with their leading dimension (ldx = nxt + 2).
+!-----------------------------------------------------------------------
+module les_gpu_mod
+use iso_c_binding
+implicit none
+private
+
+integer(c_int), parameter, public :: LES_OK = 0, LES_E_BADARG = -1, LES_E_HIP = -2, &
+                                     LES_E_CFL = -3, LES_E_NOMEM = -4, LES_E_NODEV = -5
+integer(c_int), parameter, public :: LES_FP64 = 0, LES_MIXED = 1
+integer(c_int), parameter, public :: LES_WALL_ROUGH = 0, LES_WALL_SMOOTH = 1
+integer(c_int), parameter, public :: LES_BC_WALL = 0, LES_BC_STRESS_FREE = 1
+
+! field identifiers (enum les_field)
+integer(c_int), parameter, public :: LES_F_U = 0, LES_F_V = 1, LES_F_W = 2, LES_F_THETA = 3, &
+    LES_F_RHSX = 4, LES_F_RHSY = 5, LES_F_RHSZ = 6, LES_F_RHST = 7, &
+    LES_F_RHSX_F = 8, LES_F_RHSY_F = 9, LES_F_RHSZ_F = 10, LES_F_RHST_F = 11, &
+    LES_F_DUDX = 12, LES_F_DUDY = 13, LES_F_DUDZ = 14, LES_F_DVDX = 15, LES_F_DVDY = 16, &
+    LES_F_DVDZ = 17, LES_F_DWDX = 18, LES_F_DWDY = 19, LES_F_DWDZ = 20, &
+    LES_F_TXX = 21, LES_F_TXY = 22, LES_F_TXZ = 23, LES_F_TYY = 24, LES_F_TYZ = 25, LES_F_TZZ = 26, &
+    LES_F_DIVTX = 27, LES_F_DIVTY = 28, LES_F_DIVTZ = 29, &
+    LES_F_P = 30, LES_F_DPDX = 31, LES_F_DPDY = 32, LES_F_DPDZ = 33, &
+    LES_F_CS_OPT2 = 34, LES_F_F_LM = 35, LES_F_F_MM = 36, LES_F_F_QN = 37, LES_F_F_NN = 38, &
+    LES_F_NU_T = 39, LES_F_U_LAG = 40, LES_F_V_LAG = 41, LES_F_W_LAG = 42, &
+    LES_F_SGS_T3 = 56, LES_F_PSI_M_SFC = 101
+
+type, bind(C), public :: les_params
+    integer(c_int32_t) :: nxt, nyt, nzt, model, cs_count, dyn_init, inilag, restart_vel, sgs, molec, &
+                          walltype, lbc_mom, ubc_mom, ubc, damping_method, coriolis_forcing, &
+                          use_mean_p_force, ocean_flag, theta_flag, restart_theta, theta_init_time, &
+                          passive_scalar, lbc, pad0
+    real(c_double) :: lx_tot, ly_tot, lz_tot, dt, z_i, u_scale, T_scale, nu_molec, zo, d0, Pr, wt_s, &
+                      dTdz_top, coriol, ug, vg, mean_p_force, lz_npz
+end type les_params
+
+public :: les_create, les_destroy, les_set_field, les_get_field, les_step, les_run, &
+          les_ddx, les_ddy, les_ddz_uv, les_ddz_w, les_filter_data, les_press_stag_array, &
+          les_sgs_stag, les_convec, les_theta_all_in_one, les_project, les_set_precision, &
+          les_fftn, les_ifftn, les_zero_padding, les_zero_unpadding, les_test_filter, &
+          les_obukhov, les_wallstress_bottom, les_wallstress_top, les_divstress_uv, &
+          les_divstress_w, les_set_stokes, les_get_cfl, les_check
+
+interface
+    integer(c_int) function les_create(prm, ctx) bind(C, name="les_create")
+        import :: c_int, c_ptr, les_params
+        type(les_params), intent(in) :: prm
+        type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function les_destroy(ctx) bind(C, name="les_destroy")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function les_set_precision(ctx, p) bind(C, name="les_set_precision")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: p
+    end function
+    integer(c_int) function les_set_stokes(ctx, ust, vst) bind(C, name="les_set_stokes")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: ctx
+        real(c_double), intent(in) :: ust(*), vst(*)
+    end function
+    integer(c_int) function les_set_field(ctx, f, h, ld, lev0, nlev) bind(C, name="les_set_field")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: f, ld, lev0, nlev
+        real(c_double), intent(in) :: h(*)
+    end function
+    integer(c_int) function les_get_field(ctx, f, h, ld, lev0, nlev) bind(C, name="les_get_field")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: f, ld, lev0, nlev
+        real(c_double), intent(out) :: h(*)
+    end function
+    integer(c_int) function les_ddx(ctx, d, f) bind(C, name="les_ddx")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: d, f
+    end function
+    integer(c_int) function les_ddy(ctx, d, f) bind(C, name="les_ddy")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: d, f
+    end function
+    integer(c_int) function les_ddz_uv(ctx, d, f) bind(C, name="les_ddz_uv")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: d, f
+    end function
+    integer(c_int) function les_ddz_w(ctx, d, f) bind(C, name="les_ddz_w")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: d, f
+    end function
+    integer(c_int) function les_filter_data(ctx, f) bind(C, name="les_filter_data")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: f
+    end function
+    integer(c_int) function les_test_filter(ctx, f, level, which) bind(C, name="les_test_filter")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: f, level, which
+    end function
+    integer(c_int) function les_fftn(ctx, fin, ld, fout) bind(C, name="les_fftn")
+        import :: c_int, c_ptr, c_double, c_double_complex
+        type(c_ptr), value :: ctx
+        real(c_double), intent(in) :: fin(*)
+        integer(c_int), value :: ld
+        complex(c_double_complex), intent(out) :: fout(*)
+    end function
+    integer(c_int) function les_ifftn(ctx, fin, fout, ld) bind(C, name="les_ifftn")
+        import :: c_int, c_ptr, c_double, c_double_complex
+        type(c_ptr), value :: ctx
+        complex(c_double_complex), intent(in) :: fin(*)
+        real(c_double), intent(out) :: fout(*)
+        integer(c_int), value :: ld
+    end function
+    integer(c_int) function les_zero_padding(ctx, var, ld, big) bind(C, name="les_zero_padding")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: ctx
+        real(c_double), intent(in) :: var(*)
+        integer(c_int), value :: ld
+        real(c_double), intent(out) :: big(*)
+    end function
+    integer(c_int) function les_zero_unpadding(ctx, big, var, ld) bind(C, name="les_zero_unpadding")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: ctx
+        real(c_double), intent(in) :: big(*)
+        real(c_double), intent(out) :: var(*)
+        integer(c_int), value :: ld
+    end function
+    integer(c_int) function les_obukhov(ctx, jt) bind(C, name="les_obukhov")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: jt
+    end function
+    integer(c_int) function les_wallstress_bottom(ctx) bind(C, name="les_wallstress_bottom")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function les_wallstress_top(ctx) bind(C, name="les_wallstress_top")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function les_sgs_stag(ctx, jt) bind(C, name="les_sgs_stag")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: jt
+    end function
+    integer(c_int) function les_divstress_uv(ctx, divt, tx, ty, tz) bind(C, name="les_divstress_uv")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: divt, tx, ty, tz
+    end function
+    integer(c_int) function les_divstress_w(ctx, divt, tx, ty, tz) bind(C, name="les_divstress_w")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: divt, tx, ty, tz
+    end function
+    integer(c_int) function les_convec(ctx) bind(C, name="les_convec")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function les_theta_all_in_one(ctx, jt) bind(C, name="les_theta_all_in_one")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: jt
+    end function
+    integer(c_int) function les_press_stag_array(ctx, jt) bind(C, name="les_press_stag_array")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: jt
+    end function
+    integer(c_int) function les_project(ctx) bind(C, name="les_project")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function les_step(ctx, jt) bind(C, name="les_step")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: jt
+    end function
+    integer(c_int) function les_run(ctx, jt0, nsteps, failed) bind(C, name="les_run")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: jt0, nsteps
+        integer(c_int), intent(out) :: failed
+    end function
+    integer(c_int) function les_get_cfl(ctx, cfl) bind(C, name="les_get_cfl")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: ctx
+        real(c_double), intent(out) :: cfl
+    end function
+end interface
+
+contains
+
+    ! The reference prints and stops where the library returns a status.
+    subroutine les_check(st, where)
+        integer(c_int), intent(in) :: st
+        character(*), intent(in) :: where
+        if (st /= LES_OK) then
+            if (st == LES_E_CFL) then
+                write(*,*) "CFL IS TOO LARGE, PLEASE REDUCE THE TIME STEP (", where, ")"
+            else
+                write(*,*) "les_gpu error ", st, " in ", where
+            end if
+            stop
+        end if
+    end subroutine les_check
+
+end module les_gpu_mod

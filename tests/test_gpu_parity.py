@@ -107,9 +107,9 @@ def test_routines_in_flosol_order(case, oracle_mod, gpu_lib):
     jt = P.cs_count  # an SGS update step with F initialisation
     # perturb every level so no plane is uniform (a uniform plane gives
     # MM = 0 and Cs_opt2 = 1e-24/0 in the reference as well)
-    for k, n in enumerate(("U", "V", "W")):
+    for k, n in enumerate(("U", "V", "W", "THETA")):
         a = o.get(n)
-        a[1:nz] += 0.05 * config.uniform(99, k, a[1:nz].shape)
+        a[1:nz] += (0.05 if n != "THETA" else 0.002) * config.uniform(99, k, a[1:nz].shape)
         for ctx in (o, g):
             ctx.set(n, a)
             ctx.filter_data(n)
