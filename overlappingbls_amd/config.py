@@ -116,6 +116,11 @@ def workload(name: str) -> Namelist:
     if name == "tiny_ragged":
         return Namelist(nxt=24, nyt=12, nzt=7, cs_count=2, dyn_init=2, wt_s=0.01 * 300.0,
                         coriolis_forcing=True, ug_dim=1.0, vg_dim=0.2, ubc=1, **base)
+    if name == "mid":
+        return Namelist(nxt=64, nyt=64, nzt=9, cs_count=2, dyn_init=2, wt_s=-0.005 * 300.0, **base)
+    if name == "mid_rect":
+        return Namelist(nxt=128, nyt=64, nzt=7, cs_count=2, dyn_init=2, wt_s=0.01 * 300.0,
+                        coriolis_forcing=True, ug_dim=1.0, **base)
     if name == "small":
         return Namelist(nxt=32, nyt=32, nzt=17, cs_count=3, dyn_init=3, wt_s=-0.005 * 300.0, **base)
     raise KeyError(name)

@@ -274,9 +274,12 @@ static inline int lines_per_block(int L) {
     return P < 1 ? 1 : P;
 }
 
+int g_use_ct = 1;
+
 template <typename T>
 void xline_real(hipStream_t s, const T* in, T* out, long nrows, int ld_in, int ld_out, const Twiddle& tin,
                 const Twiddle& tout, LineOp op, double kfac) {
+    if (g_use_ct && xline_real_ct<T>(s, in, out, nrows, ld_in, ld_out, tin, tout, op, kfac)) return;
     const int L = tin.n > tout.n ? tin.n : tout.n;
     const int P = lines_per_block(L);
     const long nb = (nrows + 2 * P - 1) / (2 * P);
@@ -288,6 +291,7 @@ void xline_real(hipStream_t s, const T* in, T* out, long nrows, int ld_in, int l
 template <typename T>
 void yline_real(hipStream_t s, const T* in, T* out, int nplanes, int ncols, int ld_in, int ld_out, long ps_in,
                 long ps_out, const Twiddle& tin, const Twiddle& tout, LineOp op, double kfac) {
+    if (g_use_ct && yline_real_ct<T>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, op, kfac)) return;
     const int L = tin.n > tout.n ? tin.n : tout.n;
     const int C = lines_per_block(L);
     const int nbx = (ncols + 2 * C - 1) / (2 * C);
@@ -299,6 +303,7 @@ void yline_real(hipStream_t s, const T* in, T* out, int nplanes, int ncols, int 
 template <typename T>
 void x_r2c(hipStream_t s, const T* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
            double scale) {
+    if (g_use_ct && x_r2c_ct<T>(s, in, out, nrows, ld_in, ldc, tw, scale)) return;
     const int P = lines_per_block(tw.n);
     const long nb = (nrows + 2 * P - 1) / (2 * P);
     const size_t lds = sizeof(cx<T>) * ((size_t)P * tw.n + tw.n);
@@ -309,6 +314,7 @@ void x_r2c(hipStream_t s, const T* in, cx<T>* out, long nrows, int ld_in, int ld
 template <typename T>
 void x_c2r(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int ld_out, const Twiddle& tw, int deriv,
            double kfac) {
+    if (g_use_ct && x_c2r_ct<T>(s, in, out, nrows, ldc, ld_out, tw, deriv, kfac)) return;
     const int P = lines_per_block(tw.n);
     const long nb = (nrows + 2 * P - 1) / (2 * P);
     const size_t lds = sizeof(cx<T>) * ((size_t)P * tw.n + tw.n);
@@ -319,6 +325,7 @@ void x_c2r(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int ld_o
 template <typename T>
 void yline_spec(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lhx, long ps, const Twiddle& tw,
                 SpecOp op, const double* G) {
+    if (g_use_ct && yline_spec_ct<T>(s, in, out, nplanes, lhx, ps, tw, op, G)) return;
     const int C = lines_per_block(tw.n);
     const int nbx = (lhx + C - 1) / C;
     const size_t lds = sizeof(cx<T>) * ((size_t)C * tw.n + tw.n);

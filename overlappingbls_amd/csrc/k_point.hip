@@ -414,7 +414,7 @@ void interpolag_dev(les_ctx* c) {
     const size_t bytes = sizeof(double) * c->plane * c->nlev;
     const int ids[4] = {LES_F_F_LM, LES_F_F_MM, LES_F_F_QN, LES_F_F_NN};
     for (int m = 0; m < 4; ++m)
-        hipMemcpyAsync(c->fold[m], c->f3[ids[m]], bytes, hipMemcpyDeviceToDevice, c->stream);
+        les_note(c, hipMemcpyAsync(c->fold[m], c->f3[ids[m]], bytes, hipMemcpyDeviceToDevice, c->stream));
     hipLaunchKernelGGL(k_interpolag, grid_for(p.plane * p.nz), dim3(256), 0, c->stream,
                        (const double*)c->f3[LES_F_U_LAG], (const double*)c->f3[LES_F_V_LAG],
                        (const double*)c->f3[LES_F_W_LAG], (const double*)c->ust, (const double*)c->vst,

@@ -1,6 +1,8 @@
 // Pressure Poisson solve: one Thomas (TDMA) system per (kx, ky) of the half
-// spectrum, one lane per system, wavenumber-major so that at every level the
-// lanes of a wavefront touch consecutive complex values.
+// spectrum.  The coefficients are real, so the real and imaginary parts of a
+// system are independent: lane 2q+c solves component c of wavenumber q.
+// Lanes are wavenumber-major, so at every level the lanes of a wavefront
+// touch one contiguous run of complex values.
 //
 // Restates press_stag_array's right-hand side (109_press_stag_array.f90:201-225)
 // and tdmaSolver_pipelined (:244-319) for the whole column at once: the
@@ -12,69 +14,105 @@
 //   surface  p_0 = p_1 - dz d'_0
 // The mean mode (k^2 = 0, not Nyquist) has a = -1/dz, b = 1/dz, c = 0 above
 // the wall, i.e. p_1 = 0, p_k = p_{k-1} + dz H_z(k).  Nyquist modes are zero.
+//
+// The column walk is a dependency chain, so loads are issued D levels at a
+// time ahead of the arithmetic (one memory latency per D levels).
 #include "les_internal.hpp"
 
 namespace les {
 
-__global__ __launch_bounds__(256) void k_tdma(const cx<double>* __restrict__ Hx, const cx<double>* __restrict__ Hy,
-                                               const cx<double>* __restrict__ Hz, cx<double>* __restrict__ ph,
+constexpr int TD = 8;  // levels per load batch
+
+__global__ __launch_bounds__(128) void k_tdma(const double* __restrict__ Hx, const double* __restrict__ Hy,
+                                               const double* __restrict__ Hz, double* __restrict__ ph,
                                                const double* __restrict__ kx2d, const double* __restrict__ ky2d,
                                                const double* __restrict__ k2a, const unsigned char* __restrict__ nn,
                                                const double* __restrict__ cb, const double* __restrict__ cc,
                                                long ns, int nz, double dz) {
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ns) return;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * ns) return;
+    const long q = t >> 1;
+    const int comp = (int)(t & 1);     // 0: real part, 1: imaginary part
+    const long s2 = 2 * ns;            // level stride in doubles
     if (!nn[q]) {
-        for (int k = 0; k <= nz - 1; ++k) ph[k * ns + q] = {0.0, 0.0};
+        for (int k = 0; k <= nz - 1; ++k) ph[k * s2 + t] = 0.0;
         return;
     }
     const double k2 = k2a[q], kx = kx2d[q], ky = ky2d[q];
     const bool wave = k2 > 0.0;
-    const double idz2 = 1.0 / (dz * dz);
-    const cx<double> h0 = Hz[q];
-    cx<double> hzk = Hz[ns + q];
-    cx<double> dprev = h0;
-    // forward sweep, k = 1 .. nz-1; d' stored in ph
-    for (int k = 1; k <= nz - 1; ++k) {
-        const long o = k * ns + q;
-        const cx<double> hzp = Hz[o + ns];
-        cx<double> r;
-        double a;
-        if (wave) {
-            const cx<double> hx = Hx[o], hy = Hy[o];
-            // i (kx Hx + ky Hy) + (Hz(k+1) - Hz(k) [+ Hz(0) at the wall]) / dz
-            const double sr = kx * hx.x + ky * hy.x, si = kx * hx.y + ky * hy.y;
-            cx<double> dh = {hzp.x - hzk.x, hzp.y - hzk.y};
-            if (k == 1) { dh.x += h0.x; dh.y += h0.y; }
-            r = {-si + dh.x / dz, sr + dh.y / dz};
-            a = (k == 1) ? 0.0 : idz2;
-        } else {
-            r = (k == 1) ? cx<double>{0.0, 0.0} : hzk;
-            a = (k == 1) ? 0.0 : -1.0 / dz;
+    const double idz = 1.0 / dz;
+    // i (kx Hx + ky Hy): real part -(kx Hx.im + ky Hy.im), imag part kx Hx.re + ky Hy.re
+    const long oxy = t ^ 1;            // the other component of the same element
+    const double sgn = comp ? 1.0 : -1.0;
+    const double h0 = Hz[t];
+    double hzk = Hz[s2 + t];
+    double dprev = h0;
+    // ---- forward sweep ----
+    for (int k0 = 1; k0 <= nz - 1; k0 += TD) {
+        double hx[TD], hy[TD], hzn[TD], b[TD];
+#pragma unroll
+        for (int d = 0; d < TD; ++d) {
+            const int k = k0 + d;
+            if (k <= nz - 1) {
+                const long o = k * s2;
+                hx[d] = Hx[o + oxy];
+                hy[d] = Hy[o + oxy];
+                hzn[d] = Hz[o + s2 + t];
+                b[d] = cb[k * ns + q];
+            }
         }
-        const double b = cb[o];
-        const cx<double> d = {(r.x - a * dprev.x) / b, (r.y - a * dprev.y) / b};
-        ph[o] = d;
-        dprev = d;
-        hzk = hzp;
+#pragma unroll
+        for (int d = 0; d < TD; ++d) {
+            const int k = k0 + d;
+            if (k <= nz - 1) {
+                double r, a;
+                if (wave) {
+                    double dh = hzn[d] - hzk;
+                    if (k == 1) dh += h0;
+                    r = sgn * (kx * hx[d] + ky * hy[d]) + dh * idz;
+                    a = (k == 1) ? 0.0 : idz * idz;
+                } else {
+                    r = (k == 1) ? 0.0 : hzk;
+                    a = (k == 1) ? 0.0 : -idz;
+                }
+                const double dv = (r - a * dprev) / b[d];
+                ph[k * s2 + t] = dv;
+                dprev = dv;
+                hzk = hzn[d];
+            }
+        }
     }
-    // backward sweep
-    cx<double> pn = dprev;  // level nz-1 (c'(nz-1) = 0)
-    for (int k = nz - 2; k >= 1; --k) {
-        const long o = k * ns + q;
-        const cx<double> d = ph[o];
-        const double c = cc[o];
-        pn = {d.x - c * pn.x, d.y - c * pn.y};
-        ph[o] = pn;
+    // ---- backward sweep ----
+    double pn = dprev;  // level nz-1 (c'(nz-1) = 0)
+    for (int k0 = nz - 2; k0 >= 1; k0 -= TD) {
+        double dv[TD], c[TD];
+#pragma unroll
+        for (int d = 0; d < TD; ++d) {
+            const int k = k0 - d;
+            if (k >= 1) {
+                dv[d] = ph[k * s2 + t];
+                c[d] = cc[k * ns + q];
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < TD; ++d) {
+            const int k = k0 - d;
+            if (k >= 1) {
+                pn = dv[d] - c[d] * pn;
+                ph[k * s2 + t] = pn;
+            }
+        }
     }
-    ph[q] = {pn.x - dz * h0.x, pn.y - dz * h0.y};
+    ph[t] = pn - dz * h0;
 }
 
 void tdma_press(les_ctx* c, const cx<double>* Hx, const cx<double>* Hy, const cx<double>* Hz, cx<double>* ph) {
     const long ns = (long)c->ns;
-    const int nb = (int)((ns + 255) / 256);
-    hipLaunchKernelGGL(k_tdma, dim3(nb), dim3(256), 0, c->stream, Hx, Hy, Hz, ph, c->kx2d, c->ky2d, c->k2,
-                       c->notnyq, c->coef_b, c->coef_c, ns, c->nz, c->dz);
+    const int nb = (int)((2 * ns + 127) / 128);
+    hipLaunchKernelGGL(k_tdma, dim3(nb), dim3(128), 0, c->stream, reinterpret_cast<const double*>(Hx),
+                       reinterpret_cast<const double*>(Hy), reinterpret_cast<const double*>(Hz),
+                       reinterpret_cast<double*>(ph), c->kx2d, c->ky2d, c->k2, c->notnyq, c->coef_b, c->coef_c, ns,
+                       c->nz, c->dz);
 }
 
 }  // namespace les

@@ -118,13 +118,12 @@ hipError_t init_spectral(les_ctx* c) {
     if ((e = dalloc(c, &c->notnyq, ns)) != hipSuccess) return e;
     if ((e = dalloc(c, &c->coef_b, ns * nz)) != hipSuccess) return e;
     if ((e = dalloc(c, &c->coef_c, ns * nz)) != hipSuccess) return e;
-    hipMemcpy(c->kx2d, kx2d.data(), ns * 8, hipMemcpyHostToDevice);
-    hipMemcpy(c->ky2d, ky2d.data(), ns * 8, hipMemcpyHostToDevice);
-    hipMemcpy(c->k2, k2.data(), ns * 8, hipMemcpyHostToDevice);
-    hipMemcpy(c->Gtest, G2.data(), ns * 8, hipMemcpyHostToDevice);
-    hipMemcpy(c->Gtest2, G4.data(), ns * 8, hipMemcpyHostToDevice);
-    hipMemcpy(c->notnyq, nn.data(), ns, hipMemcpyHostToDevice);
-    hipMemcpy(c->coef_b, cb.data(), ns * nz * 8, hipMemcpyHostToDevice);
+    const std::pair<void*, const void*> up[] = {{c->kx2d, kx2d.data()}, {c->ky2d, ky2d.data()}, {c->k2, k2.data()},
+                                                {c->Gtest, G2.data()}, {c->Gtest2, G4.data()}};
+    for (const auto& pr : up)
+        if ((e = hipMemcpy(pr.first, pr.second, ns * 8, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(c->notnyq, nn.data(), ns, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(c->coef_b, cb.data(), ns * nz * 8, hipMemcpyHostToDevice)) != hipSuccess) return e;
     return hipMemcpy(c->coef_c, cc.data(), ns * nz * 8, hipMemcpyHostToDevice);
 }
 
@@ -155,9 +154,11 @@ double* surf(les_ctx* c, int id) {
 
 int finish(les_ctx* c) {
     hipError_t e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) return LES_E_HIP;
-    e = hipGetLastError();
-    return e == hipSuccess ? LES_OK : LES_E_HIP;
+    les_note(c, e);
+    les_note(c, hipGetLastError());
+    const hipError_t first = c->err;
+    c->err = hipSuccess;
+    return first == hipSuccess ? LES_OK : LES_E_HIP;
 }
 
 bool valid_field(int f) { return f >= 0 && f < LES_N_FIELDS3D; }
@@ -212,9 +213,9 @@ void wallstress(les_ctx* c, int top) {
     }
     const int lev = top ? c->nz - 1 : 1;
     double* w = c->work3[7];  // two planes: u1, v1
-    hipMemcpyAsync(w, F(c, LES_F_U) + (size_t)lev * c->plane, c->plane * 8, hipMemcpyDeviceToDevice, c->stream);
-    hipMemcpyAsync(w + c->plane, F(c, LES_F_V) + (size_t)lev * c->plane, c->plane * 8, hipMemcpyDeviceToDevice,
-                   c->stream);
+    les_note(c, hipMemcpyAsync(w, F(c, LES_F_U) + (size_t)lev * c->plane, c->plane * 8, hipMemcpyDeviceToDevice, c->stream));
+    les_note(c, hipMemcpyAsync(w + c->plane, F(c, LES_F_V) + (size_t)lev * c->plane, c->plane * 8, hipMemcpyDeviceToDevice,
+                   c->stream));
     test_filter_f(c, w, 2, c->Gtest);
     wallstress_dev(c, top, w, w + c->plane);
 }
@@ -315,7 +316,7 @@ void theta_all_in_one(les_ctx* c, int jt) {
     level_means(c->stream, th, np, 0, nz, (double)(c->nx * c->ny), c->red);
     calc_beta_dev(c, c->red);
     if (jt == P.theta_init_time && !P.restart_theta)
-        hipMemcpyAsync(F(c, LES_F_RHST_F), F(c, LES_F_RHST), np * c->nlev * 8, hipMemcpyDeviceToDevice, c->stream);
+        les_note(c, hipMemcpyAsync(F(c, LES_F_RHST_F), F(c, LES_F_RHST), np * c->nlev * 8, hipMemcpyDeviceToDevice, c->stream));
     step_scalar_dev(c);
 }
 
@@ -332,7 +333,7 @@ void press_stag_array(les_ctx* c) {
     yline_spec<double>(c->stream, Hx, Hx, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
     yline_spec<double>(c->stream, Hy, Hy, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
     yline_spec<double>(c->stream, Hz, Hz, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
-    cx<double>* ph = Hx;  // each lane overwrites only the entries it has read
+    cx<double>* ph = c->spec3[3];
     tdma_press(c, Hx, Hy, Hz, ph);
     yline_spec<double>(c->stream, ph, ph, nz, c->lhx, (long)ns, c->tw_y, SP_INV, nullptr);
     x_c2r<double>(c->stream, ph, F(c, LES_F_P), (long)nz * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
@@ -405,7 +406,8 @@ int les_create(const les_params* prm, les_ctx** out) {
     c->dy = prm->ly_tot / c->ny;
     c->dz = prm->lz_tot / (c->nz - 1);
     c->precision = LES_FP64;
-    hipGetDevice(&c->dev);
+    if (hipGetDevice(&c->dev) != hipSuccess) { delete c; return LES_E_NODEV; }
+    c->err = hipSuccess;
     if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return LES_E_HIP; }
     const size_t F3 = c->plane * c->nlev;
     hipError_t e = hipSuccess;
@@ -418,7 +420,7 @@ int les_create(const les_params* prm, les_ctx** out) {
     if (e == hipSuccess) e = dalloc(c, &c->w_big, B3);
     for (int i = 0; i < 6 && e == hipSuccess; ++i) e = dalloc(c, &c->big3[i], B3);
     for (int i = 0; i < 8 && e == hipSuccess; ++i) e = dalloc(c, &c->work3[i], i == 6 ? (F3 * 3) / 2 + c->plane2 : F3);
-    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = dalloc(c, &c->spec3[i], c->ns * c->nlev);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = dalloc(c, &c->spec3[i], c->ns * c->nlev);
     const size_t np21 = (size_t)21 * c->nz;
     if (e == hipSuccess) e = dalloc(c, &c->lag_buf, 3 * np21 * c->plane);
     if (e == hipSuccess) e = dalloc(c, &c->lag_spec, 2 * np21 * c->ns);
@@ -442,16 +444,16 @@ int les_create(const les_params* prm, les_ctx** out) {
         les_destroy(c);
         return LES_E_HIP;
     }
-    for (int f = 0; f < LES_N_FIELDS3D; ++f) hipMemsetAsync(c->f3[f], 0, F3 * 8, c->stream);
-    hipMemsetAsync(c->ust, 0, c->nlev * 8, c->stream);
-    hipMemsetAsync(c->vst, 0, c->nlev * 8, c->stream);
-    hipMemsetAsync(c->psi_m, 0, c->plane * 8, c->stream);
-    hipMemsetAsync(c->psi_h, 0, c->plane * 8, c->stream);
-    hipMemsetAsync(c->ustar, 0, c->plane * 8, c->stream);
+    for (int f = 0; f < LES_N_FIELDS3D; ++f) les_note(c, hipMemsetAsync(c->f3[f], 0, F3 * 8, c->stream));
+    les_note(c, hipMemsetAsync(c->ust, 0, c->nlev * 8, c->stream));
+    les_note(c, hipMemsetAsync(c->vst, 0, c->nlev * 8, c->stream));
+    les_note(c, hipMemsetAsync(c->psi_m, 0, c->plane * 8, c->stream));
+    les_note(c, hipMemsetAsync(c->psi_h, 0, c->plane * 8, c->stream));
+    les_note(c, hipMemsetAsync(c->ustar, 0, c->plane * 8, c->stream));
     fill_levels(c->stream, c->phi_m, c->plane, 0, 1, 1.0);
     fill_levels(c->stream, c->phi_h, c->plane, 0, 1, 1.0);
     std::vector<double> sp = sponge_profile(c);
-    hipMemcpyAsync(c->sponge, sp.data(), c->nlev * 8, hipMemcpyHostToDevice, c->stream);
+    les_note(c, hipMemcpyAsync(c->sponge, sp.data(), c->nlev * 8, hipMemcpyHostToDevice, c->stream));
     if (finish(c) != LES_OK) {
         les_destroy(c);
         return LES_E_HIP;
@@ -462,10 +464,10 @@ int les_create(const les_params* prm, les_ctx** out) {
 
 int les_destroy(les_ctx* c) {
     if (!c) return LES_OK;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    for (void* p : c->allocs) hipFree(p);
-    if (c->cfl_host) hipHostFree(c->cfl_host);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->cfl_host) (void)hipHostFree(c->cfl_host);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return LES_OK;
 }
@@ -479,14 +481,14 @@ int les_set_precision(les_ctx* c, int precision) {
 
 int les_set_stokes(les_ctx* c, const double* ust, const double* vst) {
     if (!c || !ust || !vst) return LES_E_BADARG;
-    hipMemcpyAsync(c->ust, ust, c->nlev * 8, hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(c->vst, vst, c->nlev * 8, hipMemcpyHostToDevice, c->stream);
+    les_note(c, hipMemcpyAsync(c->ust, ust, c->nlev * 8, hipMemcpyHostToDevice, c->stream));
+    les_note(c, hipMemcpyAsync(c->vst, vst, c->nlev * 8, hipMemcpyHostToDevice, c->stream));
     return finish(c);
 }
 
 int les_set_sponge(les_ctx* c, const double* sp) {
     if (!c || !sp) return LES_E_BADARG;
-    hipMemcpyAsync(c->sponge, sp, c->nlev * 8, hipMemcpyHostToDevice, c->stream);
+    les_note(c, hipMemcpyAsync(c->sponge, sp, c->nlev * 8, hipMemcpyHostToDevice, c->stream));
     return finish(c);
 }
 
@@ -500,8 +502,8 @@ int les_set_field(les_ctx* c, int id, const double* h, int ld, int lev0, int nle
         d = surf(c, id);
         if (!d || lev0 != 0 || nlev != 1) return LES_E_BADARG;
     }
-    hipMemcpy2DAsync(d, (size_t)c->nx * 8, h, (size_t)ld * 8, (size_t)c->nx * 8, (size_t)nlev * c->ny,
-                     hipMemcpyHostToDevice, c->stream);
+    les_note(c, hipMemcpy2DAsync(d, (size_t)c->nx * 8, h, (size_t)ld * 8, (size_t)c->nx * 8, (size_t)nlev * c->ny,
+                     hipMemcpyHostToDevice, c->stream));
     return finish(c);
 }
 
@@ -515,8 +517,8 @@ int les_get_field(les_ctx* c, int id, double* h, int ld, int lev0, int nlev) {
         d = surf(c, id);
         if (!d || lev0 != 0 || nlev != 1) return LES_E_BADARG;
     }
-    hipMemcpy2DAsync(h, (size_t)ld * 8, d, (size_t)c->nx * 8, (size_t)c->nx * 8, (size_t)nlev * c->ny,
-                     hipMemcpyDeviceToHost, c->stream);
+    les_note(c, hipMemcpy2DAsync(h, (size_t)ld * 8, d, (size_t)c->nx * 8, (size_t)c->nx * 8, (size_t)nlev * c->ny,
+                     hipMemcpyDeviceToHost, c->stream));
     return finish(c);
 }
 
@@ -568,44 +570,44 @@ int les_test_filter(les_ctx* c, int f, int level, int which) {
 int les_fftn(les_ctx* c, const double* in, int ld, double* out_c) {
     if (!c || !in || !out_c || ld < c->nx) return LES_E_BADARG;
     double* w = c->work3[7];
-    hipMemcpy2DAsync(w, (size_t)c->nx * 8, in, (size_t)ld * 8, (size_t)c->nx * 8, c->ny, hipMemcpyHostToDevice,
-                     c->stream);
+    les_note(c, hipMemcpy2DAsync(w, (size_t)c->nx * 8, in, (size_t)ld * 8, (size_t)c->nx * 8, c->ny, hipMemcpyHostToDevice,
+                     c->stream));
     cx<double>* sp = c->spec3[2];
     x_r2c<double>(c->stream, w, sp, c->ny, c->nx, c->lhx, c->tw_x, 1.0);
     yline_spec<double>(c->stream, sp, sp, 1, c->lhx, (long)c->ns, c->tw_y, SP_FWD, nullptr);
-    hipMemcpyAsync(out_c, sp, c->ns * 16, hipMemcpyDeviceToHost, c->stream);
+    les_note(c, hipMemcpyAsync(out_c, sp, c->ns * 16, hipMemcpyDeviceToHost, c->stream));
     return finish(c);
 }
 
 int les_ifftn(les_ctx* c, const double* in_c, double* out, int ld) {
     if (!c || !in_c || !out || ld < c->nx) return LES_E_BADARG;
     cx<double>* sp = c->spec3[2];
-    hipMemcpyAsync(sp, in_c, c->ns * 16, hipMemcpyHostToDevice, c->stream);
+    les_note(c, hipMemcpyAsync(sp, in_c, c->ns * 16, hipMemcpyHostToDevice, c->stream));
     yline_spec<double>(c->stream, sp, sp, 1, c->lhx, (long)c->ns, c->tw_y, SP_INV, nullptr);
     double* w = c->work3[7];
     x_c2r<double>(c->stream, sp, w, c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
-    hipMemcpy2DAsync(out, (size_t)ld * 8, w, (size_t)c->nx * 8, (size_t)c->nx * 8, c->ny, hipMemcpyDeviceToHost,
-                     c->stream);
+    les_note(c, hipMemcpy2DAsync(out, (size_t)ld * 8, w, (size_t)c->nx * 8, (size_t)c->nx * 8, c->ny, hipMemcpyDeviceToHost,
+                     c->stream));
     return finish(c);
 }
 
 int les_zero_padding(les_ctx* c, const double* var, int ld, double* big) {
     if (!c || !var || !big || ld < c->nx) return LES_E_BADARG;
     double* w = c->work3[7];
-    hipMemcpy2DAsync(w, (size_t)c->nx * 8, var, (size_t)ld * 8, (size_t)c->nx * 8, c->ny, hipMemcpyHostToDevice,
-                     c->stream);
+    les_note(c, hipMemcpy2DAsync(w, (size_t)c->nx * 8, var, (size_t)ld * 8, (size_t)c->nx * 8, c->ny, hipMemcpyHostToDevice,
+                     c->stream));
     pad_f(c, w, c->big3[0], 1);
-    hipMemcpyAsync(big, c->big3[0], c->plane2 * 8, hipMemcpyDeviceToHost, c->stream);
+    les_note(c, hipMemcpyAsync(big, c->big3[0], c->plane2 * 8, hipMemcpyDeviceToHost, c->stream));
     return finish(c);
 }
 
 int les_zero_unpadding(les_ctx* c, const double* big, double* var, int ld) {
     if (!c || !var || !big || ld < c->nx) return LES_E_BADARG;
-    hipMemcpyAsync(c->big3[0], big, c->plane2 * 8, hipMemcpyHostToDevice, c->stream);
+    les_note(c, hipMemcpyAsync(c->big3[0], big, c->plane2 * 8, hipMemcpyHostToDevice, c->stream));
     double* w = c->work3[7];
     unpad_f(c, c->big3[0], w, 1);
-    hipMemcpy2DAsync(var, (size_t)ld * 8, w, (size_t)c->nx * 8, (size_t)c->nx * 8, c->ny, hipMemcpyDeviceToHost,
-                     c->stream);
+    les_note(c, hipMemcpy2DAsync(var, (size_t)ld * 8, w, (size_t)c->nx * 8, (size_t)c->nx * 8, c->ny, hipMemcpyDeviceToHost,
+                     c->stream));
     return finish(c);
 }
 
@@ -686,7 +688,7 @@ int les_run(les_ctx* c, int jt0, int nsteps, int* failed_jt) {
             max3_dev(c, c->cfl_dev + s);  // check_cfl of newstep (901_debug.f90:85-99)
             step_body(c, jt0 + done + s);
         }
-        hipMemcpyAsync(c->cfl_host, c->cfl_dev, sizeof(double) * chunk, hipMemcpyDeviceToHost, c->stream);
+        les_note(c, hipMemcpyAsync(c->cfl_host, c->cfl_dev, sizeof(double) * chunk, hipMemcpyDeviceToHost, c->stream));
         const int st = finish(c);
         if (st != LES_OK) return st;
         for (int s = 0; s < chunk; ++s) {
@@ -715,11 +717,11 @@ int les_time_kernel(les_ctx* c, int kernel, int reps, double* ms, double* bytes)
     double* b = c->work3[1];
     auto launch = [&]() {
         switch (kernel) {
-            case LES_K_TDMA: tdma_press(c, c->spec3[0], c->spec3[1], c->spec3[2], c->spec3[0]); break;
+            case LES_K_TDMA: tdma_press(c, c->spec3[0], c->spec3[1], c->spec3[2], c->spec3[3]); break;
             case LES_K_XDERIV: ddx_f(c, b, a, 0, (int)c->nlev); break;
             case LES_K_YDERIV: ddy_f(c, b, a, 0, (int)c->nlev); break;
             case LES_K_PAD: pad_f(c, a, c->big3[0], (int)c->nlev); break;
-            default: hipMemcpyAsync(b, a, F3 * 8, hipMemcpyDeviceToDevice, c->stream); break;
+            default: les_note(c, hipMemcpyAsync(b, a, F3 * 8, hipMemcpyDeviceToDevice, c->stream)); break;
         }
     };
     switch (kernel) {
@@ -728,16 +730,21 @@ int les_time_kernel(les_ctx* c, int kernel, int reps, double* ms, double* bytes)
         default: *bytes = 2.0 * (double)F3 * 8.0; break;
     }
     hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return LES_E_HIP;
+    if (hipEventCreate(&e0) != hipSuccess) return LES_E_HIP;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        les_note(c, hipEventDestroy(e0));
+        return LES_E_HIP;
+    }
     launch();  // warm
-    hipEventRecord(e0, c->stream);
+    les_note(c, hipEventRecord(e0, c->stream));
     for (int r = 0; r < reps; ++r) launch();
-    hipEventRecord(e1, c->stream);
+    les_note(c, hipEventRecord(e1, c->stream));
     const int st = finish(c);
     float t = 0.f;
-    hipEventElapsedTime(&t, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    const hipError_t et = hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (et != hipSuccess) return LES_E_HIP;
     *ms = (double)t / reps;
     return st;
 }

@@ -65,7 +65,7 @@ struct les_ctx {
     // work memory
     double* work3[8];          // 8 extra nlev real fields
     double* big3[6];           // 6 padded fields (nlev planes of plane2)
-    les::cx<double>* spec3[3]; // 3 half-spectrum fields (nlev planes of ns)
+    les::cx<double>* spec3[4]; // half-spectrum fields (nlev planes of ns): Hx, Hy, Hz, p_hat
     double* lag_buf;           // lagrange_Sdep: 21 fields x 3 (raw, 2d, 4d)
     les::cx<double>* lag_spec; // 21 spectra
     double* fold[4];           // interpolag copies of F_*
@@ -76,8 +76,14 @@ struct les_ctx {
     int cfl_cap;
     int f_lm_mm_init, f_qn_nn_init;
     double cfl;
+    hipError_t err;            // first asynchronous HIP error since the last finish()
     std::vector<void*> allocs;
 };
+
+// Record the first failing HIP runtime call of a context.
+inline void les_note(les_ctx* c, hipError_t e) {
+    if (e != hipSuccess && c->err == hipSuccess) c->err = e;
+}
 
 namespace les {
 
@@ -102,6 +108,26 @@ void x_c2r(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int ld_o
 template <typename T>
 void yline_spec(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lhx, long ps,
                 const Twiddle& tw, SpecOp op, const double* G);
+
+// compile-time specialised versions (k_lines_ct.hip); return false when the
+// line length has no specialisation
+template <typename T>
+bool xline_real_ct(hipStream_t s, const T* in, T* out, long nrows, int ld_in, int ld_out, const Twiddle& tin,
+                   const Twiddle& tout, LineOp op, double kfac);
+template <typename T>
+bool yline_real_ct(hipStream_t s, const T* in, T* out, int nplanes, int ncols, int ld_in, int ld_out, long ps_in,
+                   long ps_out, const Twiddle& tin, const Twiddle& tout, LineOp op, double kfac);
+template <typename T>
+bool x_r2c_ct(hipStream_t s, const T* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
+              double scale);
+template <typename T>
+bool x_c2r_ct(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int ld_out, const Twiddle& tw, int deriv,
+              double kfac);
+template <typename T>
+bool yline_spec_ct(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lhx, long ps, const Twiddle& tw,
+                   SpecOp op, const double* G);
+// process-wide switch: 0 forces the generic runtime-length kernels
+extern int g_use_ct;
 
 // ---- TDMA (k_tdma.hip) --------------------------------------------------
 void tdma_press(les_ctx* c, const cx<double>* Hx, const cx<double>* Hy, const cx<double>* Hz,
