@@ -81,8 +81,8 @@ __device__ __forceinline__ void apply_line_op(cx<T>* buf, int P, int nin, int no
     __syncthreads();
 }
 
-template <typename T>
-__global__ __launch_bounds__(NT) void k_xline_real(const T* __restrict__ in, T* __restrict__ out, long nrows,
+template <typename T, typename TI>
+__global__ __launch_bounds__(NT) void k_xline_real(const TI* __restrict__ in, T* __restrict__ out, long nrows,
                                                    int ld_in, int ld_out, FftPlan pin, FftPlan pout,
                                                    const cx<T>* __restrict__ twin_g,
                                                    const cx<T>* __restrict__ twout_g, int op, T kfac, int P) {
@@ -90,6 +90,9 @@ __global__ __launch_bounds__(NT) void k_xline_real(const T* __restrict__ in, T* 
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int nin = pin.n, nout = pout.n;
     const int L = nin > nout ? nin : nout;
+    // derivative in a lower precision than the input: subtract the line's
+    // first element before the conversion (see SHIFT_IN of the specialised kernels)
+    const bool shift = (op == OP_DERIV) && (sizeof(TI) > sizeof(T));
     cx<T>* twin = buf + P * L;
     cx<T>* twout = twin + nin;
     load_tw(twin, twin_g, nin);
@@ -98,7 +101,9 @@ __global__ __launch_bounds__(NT) void k_xline_real(const T* __restrict__ in, T* 
     for (int e = threadIdx.x; e < 2 * P * nin; e += NT) {
         const int rl = e / nin, col = e - rl * nin;
         const long gr = row0 + rl;
-        const T v = (gr < nrows) ? in[gr * ld_in + col] : T(0);
+        T v = T(0);
+        if (gr < nrows)
+            v = shift ? (T)(in[gr * ld_in + col] - in[gr * ld_in]) : (T)in[gr * ld_in + col];
         reinterpret_cast<T*>(&buf[(rl >> 1) * nin + col])[rl & 1] = v;
     }
     __syncthreads();
@@ -112,8 +117,8 @@ __global__ __launch_bounds__(NT) void k_xline_real(const T* __restrict__ in, T* 
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(NT) void k_yline_real(const T* __restrict__ in, T* __restrict__ out, int ncols,
+template <typename T, typename TI>
+__global__ __launch_bounds__(NT) void k_yline_real(const TI* __restrict__ in, T* __restrict__ out, int ncols,
                                                    int ld_in, int ld_out, long ps_in, long ps_out, FftPlan pin,
                                                    FftPlan pout, const cx<T>* __restrict__ twin_g,
                                                    const cx<T>* __restrict__ twout_g, int op, T kfac, int C) {
@@ -121,18 +126,22 @@ __global__ __launch_bounds__(NT) void k_yline_real(const T* __restrict__ in, T* 
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int nin = pin.n, nout = pout.n;
     const int L = nin > nout ? nin : nout;
+    // derivative in a lower precision than the input: subtract the line's
+    // first element before the conversion (see SHIFT_IN of the specialised kernels)
+    const bool shift = (op == OP_DERIV) && (sizeof(TI) > sizeof(T));
     cx<T>* twin = buf + C * L;
     cx<T>* twout = twin + nin;
     load_tw(twin, twin_g, nin);
     load_tw(twout, twout_g, nout);
     const int x0 = blockIdx.x * 2 * C;
-    const T* src = in + (long)blockIdx.y * ps_in;
+    const TI* src = in + (long)blockIdx.y * ps_in;
     T* dst = out + (long)blockIdx.y * ps_out;
     const int w2 = 2 * C;
     for (int e = threadIdx.x; e < nin * w2; e += NT) {
         const int y = e / w2, cl = e - y * w2;
         const int x = x0 + cl;
-        const T v = (x < ncols) ? src[(long)y * ld_in + x] : T(0);
+        T v = T(0);
+        if (x < ncols) v = shift ? (T)(src[(long)y * ld_in + x] - src[x]) : (T)src[(long)y * ld_in + x];
         reinterpret_cast<T*>(&buf[(cl >> 1) * nin + y])[cl & 1] = v;
     }
     __syncthreads();
@@ -148,8 +157,8 @@ __global__ __launch_bounds__(NT) void k_yline_real(const T* __restrict__ in, T* 
 
 // x r2c of fftn_mpi (203_fftn_mpi.f90:21-25): rows -> half spectrum, x
 // Nyquist zeroed, times `scale`.
-template <typename T>
-__global__ __launch_bounds__(NT) void k_x_r2c(const T* __restrict__ in, cx<T>* __restrict__ out, long nrows,
+template <typename T, typename TI>
+__global__ __launch_bounds__(NT) void k_x_r2c(const TI* __restrict__ in, cx<T>* __restrict__ out, long nrows,
                                               int ld_in, int ldc, FftPlan pl, const cx<T>* __restrict__ tw_g,
                                               T scale, int P) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -161,7 +170,7 @@ __global__ __launch_bounds__(NT) void k_x_r2c(const T* __restrict__ in, cx<T>* _
     for (int e = threadIdx.x; e < 2 * P * n; e += NT) {
         const int rl = e / n, col = e - rl * n;
         const long gr = row0 + rl;
-        const T v = (gr < nrows) ? in[gr * ld_in + col] : T(0);
+        const T v = (gr < nrows) ? (T)in[gr * ld_in + col] : T(0);
         reinterpret_cast<T*>(&buf[(rl >> 1) * n + col])[rl & 1] = v;
     }
     __syncthreads();
@@ -276,38 +285,38 @@ static inline int lines_per_block(int L) {
 
 int g_use_ct = 1;
 
-template <typename T>
-void xline_real(hipStream_t s, const T* in, T* out, long nrows, int ld_in, int ld_out, const Twiddle& tin,
+template <typename T, typename TI>
+void xline_real(hipStream_t s, const TI* in, T* out, long nrows, int ld_in, int ld_out, const Twiddle& tin,
                 const Twiddle& tout, LineOp op, double kfac) {
-    if (g_use_ct && xline_real_ct<T>(s, in, out, nrows, ld_in, ld_out, tin, tout, op, kfac)) return;
+    if (g_use_ct && xline_real_ct<T, TI>(s, in, out, nrows, ld_in, ld_out, tin, tout, op, kfac)) return;
     const int L = tin.n > tout.n ? tin.n : tout.n;
     const int P = lines_per_block(L);
     const long nb = (nrows + 2 * P - 1) / (2 * P);
     const size_t lds = sizeof(cx<T>) * ((size_t)P * L + tin.n + tout.n);
-    hipLaunchKernelGGL(k_xline_real<T>, dim3((unsigned)nb), dim3(NT), lds, s, in, out, nrows, ld_in, ld_out,
+    hipLaunchKernelGGL((k_xline_real<T, TI>), dim3((unsigned)nb), dim3(NT), lds, s, in, out, nrows, ld_in, ld_out,
                        tin.plan, tout.plan, twp<T>(tin), twp<T>(tout), (int)op, (T)kfac, P);
 }
 
-template <typename T>
-void yline_real(hipStream_t s, const T* in, T* out, int nplanes, int ncols, int ld_in, int ld_out, long ps_in,
+template <typename T, typename TI>
+void yline_real(hipStream_t s, const TI* in, T* out, int nplanes, int ncols, int ld_in, int ld_out, long ps_in,
                 long ps_out, const Twiddle& tin, const Twiddle& tout, LineOp op, double kfac) {
-    if (g_use_ct && yline_real_ct<T>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, op, kfac)) return;
+    if (g_use_ct && yline_real_ct<T, TI>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, op, kfac)) return;
     const int L = tin.n > tout.n ? tin.n : tout.n;
     const int C = lines_per_block(L);
     const int nbx = (ncols + 2 * C - 1) / (2 * C);
     const size_t lds = sizeof(cx<T>) * ((size_t)C * L + tin.n + tout.n);
-    hipLaunchKernelGGL(k_yline_real<T>, dim3(nbx, nplanes), dim3(NT), lds, s, in, out, ncols, ld_in, ld_out, ps_in,
+    hipLaunchKernelGGL((k_yline_real<T, TI>), dim3(nbx, nplanes), dim3(NT), lds, s, in, out, ncols, ld_in, ld_out, ps_in,
                        ps_out, tin.plan, tout.plan, twp<T>(tin), twp<T>(tout), (int)op, (T)kfac, C);
 }
 
-template <typename T>
-void x_r2c(hipStream_t s, const T* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
+template <typename T, typename TI>
+void x_r2c(hipStream_t s, const TI* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
            double scale) {
-    if (g_use_ct && x_r2c_ct<T>(s, in, out, nrows, ld_in, ldc, tw, scale)) return;
+    if (g_use_ct && x_r2c_ct<T, TI>(s, in, out, nrows, ld_in, ldc, tw, scale)) return;
     const int P = lines_per_block(tw.n);
     const long nb = (nrows + 2 * P - 1) / (2 * P);
     const size_t lds = sizeof(cx<T>) * ((size_t)P * tw.n + tw.n);
-    hipLaunchKernelGGL(k_x_r2c<T>, dim3((unsigned)nb), dim3(NT), lds, s, in, out, nrows, ld_in, ldc, tw.plan,
+    hipLaunchKernelGGL((k_x_r2c<T, TI>), dim3((unsigned)nb), dim3(NT), lds, s, in, out, nrows, ld_in, ldc, tw.plan,
                        twp<T>(tw), (T)scale, P);
 }
 
@@ -333,16 +342,16 @@ void yline_spec(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lhx
                        twp<T>(tw), (int)op, G, C);
 }
 
-template void xline_real<double>(hipStream_t, const double*, double*, long, int, int, const Twiddle&,
-                                 const Twiddle&, LineOp, double);
-template void xline_real<float>(hipStream_t, const float*, float*, long, int, int, const Twiddle&,
-                                const Twiddle&, LineOp, double);
-template void yline_real<double>(hipStream_t, const double*, double*, int, int, int, int, long, long,
-                                 const Twiddle&, const Twiddle&, LineOp, double);
-template void yline_real<float>(hipStream_t, const float*, float*, int, int, int, int, long, long,
-                                const Twiddle&, const Twiddle&, LineOp, double);
-template void x_r2c<double>(hipStream_t, const double*, cx<double>*, long, int, int, const Twiddle&, double);
-template void x_r2c<float>(hipStream_t, const float*, cx<float>*, long, int, int, const Twiddle&, double);
+#define LES_INST_LINES(T, TI)                                                                                   \
+    template void xline_real<T, TI>(hipStream_t, const TI*, T*, long, int, int, const Twiddle&, const Twiddle&,      \
+                                    LineOp, double);                                                               \
+    template void yline_real<T, TI>(hipStream_t, const TI*, T*, int, int, int, int, long, long, const Twiddle&,      \
+                                    const Twiddle&, LineOp, double);                                               \
+    template void x_r2c<T, TI>(hipStream_t, const TI*, cx<T>*, long, int, int, const Twiddle&, double);
+LES_INST_LINES(double, double)
+LES_INST_LINES(float, float)
+LES_INST_LINES(float, double)
+#undef LES_INST_LINES
 template void x_c2r<double>(hipStream_t, const cx<double>*, double*, long, int, int, const Twiddle&, int, double);
 template void x_c2r<float>(hipStream_t, const cx<float>*, float*, long, int, int, const Twiddle&, int, double);
 template void yline_spec<double>(hipStream_t, const cx<double>*, cx<double>*, int, int, long, const Twiddle&,

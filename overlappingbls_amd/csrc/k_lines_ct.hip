@@ -91,8 +91,14 @@ __device__ __forceinline__ void load_tw_ct(cx<T>* lds, const cx<T>* g) {
     for (int m = threadIdx.x; m < N; m += NTC) lds[m] = g[m];
 }
 
-template <typename T, int NIN, int NOUT, int OP>
-__global__ __launch_bounds__(NTC) void kc_xline(const T* __restrict__ in, T* __restrict__ out, long nrows,
+// A derivative computed in a lower precision than its input is taken of the
+// line minus its first element (the derivative of a constant is zero), so the
+// conversion rounds the fluctuation about the line, not the mean flow.
+template <typename T, typename TI, int OP>
+constexpr bool SHIFT_IN = (OP == OP_DERIV) && (sizeof(TI) > sizeof(T));
+
+template <typename T, typename TI, int NIN, int NOUT, int OP>
+__global__ __launch_bounds__(NTC) void kc_xline(const TI* __restrict__ in, T* __restrict__ out, long nrows,
                                                 int ld_in, int ld_out, const cx<T>* __restrict__ twin_g,
                                                 const cx<T>* __restrict__ twout_g, T kfac) {
     constexpr int L = NIN > NOUT ? NIN : NOUT;
@@ -107,7 +113,11 @@ __global__ __launch_bounds__(NTC) void kc_xline(const T* __restrict__ in, T* __r
     for (int e = threadIdx.x; e < 2 * P * NIN; e += NTC) {
         const int rl = e / NIN, col = e - rl * NIN;
         const long gr = row0 + rl;
-        const T v = (gr < nrows) ? in[gr * ld_in + col] : T(0);
+        T v = T(0);
+        if (gr < nrows) {
+            if constexpr (SHIFT_IN<T, TI, OP>) v = (T)(in[gr * ld_in + col] - in[gr * ld_in]);
+            else v = (T)in[gr * ld_in + col];
+        }
         reinterpret_cast<T*>(&buf[(rl >> 1) * NIN + col])[rl & 1] = v;
     }
     __syncthreads();
@@ -122,8 +132,8 @@ __global__ __launch_bounds__(NTC) void kc_xline(const T* __restrict__ in, T* __r
     }
 }
 
-template <typename T, int NIN, int NOUT, int OP>
-__global__ __launch_bounds__(NTC) void kc_yline(const T* __restrict__ in, T* __restrict__ out, int ncols, int ld_in,
+template <typename T, typename TI, int NIN, int NOUT, int OP>
+__global__ __launch_bounds__(NTC) void kc_yline(const TI* __restrict__ in, T* __restrict__ out, int ncols, int ld_in,
                                                 int ld_out, long ps_in, long ps_out, const cx<T>* __restrict__ twin_g,
                                                 const cx<T>* __restrict__ twout_g, T kfac) {
     constexpr int L = NIN > NOUT ? NIN : NOUT;
@@ -135,14 +145,18 @@ __global__ __launch_bounds__(NTC) void kc_yline(const T* __restrict__ in, T* __r
     load_tw_ct<T, NIN>(twi, twin_g);
     load_tw_ct<T, NOUT>(two, twout_g);
     const int x0 = blockIdx.x * 2 * C;
-    const T* src = in + (long)blockIdx.y * ps_in;
+    const TI* src = in + (long)blockIdx.y * ps_in;
     T* dst = out + (long)blockIdx.y * ps_out;
     constexpr int w2 = 2 * C;
 #pragma unroll 4
     for (int e = threadIdx.x; e < NIN * w2; e += NTC) {
         const int y = e / w2, cl = e - y * w2;
         const int x = x0 + cl;
-        const T v = (x < ncols) ? src[(long)y * ld_in + x] : T(0);
+        T v = T(0);
+        if (x < ncols) {
+            if constexpr (SHIFT_IN<T, TI, OP>) v = (T)(src[(long)y * ld_in + x] - src[x]);
+            else v = (T)src[(long)y * ld_in + x];
+        }
         reinterpret_cast<T*>(&buf[(cl >> 1) * LSI + y])[cl & 1] = v;
     }
     __syncthreads();
@@ -157,8 +171,8 @@ __global__ __launch_bounds__(NTC) void kc_yline(const T* __restrict__ in, T* __r
     }
 }
 
-template <typename T, int N>
-__global__ __launch_bounds__(NTC) void kc_x_r2c(const T* __restrict__ in, cx<T>* __restrict__ out, long nrows,
+template <typename T, typename TI, int N>
+__global__ __launch_bounds__(NTC) void kc_x_r2c(const TI* __restrict__ in, cx<T>* __restrict__ out, long nrows,
                                                 int ld_in, int ldc, const cx<T>* __restrict__ tw_g, T scale) {
     constexpr int P = lines_for(N);
     constexpr int LH = N / 2 + 1;
@@ -170,7 +184,7 @@ __global__ __launch_bounds__(NTC) void kc_x_r2c(const T* __restrict__ in, cx<T>*
     for (int e = threadIdx.x; e < 2 * P * N; e += NTC) {
         const int rl = e / N, col = e - rl * N;
         const long gr = row0 + rl;
-        const T v = (gr < nrows) ? in[gr * ld_in + col] : T(0);
+        const T v = (gr < nrows) ? (T)in[gr * ld_in + col] : T(0);
         reinterpret_cast<T*>(&buf[(rl >> 1) * N + col])[rl & 1] = v;
     }
     __syncthreads();
@@ -276,60 +290,60 @@ template <>
 const cx<float>* twp_ct<float>(const Twiddle& t) { return t.f; }
 
 // ---- dispatch helpers ------------------------------------------------------
-template <typename T, int NIN, int NOUT, int OP>
-void launch_x(hipStream_t s, const T* in, T* out, long nrows, int ld_in, int ld_out, const Twiddle& tin,
+template <typename T, typename TI, int NIN, int NOUT, int OP>
+void launch_x(hipStream_t s, const TI* in, T* out, long nrows, int ld_in, int ld_out, const Twiddle& tin,
               const Twiddle& tout, double kfac) {
     constexpr int P = lines_for(NIN > NOUT ? NIN : NOUT);
     const long nb = (nrows + 2 * P - 1) / (2 * P);
-    hipLaunchKernelGGL((kc_xline<T, NIN, NOUT, OP>), dim3((unsigned)nb), dim3(NTC), 0, s, in, out, nrows, ld_in,
+    hipLaunchKernelGGL((kc_xline<T, TI, NIN, NOUT, OP>), dim3((unsigned)nb), dim3(NTC), 0, s, in, out, nrows, ld_in,
                        ld_out, twp_ct<T>(tin), twp_ct<T>(tout), (T)kfac);
 }
 
-template <typename T, int NIN, int NOUT, int OP>
-void launch_y(hipStream_t s, const T* in, T* out, int nplanes, int ncols, int ld_in, int ld_out, long ps_in,
+template <typename T, typename TI, int NIN, int NOUT, int OP>
+void launch_y(hipStream_t s, const TI* in, T* out, int nplanes, int ncols, int ld_in, int ld_out, long ps_in,
               long ps_out, const Twiddle& tin, const Twiddle& tout, double kfac) {
     constexpr int C = ycols<T>((NIN > NOUT ? NIN : NOUT) + 1);
     const int nbx = (ncols + 2 * C - 1) / (2 * C);
-    hipLaunchKernelGGL((kc_yline<T, NIN, NOUT, OP>), dim3(nbx, nplanes), dim3(NTC), 0, s, in, out, ncols, ld_in,
+    hipLaunchKernelGGL((kc_yline<T, TI, NIN, NOUT, OP>), dim3(nbx, nplanes), dim3(NTC), 0, s, in, out, ncols, ld_in,
                        ld_out, ps_in, ps_out, twp_ct<T>(tin), twp_ct<T>(tout), (T)kfac);
 }
 
 #define LES_SIZES(X) X(64) X(128) X(256) X(512)
 
-template <typename T>
-bool line_dispatch_x(hipStream_t s, const T* in, T* out, long nrows, int ld_in, int ld_out, const Twiddle& tin,
+template <typename T, typename TI>
+bool line_dispatch_x(hipStream_t s, const TI* in, T* out, long nrows, int ld_in, int ld_out, const Twiddle& tin,
                      const Twiddle& tout, LineOp op, double kfac) {
     const int a = tin.n, b = tout.n;
 #define XCASE(n)                                                                                    \
-    if (a == n && b == n && op == OP_DERIV) { launch_x<T, n, n, OP_DERIV>(s, in, out, nrows, ld_in, ld_out, tin, tout, kfac); return true; } \
-    if (a == n && b == n && op == OP_NYQ) { launch_x<T, n, n, OP_NYQ>(s, in, out, nrows, ld_in, ld_out, tin, tout, kfac); return true; }     \
-    if (a == n && b == 3 * n / 2 && op == OP_PAD) { launch_x<T, n, 3 * n / 2, OP_PAD>(s, in, out, nrows, ld_in, ld_out, tin, tout, kfac); return true; } \
-    if (a == 3 * n / 2 && b == n && op == OP_UNPAD) { launch_x<T, 3 * n / 2, n, OP_UNPAD>(s, in, out, nrows, ld_in, ld_out, tin, tout, kfac); return true; }
+    if (a == n && b == n && op == OP_DERIV) { launch_x<T, TI, n, n, OP_DERIV>(s, in, out, nrows, ld_in, ld_out, tin, tout, kfac); return true; } \
+    if (a == n && b == n && op == OP_NYQ) { launch_x<T, TI, n, n, OP_NYQ>(s, in, out, nrows, ld_in, ld_out, tin, tout, kfac); return true; }     \
+    if (a == n && b == 3 * n / 2 && op == OP_PAD) { launch_x<T, TI, n, 3 * n / 2, OP_PAD>(s, in, out, nrows, ld_in, ld_out, tin, tout, kfac); return true; } \
+    if (a == 3 * n / 2 && b == n && op == OP_UNPAD) { launch_x<T, TI, 3 * n / 2, n, OP_UNPAD>(s, in, out, nrows, ld_in, ld_out, tin, tout, kfac); return true; }
     LES_SIZES(XCASE)
 #undef XCASE
     return false;
 }
 
-template <typename T>
-bool line_dispatch_y(hipStream_t s, const T* in, T* out, int nplanes, int ncols, int ld_in, int ld_out, long ps_in,
+template <typename T, typename TI>
+bool line_dispatch_y(hipStream_t s, const TI* in, T* out, int nplanes, int ncols, int ld_in, int ld_out, long ps_in,
                      long ps_out, const Twiddle& tin, const Twiddle& tout, LineOp op, double kfac) {
     const int a = tin.n, b = tout.n;
 #define YCASE(n)                                                                                    \
-    if (a == n && b == n && op == OP_DERIV) { launch_y<T, n, n, OP_DERIV>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, kfac); return true; } \
-    if (a == n && b == n && op == OP_NYQ) { launch_y<T, n, n, OP_NYQ>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, kfac); return true; }     \
-    if (a == n && b == 3 * n / 2 && op == OP_PAD) { launch_y<T, n, 3 * n / 2, OP_PAD>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, kfac); return true; } \
-    if (a == 3 * n / 2 && b == n && op == OP_UNPAD) { launch_y<T, 3 * n / 2, n, OP_UNPAD>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, kfac); return true; }
+    if (a == n && b == n && op == OP_DERIV) { launch_y<T, TI, n, n, OP_DERIV>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, kfac); return true; } \
+    if (a == n && b == n && op == OP_NYQ) { launch_y<T, TI, n, n, OP_NYQ>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, kfac); return true; }     \
+    if (a == n && b == 3 * n / 2 && op == OP_PAD) { launch_y<T, TI, n, 3 * n / 2, OP_PAD>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, kfac); return true; } \
+    if (a == 3 * n / 2 && b == n && op == OP_UNPAD) { launch_y<T, TI, 3 * n / 2, n, OP_UNPAD>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, kfac); return true; }
     LES_SIZES(YCASE)
 #undef YCASE
     return false;
 }
 
-template <typename T, int N>
-void launch_r2c(hipStream_t s, const T* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
+template <typename T, typename TI, int N>
+void launch_r2c(hipStream_t s, const TI* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
                 double scale) {
     constexpr int P = lines_for(N);
     const long nb = (nrows + 2 * P - 1) / (2 * P);
-    hipLaunchKernelGGL((kc_x_r2c<T, N>), dim3((unsigned)nb), dim3(NTC), 0, s, in, out, nrows, ld_in, ldc,
+    hipLaunchKernelGGL((kc_x_r2c<T, TI, N>), dim3((unsigned)nb), dim3(NTC), 0, s, in, out, nrows, ld_in, ldc,
                        twp_ct<T>(tw), (T)scale);
 }
 
@@ -351,11 +365,11 @@ void launch_spec(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lh
                        G);
 }
 
-template <typename T>
-bool r2c_dispatch(hipStream_t s, const T* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
+template <typename T, typename TI>
+bool r2c_dispatch(hipStream_t s, const TI* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
                   double scale) {
 #define RCASE(NN_) \
-    if (tw.n == NN_) { launch_r2c<T, NN_>(s, in, out, nrows, ld_in, ldc, tw, scale); return true; }
+    if (tw.n == NN_) { launch_r2c<T, TI, NN_>(s, in, out, nrows, ld_in, ldc, tw, scale); return true; }
     LES_SIZES(RCASE)
 #undef RCASE
     return false;
@@ -396,20 +410,20 @@ bool spec_dispatch(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int 
 }  // namespace
 
 // exported to k_lines.hip
-template <typename T>
-bool xline_real_ct(hipStream_t s, const T* in, T* out, long nrows, int ld_in, int ld_out, const Twiddle& tin,
+template <typename T, typename TI>
+bool xline_real_ct(hipStream_t s, const TI* in, T* out, long nrows, int ld_in, int ld_out, const Twiddle& tin,
                    const Twiddle& tout, LineOp op, double kfac) {
-    return line_dispatch_x<T>(s, in, out, nrows, ld_in, ld_out, tin, tout, op, kfac);
+    return line_dispatch_x<T, TI>(s, in, out, nrows, ld_in, ld_out, tin, tout, op, kfac);
 }
-template <typename T>
-bool yline_real_ct(hipStream_t s, const T* in, T* out, int nplanes, int ncols, int ld_in, int ld_out, long ps_in,
+template <typename T, typename TI>
+bool yline_real_ct(hipStream_t s, const TI* in, T* out, int nplanes, int ncols, int ld_in, int ld_out, long ps_in,
                    long ps_out, const Twiddle& tin, const Twiddle& tout, LineOp op, double kfac) {
-    return line_dispatch_y<T>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, op, kfac);
+    return line_dispatch_y<T, TI>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, op, kfac);
 }
-template <typename T>
-bool x_r2c_ct(hipStream_t s, const T* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
+template <typename T, typename TI>
+bool x_r2c_ct(hipStream_t s, const TI* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
               double scale) {
-    return r2c_dispatch<T>(s, in, out, nrows, ld_in, ldc, tw, scale);
+    return r2c_dispatch<T, TI>(s, in, out, nrows, ld_in, ldc, tw, scale);
 }
 template <typename T>
 bool x_c2r_ct(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int ld_out, const Twiddle& tw, int deriv,
@@ -422,17 +436,21 @@ bool yline_spec_ct(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int 
     return spec_dispatch<T>(s, in, out, nplanes, lhx, ps, tw, op, G);
 }
 
-#define INST(T)                                                                                                     \
-    template bool xline_real_ct<T>(hipStream_t, const T*, T*, long, int, int, const Twiddle&, const Twiddle&, LineOp, \
-                                   double);                                                                       \
-    template bool yline_real_ct<T>(hipStream_t, const T*, T*, int, int, int, int, long, long, const Twiddle&,       \
-                                   const Twiddle&, LineOp, double);                                               \
-    template bool x_r2c_ct<T>(hipStream_t, const T*, cx<T>*, long, int, int, const Twiddle&, double);             \
-    template bool x_c2r_ct<T>(hipStream_t, const cx<T>*, T*, long, int, int, const Twiddle&, int, double);        \
-    template bool yline_spec_ct<T>(hipStream_t, const cx<T>*, cx<T>*, int, int, long, const Twiddle&, SpecOp,     \
-                                   const double*);
-INST(double)
-INST(float)
-#undef INST
+#define INST_IO(T, TI)                                                                                       \
+    template bool xline_real_ct<T, TI>(hipStream_t, const TI*, T*, long, int, int, const Twiddle&, const Twiddle&, \
+                                       LineOp, double);                                                          \
+    template bool yline_real_ct<T, TI>(hipStream_t, const TI*, T*, int, int, int, int, long, long, const Twiddle&, \
+                                       const Twiddle&, LineOp, double);                                          \
+    template bool x_r2c_ct<T, TI>(hipStream_t, const TI*, cx<T>*, long, int, int, const Twiddle&, double);
+INST_IO(double, double)
+INST_IO(float, float)
+INST_IO(float, double)
+#undef INST_IO
+template bool x_c2r_ct<double>(hipStream_t, const cx<double>*, double*, long, int, int, const Twiddle&, int, double);
+template bool x_c2r_ct<float>(hipStream_t, const cx<float>*, float*, long, int, int, const Twiddle&, int, double);
+template bool yline_spec_ct<double>(hipStream_t, const cx<double>*, cx<double>*, int, int, long, const Twiddle&,
+                                    SpecOp, const double*);
+template bool yline_spec_ct<float>(hipStream_t, const cx<float>*, cx<float>*, int, int, long, const Twiddle&,
+                                   SpecOp, const double*);
 
 }  // namespace les

@@ -40,14 +40,31 @@ __device__ __forceinline__ double smag(double s11, double s12, double s13, doubl
 }
 
 // ---- fills and reductions ------------------------------------------------
-__global__ void k_fill(double* f, long n, double v) {
-    GRID_LOOP(i, n) f[i] = v;
+template <typename T>
+__global__ void k_fill(T* f, long n, double v) {
+    GRID_LOOP(i, n) f[i] = (T)v;
 }
 
-void fill_levels(hipStream_t s, double* f, size_t plane, int lev0, int nlev, double v) {
+template <typename T>
+void fill_levels(hipStream_t s, T* f, size_t plane, int lev0, int nlev, double v) {
     const long n = (long)plane * nlev;
-    hipLaunchKernelGGL(k_fill, grid_for(n), dim3(256), 0, s, f + (size_t)lev0 * plane, n, v);
+    hipLaunchKernelGGL(k_fill<T>, grid_for(n), dim3(256), 0, s, f + (size_t)lev0 * plane, n, v);
 }
+template void fill_levels<double>(hipStream_t, double*, size_t, int, int, double);
+template void fill_levels<float>(hipStream_t, float*, size_t, int, int, double);
+
+// element-wise conversion between storage precisions
+template <typename TO, typename TF>
+__global__ void k_convert(TO* __restrict__ out, const TF* __restrict__ in, long n) {
+    GRID_LOOP(i, n) out[i] = (TO)in[i];
+}
+
+template <typename TO, typename TF>
+void convert(hipStream_t s, TO* out, const TF* in, long n) {
+    hipLaunchKernelGGL((k_convert<TO, TF>), grid_for(n), dim3(256), 0, s, out, in, n);
+}
+template void convert<float, double>(hipStream_t, float*, const double*, long);
+template void convert<double, float>(hipStream_t, double*, const float*, long);
 
 // one block per level: out[l] = sum(plane l) / nn, deterministic tree order
 __global__ __launch_bounds__(256) void k_level_means(const double* __restrict__ f, long plane, double nn,
@@ -169,7 +186,8 @@ void obukhov_dev(les_ctx* c, int jt) {
 }
 
 // ---- wall stress, 104_wallstress.f90:98-260 ------------------------------
-__global__ void k_wall(double* txz, double* tyz, double* dudz, double* dvdz, double* ustar,
+template <typename W>
+__global__ void k_wall(W* txz, W* tyz, W* dudz, W* dvdz, double* ustar,
                        const double* __restrict__ u, const double* __restrict__ v,
                        const double* __restrict__ u1, const double* __restrict__ v1,
                        const double* __restrict__ psi_m, const double* __restrict__ phi_m, long plane, int top,
@@ -201,15 +219,16 @@ __global__ void k_wall(double* txz, double* tyz, double* dudz, double* dvdz, dou
     }
 }
 
+template <typename W>
 void wallstress_dev(les_ctx* c, int top, const double* u1, const double* v1) {
     const les_params& P = c->prm;
     const long np = (long)c->plane;
     const int lev = top ? c->nz : 1;
     const int ulev = top ? c->nz - 1 : 1;
-    double* txz = c->f3[LES_F_TXZ] + lev * np;
-    double* tyz = c->f3[LES_F_TYZ] + lev * np;
-    double* dudz = c->f3[LES_F_DUDZ] + lev * np;
-    double* dvdz = c->f3[LES_F_DVDZ] + lev * np;
+    W* txz = WF<W>(c, LES_F_TXZ) + lev * np;
+    W* tyz = WF<W>(c, LES_F_TYZ) + lev * np;
+    W* dudz = WF<W>(c, LES_F_DUDZ) + lev * np;
+    W* dvdz = WF<W>(c, LES_F_DVDZ) + lev * np;
     const int sf = top ? (P.ubc_mom == LES_BC_STRESS_FREE) : (P.lbc_mom == LES_BC_STRESS_FREE);
     if (sf) {
         fill_levels(c->stream, txz, c->plane, 0, 1, 0.0);
@@ -218,22 +237,25 @@ void wallstress_dev(les_ctx* c, int top, const double* u1, const double* v1) {
         fill_levels(c->stream, dvdz, c->plane, 0, 1, 0.0);
         return;
     }
-    hipLaunchKernelGGL(k_wall, grid_for(np), dim3(256), 0, c->stream, txz, tyz, dudz, dvdz, c->ustar,
+    hipLaunchKernelGGL(k_wall<W>, grid_for(np), dim3(256), 0, c->stream, txz, tyz, dudz, dvdz, c->ustar,
                        (const double*)(c->f3[LES_F_U] + ulev * np), (const double*)(c->f3[LES_F_V] + ulev * np),
                        u1, v1, (const double*)c->psi_m, (const double*)c->phi_m, np, top,
                        (int)(P.walltype == LES_WALL_SMOOTH), c->dz, P.d0, P.zo, P.nu_molec / P.u_scale / P.z_i);
 }
 
 // ---- sgs_stag, 106_sgs_stag.f90 ---------------------------------------------
+template <typename W>
 struct Grad {
-    const double *ux, *uy, *uz, *vx, *vy, *vz, *wx, *wy, *wz;
+    const W *ux, *uy, *uz, *vx, *vy, *vz, *wx, *wy, *wz;
 };
+template <typename W>
 struct Strain {
-    double *s11, *s12, *s13, *s22, *s23, *s33;
+    W *s11, *s12, *s13, *s22, *s23, *s33;
 };
 
 // S_ij on w-nodes, uvp nodes at the wall (:47-68) and the top (:83-104)
-__global__ void k_strain(Grad g, Strain S, KP p) {
+template <typename W>
+__global__ void k_strain(Grad<W> g, Strain<W> S, KP p) {
     const long n = p.plane * p.nz;  // levels 1..nz
     GRID_LOOP(t, n) {
         const long i = p.plane + t;
@@ -261,18 +283,23 @@ __global__ void k_strain(Grad g, Strain S, KP p) {
     }
 }
 
-static Grad grads(les_ctx* c) {
-    return {c->f3[LES_F_DUDX], c->f3[LES_F_DUDY], c->f3[LES_F_DUDZ], c->f3[LES_F_DVDX], c->f3[LES_F_DVDY],
-            c->f3[LES_F_DVDZ], c->f3[LES_F_DWDX], c->f3[LES_F_DWDY], c->f3[LES_F_DWDZ]};
+template <typename W>
+static Grad<W> grads(les_ctx* c) {
+    return {WF<W>(c, LES_F_DUDX), WF<W>(c, LES_F_DUDY), WF<W>(c, LES_F_DUDZ), WF<W>(c, LES_F_DVDX),
+            WF<W>(c, LES_F_DVDY), WF<W>(c, LES_F_DVDZ), WF<W>(c, LES_F_DWDX), WF<W>(c, LES_F_DWDY),
+            WF<W>(c, LES_F_DWDZ)};
 }
-static Strain strains(les_ctx* c) {
-    return {c->f3[LES_F_S11], c->f3[LES_F_S12], c->f3[LES_F_S13], c->f3[LES_F_S22], c->f3[LES_F_S23],
-            c->f3[LES_F_S33]};
+template <typename W>
+static Strain<W> strains(les_ctx* c) {
+    return {WF<W>(c, LES_F_S11), WF<W>(c, LES_F_S12), WF<W>(c, LES_F_S13), WF<W>(c, LES_F_S22),
+            WF<W>(c, LES_F_S23), WF<W>(c, LES_F_S33)};
 }
 
+template <typename W>
 void strain_dev(les_ctx* c) {
     KP p = kp(c);
-    hipLaunchKernelGGL(k_strain, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, grads(c), strains(c), p);
+    hipLaunchKernelGGL(k_strain<W>, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, grads<W>(c), strains<W>(c),
+                       p);
 }
 
 __global__ void k_lag_acc(double* ul, double* vl, double* wl, const double* __restrict__ u,
@@ -294,8 +321,9 @@ void lag_accumulate_dev(les_ctx* c) {
 void cs_init_dev(les_ctx* c) { fill_levels(c->stream, c->f3[LES_F_CS_OPT2], c->plane, 1, c->nz, 0.03); }
 
 // Nu_t (:206-216) and tau_ij (:219-315)
-__global__ void k_stress(Strain S, const double* __restrict__ Cs, double* Nu, double* txx, double* txy,
-                         double* txz, double* tyy, double* tyz, double* tzz, KP p) {
+template <typename W>
+__global__ void k_stress(Strain<W> S, const double* __restrict__ Cs, W* Nu, W* txx, W* txy, W* txz, W* tyy, W* tyz,
+                         W* tzz, KP p) {
     // pass 1 is Nu_t; it is computed per point of the levels this point reads
     const long n = p.plane * (p.nz + 1);
     const double l2 = p.delta * p.delta;
@@ -336,11 +364,13 @@ __global__ void k_stress(Strain S, const double* __restrict__ Cs, double* Nu, do
     }
 }
 
+template <typename W>
 void nu_t_stress_dev(les_ctx* c) {
     KP p = kp(c);
-    hipLaunchKernelGGL(k_stress, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, strains(c),
-                       (const double*)c->f3[LES_F_CS_OPT2], c->f3[LES_F_NU_T], c->f3[LES_F_TXX],
-                       c->f3[LES_F_TXY], c->f3[LES_F_TXZ], c->f3[LES_F_TYY], c->f3[LES_F_TYZ], c->f3[LES_F_TZZ], p);
+    hipLaunchKernelGGL(k_stress<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, strains<W>(c),
+                       (const double*)c->f3[LES_F_CS_OPT2], WF<W>(c, LES_F_NU_T), WF<W>(c, LES_F_TXX),
+                       WF<W>(c, LES_F_TXY), WF<W>(c, LES_F_TXZ), WF<W>(c, LES_F_TYY), WF<W>(c, LES_F_TYZ),
+                       WF<W>(c, LES_F_TZZ), p);
 }
 
 // ---- interpolag_Sdep, 106_06_interpolag_Sdep.f90:4-336 -------------------
@@ -422,17 +452,24 @@ void interpolag_dev(les_ctx* c) {
                        (const double*)c->fold[3], c->f3[ids[0]], c->f3[ids[1]], c->f3[ids[2]], c->f3[ids[3]], p,
                        (double)c->prm.cs_count);
     const long n = (long)(c->plane * c->nlev);                              // :330-332
-    hipLaunchKernelGGL(k_fill, grid_for(n), dim3(256), 0, c->stream, c->f3[LES_F_U_LAG], n, 0.0);
-    hipLaunchKernelGGL(k_fill, grid_for(n), dim3(256), 0, c->stream, c->f3[LES_F_V_LAG], n, 0.0);
-    hipLaunchKernelGGL(k_fill, grid_for(n), dim3(256), 0, c->stream, c->f3[LES_F_W_LAG], n, 0.0);
+    hipLaunchKernelGGL(k_fill<double>, grid_for(n), dim3(256), 0, c->stream, c->f3[LES_F_U_LAG], n, 0.0);
+    hipLaunchKernelGGL(k_fill<double>, grid_for(n), dim3(256), 0, c->stream, c->f3[LES_F_V_LAG], n, 0.0);
+    hipLaunchKernelGGL(k_fill<double>, grid_for(n), dim3(256), 0, c->stream, c->f3[LES_F_W_LAG], n, 0.0);
 }
 
 // ---- lagrange_Sdep, 106_03_lagrange_Sdep.f90:70-331 -------------------------
 // buf21 layout: field f (0..20) of level jz (1..nz) at ((f * nz) + jz-1) * plane.
 // f: 0 u_bar, 1 v_bar, 2 w_bar, 3..8 products p11 p12 p13 p22 p23 p33,
 //    9..14 S_ij, 15..20 |S| S_ij  (ij order 11 12 13 22 23 33)
+// With um/vm (plane means of u, v; FP32 work precision) the w-node velocities
+// are taken relative to their plane mean.  L_ij = bar(u_i u_j) - bar(u_i)
+// bar(u_j) and Q_ij are unchanged by a per-level constant shift (the test
+// filters keep the mean), and the FP32 products then carry the fluctuation
+// instead of the mean flow, avoiding the cancellation in L_ij.
+template <typename W>
 __global__ void k_lag_prep(const double* __restrict__ u, const double* __restrict__ v,
-                           const double* __restrict__ w, Strain S, double* __restrict__ buf, KP p) {
+                           const double* __restrict__ w, Strain<W> S, W* __restrict__ buf, KP p,
+                           const double* __restrict__ um, const double* __restrict__ vm) {
     const long n = p.plane * p.nz;
     GRID_LOOP(t, n) {
         const long i = p.plane + t;
@@ -441,10 +478,15 @@ __global__ void k_lag_prep(const double* __restrict__ u, const double* __restric
         if (k == 1) { ub = u[i]; vb = v[i]; wb = 0.25 * w[i + p.plane]; }
         else if (k == p.nz) { ub = u[i - p.plane]; vb = v[i - p.plane]; wb = 0.25 * w[i - p.plane]; }
         else { ub = 0.5 * (u[i] + u[i - p.plane]); vb = 0.5 * (v[i] + v[i - p.plane]); wb = w[i]; }
+        if (um) {
+            if (k == 1) { ub -= um[1]; vb -= vm[1]; }
+            else if (k == p.nz) { ub -= um[k - 1]; vb -= vm[k - 1]; }
+            else { ub -= 0.5 * (um[k] + um[k - 1]); vb -= 0.5 * (vm[k] + vm[k - 1]); }
+        }
         const double s[6] = {S.s11[i], S.s12[i], S.s13[i], S.s22[i], S.s23[i], S.s33[i]};
         const double sm = smag(s[0], s[1], s[2], s[3], s[4], s[5]);
         const long fs = p.plane * p.nz;
-        double* o = buf + t;
+        W* o = buf + t;
         o[0 * fs] = ub; o[1 * fs] = vb; o[2 * fs] = wb;
         o[3 * fs] = ub * ub; o[4 * fs] = ub * vb; o[5 * fs] = ub * wb;
         o[6 * fs] = vb * vb; o[7 * fs] = vb * wb; o[8 * fs] = wb * wb;
@@ -456,11 +498,20 @@ __global__ void k_lag_prep(const double* __restrict__ u, const double* __restric
     }
 }
 
-void lagrange_prep_dev(les_ctx* c, double* buf21) {
+template <typename W>
+void lagrange_prep_dev(les_ctx* c, W* buf21) {
     KP p = kp(c);
-    hipLaunchKernelGGL(k_lag_prep, grid_for(p.plane * p.nz), dim3(256), 0, c->stream,
+    const double *um = nullptr, *vm = nullptr;
+    if (sizeof(W) < sizeof(double)) {
+        const double nn = (double)(c->nx * c->ny);
+        level_means(c->stream, c->f3[LES_F_U], c->plane, 0, (int)c->nlev, nn, c->lmean);
+        level_means(c->stream, c->f3[LES_F_V], c->plane, 0, (int)c->nlev, nn, c->lmean + c->nlev);
+        um = c->lmean;
+        vm = c->lmean + c->nlev;
+    }
+    hipLaunchKernelGGL(k_lag_prep<W>, grid_for(p.plane * p.nz), dim3(256), 0, c->stream,
                        (const double*)c->f3[LES_F_U], (const double*)c->f3[LES_F_V], (const double*)c->f3[LES_F_W],
-                       strains(c), buf21, p);
+                       strains<W>(c), buf21, p, um, vm);
 }
 
 __device__ __forceinline__ void lag_relax(double& F1, double& F2, double A, double B, double opftdelta,
@@ -476,7 +527,8 @@ __device__ __forceinline__ void lag_relax(double& F1, double& F2, double A, doub
     F1 = fmax(CLIP, F1);
 }
 
-__global__ void k_lag_contract(const double* __restrict__ b2, const double* __restrict__ b4, double* FLM,
+template <typename W>
+__global__ void k_lag_contract(const W* __restrict__ b2, const W* __restrict__ b4, double* FLM,
                                double* FMM, double* FQN, double* FNN, double* Cs, KP p, int init_lm, int init_qn,
                                double lagran_dt, double bexp) {
     const long n = p.plane * p.nz;
@@ -487,8 +539,8 @@ __global__ void k_lag_contract(const double* __restrict__ b2, const double* __re
     GRID_LOOP(t, n) {
         const long i = p.plane + t;
         const int k = (int)(i / p.plane);
-        const double* a = b2 + t;
-        const double* h = b4 + t;
+        const W* a = b2 + t;
+        const W* h = b4 + t;
         const double ub = a[0], vb = a[fs], wb = a[2 * fs];
         const double uh = h[0], vh = h[fs], wh = h[2 * fs];
         const double l11 = a[3 * fs] - ub * ub, l12 = a[4 * fs] - ub * vb, l13 = a[5 * fs] - ub * wb;
@@ -525,101 +577,108 @@ __global__ void k_lag_contract(const double* __restrict__ b2, const double* __re
     }
 }
 
-void lagrange_contract_dev(les_ctx* c, const double* b2, const double* b4, const double* raw, int init_lm,
-                           int init_qn) {
-    (void)raw;
+template <typename W>
+void lagrange_contract_dev(les_ctx* c, const W* b2, const W* b4, int init_lm, int init_qn) {
     KP p = kp(c);
     const double bexp = log(2.0) / (log(4.0) - log(2.0));
-    hipLaunchKernelGGL(k_lag_contract, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, b2, b4,
+    hipLaunchKernelGGL(k_lag_contract<W>, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, b2, b4,
                        c->f3[LES_F_F_LM], c->f3[LES_F_F_MM], c->f3[LES_F_F_QN], c->f3[LES_F_F_NN],
                        c->f3[LES_F_CS_OPT2], p, init_lm, init_qn, c->prm.dt * (double)c->prm.cs_count, bexp);
 }
 
 // ---- vertical differences, 004_derivatives.f90:85-183 --------------------
-__global__ void k_ddz_uv(double* d, const double* __restrict__ f, KP p) {
+template <typename W, typename TI>
+__global__ void k_ddz_uv(W* d, const TI* __restrict__ f, KP p) {
     const long n = p.plane * (p.nz + 1);
     const double cst = 1.0 / p.dz;
     GRID_LOOP(i, n) {
         const int k = (int)(i / p.plane);
         if (k == 0) d[i] = BOGUS;
-        else if (k >= 2 && k <= p.nz - 1) d[i] = cst * (f[i] - f[i - p.plane]);
+        else if (k >= 2 && k <= p.nz - 1) d[i] = cst * ((double)f[i] - (double)f[i - p.plane]);
     }
 }
 
-__global__ void k_ddz_w(double* d, const double* __restrict__ f, KP p) {
+template <typename W, typename TI>
+__global__ void k_ddz_w(W* d, const TI* __restrict__ f, KP p) {
     const long n = p.plane * (p.nz + 1);
     const double cst = 1.0 / p.dz;
     GRID_LOOP(i, n) {
         const int k = (int)(i / p.plane);
         if (k == 0) d[i] = BOGUS;
         else if (k == p.nz) d[i] = 0.0;
-        else d[i] = cst * (f[i + p.plane] - f[i]);
+        else d[i] = cst * ((double)f[i + p.plane] - (double)f[i]);
     }
 }
 
-void ddz_uv_dev(les_ctx* c, double* d, const double* f) {
+template <typename W, typename TI>
+void ddz_uv_dev(les_ctx* c, W* d, const TI* f) {
     KP p = kp(c);
-    hipLaunchKernelGGL(k_ddz_uv, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, d, f, p);
+    hipLaunchKernelGGL((k_ddz_uv<W, TI>), grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, d, f, p);
 }
-void ddz_w_dev(les_ctx* c, double* d, const double* f) {
+template <typename W, typename TI>
+void ddz_w_dev(les_ctx* c, W* d, const TI* f) {
     KP p = kp(c);
-    hipLaunchKernelGGL(k_ddz_w, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, d, f, p);
+    hipLaunchKernelGGL((k_ddz_w<W, TI>), grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, d, f, p);
 }
 
 // ---- divstress, 107_divstress.f90:36-102 ---------------------------------------
 // a = d/dx tx, b = d/dy ty, f = tz (FD taken here: ddz_w for uv, ddz_uv for w)
-__global__ void k_divstress(double* out, const double* __restrict__ a, const double* __restrict__ b,
-                            const double* __restrict__ f, KP p, int wver) {
+template <typename W>
+__global__ void k_divstress(W* out, const W* __restrict__ a, const W* __restrict__ b, const W* __restrict__ f, KP p,
+                            int wver) {
     const long n = p.plane * (p.nz + 1);
     const double cst = 1.0 / p.dz;
     GRID_LOOP(i, n) {
         const int k = (int)(i / p.plane);
         if (!wver) {
             if (k == 0 || k == p.nz) out[i] = BOGUS;
-            else out[i] = a[i] + b[i] + cst * (f[i + p.plane] - f[i]);
+            else out[i] = (double)a[i] + (double)b[i] + cst * ((double)f[i + p.plane] - (double)f[i]);
         } else {
             if (k == 0) out[i] = BOGUS;
-            else if (k == 1 || k == p.nz) out[i] = a[i] + b[i];
-            else out[i] = a[i] + b[i] + cst * (f[i] - f[i - p.plane]);
+            else if (k == 1 || k == p.nz) out[i] = (double)a[i] + (double)b[i];
+            else out[i] = (double)a[i] + (double)b[i] + cst * ((double)f[i] - (double)f[i - p.plane]);
         }
     }
 }
 
-void divstress_combine_dev(les_ctx* c, double* out, const double* a, const double* b, const double* d, int wver) {
+template <typename W>
+void divstress_combine_dev(les_ctx* c, W* out, const W* a, const W* b, const W* d, int wver) {
     KP p = kp(c);
-    hipLaunchKernelGGL(k_divstress, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, out, a, b, d, p, wver);
+    hipLaunchKernelGGL(k_divstress<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, out, a, b, d, p, wver);
 }
 
 // ---- convec, 108_convec.f90:44-69 and cross_product :103-202 -------------------
-__global__ void k_vort(double* v1, double* v2, double* v3, Grad g, KP p) {
+template <typename W>
+__global__ void k_vort(W* v1, W* v2, W* v3, Grad<W> g, KP p) {
     const long n = p.plane * p.nz;
     GRID_LOOP(t, n) {
         const long i = p.plane + t;
         const int k = (int)(i / p.plane);
         if (k == 1) {
-            v1[i] = 0.5 * (g.wy[i] + g.wy[i + p.plane]) - g.vz[i];
-            v2[i] = g.uz[i] - 0.5 * (g.wx[i] + g.wx[i + p.plane]);
+            v1[i] = 0.5 * ((double)g.wy[i] + (double)g.wy[i + p.plane]) - (double)g.vz[i];
+            v2[i] = (double)g.uz[i] - 0.5 * ((double)g.wx[i] + (double)g.wx[i + p.plane]);
         } else if (k == p.nz) {
-            v1[i] = 0.5 * (g.wy[i] + g.wy[i - p.plane]) - g.vz[i];
-            v2[i] = g.uz[i] - 0.5 * (g.wx[i] + g.wx[i - p.plane]);
+            v1[i] = 0.5 * ((double)g.wy[i] + (double)g.wy[i - p.plane]) - (double)g.vz[i];
+            v2[i] = (double)g.uz[i] - 0.5 * ((double)g.wx[i] + (double)g.wx[i - p.plane]);
         } else {
-            v1[i] = g.wy[i] - g.vz[i];
-            v2[i] = g.uz[i] - g.wx[i];
+            v1[i] = (double)g.wy[i] - (double)g.vz[i];
+            v2[i] = (double)g.uz[i] - (double)g.wx[i];
         }
-        v3[i] = g.vx[i] - g.uy[i];
+        v3[i] = (double)g.vx[i] - (double)g.uy[i];
     }
 }
 
-void vorticity_dev(les_ctx* c, double* v1, double* v2, double* v3) {
+template <typename W>
+void vorticity_dev(les_ctx* c, W* v1, W* v2, W* v3) {
     KP p = kp(c);
-    hipLaunchKernelGGL(k_vort, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, v1, v2, v3, grads(c), p);
+    hipLaunchKernelGGL(k_vort<W>, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, v1, v2, v3, grads<W>(c), p);
 }
 
-__global__ void k_cross(const double* __restrict__ ub, const double* __restrict__ vb,
-                        const double* __restrict__ wb, const double* __restrict__ b1,
-                        const double* __restrict__ b2, const double* __restrict__ b3,
-                        const double* __restrict__ ust, const double* __restrict__ vst, double* cxb, double* cyb,
-                        double* czb, KP p) {
+template <typename W>
+__global__ void k_cross(const W* __restrict__ ub, const W* __restrict__ vb, const W* __restrict__ wb,
+                        const W* __restrict__ b1, const W* __restrict__ b2, const W* __restrict__ b3,
+                        const double* __restrict__ ust, const double* __restrict__ vst, W* cxb, W* cyb, W* czb,
+                        KP p) {
     const long P2 = p.plane2;
     const long n = P2 * (p.nz - 1);  // levels 1..nz-1
     GRID_LOOP(t, n) {
@@ -638,16 +697,17 @@ __global__ void k_cross(const double* __restrict__ ub, const double* __restrict_
     }
 }
 
-void cross_product_dev(les_ctx* c, const double* b1, const double* b2, const double* b3, double* cxb,
-                       double* cyb, double* czb) {
+template <typename W>
+void cross_product_dev(les_ctx* c, const W* b1, const W* b2, const W* b3, W* cxb, W* cyb, W* czb) {
     KP p = kp(c);
-    hipLaunchKernelGGL(k_cross, grid_for(p.plane2 * (p.nz - 1)), dim3(256), 0, c->stream,
-                       (const double*)c->u_big, (const double*)c->v_big, (const double*)c->w_big, b1, b2, b3,
+    hipLaunchKernelGGL(k_cross<W>, grid_for(p.plane2 * (p.nz - 1)), dim3(256), 0, c->stream,
+                       (const W*)c->u_big, (const W*)c->v_big, (const W*)c->w_big, b1, b2, b3,
                        (const double*)c->ust, (const double*)c->vst, cxb, cyb, czb, p);
 }
 
 // ---- scalar transport, 010_scalars.f90:111-671 ----------------------------------
-__global__ void k_theta_prep(double* dTdz, double* rhsf, const double* __restrict__ rhs, KP p, double top) {
+template <typename W>
+__global__ void k_theta_prep(W* dTdz, double* rhsf, const double* __restrict__ rhs, KP p, double top) {
     const long n = p.plane * (p.nz + 1);
     GRID_LOOP(i, n) {
         rhsf[i] = rhs[i];
@@ -655,18 +715,19 @@ __global__ void k_theta_prep(double* dTdz, double* rhsf, const double* __restric
     }
 }
 
+template <typename W>
 void theta_prep_dev(les_ctx* c) {
     KP p = kp(c);
     const les_params& P = c->prm;
-    hipLaunchKernelGGL(k_theta_prep, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, c->f3[LES_F_DTDZ],
+    hipLaunchKernelGGL(k_theta_prep<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, WF<W>(c, LES_F_DTDZ),
                        c->f3[LES_F_RHST_F], (const double*)c->f3[LES_F_RHST], p, P.dTdz_top / P.T_scale * P.z_i);
 }
 
 // dealiased advection on the 3/2 grid, levels 1..nz-1 (:466-484)
-__global__ void k_scalar_adv(const double* __restrict__ ub, const double* __restrict__ vb,
-                             const double* __restrict__ wb, const double* __restrict__ sx,
-                             const double* __restrict__ sy, const double* __restrict__ sz,
-                             const double* __restrict__ ust, const double* __restrict__ vst, double* rm, KP p) {
+template <typename W>
+__global__ void k_scalar_adv(const W* __restrict__ ub, const W* __restrict__ vb, const W* __restrict__ wb,
+                             const W* __restrict__ sx, const W* __restrict__ sy, const W* __restrict__ sz,
+                             const double* __restrict__ ust, const double* __restrict__ vst, W* rm, KP p) {
     const long P2 = p.plane2;
     const long n = P2 * (p.nz - 1);
     GRID_LOOP(t, n) {
@@ -675,20 +736,23 @@ __global__ void k_scalar_adv(const double* __restrict__ ub, const double* __rest
         if (k == 1)
             rm[i] = (ub[i] + ust[1]) * sx[i] + (vb[i] + vst[1]) * sy[i] + (0.5 * wb[i + P2]) * sz[i + P2];
         else
-            rm[i] = (ub[i] + ust[k]) * sx[i] + (vb[i] + vst[k]) * sy[i] + (wb[i] * sz[i] + wb[i + P2] * sz[i + P2]) / 2.0;
+            rm[i] = (ub[i] + ust[k]) * sx[i] + (vb[i] + vst[k]) * sy[i] +
+                    ((double)wb[i] * sz[i] + (double)wb[i + P2] * sz[i + P2]) / 2.0;
     }
 }
 
-void scalar_adv_dev(les_ctx* c, const double* dsx, const double* dsy, const double* dsz, double* rm) {
+template <typename W>
+void scalar_adv_dev(les_ctx* c, const W* dsx, const W* dsy, const W* dsz, W* rm) {
     KP p = kp(c);
-    hipLaunchKernelGGL(k_scalar_adv, grid_for(p.plane2 * (p.nz - 1)), dim3(256), 0, c->stream,
-                       (const double*)c->u_big, (const double*)c->v_big, (const double*)c->w_big, dsx, dsy, dsz,
+    hipLaunchKernelGGL(k_scalar_adv<W>, grid_for(p.plane2 * (p.nz - 1)), dim3(256), 0, c->stream,
+                       (const W*)c->u_big, (const W*)c->v_big, (const W*)c->w_big, dsx, dsy, dsz,
                        (const double*)c->ust, (const double*)c->vst, rm, p);
 }
 
 // SGS flux components: comp 0 x (:513-541), 1 y (:552-569), 2 z (:578-594)
-__global__ void k_scalar_flux(double* temp, double* sgs_t3, const double* __restrict__ Nu,
-                              const double* __restrict__ ds, KP p, int comp, double iPr, double surf_flux) {
+template <typename W>
+__global__ void k_scalar_flux(W* temp, double* sgs_t3, const W* __restrict__ Nu, const W* __restrict__ ds, KP p,
+                              int comp, double iPr, double surf_flux) {
     const long n = p.plane * (p.nz + 1);
     GRID_LOOP(i, n) {
         const int k = (int)(i / p.plane);
@@ -696,7 +760,7 @@ __global__ void k_scalar_flux(double* temp, double* sgs_t3, const double* __rest
         if (comp < 2) {
             if (k == 1) t = iPr * Nu[i] * ds[i];
             else if (k == p.nz - 1) t = iPr * Nu[i + p.plane] * ds[i];
-            else if (k >= 2 && k <= p.nz - 2) t = iPr * 0.5 * (Nu[i] + Nu[i + p.plane]) * ds[i];
+            else if (k >= 2 && k <= p.nz - 2) t = iPr * 0.5 * ((double)Nu[i] + (double)Nu[i + p.plane]) * ds[i];
         } else {
             if (k == 1) { t = -1.0 * surf_flux; sgs_t3[i] = surf_flux; }
             else if (k >= 2) { t = iPr * Nu[i] * ds[i]; sgs_t3[i] = -1.0 * t; }
@@ -705,32 +769,37 @@ __global__ void k_scalar_flux(double* temp, double* sgs_t3, const double* __rest
     }
 }
 
-void scalar_flux_dev(les_ctx* c, double* temp, int comp) {
+template <typename W>
+void scalar_flux_dev(les_ctx* c, W* temp, int comp) {
     KP p = kp(c);
     const les_params& P = c->prm;
-    const double* ds = c->f3[comp == 0 ? LES_F_DTDX : comp == 1 ? LES_F_DTDY : LES_F_DTDZ];
-    hipLaunchKernelGGL(k_scalar_flux, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, temp,
-                       c->f3[LES_F_SGS_T3], (const double*)c->f3[LES_F_NU_T], ds, p, comp, 1.0 / P.Pr,
+    const W* ds = WF<W>(c, comp == 0 ? LES_F_DTDX : comp == 1 ? LES_F_DTDY : LES_F_DTDZ);
+    hipLaunchKernelGGL(k_scalar_flux<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, temp,
+                       c->f3[LES_F_SGS_T3], (const W*)WF<W>(c, LES_F_NU_T), ds, p, comp, 1.0 / P.Pr,
                        P.wt_s / P.T_scale / P.u_scale);
 }
 
-// mode 0: RHS = -RHS + d (first flux), 1: RHS = RHS + d; levels 1..nz-1
-__global__ void k_scalar_accum(double* rhs, const double* __restrict__ d, KP p, int mode) {
+// adv != null: RHS = -adv + d (first flux, adv = the advection term), else
+// RHS = RHS + d; levels 1..nz-1
+template <typename W>
+__global__ void k_scalar_accum(double* rhs, const W* __restrict__ d, const W* __restrict__ adv, KP p) {
     const long n = p.plane * (p.nz - 1);
     GRID_LOOP(t, n) {
         const long i = p.plane + t;
-        rhs[i] = mode == 0 ? (-1.0 * rhs[i] + d[i]) : (rhs[i] + d[i]);
+        rhs[i] = adv ? (-1.0 * (double)adv[i] + d[i]) : (rhs[i] + d[i]);
     }
 }
 
-void scalar_accum_dev(les_ctx* c, const double* dtemp, int mode) {
+template <typename W>
+void scalar_accum_dev(les_ctx* c, const W* dtemp, const W* adv) {
     KP p = kp(c);
-    hipLaunchKernelGGL(k_scalar_accum, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream,
-                       c->f3[LES_F_RHST], dtemp, p, mode);
+    hipLaunchKernelGGL(k_scalar_accum<W>, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream,
+                       c->f3[LES_F_RHST], dtemp, adv, p);
 }
 
 // calc_beta, land branch (:214-258); means[k] = plane mean of theta(k)
-__global__ void k_calc_beta(double* beta, const double* __restrict__ th, const double* __restrict__ bar, KP p,
+template <typename W>
+__global__ void k_calc_beta(W* beta, const double* __restrict__ th, const double* __restrict__ bar, KP p,
                             double g_hat) {
     const long n = p.plane * (p.nz + 1);
     GRID_LOOP(i, n) {
@@ -745,11 +814,12 @@ __global__ void k_calc_beta(double* beta, const double* __restrict__ th, const d
     }
 }
 
+template <typename W>
 void calc_beta_dev(les_ctx* c, const double* means) {
     KP p = kp(c);
     const les_params& P = c->prm;
-    hipLaunchKernelGGL(k_calc_beta, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream,
-                       c->f3[LES_F_BETA_SCAL], (const double*)c->f3[LES_F_THETA], means, p,
+    hipLaunchKernelGGL(k_calc_beta<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream,
+                       WF<W>(c, LES_F_BETA_SCAL), (const double*)c->f3[LES_F_THETA], means, p,
                        GRAV * (P.z_i / (P.u_scale * P.u_scale)));
 }
 
@@ -783,9 +853,13 @@ void step_scalar_dev(les_ctx* c) {
 }
 
 // ---- pressure, 109_press_stag_array.f90:158-199 -------------------------------
-__global__ void k_press_prep(double* rx, double* ry, double* rz, const double* __restrict__ u,
-                             const double* __restrict__ v, const double* __restrict__ w,
-                             const double* __restrict__ divtz, KP p, double cst) {
+// shift != 0 (reduced-precision work fields): u, v, w are taken relative to
+// their value at the level's first point before rounding.  This changes only
+// the kx = ky = 0 coefficients, which the solve does not read from these
+// transforms (Hx, Hy: unused; Hz: replaced by the FP64 plane sums).
+template <typename W>
+__global__ void k_press_prep(W* rx, W* ry, W* rz, const double* __restrict__ u, const double* __restrict__ v,
+                             const double* __restrict__ w, const W* __restrict__ divtz, KP p, double cst, int shift) {
     const long n = p.plane * (p.nz + 1);
     const double tadv1 = 1.5;
     GRID_LOOP(i, n) {
@@ -797,49 +871,91 @@ __global__ void k_press_prep(double* rx, double* ry, double* rz, const double* _
             rz[i] = cst * divtz[i];
             rx[i] = 0.0; ry[i] = 0.0;
         } else {
-            rx[i] = cst / tadv1 * (u[i] / p.dt);
-            ry[i] = cst / tadv1 * (v[i] / p.dt);
-            rz[i] = cst / tadv1 * (w[i] / p.dt);
+            const long i0 = (long)k * p.plane;
+            const double su = shift ? u[i0] : 0.0, sv = shift ? v[i0] : 0.0, sw = shift ? w[i0] : 0.0;
+            rx[i] = cst / tadv1 * ((u[i] - su) / p.dt);
+            ry[i] = cst / tadv1 * ((v[i] - sv) / p.dt);
+            rz[i] = cst / tadv1 * ((w[i] - sw) / p.dt);
         }
     }
 }
 
-void press_rhs_prep_dev(les_ctx* c, double* rx, double* ry, double* rz) {
+template <typename W>
+void press_rhs_prep_dev(les_ctx* c, W* rx, W* ry, W* rz) {
     KP p = kp(c);
-    hipLaunchKernelGGL(k_press_prep, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, rx, ry, rz,
+    hipLaunchKernelGGL(k_press_prep<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, rx, ry, rz,
                        (const double*)c->f3[LES_F_U], (const double*)c->f3[LES_F_V], (const double*)c->f3[LES_F_W],
-                       (const double*)c->f3[LES_F_DIVTZ], p, 1.0 / (double)(c->nx * c->ny));
+                       (const W*)WF<W>(c, LES_F_DIVTZ), p, 1.0 / (double)(c->nx * c->ny),
+                       (int)(sizeof(W) < sizeof(double)));
 }
 
-__global__ void k_dpdz(double* dpdz, const double* __restrict__ pr, KP p) {
+// Plane sums of the z right-hand side of the pressure equation in FP64, one
+// block per level 0..nz (the kx = ky = 0 coefficient of its forward
+// transform, formed from the FP64 velocity rather than from a reduced-
+// precision transform).
+template <typename W>
+__global__ __launch_bounds__(256) void k_hz_sums(const double* __restrict__ w, const W* __restrict__ divtz, KP p,
+                                                  double cst, double* __restrict__ out) {
+    __shared__ double sh[256];
+    const int k = blockIdx.x;
+    const double tadv1 = 1.5;
+    double s = 0.0;
+    for (long q = threadIdx.x; q < p.plane; q += 256) {
+        const long i = (long)k * p.plane + q;
+        if (k == 0) s += cst * divtz[i + p.plane];
+        else if (k == p.nz) s += cst * divtz[i];
+        else s += cst / tadv1 * (w[i] / p.dt);
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1) {
+        if (threadIdx.x < m) sh[threadIdx.x] += sh[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = sh[0];
+}
+
+template <typename W>
+void press_hz_sums_dev(les_ctx* c, double* out) {
+    KP p = kp(c);
+    hipLaunchKernelGGL(k_hz_sums<W>, dim3(p.nz + 1), dim3(256), 0, c->stream, (const double*)c->f3[LES_F_W],
+                       (const W*)WF<W>(c, LES_F_DIVTZ), p, 1.0 / (double)(c->nx * c->ny), out);
+}
+
+template <typename W>
+__global__ void k_dpdz(W* dpdz, const W* __restrict__ pr, KP p) {
     const long n = p.plane * (p.nz - 1);
     GRID_LOOP(t, n) {
         const long i = p.plane + t;
-        dpdz[i] = (pr[i] - pr[i - p.plane]) / p.dz;
+        dpdz[i] = ((double)pr[i] - (double)pr[i - p.plane]) / p.dz;
     }
 }
 
+template <typename W>
 void dpdz_dev(les_ctx* c) {
     KP p = kp(c);
-    hipLaunchKernelGGL(k_dpdz, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, c->f3[LES_F_DPDZ],
-                       (const double*)c->f3[LES_F_P], p);
+    hipLaunchKernelGGL(k_dpdz<W>, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, WF<W>(c, LES_F_DPDZ),
+                       (const W*)WF<W>(c, LES_F_P), p);
 }
 
 // ---- RHS assembly and AB2, 103_flosol.f90:115-162, 319-389 -----------------
+template <typename W>
 struct RhsArgs {
     double *rx, *ry, *rz, *rxf, *ryf, *rzf;
-    const double *cx, *cy, *cz, *dx, *dy, *dz, *beta, *u, *v, *w, *sponge;
+    const W *cx, *cy, *cz, *dx, *dy, *dz, *beta;
+    const double *u, *v, *w, *sponge;
 };
 
-__global__ void k_rhs(RhsArgs a, KP p, int buoy, int cori, double coriol, double ug, double vg, int damp,
+template <typename W>
+__global__ void k_rhs(RhsArgs<W> a, KP p, int buoy, int cori, double coriol, double ug, double vg, int damp,
                       int euler) {
     const long n = p.plane * (p.nz - 1);
     GRID_LOOP(t, n) {
         const long i = p.plane + t;
         const int k = (int)(i / p.plane);
-        double rx = -a.cx[i] - a.dx[i];
-        double ry = -a.cy[i] - a.dy[i];
-        double rz = -a.cz[i] - a.dz[i];
+        double rx = -(double)a.cx[i] - (double)a.dx[i];
+        double ry = -(double)a.cy[i] - (double)a.dy[i];
+        double rz = -(double)a.cz[i] - (double)a.dz[i];
         if (buoy) rz = rz + a.beta[i];
         if (cori) {
             rx = rx + coriol * (a.v[i] - vg);
@@ -851,18 +967,19 @@ __global__ void k_rhs(RhsArgs a, KP p, int buoy, int cori, double coriol, double
     }
 }
 
+template <typename W>
 void rhs_assembly_dev(les_ctx* c, int jt) {
     KP p = kp(c);
     const les_params& P = c->prm;
-    RhsArgs a;
+    RhsArgs<W> a;
     a.rx = c->f3[LES_F_RHSX]; a.ry = c->f3[LES_F_RHSY]; a.rz = c->f3[LES_F_RHSZ];
     a.rxf = c->f3[LES_F_RHSX_F]; a.ryf = c->f3[LES_F_RHSY_F]; a.rzf = c->f3[LES_F_RHSZ_F];
-    a.cx = c->f3[LES_F_CX]; a.cy = c->f3[LES_F_CY]; a.cz = c->f3[LES_F_CZ];
-    a.dx = c->f3[LES_F_DIVTX]; a.dy = c->f3[LES_F_DIVTY]; a.dz = c->f3[LES_F_DIVTZ];
-    a.beta = c->f3[LES_F_BETA_SCAL];
+    a.cx = WF<W>(c, LES_F_CX); a.cy = WF<W>(c, LES_F_CY); a.cz = WF<W>(c, LES_F_CZ);
+    a.dx = WF<W>(c, LES_F_DIVTX); a.dy = WF<W>(c, LES_F_DIVTY); a.dz = WF<W>(c, LES_F_DIVTZ);
+    a.beta = WF<W>(c, LES_F_BETA_SCAL);
     a.u = c->f3[LES_F_U]; a.v = c->f3[LES_F_V]; a.w = c->f3[LES_F_W]; a.sponge = c->sponge;
     const int damp = (P.ubc_mom == LES_BC_STRESS_FREE && P.ubc == 1 && P.damping_method == 1);
-    hipLaunchKernelGGL(k_rhs, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, a, p,
+    hipLaunchKernelGGL(k_rhs<W>, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, a, p,
                        (int)(P.theta_flag && !P.passive_scalar), P.coriolis_forcing, P.coriol, P.ug, P.vg, damp,
                        (int)(jt == 1 && !P.restart_vel));
 }
@@ -893,8 +1010,9 @@ void ab2_dev(les_ctx* c) {
                        P.use_mean_p_force ? P.mean_p_force : 0.0);
 }
 
-__global__ void k_rhs_gradp(double* rx, double* ry, double* rz, const double* __restrict__ px,
-                            const double* __restrict__ py, const double* __restrict__ pz, KP p) {
+template <typename W>
+__global__ void k_rhs_gradp(double* rx, double* ry, double* rz, const W* __restrict__ px, const W* __restrict__ py,
+                            const W* __restrict__ pz, KP p) {
     const long n = p.plane * (p.nz - 1);
     GRID_LOOP(t, n) {
         const long i = p.plane + t;
@@ -904,16 +1022,18 @@ __global__ void k_rhs_gradp(double* rx, double* ry, double* rz, const double* __
     }
 }
 
+template <typename W>
 void rhs_minus_gradp_dev(les_ctx* c) {
     KP p = kp(c);
-    hipLaunchKernelGGL(k_rhs_gradp, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, c->f3[LES_F_RHSX],
-                       c->f3[LES_F_RHSY], c->f3[LES_F_RHSZ], (const double*)c->f3[LES_F_DPDX],
-                       (const double*)c->f3[LES_F_DPDY], (const double*)c->f3[LES_F_DPDZ], p);
+    hipLaunchKernelGGL(k_rhs_gradp<W>, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, c->f3[LES_F_RHSX],
+                       c->f3[LES_F_RHSY], c->f3[LES_F_RHSZ], (const W*)WF<W>(c, LES_F_DPDX),
+                       (const W*)WF<W>(c, LES_F_DPDY), (const W*)WF<W>(c, LES_F_DPDZ), p);
 }
 
 // project (110_project.f90:16-40) and the boundary rows that follow
-__global__ void k_project(double* u, double* v, double* w, const double* __restrict__ px,
-                          const double* __restrict__ py, const double* __restrict__ pz, KP p) {
+template <typename W>
+__global__ void k_project(double* u, double* v, double* w, const W* __restrict__ px, const W* __restrict__ py,
+                          const W* __restrict__ pz, KP p) {
     const long n = p.plane * (p.nz - 1);
     const double tadv1 = 1.5;
     GRID_LOOP(t, n) {
@@ -945,12 +1065,13 @@ __global__ void k_project_bc(double* u, double* v, double* w, double* th, KP p, 
     }
 }
 
+template <typename W>
 void project_dev(les_ctx* c) {
     KP p = kp(c);
     const les_params& P = c->prm;
-    hipLaunchKernelGGL(k_project, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, c->f3[LES_F_U],
-                       c->f3[LES_F_V], c->f3[LES_F_W], (const double*)c->f3[LES_F_DPDX],
-                       (const double*)c->f3[LES_F_DPDY], (const double*)c->f3[LES_F_DPDZ], p);
+    hipLaunchKernelGGL(k_project<W>, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, c->f3[LES_F_U],
+                       c->f3[LES_F_V], c->f3[LES_F_W], (const W*)WF<W>(c, LES_F_DPDX),
+                       (const W*)WF<W>(c, LES_F_DPDY), (const W*)WF<W>(c, LES_F_DPDZ), p);
     hipLaunchKernelGGL(k_project_bc, grid_for(p.plane), dim3(256), 0, c->stream, c->f3[LES_F_U], c->f3[LES_F_V],
                        c->f3[LES_F_W], c->f3[LES_F_THETA], p, (int)(P.ubc_mom == LES_BC_STRESS_FREE),
                        P.theta_flag);
@@ -1015,5 +1136,36 @@ void max3_dev(les_ctx* c, double* out) {
     hipLaunchKernelGGL(k_max3_final, dim3(1), dim3(256), 0, c->stream, (const double*)part, (int)nb, out,
                        dt / c->dx, dt / c->dy, dt / c->dz);
 }
+
+// ---- instantiations for both work precisions --------------------------------
+#define LES_INST_POINT(W)                                                                                   \
+    template void wallstress_dev<W>(les_ctx*, int, const double*, const double*);                           \
+    template void strain_dev<W>(les_ctx*);                                                                  \
+    template void nu_t_stress_dev<W>(les_ctx*);                                                             \
+    template void lagrange_prep_dev<W>(les_ctx*, W*);                                                       \
+    template void lagrange_contract_dev<W>(les_ctx*, const W*, const W*, int, int);                         \
+    template void divstress_combine_dev<W>(les_ctx*, W*, const W*, const W*, const W*, int);                \
+    template void vorticity_dev<W>(les_ctx*, W*, W*, W*);                                                   \
+    template void cross_product_dev<W>(les_ctx*, const W*, const W*, const W*, W*, W*, W*);                 \
+    template void theta_prep_dev<W>(les_ctx*);                                                              \
+    template void scalar_adv_dev<W>(les_ctx*, const W*, const W*, const W*, W*);                            \
+    template void scalar_flux_dev<W>(les_ctx*, W*, int);                                                    \
+    template void scalar_accum_dev<W>(les_ctx*, const W*, const W*);                                        \
+    template void calc_beta_dev<W>(les_ctx*, const double*);                                                \
+    template void press_rhs_prep_dev<W>(les_ctx*, W*, W*, W*);                                              \
+    template void press_hz_sums_dev<W>(les_ctx*, double*);                                                  \
+    template void dpdz_dev<W>(les_ctx*);                                                                    \
+    template void rhs_assembly_dev<W>(les_ctx*, int);                                                       \
+    template void rhs_minus_gradp_dev<W>(les_ctx*);                                                         \
+    template void project_dev<W>(les_ctx*);
+LES_INST_POINT(double)
+LES_INST_POINT(float)
+#undef LES_INST_POINT
+template void ddz_uv_dev<double, double>(les_ctx*, double*, const double*);
+template void ddz_w_dev<double, double>(les_ctx*, double*, const double*);
+template void ddz_uv_dev<float, double>(les_ctx*, float*, const double*);
+template void ddz_w_dev<float, double>(les_ctx*, float*, const double*);
+template void ddz_uv_dev<float, float>(les_ctx*, float*, const float*);
+template void ddz_w_dev<float, float>(les_ctx*, float*, const float*);
 
 }  // namespace les

@@ -17,14 +17,23 @@
 //
 // The column walk is a dependency chain, so loads are issued D levels at a
 // time ahead of the arithmetic (one memory latency per D levels).
+//
+// The right-hand sides H may be FP32 (mixed precision); the recurrences, the
+// forward-sweep store d' and the mean mode are FP64 in both precisions.  With
+// hzm != null the mean mode reads its z right-hand side from hzm (FP64 plane
+// sums, level 0..nz) instead of the transform's (0,0) coefficient.
 #include "les_internal.hpp"
 
 namespace les {
 
 constexpr int TD = 8;  // levels per load batch
 
-__global__ __launch_bounds__(128) void k_tdma(const double* __restrict__ Hx, const double* __restrict__ Hy,
-                                               const double* __restrict__ Hz, double* __restrict__ ph,
+// dsc holds d' (FP64); out receives p.  For H of type double both may be the
+// same buffer: each element is read by its own lane before it is overwritten.
+template <typename TH>
+__global__ __launch_bounds__(128) void k_tdma(const TH* __restrict__ Hx, const TH* __restrict__ Hy,
+                                               const TH* __restrict__ Hz, double* dsc, TH* out,
+                                               const double* __restrict__ hzm,
                                                const double* __restrict__ kx2d, const double* __restrict__ ky2d,
                                                const double* __restrict__ k2a, const unsigned char* __restrict__ nn,
                                                const double* __restrict__ cb, const double* __restrict__ cc,
@@ -35,7 +44,7 @@ __global__ __launch_bounds__(128) void k_tdma(const double* __restrict__ Hx, con
     const int comp = (int)(t & 1);     // 0: real part, 1: imaginary part
     const long s2 = 2 * ns;            // level stride in doubles
     if (!nn[q]) {
-        for (int k = 0; k <= nz - 1; ++k) ph[k * s2 + t] = 0.0;
+        for (int k = 0; k <= nz - 1; ++k) out[k * s2 + t] = (TH)0.0;
         return;
     }
     const double k2 = k2a[q], kx = kx2d[q], ky = ky2d[q];
@@ -44,8 +53,9 @@ __global__ __launch_bounds__(128) void k_tdma(const double* __restrict__ Hx, con
     // i (kx Hx + ky Hy): real part -(kx Hx.im + ky Hy.im), imag part kx Hx.re + ky Hy.re
     const long oxy = t ^ 1;            // the other component of the same element
     const double sgn = comp ? 1.0 : -1.0;
-    const double h0 = Hz[t];
-    double hzk = Hz[s2 + t];
+    const bool m64 = (hzm != nullptr) && (t == 0);
+    const double h0 = m64 ? hzm[0] : (double)Hz[t];
+    double hzk = m64 ? hzm[1] : (double)Hz[s2 + t];
     double dprev = h0;
     // ---- forward sweep ----
     for (int k0 = 1; k0 <= nz - 1; k0 += TD) {
@@ -57,7 +67,7 @@ __global__ __launch_bounds__(128) void k_tdma(const double* __restrict__ Hx, con
                 const long o = k * s2;
                 hx[d] = Hx[o + oxy];
                 hy[d] = Hy[o + oxy];
-                hzn[d] = Hz[o + s2 + t];
+                hzn[d] = m64 ? hzm[k + 1] : (double)Hz[o + s2 + t];
                 b[d] = cb[k * ns + q];
             }
         }
@@ -76,7 +86,7 @@ __global__ __launch_bounds__(128) void k_tdma(const double* __restrict__ Hx, con
                     a = (k == 1) ? 0.0 : -idz;
                 }
                 const double dv = (r - a * dprev) / b[d];
-                ph[k * s2 + t] = dv;
+                dsc[k * s2 + t] = dv;
                 dprev = dv;
                 hzk = hzn[d];
             }
@@ -84,13 +94,14 @@ __global__ __launch_bounds__(128) void k_tdma(const double* __restrict__ Hx, con
     }
     // ---- backward sweep ----
     double pn = dprev;  // level nz-1 (c'(nz-1) = 0)
+    if (nz - 1 >= 1) out[(nz - 1) * s2 + t] = (TH)pn;
     for (int k0 = nz - 2; k0 >= 1; k0 -= TD) {
         double dv[TD], c[TD];
 #pragma unroll
         for (int d = 0; d < TD; ++d) {
             const int k = k0 - d;
             if (k >= 1) {
-                dv[d] = ph[k * s2 + t];
+                dv[d] = dsc[k * s2 + t];
                 c[d] = cc[k * ns + q];
             }
         }
@@ -99,20 +110,26 @@ __global__ __launch_bounds__(128) void k_tdma(const double* __restrict__ Hx, con
             const int k = k0 - d;
             if (k >= 1) {
                 pn = dv[d] - c[d] * pn;
-                ph[k * s2 + t] = pn;
+                out[k * s2 + t] = (TH)pn;
             }
         }
     }
-    ph[t] = pn - dz * h0;
+    out[t] = (TH)(pn - dz * h0);
 }
 
-void tdma_press(les_ctx* c, const cx<double>* Hx, const cx<double>* Hy, const cx<double>* Hz, cx<double>* ph) {
+template <typename T>
+void tdma_press(les_ctx* c, const cx<T>* Hx, const cx<T>* Hy, const cx<T>* Hz, cx<T>* ph, double* dsc,
+                const double* hzm) {
     const long ns = (long)c->ns;
     const int nb = (int)((2 * ns + 127) / 128);
-    hipLaunchKernelGGL(k_tdma, dim3(nb), dim3(128), 0, c->stream, reinterpret_cast<const double*>(Hx),
-                       reinterpret_cast<const double*>(Hy), reinterpret_cast<const double*>(Hz),
-                       reinterpret_cast<double*>(ph), c->kx2d, c->ky2d, c->k2, c->notnyq, c->coef_b, c->coef_c, ns,
+    hipLaunchKernelGGL(k_tdma<T>, dim3(nb), dim3(128), 0, c->stream, reinterpret_cast<const T*>(Hx),
+                       reinterpret_cast<const T*>(Hy), reinterpret_cast<const T*>(Hz), dsc,
+                       reinterpret_cast<T*>(ph), hzm, c->kx2d, c->ky2d, c->k2, c->notnyq, c->coef_b, c->coef_c, ns,
                        c->nz, c->dz);
 }
+template void tdma_press<double>(les_ctx*, const cx<double>*, const cx<double>*, const cx<double>*, cx<double>*,
+                                 double*, const double*);
+template void tdma_press<float>(les_ctx*, const cx<float>*, const cx<float>*, const cx<float>*, cx<float>*,
+                                double*, const double*);
 
 }  // namespace les

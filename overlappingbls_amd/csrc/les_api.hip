@@ -141,6 +141,33 @@ std::vector<double> sponge_profile(const les_ctx* c) {
 
 double* F(les_ctx* c, int id) { return c->f3[id]; }
 
+// Work fields: recomputed inside every step, stored in the precision of the
+// mode (FP32 in LES_MIXED).  Every other 3-D field is FP64 state.
+bool is_work_field(int id) {
+    return (id >= LES_F_DUDX && id <= LES_F_DPDZ) || id == LES_F_NU_T || (id >= LES_F_S11 && id <= LES_F_BETA_SCAL) ||
+           (id >= LES_F_WORK0 && id <= LES_F_WORK2);
+}
+
+// storage of a 3-D field: double or float pointer
+struct FieldRef {
+    double* d;
+    float* f;
+};
+FieldRef field_ref(les_ctx* c, int id) {
+    if (c->precision == LES_MIXED && is_work_field(id)) return {nullptr, c->f32buf[id]};
+    return {c->f3[id], nullptr};
+}
+
+// scratch memory (allocated for FP64) viewed in precision W
+template <typename W>
+W* scr(double* p) {
+    return reinterpret_cast<W*>(p);
+}
+template <typename W>
+cx<W>* scr(cx<double>* p) {
+    return reinterpret_cast<cx<W>*>(p);
+}
+
 double* surf(les_ctx* c, int id) {
     switch (id) {
         case LES_F_USTAR_SFC: return c->ustar;
@@ -167,182 +194,224 @@ bool valid_field(int f) { return f >= 0 && f < LES_N_FIELDS3D; }
 double kfx(const les_ctx* c) { return 2.0 * PI / c->prm.lx_tot; }
 double kfy(const les_ctx* c) { return 2.0 * PI / c->prm.ly_tot; }
 
-void ddx_f(les_ctx* c, double* d, const double* f, int lev0, int nlev) {
-    xline_real<double>(c->stream, f + (size_t)lev0 * c->plane, d + (size_t)lev0 * c->plane, (long)nlev * c->ny,
-                       c->nx, c->nx, c->tw_x, c->tw_x, OP_DERIV, kfx(c));
+template <typename W, typename TI>
+void ddx_f(les_ctx* c, W* d, const TI* f, int lev0, int nlev) {
+    xline_real<W, TI>(c->stream, f + (size_t)lev0 * c->plane, d + (size_t)lev0 * c->plane, (long)nlev * c->ny, c->nx,
+                      c->nx, c->tw_x, c->tw_x, OP_DERIV, kfx(c));
 }
-void ddy_f(les_ctx* c, double* d, const double* f, int lev0, int nlev) {
-    yline_real<double>(c->stream, f + (size_t)lev0 * c->plane, d + (size_t)lev0 * c->plane, nlev, c->nx, c->nx,
-                       c->nx, (long)c->plane, (long)c->plane, c->tw_y, c->tw_y, OP_DERIV, kfy(c));
+template <typename W, typename TI>
+void ddy_f(les_ctx* c, W* d, const TI* f, int lev0, int nlev) {
+    yline_real<W, TI>(c->stream, f + (size_t)lev0 * c->plane, d + (size_t)lev0 * c->plane, nlev, c->nx, c->nx, c->nx,
+                      (long)c->plane, (long)c->plane, c->tw_y, c->tw_y, OP_DERIV, kfy(c));
 }
-void filter_f(les_ctx* c, double* f, int lev0, int nlev) {
-    double* p = f + (size_t)lev0 * c->plane;
-    xline_real<double>(c->stream, p, p, (long)nlev * c->ny, c->nx, c->nx, c->tw_x, c->tw_x, OP_NYQ, 0.0);
-    yline_real<double>(c->stream, p, p, nlev, c->nx, c->nx, c->nx, (long)c->plane, (long)c->plane, c->tw_y, c->tw_y,
-                       OP_NYQ, 0.0);
+template <typename T>
+void filter_f(les_ctx* c, T* f, int lev0, int nlev) {
+    T* p = f + (size_t)lev0 * c->plane;
+    xline_real<T>(c->stream, p, p, (long)nlev * c->ny, c->nx, c->nx, c->tw_x, c->tw_x, OP_NYQ, 0.0);
+    yline_real<T>(c->stream, p, p, nlev, c->nx, c->nx, c->nx, (long)c->plane, (long)c->plane, c->tw_y, c->tw_y, OP_NYQ,
+                  0.0);
 }
 // zero_padding of nlev planes: src (nx x ny planes) -> dst (nx2 x ny2 planes)
-void pad_f(les_ctx* c, const double* src, double* dst, int nlev) {
-    double* tmp = c->work3[6];  // >= nlev * ny * nx2
-    xline_real<double>(c->stream, src, tmp, (long)nlev * c->ny, c->nx, c->nx2, c->tw_x, c->tw_x2, OP_PAD, 0.0);
-    yline_real<double>(c->stream, tmp, dst, nlev, c->nx2, c->nx2, c->nx2, (long)c->ny * c->nx2, (long)c->plane2,
-                       c->tw_y, c->tw_y2, OP_PAD, 0.0);
+template <typename W, typename TI>
+void pad_f(les_ctx* c, const TI* src, W* dst, int nlev) {
+    W* tmp = scr<W>(c->work3[6]);  // >= nlev * ny * nx2
+    xline_real<W, TI>(c->stream, src, tmp, (long)nlev * c->ny, c->nx, c->nx2, c->tw_x, c->tw_x2, OP_PAD, 0.0);
+    yline_real<W>(c->stream, tmp, dst, nlev, c->nx2, c->nx2, c->nx2, (long)c->ny * c->nx2, (long)c->plane2, c->tw_y,
+                  c->tw_y2, OP_PAD, 0.0);
 }
 // zero_unpadding of nlev planes: src (nx2 x ny2) -> dst (nx x ny)
-void unpad_f(les_ctx* c, const double* src, double* dst, int nlev) {
-    double* tmp = c->work3[6];  // >= nlev * ny2 * nx
-    xline_real<double>(c->stream, src, tmp, (long)nlev * c->ny2, c->nx2, c->nx, c->tw_x2, c->tw_x, OP_UNPAD, 0.0);
-    yline_real<double>(c->stream, tmp, dst, nlev, c->nx, c->nx, c->nx, (long)c->ny2 * c->nx, (long)c->plane,
-                       c->tw_y2, c->tw_y, OP_UNPAD, 0.0);
+template <typename W>
+void unpad_f(les_ctx* c, const W* src, W* dst, int nlev) {
+    W* tmp = scr<W>(c->work3[6]);  // >= nlev * ny2 * nx
+    xline_real<W>(c->stream, src, tmp, (long)nlev * c->ny2, c->nx2, c->nx, c->tw_x2, c->tw_x, OP_UNPAD, 0.0);
+    yline_real<W>(c->stream, tmp, dst, nlev, c->nx, c->nx, c->nx, (long)c->ny2 * c->nx, (long)c->plane, c->tw_y2,
+                  c->tw_y, OP_UNPAD, 0.0);
 }
 // test_filter of nplanes consecutive planes in place
-void test_filter_f(les_ctx* c, double* p, int nplanes, const double* G) {
-    cx<double>* sp = c->spec3[2];  // >= nplanes planes of ns
-    x_r2c<double>(c->stream, p, sp, (long)nplanes * c->ny, c->nx, c->lhx, c->tw_x, 1.0);
-    yline_spec<double>(c->stream, sp, sp, nplanes, c->lhx, (long)c->ns, c->tw_y, SP_FILT, G);
-    x_c2r<double>(c->stream, sp, p, (long)nplanes * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
+template <typename T>
+void test_filter_f(les_ctx* c, T* p, int nplanes, const double* G) {
+    cx<T>* sp = scr<T>(c->spec3[2]);  // >= nplanes planes of ns
+    x_r2c<T>(c->stream, p, sp, (long)nplanes * c->ny, c->nx, c->lhx, c->tw_x, 1.0);
+    yline_spec<T>(c->stream, sp, sp, nplanes, c->lhx, (long)c->ns, c->tw_y, SP_FILT, G);
+    x_c2r<T>(c->stream, sp, p, (long)nplanes * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
 }
 
 // ---- routines ---------------------------------------------------------------
+// The wall model's filtered first-level velocity is FP64 in both modes.
+template <typename W>
 void wallstress(les_ctx* c, int top) {
     const les_params& P = c->prm;
     const int sf = top ? (P.ubc_mom == LES_BC_STRESS_FREE) : (P.lbc_mom == LES_BC_STRESS_FREE);
     if (sf) {
-        wallstress_dev(c, top, nullptr, nullptr);
+        wallstress_dev<W>(c, top, nullptr, nullptr);
         return;
     }
     const int lev = top ? c->nz - 1 : 1;
     double* w = c->work3[7];  // two planes: u1, v1
-    les_note(c, hipMemcpyAsync(w, F(c, LES_F_U) + (size_t)lev * c->plane, c->plane * 8, hipMemcpyDeviceToDevice, c->stream));
-    les_note(c, hipMemcpyAsync(w + c->plane, F(c, LES_F_V) + (size_t)lev * c->plane, c->plane * 8, hipMemcpyDeviceToDevice,
-                   c->stream));
-    test_filter_f(c, w, 2, c->Gtest);
-    wallstress_dev(c, top, w, w + c->plane);
+    les_note(c, hipMemcpyAsync(w, F(c, LES_F_U) + (size_t)lev * c->plane, c->plane * 8, hipMemcpyDeviceToDevice,
+                               c->stream));
+    les_note(c, hipMemcpyAsync(w + c->plane, F(c, LES_F_V) + (size_t)lev * c->plane, c->plane * 8,
+                               hipMemcpyDeviceToDevice, c->stream));
+    test_filter_f<double>(c, w, 2, c->Gtest);
+    wallstress_dev<W>(c, top, w, w + c->plane);
 }
 
+template <typename W>
 void lagrange(les_ctx* c, int jt) {
     const les_params& P = c->prm;
     const int nz = c->nz;
     const long np21 = 21L * nz;  // planes
     interpolag_dev(c);
-    double* raw = c->lag_buf;
-    double* b2 = raw + (size_t)np21 * c->plane;
-    double* b4 = b2 + (size_t)np21 * c->plane;
-    cx<double>* spf = c->lag_spec;
-    cx<double>* spt = spf + (size_t)np21 * c->ns;
-    lagrange_prep_dev(c, raw);
-    x_r2c<double>(c->stream, raw, spf, np21 * c->ny, c->nx, c->lhx, c->tw_x, 1.0);
-    yline_spec<double>(c->stream, spf, spf, (int)np21, c->lhx, (long)c->ns, c->tw_y, SP_FWD, nullptr);
-    yline_spec<double>(c->stream, spf, spt, (int)np21, c->lhx, (long)c->ns, c->tw_y, SP_MULINV, c->Gtest);
-    x_c2r<double>(c->stream, spt, b2, np21 * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
-    yline_spec<double>(c->stream, spf, spt, (int)np21, c->lhx, (long)c->ns, c->tw_y, SP_MULINV, c->Gtest2);
-    x_c2r<double>(c->stream, spt, b4, np21 * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
+    W* raw = scr<W>(c->lag_buf);
+    W* b2 = raw + (size_t)np21 * c->plane;
+    W* b4 = b2 + (size_t)np21 * c->plane;
+    cx<W>* spf = scr<W>(c->lag_spec);
+    cx<W>* spt = spf + (size_t)np21 * c->ns;
+    lagrange_prep_dev<W>(c, raw);
+    x_r2c<W>(c->stream, raw, spf, np21 * c->ny, c->nx, c->lhx, c->tw_x, 1.0);
+    yline_spec<W>(c->stream, spf, spf, (int)np21, c->lhx, (long)c->ns, c->tw_y, SP_FWD, nullptr);
+    yline_spec<W>(c->stream, spf, spt, (int)np21, c->lhx, (long)c->ns, c->tw_y, SP_MULINV, c->Gtest);
+    x_c2r<W>(c->stream, spt, b2, np21 * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
+    yline_spec<W>(c->stream, spf, spt, (int)np21, c->lhx, (long)c->ns, c->tw_y, SP_MULINV, c->Gtest2);
+    x_c2r<W>(c->stream, spt, b4, np21 * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
     const int init_lm = P.inilag && !c->f_lm_mm_init && (jt == P.cs_count || jt == P.dyn_init);
     const int init_qn = P.inilag && !c->f_qn_nn_init && (jt == P.cs_count || jt == P.dyn_init);
-    lagrange_contract_dev(c, b2, b4, raw, init_lm, init_qn);
+    lagrange_contract_dev<W>(c, b2, b4, init_lm, init_qn);
     if (init_lm) c->f_lm_mm_init = 1;
     if (init_qn) c->f_qn_nn_init = 1;
 }
 
+template <typename W>
 void sgs_stag(les_ctx* c, int jt) {
     const les_params& P = c->prm;
-    strain_dev(c);
+    strain_dev<W>(c);
     lag_accumulate_dev(c);
     if (jt == 1 && P.inilag) cs_init_dev(c);
-    else if ((jt >= P.dyn_init || P.restart_vel) && (jt % P.cs_count) == 0) lagrange(c, jt);
-    nu_t_stress_dev(c);
+    else if ((jt >= P.dyn_init || P.restart_vel) && (jt % P.cs_count) == 0) lagrange<W>(c, jt);
+    nu_t_stress_dev<W>(c);
 }
 
-void divstress(les_ctx* c, double* out, const double* tx, const double* ty, const double* tz, int wver) {
-    double* a = c->work3[0];
-    double* b = c->work3[1];
-    ddx_f(c, a, tx, 0, (int)c->nlev);
-    ddy_f(c, b, ty, 0, (int)c->nlev);
-    divstress_combine_dev(c, out, a, b, tz, wver);
+template <typename W>
+void divstress(les_ctx* c, W* out, const W* tx, const W* ty, const W* tz, int wver) {
+    W* a = scr<W>(c->work3[0]);
+    W* b = scr<W>(c->work3[1]);
+    ddx_f<W, W>(c, a, tx, 0, (int)c->nlev);
+    ddy_f<W, W>(c, b, ty, 0, (int)c->nlev);
+    divstress_combine_dev<W>(c, out, a, b, tz, wver);
 }
 
+template <typename W>
 void convec(les_ctx* c) {
     const int nz = c->nz;
     const size_t np = c->plane, np2 = c->plane2;
-    pad_f(c, F(c, LES_F_U), c->u_big, (int)c->nlev);
-    pad_f(c, F(c, LES_F_V), c->v_big, (int)c->nlev);
-    pad_f(c, F(c, LES_F_W), c->w_big, (int)c->nlev);
-    double *v1 = c->work3[2], *v2 = c->work3[3], *v3 = c->work3[4];
-    vorticity_dev(c, v1, v2, v3);
-    pad_f(c, v1 + np, c->big3[0] + np2, nz);
-    pad_f(c, v2 + np, c->big3[1] + np2, nz);
-    pad_f(c, v3 + np, c->big3[2] + np2, nz);
-    cross_product_dev(c, c->big3[0], c->big3[1], c->big3[2], c->big3[3], c->big3[4], c->big3[5]);
-    unpad_f(c, c->big3[3] + np2, F(c, LES_F_CX) + np, nz - 1);
-    unpad_f(c, c->big3[4] + np2, F(c, LES_F_CY) + np, nz - 1);
-    unpad_f(c, c->big3[5] + np2, F(c, LES_F_CZ) + np, nz - 1);
-    fill_levels(c->stream, F(c, LES_F_CX), np, 0, 1, BOGUS);
-    fill_levels(c->stream, F(c, LES_F_CY), np, 0, 1, BOGUS);
-    fill_levels(c->stream, F(c, LES_F_CZ), np, 0, 1, BOGUS);
-    fill_levels(c->stream, F(c, LES_F_CX), np, nz, 1, BOGUS);
-    fill_levels(c->stream, F(c, LES_F_CY), np, nz, 1, BOGUS);
+    W* ub = scr<W>(c->u_big);
+    W* vb = scr<W>(c->v_big);
+    W* wb = scr<W>(c->w_big);
+    W* big[6];
+    for (int i = 0; i < 6; ++i) big[i] = scr<W>(c->big3[i]);
+    pad_f<W, double>(c, F(c, LES_F_U), ub, (int)c->nlev);
+    pad_f<W, double>(c, F(c, LES_F_V), vb, (int)c->nlev);
+    pad_f<W, double>(c, F(c, LES_F_W), wb, (int)c->nlev);
+    W *v1 = scr<W>(c->work3[2]), *v2 = scr<W>(c->work3[3]), *v3 = scr<W>(c->work3[4]);
+    vorticity_dev<W>(c, v1, v2, v3);
+    pad_f<W, W>(c, v1 + np, big[0] + np2, nz);
+    pad_f<W, W>(c, v2 + np, big[1] + np2, nz);
+    pad_f<W, W>(c, v3 + np, big[2] + np2, nz);
+    cross_product_dev<W>(c, big[0], big[1], big[2], big[3], big[4], big[5]);
+    W* cxf = WF<W>(c, LES_F_CX);
+    W* cyf = WF<W>(c, LES_F_CY);
+    W* czf = WF<W>(c, LES_F_CZ);
+    unpad_f<W>(c, big[3] + np2, cxf + np, nz - 1);
+    unpad_f<W>(c, big[4] + np2, cyf + np, nz - 1);
+    unpad_f<W>(c, big[5] + np2, czf + np, nz - 1);
+    fill_levels(c->stream, cxf, np, 0, 1, BOGUS);
+    fill_levels(c->stream, cyf, np, 0, 1, BOGUS);
+    fill_levels(c->stream, czf, np, 0, 1, BOGUS);
+    fill_levels(c->stream, cxf, np, nz, 1, BOGUS);
+    fill_levels(c->stream, cyf, np, nz, 1, BOGUS);
 }
 
+template <typename W>
 void theta_all_in_one(les_ctx* c, int jt) {
     const les_params& P = c->prm;
     const int nz = c->nz;
     const size_t np = c->plane, np2 = c->plane2;
     if (!(P.theta_flag && jt >= P.theta_init_time)) {
-        fill_levels(c->stream, F(c, LES_F_BETA_SCAL), np, 0, (int)c->nlev, 0.0);
+        fill_levels(c->stream, WF<W>(c, LES_F_BETA_SCAL), np, 0, (int)c->nlev, 0.0);
         return;
     }
     double* th = F(c, LES_F_THETA);
-    filter_f(c, th, 0, (int)c->nlev);
-    ddx_f(c, F(c, LES_F_DTDX), th, 0, (int)c->nlev);
-    ddy_f(c, F(c, LES_F_DTDY), th, 0, (int)c->nlev);
-    ddz_uv_dev(c, F(c, LES_F_DTDZ), th);
-    theta_prep_dev(c);
-    pad_f(c, F(c, LES_F_DTDX) + np, c->big3[0] + np2, nz - 1);
-    pad_f(c, F(c, LES_F_DTDY) + np, c->big3[1] + np2, nz - 1);
-    pad_f(c, F(c, LES_F_DTDZ) + 2 * np, c->big3[2] + 2 * np2, nz - 1);
-    scalar_adv_dev(c, c->big3[0], c->big3[1], c->big3[2], c->big3[3]);
-    unpad_f(c, c->big3[3] + np2, F(c, LES_F_RHST) + np, nz - 1);
-    double* temp = c->work3[0];
-    double* dtemp = c->work3[1];
-    scalar_flux_dev(c, temp, 0);
-    ddx_f(c, dtemp, temp, 0, (int)c->nlev);
-    scalar_accum_dev(c, dtemp, 0);
-    scalar_flux_dev(c, temp, 1);
-    ddy_f(c, dtemp, temp, 0, (int)c->nlev);
-    scalar_accum_dev(c, dtemp, 1);
-    scalar_flux_dev(c, temp, 2);
-    ddz_w_dev(c, dtemp, temp);
-    scalar_accum_dev(c, dtemp, 1);
+    W* dtdx = WF<W>(c, LES_F_DTDX);
+    W* dtdy = WF<W>(c, LES_F_DTDY);
+    W* dtdz = WF<W>(c, LES_F_DTDZ);
+    W* big[4];
+    for (int i = 0; i < 4; ++i) big[i] = scr<W>(c->big3[i]);
+    filter_f<double>(c, th, 0, (int)c->nlev);
+    ddx_f<W, double>(c, dtdx, th, 0, (int)c->nlev);
+    ddy_f<W, double>(c, dtdy, th, 0, (int)c->nlev);
+    ddz_uv_dev<W, double>(c, dtdz, th);
+    theta_prep_dev<W>(c);
+    pad_f<W, W>(c, dtdx + np, big[0] + np2, nz - 1);
+    pad_f<W, W>(c, dtdy + np, big[1] + np2, nz - 1);
+    pad_f<W, W>(c, dtdz + 2 * np, big[2] + 2 * np2, nz - 1);
+    scalar_adv_dev<W>(c, big[0], big[1], big[2], big[3]);
+    W* adv = scr<W>(c->work3[2]);
+    unpad_f<W>(c, big[3] + np2, adv + np, nz - 1);
+    W* temp = scr<W>(c->work3[0]);
+    W* dtemp = scr<W>(c->work3[1]);
+    scalar_flux_dev<W>(c, temp, 0);
+    ddx_f<W, W>(c, dtemp, temp, 0, (int)c->nlev);
+    scalar_accum_dev<W>(c, dtemp, adv);
+    scalar_flux_dev<W>(c, temp, 1);
+    ddy_f<W, W>(c, dtemp, temp, 0, (int)c->nlev);
+    scalar_accum_dev<W>(c, dtemp, (const W*)nullptr);
+    scalar_flux_dev<W>(c, temp, 2);
+    ddz_w_dev<W, W>(c, dtemp, temp);
+    scalar_accum_dev<W>(c, dtemp, (const W*)nullptr);
     level_means(c->stream, th, np, 0, nz, (double)(c->nx * c->ny), c->red);
-    calc_beta_dev(c, c->red);
+    calc_beta_dev<W>(c, c->red);
     if (jt == P.theta_init_time && !P.restart_theta)
-        les_note(c, hipMemcpyAsync(F(c, LES_F_RHST_F), F(c, LES_F_RHST), np * c->nlev * 8, hipMemcpyDeviceToDevice, c->stream));
+        les_note(c, hipMemcpyAsync(F(c, LES_F_RHST_F), F(c, LES_F_RHST), np * c->nlev * 8, hipMemcpyDeviceToDevice,
+                                   c->stream));
     step_scalar_dev(c);
 }
 
+// FP64: H spectra in spec3[0..2], p_hat (and d') in spec3[3].  Mixed: FP32
+// H spectra in spec3[0..2], d' in spec3[3] (FP64), FP32 p_hat in phf, and the
+// mean mode from FP64 plane sums.
+template <typename W>
 void press_stag_array(les_ctx* c) {
     const int nz = c->nz;
     const size_t np = c->plane, ns = c->ns;
-    double *rx = c->work3[0], *ry = c->work3[1], *rz = c->work3[2];
-    press_rhs_prep_dev(c, rx, ry, rz);
-    cx<double>*Hx = c->spec3[0], *Hy = c->spec3[1], *Hz = c->spec3[2];
+    W *rx = scr<W>(c->work3[0]), *ry = scr<W>(c->work3[1]), *rz = scr<W>(c->work3[2]);
+    press_rhs_prep_dev<W>(c, rx, ry, rz);
+    cx<W>*Hx = scr<W>(c->spec3[0]), *Hy = scr<W>(c->spec3[1]), *Hz = scr<W>(c->spec3[2]);
     const long rows = (long)(nz + 1) * c->ny;
-    x_r2c<double>(c->stream, rx, Hx, rows, c->nx, c->lhx, c->tw_x, 1.0);
-    x_r2c<double>(c->stream, ry, Hy, rows, c->nx, c->lhx, c->tw_x, 1.0);
-    x_r2c<double>(c->stream, rz, Hz, rows, c->nx, c->lhx, c->tw_x, 1.0);
-    yline_spec<double>(c->stream, Hx, Hx, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
-    yline_spec<double>(c->stream, Hy, Hy, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
-    yline_spec<double>(c->stream, Hz, Hz, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
-    cx<double>* ph = c->spec3[3];
-    tdma_press(c, Hx, Hy, Hz, ph);
-    yline_spec<double>(c->stream, ph, ph, nz, c->lhx, (long)ns, c->tw_y, SP_INV, nullptr);
-    x_c2r<double>(c->stream, ph, F(c, LES_F_P), (long)nz * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
-    x_c2r<double>(c->stream, ph + ns, F(c, LES_F_DPDX) + np, (long)(nz - 1) * c->ny, c->lhx, c->nx, c->tw_x, 1,
-                  kfx(c));
-    ddy_f(c, F(c, LES_F_DPDY), F(c, LES_F_P), 1, nz - 1);
-    dpdz_dev(c);
+    x_r2c<W>(c->stream, rx, Hx, rows, c->nx, c->lhx, c->tw_x, 1.0);
+    x_r2c<W>(c->stream, ry, Hy, rows, c->nx, c->lhx, c->tw_x, 1.0);
+    x_r2c<W>(c->stream, rz, Hz, rows, c->nx, c->lhx, c->tw_x, 1.0);
+    yline_spec<W>(c->stream, Hx, Hx, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
+    yline_spec<W>(c->stream, Hy, Hy, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
+    yline_spec<W>(c->stream, Hz, Hz, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
+    cx<W>* ph;
+    const double* hzm = nullptr;
+    if (sizeof(W) == sizeof(double)) {
+        ph = scr<W>(c->spec3[3]);
+    } else {
+        ph = reinterpret_cast<cx<W>*>(c->phf);
+        press_hz_sums_dev<W>(c, c->hzm);
+        hzm = c->hzm;
+    }
+    tdma_press<W>(c, Hx, Hy, Hz, ph, reinterpret_cast<double*>(c->spec3[3]), hzm);
+    W* pf = WF<W>(c, LES_F_P);
+    yline_spec<W>(c->stream, ph, ph, nz, c->lhx, (long)ns, c->tw_y, SP_INV, nullptr);
+    x_c2r<W>(c->stream, ph, pf, (long)nz * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
+    x_c2r<W>(c->stream, ph + ns, WF<W>(c, LES_F_DPDX) + np, (long)(nz - 1) * c->ny, c->lhx, c->nx, c->tw_x, 1, kfx(c));
+    ddy_f<W, W>(c, WF<W>(c, LES_F_DPDY), pf, 1, nz - 1);
+    dpdz_dev<W>(c);
 }
 
+template <typename W>
 void step_body(les_ctx* c, int jt) {
     // newstep: RHS_f = RHS (103_flosol.f90:311-313) as a pointer exchange
     std::swap(c->f3[LES_F_RHSX], c->f3[LES_F_RHSX_F]);
@@ -350,31 +419,93 @@ void step_body(les_ctx* c, int jt) {
     std::swap(c->f3[LES_F_RHSZ], c->f3[LES_F_RHSZ_F]);
     obukhov_dev(c, jt);
     const int nl = (int)c->nlev;
-    ddx_f(c, F(c, LES_F_DUDX), F(c, LES_F_U), 0, nl);
-    ddy_f(c, F(c, LES_F_DUDY), F(c, LES_F_U), 0, nl);
-    ddx_f(c, F(c, LES_F_DVDX), F(c, LES_F_V), 0, nl);
-    ddy_f(c, F(c, LES_F_DVDY), F(c, LES_F_V), 0, nl);
-    ddx_f(c, F(c, LES_F_DWDX), F(c, LES_F_W), 0, nl);
-    ddy_f(c, F(c, LES_F_DWDY), F(c, LES_F_W), 0, nl);
-    ddz_uv_dev(c, F(c, LES_F_DUDZ), F(c, LES_F_U));
-    ddz_uv_dev(c, F(c, LES_F_DVDZ), F(c, LES_F_V));
-    ddz_w_dev(c, F(c, LES_F_DWDZ), F(c, LES_F_W));
-    wallstress(c, 0);
-    wallstress(c, 1);
-    sgs_stag(c, jt);
-    divstress(c, F(c, LES_F_DIVTX), F(c, LES_F_TXX), F(c, LES_F_TXY), F(c, LES_F_TXZ), 0);
-    divstress(c, F(c, LES_F_DIVTY), F(c, LES_F_TXY), F(c, LES_F_TYY), F(c, LES_F_TYZ), 0);
-    divstress(c, F(c, LES_F_DIVTZ), F(c, LES_F_TXZ), F(c, LES_F_TYZ), F(c, LES_F_TZZ), 1);
-    convec(c);
-    theta_all_in_one(c, jt);
-    rhs_assembly_dev(c, jt);
+    const double *u = F(c, LES_F_U), *v = F(c, LES_F_V), *w = F(c, LES_F_W);
+    ddx_f<W, double>(c, WF<W>(c, LES_F_DUDX), u, 0, nl);
+    ddy_f<W, double>(c, WF<W>(c, LES_F_DUDY), u, 0, nl);
+    ddx_f<W, double>(c, WF<W>(c, LES_F_DVDX), v, 0, nl);
+    ddy_f<W, double>(c, WF<W>(c, LES_F_DVDY), v, 0, nl);
+    ddx_f<W, double>(c, WF<W>(c, LES_F_DWDX), w, 0, nl);
+    ddy_f<W, double>(c, WF<W>(c, LES_F_DWDY), w, 0, nl);
+    ddz_uv_dev<W, double>(c, WF<W>(c, LES_F_DUDZ), u);
+    ddz_uv_dev<W, double>(c, WF<W>(c, LES_F_DVDZ), v);
+    ddz_w_dev<W, double>(c, WF<W>(c, LES_F_DWDZ), w);
+    wallstress<W>(c, 0);
+    wallstress<W>(c, 1);
+    sgs_stag<W>(c, jt);
+    W *txx = WF<W>(c, LES_F_TXX), *txy = WF<W>(c, LES_F_TXY), *txz = WF<W>(c, LES_F_TXZ);
+    W *tyy = WF<W>(c, LES_F_TYY), *tyz = WF<W>(c, LES_F_TYZ), *tzz = WF<W>(c, LES_F_TZZ);
+    divstress<W>(c, WF<W>(c, LES_F_DIVTX), txx, txy, txz, 0);
+    divstress<W>(c, WF<W>(c, LES_F_DIVTY), txy, tyy, tyz, 0);
+    divstress<W>(c, WF<W>(c, LES_F_DIVTZ), txz, tyz, tzz, 1);
+    convec<W>(c);
+    theta_all_in_one<W>(c, jt);
+    rhs_assembly_dev<W>(c, jt);
     ab2_dev(c);
-    press_stag_array(c);
-    rhs_minus_gradp_dev(c);
-    project_dev(c);
-    filter_f(c, F(c, LES_F_U), 0, nl);
-    filter_f(c, F(c, LES_F_V), 0, nl);
-    filter_f(c, F(c, LES_F_W), 0, nl);
+    press_stag_array<W>(c);
+    rhs_minus_gradp_dev<W>(c);
+    project_dev<W>(c);
+    filter_f<double>(c, F(c, LES_F_U), 0, nl);
+    filter_f<double>(c, F(c, LES_F_V), 0, nl);
+    filter_f<double>(c, F(c, LES_F_W), 0, nl);
+}
+
+bool mixed(const les_ctx* c) { return c->precision == LES_MIXED; }
+
+// Run a routine templated on the work precision in the current mode.
+#define LES_DISPATCH(fn, ...)                          \
+    do {                                               \
+        if (mixed(c)) fn<float>(__VA_ARGS__);          \
+        else fn<double>(__VA_ARGS__);                  \
+    } while (0)
+
+// A derivative writes an FP64 field from an FP64 source, or a work field in
+// the mode's precision from either; an FP64 destination with an FP32 source
+// (mixed mode) is rejected.
+enum DerivKind { D_X, D_Y, D_ZUV, D_ZW };
+int deriv_entry(les_ctx* c, int d, int f, DerivKind k) {
+    if (!c || !valid_field(d) || !valid_field(f)) return LES_E_BADARG;
+    if ((k == D_ZUV || k == D_ZW) && d == f) return LES_E_BADARG;
+    const FieldRef rd = field_ref(c, d), rf = field_ref(c, f);
+    const int nl = (int)c->nlev;
+    if (rd.d && rf.d) {
+        switch (k) {
+            case D_X: ddx_f<double, double>(c, rd.d, rf.d, 0, nl); break;
+            case D_Y: ddy_f<double, double>(c, rd.d, rf.d, 0, nl); break;
+            case D_ZUV: ddz_uv_dev<double, double>(c, rd.d, rf.d); break;
+            default: ddz_w_dev<double, double>(c, rd.d, rf.d); break;
+        }
+    } else if (rd.f && rf.d) {
+        switch (k) {
+            case D_X: ddx_f<float, double>(c, rd.f, rf.d, 0, nl); break;
+            case D_Y: ddy_f<float, double>(c, rd.f, rf.d, 0, nl); break;
+            case D_ZUV: ddz_uv_dev<float, double>(c, rd.f, rf.d); break;
+            default: ddz_w_dev<float, double>(c, rd.f, rf.d); break;
+        }
+    } else if (rd.f && rf.f) {
+        switch (k) {
+            case D_X: ddx_f<float, float>(c, rd.f, rf.f, 0, nl); break;
+            case D_Y: ddy_f<float, float>(c, rd.f, rf.f, 0, nl); break;
+            case D_ZUV: ddz_uv_dev<float, float>(c, rd.f, rf.f); break;
+            default: ddz_w_dev<float, float>(c, rd.f, rf.f); break;
+        }
+    } else {
+        return LES_E_BADARG;
+    }
+    return finish(c);
+}
+// divstress on four fields that share one storage precision
+int divstress_entry(les_ctx* c, int divt, int tx, int ty, int tz, int wver) {
+    if (!c || !valid_field(divt) || !valid_field(tx) || !valid_field(ty) || !valid_field(tz)) return LES_E_BADARG;
+    const FieldRef r[4] = {field_ref(c, divt), field_ref(c, tx), field_ref(c, ty), field_ref(c, tz)};
+    bool all_d = true, all_f = true;
+    for (const FieldRef& x : r) {
+        all_d = all_d && x.d;
+        all_f = all_f && x.f;
+    }
+    if (all_d) divstress<double>(c, r[0].d, r[1].d, r[2].d, r[3].d, wver);
+    else if (all_f) divstress<float>(c, r[0].f, r[1].f, r[2].f, r[3].f, wver);
+    else return LES_E_BADARG;
+    return finish(c);
 }
 
 }  // namespace
@@ -445,6 +576,7 @@ int les_create(const les_params* prm, les_ctx** out) {
         return LES_E_HIP;
     }
     for (int f = 0; f < LES_N_FIELDS3D; ++f) les_note(c, hipMemsetAsync(c->f3[f], 0, F3 * 8, c->stream));
+    for (int f = 0; f < LES_N_FIELDS3D; ++f) c->wk[f] = is_work_field(f) ? (void*)c->f3[f] : nullptr;
     les_note(c, hipMemsetAsync(c->ust, 0, c->nlev * 8, c->stream));
     les_note(c, hipMemsetAsync(c->vst, 0, c->nlev * 8, c->stream));
     les_note(c, hipMemsetAsync(c->psi_m, 0, c->plane * 8, c->stream));
@@ -472,11 +604,32 @@ int les_destroy(les_ctx* c) {
     return LES_OK;
 }
 
+// LES_MIXED: FFTs, nonlinear terms, SGS and scalar RHS on FP32 work fields;
+// the prognostic and SGS state, the TDMA, the mean mode, the wall model and
+// the surface statistics stay FP64.  The FP32 buffers are allocated on the
+// first switch and kept until les_destroy.
 int les_set_precision(les_ctx* c, int precision) {
-    if (!c) return LES_E_BADARG;
-    if (precision != LES_FP64) return LES_E_BADARG;
+    if (!c || (precision != LES_FP64 && precision != LES_MIXED)) return LES_E_BADARG;
+    if (precision == LES_MIXED && !c->phf) {
+        const size_t F3 = c->plane * c->nlev;
+        hipError_t e = hipSuccess;
+        for (int f = 0; f < LES_N_FIELDS3D && e == hipSuccess; ++f)
+            if (is_work_field(f)) e = dalloc(c, &c->f32buf[f], F3);
+        if (e == hipSuccess) e = dalloc(c, &c->hzm, c->nlev);
+        if (e == hipSuccess) e = dalloc(c, &c->lmean, 2 * c->nlev);
+        cx<float>* ph = nullptr;
+        if (e == hipSuccess) e = dalloc(c, &ph, c->ns * c->nlev);
+        if (e != hipSuccess) return LES_E_NOMEM;
+        for (int f = 0; f < LES_N_FIELDS3D; ++f)
+            if (c->f32buf[f]) les_note(c, hipMemsetAsync(c->f32buf[f], 0, F3 * 4, c->stream));
+        c->phf = ph;
+    }
     c->precision = precision;
-    return LES_OK;
+    for (int f = 0; f < LES_N_FIELDS3D; ++f) {
+        if (!is_work_field(f)) c->wk[f] = nullptr;
+        else c->wk[f] = (precision == LES_MIXED) ? (void*)c->f32buf[f] : (void*)c->f3[f];
+    }
+    return finish(c);
 }
 
 int les_set_stokes(les_ctx* c, const double* ust, const double* vst) {
@@ -492,18 +645,28 @@ int les_set_sponge(les_ctx* c, const double* sp) {
     return finish(c);
 }
 
+// FP32 work fields of the mixed mode are converted through an FP64 staging
+// field (work3[7], one full 3-D field).
 int les_set_field(les_ctx* c, int id, const double* h, int ld, int lev0, int nlev) {
     if (!c || !h || ld < c->nx) return LES_E_BADARG;
     double* d;
+    float* df = nullptr;
     if (valid_field(id)) {
         if (lev0 < 0 || nlev < 0 || lev0 + nlev > (int)c->nlev) return LES_E_BADARG;
-        d = c->f3[id] + (size_t)lev0 * c->plane;
+        const FieldRef r = field_ref(c, id);
+        if (r.f) {
+            df = r.f + (size_t)lev0 * c->plane;
+            d = c->work3[7];
+        } else {
+            d = r.d + (size_t)lev0 * c->plane;
+        }
     } else {
         d = surf(c, id);
         if (!d || lev0 != 0 || nlev != 1) return LES_E_BADARG;
     }
     les_note(c, hipMemcpy2DAsync(d, (size_t)c->nx * 8, h, (size_t)ld * 8, (size_t)c->nx * 8, (size_t)nlev * c->ny,
                      hipMemcpyHostToDevice, c->stream));
+    if (df) convert(c->stream, df, (const double*)d, (long)nlev * (long)c->plane);
     return finish(c);
 }
 
@@ -512,7 +675,13 @@ int les_get_field(les_ctx* c, int id, double* h, int ld, int lev0, int nlev) {
     const double* d;
     if (valid_field(id)) {
         if (lev0 < 0 || nlev < 0 || lev0 + nlev > (int)c->nlev) return LES_E_BADARG;
-        d = c->f3[id] + (size_t)lev0 * c->plane;
+        const FieldRef r = field_ref(c, id);
+        if (r.f) {
+            convert(c->stream, c->work3[7], (const float*)(r.f + (size_t)lev0 * c->plane), (long)nlev * (long)c->plane);
+            d = c->work3[7];
+        } else {
+            d = r.d + (size_t)lev0 * c->plane;
+        }
     } else {
         d = surf(c, id);
         if (!d || lev0 != 0 || nlev != 1) return LES_E_BADARG;
@@ -536,34 +705,23 @@ int les_get_sgs_init_flags(les_ctx* c, int* a, int* b) {
     return LES_OK;
 }
 
-int les_ddx(les_ctx* c, int d, int f) {
-    if (!c || !valid_field(d) || !valid_field(f)) return LES_E_BADARG;
-    ddx_f(c, F(c, d), F(c, f), 0, (int)c->nlev);
-    return finish(c);
-}
-int les_ddy(les_ctx* c, int d, int f) {
-    if (!c || !valid_field(d) || !valid_field(f)) return LES_E_BADARG;
-    ddy_f(c, F(c, d), F(c, f), 0, (int)c->nlev);
-    return finish(c);
-}
-int les_ddz_uv(les_ctx* c, int d, int f) {
-    if (!c || !valid_field(d) || !valid_field(f) || d == f) return LES_E_BADARG;
-    ddz_uv_dev(c, F(c, d), F(c, f));
-    return finish(c);
-}
-int les_ddz_w(les_ctx* c, int d, int f) {
-    if (!c || !valid_field(d) || !valid_field(f) || d == f) return LES_E_BADARG;
-    ddz_w_dev(c, F(c, d), F(c, f));
-    return finish(c);
-}
+int les_ddx(les_ctx* c, int d, int f) { return deriv_entry(c, d, f, D_X); }
+int les_ddy(les_ctx* c, int d, int f) { return deriv_entry(c, d, f, D_Y); }
+int les_ddz_uv(les_ctx* c, int d, int f) { return deriv_entry(c, d, f, D_ZUV); }
+int les_ddz_w(les_ctx* c, int d, int f) { return deriv_entry(c, d, f, D_ZW); }
 int les_filter_data(les_ctx* c, int f) {
     if (!c || !valid_field(f)) return LES_E_BADARG;
-    filter_f(c, F(c, f), 0, (int)c->nlev);
+    const FieldRef r = field_ref(c, f);
+    if (r.f) filter_f<float>(c, r.f, 0, (int)c->nlev);
+    else filter_f<double>(c, r.d, 0, (int)c->nlev);
     return finish(c);
 }
 int les_test_filter(les_ctx* c, int f, int level, int which) {
     if (!c || !valid_field(f) || level < 0 || level > c->nz || (which != 2 && which != 4)) return LES_E_BADARG;
-    test_filter_f(c, F(c, f) + (size_t)level * c->plane, 1, which == 2 ? c->Gtest : c->Gtest2);
+    const FieldRef r = field_ref(c, f);
+    const double* G = which == 2 ? c->Gtest : c->Gtest2;
+    if (r.f) test_filter_f<float>(c, r.f + (size_t)level * c->plane, 1, G);
+    else test_filter_f<double>(c, r.d + (size_t)level * c->plane, 1, G);
     return finish(c);
 }
 
@@ -596,7 +754,7 @@ int les_zero_padding(les_ctx* c, const double* var, int ld, double* big) {
     double* w = c->work3[7];
     les_note(c, hipMemcpy2DAsync(w, (size_t)c->nx * 8, var, (size_t)ld * 8, (size_t)c->nx * 8, c->ny, hipMemcpyHostToDevice,
                      c->stream));
-    pad_f(c, w, c->big3[0], 1);
+    pad_f<double, double>(c, w, c->big3[0], 1);
     les_note(c, hipMemcpyAsync(big, c->big3[0], c->plane2 * 8, hipMemcpyDeviceToHost, c->stream));
     return finish(c);
 }
@@ -605,7 +763,7 @@ int les_zero_unpadding(les_ctx* c, const double* big, double* var, int ld) {
     if (!c || !var || !big || ld < c->nx) return LES_E_BADARG;
     les_note(c, hipMemcpyAsync(c->big3[0], big, c->plane2 * 8, hipMemcpyHostToDevice, c->stream));
     double* w = c->work3[7];
-    unpad_f(c, c->big3[0], w, 1);
+    unpad_f<double>(c, c->big3[0], w, 1);
     les_note(c, hipMemcpy2DAsync(var, (size_t)ld * 8, w, (size_t)c->nx * 8, (size_t)c->nx * 8, c->ny, hipMemcpyDeviceToHost,
                      c->stream));
     return finish(c);
@@ -618,22 +776,22 @@ int les_obukhov(les_ctx* c, int jt) {
 }
 int les_wallstress_bottom(les_ctx* c) {
     if (!c) return LES_E_BADARG;
-    wallstress(c, 0);
+    LES_DISPATCH(wallstress, c, 0);
     return finish(c);
 }
 int les_wallstress_top(les_ctx* c) {
     if (!c) return LES_E_BADARG;
-    wallstress(c, 1);
+    LES_DISPATCH(wallstress, c, 1);
     return finish(c);
 }
 int les_sgs_stag(les_ctx* c, int jt) {
     if (!c) return LES_E_BADARG;
-    sgs_stag(c, jt);
+    LES_DISPATCH(sgs_stag, c, jt);
     return finish(c);
 }
 int les_lagrange_sdep(les_ctx* c, int jt) {
     if (!c) return LES_E_BADARG;
-    lagrange(c, jt);
+    LES_DISPATCH(lagrange, c, jt);
     return finish(c);
 }
 int les_interpolag_sdep(les_ctx* c) {
@@ -641,35 +799,27 @@ int les_interpolag_sdep(les_ctx* c) {
     interpolag_dev(c);
     return finish(c);
 }
-int les_divstress_uv(les_ctx* c, int divt, int tx, int ty, int tz) {
-    if (!c || !valid_field(divt) || !valid_field(tx) || !valid_field(ty) || !valid_field(tz)) return LES_E_BADARG;
-    divstress(c, F(c, divt), F(c, tx), F(c, ty), F(c, tz), 0);
-    return finish(c);
-}
-int les_divstress_w(les_ctx* c, int divt, int tx, int ty, int tz) {
-    if (!c || !valid_field(divt) || !valid_field(tx) || !valid_field(ty) || !valid_field(tz)) return LES_E_BADARG;
-    divstress(c, F(c, divt), F(c, tx), F(c, ty), F(c, tz), 1);
-    return finish(c);
-}
+int les_divstress_uv(les_ctx* c, int divt, int tx, int ty, int tz) { return divstress_entry(c, divt, tx, ty, tz, 0); }
+int les_divstress_w(les_ctx* c, int divt, int tx, int ty, int tz) { return divstress_entry(c, divt, tx, ty, tz, 1); }
 int les_convec(les_ctx* c) {
     if (!c) return LES_E_BADARG;
-    convec(c);
+    LES_DISPATCH(convec, c);
     return finish(c);
 }
 int les_theta_all_in_one(les_ctx* c, int jt) {
     if (!c) return LES_E_BADARG;
-    theta_all_in_one(c, jt);
+    LES_DISPATCH(theta_all_in_one, c, jt);
     return finish(c);
 }
 int les_press_stag_array(les_ctx* c, int jt) {
     (void)jt;
     if (!c) return LES_E_BADARG;
-    press_stag_array(c);
+    LES_DISPATCH(press_stag_array, c);
     return finish(c);
 }
 int les_project(les_ctx* c) {
     if (!c) return LES_E_BADARG;
-    project_dev(c);
+    LES_DISPATCH(project_dev, c);
     return finish(c);
 }
 
@@ -686,7 +836,7 @@ int les_run(les_ctx* c, int jt0, int nsteps, int* failed_jt) {
         const int chunk = (nsteps - done) < c->cfl_cap ? (nsteps - done) : c->cfl_cap;
         for (int s = 0; s < chunk; ++s) {
             max3_dev(c, c->cfl_dev + s);  // check_cfl of newstep (901_debug.f90:85-99)
-            step_body(c, jt0 + done + s);
+            LES_DISPATCH(step_body, c, jt0 + done + s);
         }
         les_note(c, hipMemcpyAsync(c->cfl_host, c->cfl_dev, sizeof(double) * chunk, hipMemcpyDeviceToHost, c->stream));
         const int st = finish(c);
@@ -713,21 +863,43 @@ int les_time_kernel(les_ctx* c, int kernel, int reps, double* ms, double* bytes)
     if (!c || !ms || !bytes || reps < 1 || kernel < 0 || kernel > 4) return LES_E_BADARG;
     const size_t F3 = c->plane * c->nlev;
     const int nz = c->nz;
+    const bool m = mixed(c);
+    const double ws = m ? 4.0 : 8.0;  // work-field word size
     double* a = c->work3[0];
     double* b = c->work3[1];
     auto launch = [&]() {
         switch (kernel) {
-            case LES_K_TDMA: tdma_press(c, c->spec3[0], c->spec3[1], c->spec3[2], c->spec3[3]); break;
-            case LES_K_XDERIV: ddx_f(c, b, a, 0, (int)c->nlev); break;
-            case LES_K_YDERIV: ddy_f(c, b, a, 0, (int)c->nlev); break;
-            case LES_K_PAD: pad_f(c, a, c->big3[0], (int)c->nlev); break;
-            default: les_note(c, hipMemcpyAsync(b, a, F3 * 8, hipMemcpyDeviceToDevice, c->stream)); break;
+            case LES_K_TDMA:
+                if (m)
+                    tdma_press<float>(c, scr<float>(c->spec3[0]), scr<float>(c->spec3[1]), scr<float>(c->spec3[2]),
+                                      c->phf, reinterpret_cast<double*>(c->spec3[3]), c->hzm);
+                else
+                    tdma_press<double>(c, c->spec3[0], c->spec3[1], c->spec3[2], c->spec3[3],
+                                       reinterpret_cast<double*>(c->spec3[3]), nullptr);
+                break;
+            case LES_K_XDERIV:
+                if (m) ddx_f<float, float>(c, scr<float>(b), scr<float>(a), 0, (int)c->nlev);
+                else ddx_f<double, double>(c, b, a, 0, (int)c->nlev);
+                break;
+            case LES_K_YDERIV:
+                if (m) ddy_f<float, float>(c, scr<float>(b), scr<float>(a), 0, (int)c->nlev);
+                else ddy_f<double, double>(c, b, a, 0, (int)c->nlev);
+                break;
+            case LES_K_PAD:
+                if (m) pad_f<float, float>(c, scr<float>(a), scr<float>(c->big3[0]), (int)c->nlev);
+                else pad_f<double, double>(c, a, c->big3[0], (int)c->nlev);
+                break;
+            default: les_note(c, hipMemcpyAsync(b, a, F3 * (size_t)ws, hipMemcpyDeviceToDevice, c->stream)); break;
         }
     };
+    // algorithmic bytes: TDMA reads Hx, Hy (levels 1..nz-1) and Hz (0..nz),
+    // writes p (0..nz-1); the mixed TDMA also reads the FP64 plane sums
     switch (kernel) {
-        case LES_K_TDMA: *bytes = (double)c->ns * 16.0 * (double)(2 * (nz - 1) + (nz + 1) + nz); break;
-        case LES_K_PAD: *bytes = (double)c->nlev * 8.0 * ((double)c->plane + (double)c->plane2); break;
-        default: *bytes = 2.0 * (double)F3 * 8.0; break;
+        case LES_K_TDMA:
+            *bytes = (double)c->ns * 2.0 * ws * (double)(2 * (nz - 1) + (nz + 1) + nz) + (m ? 8.0 * (nz + 1) : 0.0);
+            break;
+        case LES_K_PAD: *bytes = (double)c->nlev * ws * ((double)c->plane + (double)c->plane2); break;
+        default: *bytes = 2.0 * (double)F3 * ws; break;
     }
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return LES_E_HIP;

@@ -54,6 +54,13 @@ struct les_ctx {
     hipStream_t stream;
     // device fields (FP64), nlev planes each
     double* f3[LES_N_FIELDS3D];
+    // work fields in the precision of the current mode (double: f3[id];
+    // mixed: FP32 copies in f32buf); null for the FP64 state fields
+    void* wk[LES_N_FIELDS3D];
+    float* f32buf[LES_N_FIELDS3D];  // FP32 work fields, allocated on the first switch to LES_MIXED
+    les::cx<float>* phf;            // FP32 pressure spectrum of the mixed mode
+    double* hzm;                    // FP64 plane sums of the z pressure RHS (nlev)
+    double* lmean;                  // plane means of u and v (2 nlev), mixed-mode lagrange_Sdep
     double *ustar, *psi_m, *phi_m, *psi_h, *phi_h;  // one plane each
     double *u_big, *v_big, *w_big;                  // nlev planes of nx2*ny2
     // wavenumber tables (half spectrum, 002_fft.f90:143-167)
@@ -85,20 +92,26 @@ inline void les_note(les_ctx* c, hipError_t e) {
     if (e != hipSuccess && c->err == hipSuccess) c->err = e;
 }
 
+// Work field id in the precision W of the current mode.
+template <typename W>
+inline W* WF(les_ctx* c, int id) {
+    return static_cast<W*>(c->wk[id]);
+}
+
 namespace les {
 
 // ---- line transforms (k_lines.hip) ---------------------------------------
 // Real x-lines: nrows rows, row r at in + r*ld_in (length nin) -> out + r*ld_out.
-template <typename T>
-void xline_real(hipStream_t s, const T* in, T* out, long nrows, int ld_in, int ld_out,
+template <typename T, typename TI = T>
+void xline_real(hipStream_t s, const TI* in, T* out, long nrows, int ld_in, int ld_out,
                 const Twiddle& tin, const Twiddle& tout, LineOp op, double kfac);
 // Real y-lines on nplanes planes: (nin rows x ncols) -> (nout rows x ncols).
-template <typename T>
-void yline_real(hipStream_t s, const T* in, T* out, int nplanes, int ncols, int ld_in, int ld_out,
+template <typename T, typename TI = T>
+void yline_real(hipStream_t s, const TI* in, T* out, int nplanes, int ncols, int ld_in, int ld_out,
                 long ps_in, long ps_out, const Twiddle& tin, const Twiddle& tout, LineOp op, double kfac);
 // x r2c (fftn_mpi first half): nrows real rows (length n) -> half-spectrum rows (lhx).
-template <typename T>
-void x_r2c(hipStream_t s, const T* in, cx<T>* out, long nrows, int ld_in, int ldc,
+template <typename T, typename TI = T>
+void x_r2c(hipStream_t s, const TI* in, cx<T>* out, long nrows, int ld_in, int ldc,
            const Twiddle& tw, double scale);
 // x c2r (ifftn_mpi second half); deriv != 0 applies i*kx (kx Nyquist = 0) first.
 template <typename T>
@@ -111,14 +124,14 @@ void yline_spec(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lhx
 
 // compile-time specialised versions (k_lines_ct.hip); return false when the
 // line length has no specialisation
-template <typename T>
-bool xline_real_ct(hipStream_t s, const T* in, T* out, long nrows, int ld_in, int ld_out, const Twiddle& tin,
+template <typename T, typename TI>
+bool xline_real_ct(hipStream_t s, const TI* in, T* out, long nrows, int ld_in, int ld_out, const Twiddle& tin,
                    const Twiddle& tout, LineOp op, double kfac);
-template <typename T>
-bool yline_real_ct(hipStream_t s, const T* in, T* out, int nplanes, int ncols, int ld_in, int ld_out, long ps_in,
+template <typename T, typename TI>
+bool yline_real_ct(hipStream_t s, const TI* in, T* out, int nplanes, int ncols, int ld_in, int ld_out, long ps_in,
                    long ps_out, const Twiddle& tin, const Twiddle& tout, LineOp op, double kfac);
-template <typename T>
-bool x_r2c_ct(hipStream_t s, const T* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
+template <typename T, typename TI>
+bool x_r2c_ct(hipStream_t s, const TI* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
               double scale);
 template <typename T>
 bool x_c2r_ct(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int ld_out, const Twiddle& tw, int deriv,
@@ -130,39 +143,49 @@ bool yline_spec_ct(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int 
 extern int g_use_ct;
 
 // ---- TDMA (k_tdma.hip) --------------------------------------------------
-void tdma_press(les_ctx* c, const cx<double>* Hx, const cx<double>* Hy, const cx<double>* Hz,
-                cx<double>* ph);
+// Pressure TDMA; dsc is an FP64 store for d' (may equal ph when T = double),
+// hzm optional FP64 plane sums for the mean mode.
+template <typename T>
+void tdma_press(les_ctx* c, const cx<T>* Hx, const cx<T>* Hy, const cx<T>* Hz, cx<T>* ph, double* dsc,
+                const double* hzm);
 
 // ---- pointwise / reductions (k_point.hip) -------------------------------
-void fill_levels(hipStream_t s, double* f, size_t plane, int lev0, int nlev, double v);
+// Kernels templated on W read and write the work fields in precision W
+// (double, or float in the mixed mode); the prognostic state, the SGS
+// Lagrangian state, the surface planes and the reductions are FP64 always.
+template <typename T>
+void fill_levels(hipStream_t s, T* f, size_t plane, int lev0, int nlev, double v);
+template <typename TO, typename TF>
+void convert(hipStream_t s, TO* out, const TF* in, long n);
 void level_means(hipStream_t s, const double* f, size_t plane, int lev0, int nlev, double inv_n, double* out);
 void obukhov_dev(les_ctx* c, int jt);
-void wallstress_dev(les_ctx* c, int top, const double* u1, const double* v1);
-void strain_dev(les_ctx* c);
+template <typename W> void wallstress_dev(les_ctx* c, int top, const double* u1, const double* v1);
+template <typename W> void strain_dev(les_ctx* c);
 void lag_accumulate_dev(les_ctx* c);
-void nu_t_stress_dev(les_ctx* c);
+template <typename W> void nu_t_stress_dev(les_ctx* c);
 void cs_init_dev(les_ctx* c);
 void interpolag_dev(les_ctx* c);
-void lagrange_prep_dev(les_ctx* c, double* buf21);
-void lagrange_contract_dev(les_ctx* c, const double* b2, const double* b4, const double* raw, int init_lm, int init_qn);
-void ddz_uv_dev(les_ctx* c, double* d, const double* f);
-void ddz_w_dev(les_ctx* c, double* d, const double* f);
-void divstress_combine_dev(les_ctx* c, double* out, const double* a, const double* b, const double* d, int w_version);
-void vorticity_dev(les_ctx* c, double* v1, double* v2, double* v3);
-void cross_product_dev(les_ctx* c, const double* b1, const double* b2, const double* b3,
-                       double* cx, double* cy, double* cz);
-void theta_prep_dev(les_ctx* c);
-void scalar_adv_dev(les_ctx* c, const double* dsx, const double* dsy, const double* dsz, double* rm);
-void scalar_flux_dev(les_ctx* c, double* temp, int comp);
-void scalar_accum_dev(les_ctx* c, const double* dtemp, int mode);
-void calc_beta_dev(les_ctx* c, const double* means);
+template <typename W> void lagrange_prep_dev(les_ctx* c, W* buf21);
+template <typename W> void lagrange_contract_dev(les_ctx* c, const W* b2, const W* b4, int init_lm, int init_qn);
+template <typename W, typename TI> void ddz_uv_dev(les_ctx* c, W* d, const TI* f);
+template <typename W, typename TI> void ddz_w_dev(les_ctx* c, W* d, const TI* f);
+template <typename W> void divstress_combine_dev(les_ctx* c, W* out, const W* a, const W* b, const W* d, int w_version);
+template <typename W> void vorticity_dev(les_ctx* c, W* v1, W* v2, W* v3);
+template <typename W>
+void cross_product_dev(les_ctx* c, const W* b1, const W* b2, const W* b3, W* cx, W* cy, W* cz);
+template <typename W> void theta_prep_dev(les_ctx* c);
+template <typename W> void scalar_adv_dev(les_ctx* c, const W* dsx, const W* dsy, const W* dsz, W* rm);
+template <typename W> void scalar_flux_dev(les_ctx* c, W* temp, int comp);
+template <typename W> void scalar_accum_dev(les_ctx* c, const W* dtemp, const W* adv);
+template <typename W> void calc_beta_dev(les_ctx* c, const double* means);
 void step_scalar_dev(les_ctx* c);
-void press_rhs_prep_dev(les_ctx* c, double* rx, double* ry, double* rz);
-void dpdz_dev(les_ctx* c);
-void rhs_assembly_dev(les_ctx* c, int jt);
+template <typename W> void press_rhs_prep_dev(les_ctx* c, W* rx, W* ry, W* rz);
+template <typename W> void press_hz_sums_dev(les_ctx* c, double* out);
+template <typename W> void dpdz_dev(les_ctx* c);
+template <typename W> void rhs_assembly_dev(les_ctx* c, int jt);
 void ab2_dev(les_ctx* c);
-void rhs_minus_gradp_dev(les_ctx* c);
-void project_dev(les_ctx* c);
+template <typename W> void rhs_minus_gradp_dev(les_ctx* c);
+template <typename W> void project_dev(les_ctx* c);
 void max3_dev(les_ctx* c, double* out);
 
 }  // namespace les
