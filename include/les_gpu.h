@@ -201,7 +201,8 @@ int les_press_stag_array(les_ctx* ctx, int jt);
 int les_project(les_ctx* ctx);
 
 /* ---- whole step --------------------------------------------------------- */
-/* flosol, 103_flosol.f90:4-290, minus file I/O and the ke/dudt diagnostics.
+/* flosol, 103_flosol.f90:4-290, minus file I/O (the ke/dudt diagnostics are
+ * computed only on the steps that sample the time-averaged slices).
  * Returns LES_E_CFL if check_cfl would abort (the state is then undefined,
  * as the reference stops the program). */
 int les_step(les_ctx* ctx, int jt);
@@ -211,6 +212,34 @@ int les_step(les_ctx* ctx, int jt);
 int les_run(les_ctx* ctx, int jt0, int nsteps, int* failed_jt);
 /* Last CFL value computed by newstep/check_cfl (901_debug.f90:66-143). */
 int les_get_cfl(les_ctx* ctx, double* cfl);
+
+/* ---- time-averaged x-z slices ----------------------------------------------- */
+/* y-averaged slices of flow_slice (011_scalars2.f90:28-242) and theta_slice
+ * (:246-318), accumulated with weight fr = c_count / p_count
+ * (102_initial.f90:170) by the step when use_avgslice and mod(jt, c_count)
+ * == 0 (103_flosol.f90:231-236); the theta slices only when theta_flag and
+ * jt >= theta_init_time.  On a step with mod(jt, p_count) == 0 the running
+ * sums close a window: they are copied to the window set (what the reference
+ * writes with write_avgslice) and restart from zero.  The d/dt slices need
+ * the flosol diagnostics dudt, dvdt, dwdt (103_flosol.f90:119-213, energy
+ * :393-429), which the step then computes on those steps.  Every slice is
+ * nxt x (nzt-1) values, x fastest, levels 1..nzt-1, FP64. */
+enum les_slice {
+    LES_S_U = 0, LES_S_V, LES_S_W, LES_S_P, LES_S_DUDT, LES_S_DVDT, LES_S_DWDT, LES_S_DUDZ, LES_S_DVDZ,
+    LES_S_U2, LES_S_V2, LES_S_W2, LES_S_P2, LES_S_UV, LES_S_UW, LES_S_VW, LES_S_WP,
+    LES_S_TXX, LES_S_TXY, LES_S_TXZ, LES_S_TYY, LES_S_TYZ, LES_S_TZZ,
+    LES_S_U3, LES_S_V3, LES_S_W3,
+    LES_S_U2_FLUC, LES_S_V2_FLUC, LES_S_UV_FLUC, LES_S_U2W_FLUC, LES_S_V2W_FLUC, LES_S_DISSIP,
+    LES_S_UTXZ_FLUC, LES_S_VTYZ_FLUC, LES_S_WTZZ, LES_S_CS,
+    LES_N_FLOW_SLICES,
+    LES_S_THETA = LES_N_FLOW_SLICES, LES_S_THETA2, LES_S_T2_FLUC, LES_S_DTDZ, LES_S_SGS_T3, LES_S_WT, LES_S_NU_T,
+    LES_N_SLICES
+};
+/* use_avgslice, c_count, p_count of the namelist (001_param.f90:73,140-141);
+ * (re)starts both sums and windows from zero. */
+int les_set_avgslice(les_ctx* ctx, int use_avgslice, int c_count, int p_count);
+/* window 0: running sums; 1: the last closed window. */
+int les_get_avgslice(les_ctx* ctx, int slice, int window, double* host);
 
 /* ---- timing support ------------------------------------------------------ */
 /* Kernel families timed by les_time_kernel (no reference counterpart: the

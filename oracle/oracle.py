@@ -18,31 +18,40 @@ from overlappingbls_amd.abi import LesContext, LesParams, bind
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liblesoracle.so")
-_lib = None
-_fns = None
+LIB_FMA = os.path.join(HERE, "liblesoracle_fma.so")
+LIB_R2 = os.path.join(HERE, "liblesoracle_r2.so")
+_PATHS = {"": LIB, "fma": LIB_FMA, "r2": LIB_R2}
+_libs = {}
+_fns = {}
 
 
 def build(force: bool = False) -> str:
-    """Compile the oracle with gcc via oracle/Makefile."""
-    if force and os.path.exists(LIB):
-        os.remove(LIB)
+    """Compile both oracle builds with gcc via oracle/Makefile."""
+    if force:
+        for p in _PATHS.values():
+            if os.path.exists(p):
+                os.remove(p)
     subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB
 
 
-def _load():
-    global _lib, _fns
-    if _fns is None:
-        if not os.path.exists(LIB):
+def _load(variant: str):
+    if variant not in _fns:
+        path = _PATHS[variant]
+        if not os.path.exists(path):
             build()
-        _lib = C.CDLL(LIB)
-        _fns = bind(_lib, "orc_")
-    return _fns
+        _libs[variant] = C.CDLL(path)
+        _fns[variant] = bind(_libs[variant], "orc_")
+    return _fns[variant]
 
 
-def create(prm: LesParams) -> LesContext:
-    """Oracle context with the same method surface as the HIP library's."""
-    return LesContext(_load(), prm, "oracle")
+def create(prm: LesParams, fma: bool = False, variant: str = "") -> LesContext:
+    """Oracle context with the same method surface as the HIP library's.
+    Variants measure the algorithm's own rounding sensitivity: "fma" (fused
+    multiply-adds, fma=True) and "r2" (radix-2 transforms: another FFT
+    summation order)."""
+    v = "fma" if fma else variant
+    return LesContext(_load(v), prm, "oracle_" + v if v else "oracle")
 
 
 def threads() -> int:

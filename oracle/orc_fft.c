@@ -24,8 +24,15 @@ int orc_plan_init(orc_plan* p, int n) {
     int m = n;
     while (m > 1) {
         int r;
+#ifdef ORC_RADIX2
+        /* alternative factorisation (test builds): FFTW chooses its plan at run
+         * time, so the transform's summation order is not fixed by the
+         * reference; this build measures the sensitivity to it */
+        if (m % 2 == 0) r = 2;
+#else
         if (m % 4 == 0) r = 4;
         else if (m % 2 == 0) r = 2;
+#endif
         else if (m % 3 == 0) r = 3;
         else return -1;
         p->radix[p->nstage++] = r;

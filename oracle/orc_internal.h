@@ -61,6 +61,12 @@ struct les_ctx {
     int f_lm_mm_init, f_qn_nn_init;
     double cfl;
     int nthreads;
+    /* time-averaged slices (011_scalars2.f90) and the flosol diagnostics
+     * they read (103_flosol.f90:119-213) */
+    int use_avgslice, c_count, p_count;
+    double fr;
+    double *dudt, *dvdt, *dwdt, *ke, *ke_temp, *dke;   /* nlev planes each */
+    double *sl_run, *sl_win;   /* LES_N_SLICES x (nz-1) x nx */
 };
 
 #define IDX(c, k, j, i) (((size_t)(k) * (c)->ny + (size_t)(j)) * (c)->nx + (size_t)(i))

@@ -196,6 +196,8 @@ int orc_destroy(les_ctx* c) {
     for (int f = 0; f < LES_N_FIELDS3D; ++f) free(c->f3[f]);
     free(c->ustar_avg); free(c->psi_m); free(c->phi_m); free(c->psi_h); free(c->phi_h);
     free(c->u_big); free(c->v_big); free(c->w_big);
+    free(c->dudt); free(c->dvdt); free(c->dwdt); free(c->ke); free(c->ke_temp); free(c->dke);
+    free(c->sl_run); free(c->sl_win);
     free(c);
     return LES_OK;
 }
@@ -1216,6 +1218,198 @@ int orc_project(les_ctx* c) {
 /* --------------------------------------------------------------------------
  * flosol, 103_flosol.f90:4-290
  * ------------------------------------------------------------------------ */
+/* ---- time-averaged slices ------------------------------------------------ */
+static size_t slice_len(const les_ctx* c) { return (size_t)c->nx * (size_t)(c->nz - 1); }
+
+int orc_set_avgslice(les_ctx* c, int use_avgslice, int c_count, int p_count) {
+    if (c_count < 1 || p_count < 1) return LES_E_BADARG;
+    c->use_avgslice = use_avgslice;
+    c->c_count = c_count;
+    c->p_count = p_count;
+    c->fr = (1.0 / (double)p_count) * (double)c_count;             /* 102_initial.f90:170 */
+    const size_t F3 = c->plane * c->nlev, S = (size_t)LES_N_SLICES * slice_len(c);
+    double** bufs[6] = {&c->dudt, &c->dvdt, &c->dwdt, &c->ke, &c->ke_temp, &c->dke};
+    for (int b = 0; b < 6; ++b) {
+        free(*bufs[b]);
+        *bufs[b] = (double*)calloc(F3, sizeof(double));
+    }
+    free(c->sl_run); free(c->sl_win);
+    c->sl_run = (double*)calloc(S, sizeof(double));
+    c->sl_win = (double*)calloc(S, sizeof(double));
+    return LES_OK;
+}
+
+int orc_get_avgslice(les_ctx* c, int slice, int window, double* host) {
+    if (!c->sl_run || slice < 0 || slice >= LES_N_SLICES || (window != 0 && window != 1)) return LES_E_BADARG;
+    const size_t L = slice_len(c);
+    memcpy(host, (window ? c->sl_win : c->sl_run) + (size_t)slice * L, sizeof(double) * L);
+    return LES_OK;
+}
+
+static int stats_step(const les_ctx* c, int jt) {
+    return c->use_avgslice && c->sl_run && (jt % c->c_count) == 0;    /* 103_flosol.f90:231 */
+}
+
+/* avg_y3d (202_parallel.f90:966-989): out(jx, jz) = sum over y / nyt, for the
+ * expression e(k, j, i) at levels 1..nz-1; accumulated into slice s with
+ * weight fr. */
+#define SLICE_ACC(c, s, EXPR)                                                        \
+    do {                                                                             \
+        double* acc_ = (c)->sl_run + (size_t)(s) * slice_len(c);                     \
+        for (int k = 1; k <= (c)->nz - 1; ++k)                                       \
+            for (int i = 0; i < (c)->nx; ++i) {                                      \
+                double t_ = 0.0;                                                     \
+                for (int j = 0; j < (c)->ny; ++j) {                                  \
+                    const size_t o = IDX(c, k, j, i);                                \
+                    (void)o;                                                         \
+                    t_ += (EXPR);                                                    \
+                }                                                                    \
+                acc_[(size_t)(k - 1) * (c)->nx + i] += (c)->fr * (t_ / (c)->ny);     \
+            }                                                                        \
+    } while (0)
+
+static void close_window(les_ctx* c, int s0, int s1) {
+    const size_t L = slice_len(c);
+    memcpy(c->sl_win + (size_t)s0 * L, c->sl_run + (size_t)s0 * L, sizeof(double) * L * (size_t)(s1 - s0));
+    memset(c->sl_run + (size_t)s0 * L, 0, sizeof(double) * L * (size_t)(s1 - s0));
+}
+
+/* flow_slice, 011_scalars2.f90:28-242 (coordz == 0 branches at level 1) */
+static void flow_slice(les_ctx* c, int jt) {
+    const int nz = c->nz;
+    const size_t np = c->plane, nl = c->nlev;
+    const double *u = F(c, LES_F_U), *v = F(c, LES_F_V), *w = F(c, LES_F_W), *p = F(c, LES_F_P);
+    const double *ust = c->ust, *vst = c->vst;
+    double* po = (double*)malloc(sizeof(double) * np * nl);   /* p_org, :63-70 */
+    for (int k = 0; k <= nz - 1; ++k)
+        for (size_t q = 0; q < np; ++q) {
+            const size_t o = (size_t)k * np + q;
+            po[o] = p[o] - 0.5 * (u[o] * u[o] + v[o] * v[o] + w[o] * w[o]) - (u[o] * ust[k] + v[o] * vst[k]) -
+                    0.5 * (ust[k] * ust[k] + vst[k] * vst[k]);
+        }
+    double* ub = (double*)malloc(sizeof(double) * nl);
+    double* vb = (double*)malloc(sizeof(double) * nl);
+    for (int k = 0; k <= nz - 1; ++k) {                           /* calc_hor_avg, :134-135 */
+        ub[k] = orc_hor_avg(c, LEV(c, u, k));
+        vb[k] = orc_hor_avg(c, LEV(c, v, k));
+    }
+    const double* dudz = F(c, LES_F_DUDZ);
+    const double* dvdz = F(c, LES_F_DVDZ);
+    const double *txx = F(c, LES_F_TXX), *txy = F(c, LES_F_TXY), *txz = F(c, LES_F_TXZ);
+    const double *tyy = F(c, LES_F_TYY), *tyz = F(c, LES_F_TYZ), *tzz = F(c, LES_F_TZZ);
+    const double* cs = F(c, LES_F_CS_OPT2);
+    const double *s11 = F(c, LES_F_S11), *s12 = F(c, LES_F_S12), *s13 = F(c, LES_F_S13);
+    const double *s22 = F(c, LES_F_S22), *s23 = F(c, LES_F_S23), *s33 = F(c, LES_F_S33);
+    const double delta = 1.0 * pow(c->dx * c->dy * c->dz, 1.0 / 3.0);   /* l = delta, 106_sgs_stag.f90:33,181 */
+    const double l2 = delta * delta;
+    const double *du = c->dudt, *dv = c->dvdt, *dw = c->dwdt;
+    const size_t P = np;
+    /* arg1, arg2 (:77-86): staggered u, v at w-nodes, 0 at the wall level */
+#define ARG1(o, k) ((k) == 1 ? 0.0 : (u[o] + u[(o) - P]) / 2.0)
+#define ARG2(o, k) ((k) == 1 ? 0.0 : (v[o] + v[(o) - P]) / 2.0)
+#define ARG3(o) ((po[o] + po[(o) - P]) / 2.0)
+    /* arg7, arg8 (:140-157): staggered fluctuations, 0 at the wall level */
+#define ARG7(o, k) ((k) == 1 ? 0.0 : (u[o] - ub[k] + u[(o) - P] - ub[(k) - 1]) / 2.0)
+#define ARG8(o, k) ((k) == 1 ? 0.0 : (v[o] - vb[k] + v[(o) - P] - vb[(k) - 1]) / 2.0)
+#define MAGS(o) sqrt(2.0 * (s11[o] * s11[o] + s22[o] * s22[o] + s33[o] * s33[o] + \
+                            2.0 * (s12[o] * s12[o] + s13[o] * s13[o] + s23[o] * s23[o])))
+    SLICE_ACC(c, LES_S_U, u[o]);
+    SLICE_ACC(c, LES_S_V, v[o]);
+    SLICE_ACC(c, LES_S_W, w[o]);
+    SLICE_ACC(c, LES_S_P, po[o]);
+    SLICE_ACC(c, LES_S_DUDT, du[o]);
+    SLICE_ACC(c, LES_S_DVDT, dv[o]);
+    SLICE_ACC(c, LES_S_DWDT, dw[o]);
+    SLICE_ACC(c, LES_S_DUDZ, dudz[o]);
+    SLICE_ACC(c, LES_S_DVDZ, dvdz[o]);
+    SLICE_ACC(c, LES_S_U2, u[o] * u[o]);
+    SLICE_ACC(c, LES_S_V2, v[o] * v[o]);
+    SLICE_ACC(c, LES_S_W2, w[o] * w[o]);
+    SLICE_ACC(c, LES_S_P2, po[o] * po[o]);
+    SLICE_ACC(c, LES_S_UV, u[o] * v[o]);
+    SLICE_ACC(c, LES_S_UW, ARG1(o, k) * w[o]);
+    SLICE_ACC(c, LES_S_VW, ARG2(o, k) * w[o]);
+    SLICE_ACC(c, LES_S_WP, ARG3(o) * w[o]);
+    SLICE_ACC(c, LES_S_TXX, txx[o]);
+    SLICE_ACC(c, LES_S_TXY, txy[o]);
+    SLICE_ACC(c, LES_S_TXZ, txz[o]);
+    SLICE_ACC(c, LES_S_TYY, tyy[o]);
+    SLICE_ACC(c, LES_S_TYZ, tyz[o]);
+    SLICE_ACC(c, LES_S_TZZ, tzz[o]);
+    SLICE_ACC(c, LES_S_U3, u[o] * u[o] * u[o]);
+    SLICE_ACC(c, LES_S_V3, v[o] * v[o] * v[o]);
+    SLICE_ACC(c, LES_S_W3, w[o] * w[o] * w[o]);
+    SLICE_ACC(c, LES_S_U2_FLUC, (u[o] - ub[k]) * (u[o] - ub[k]));
+    SLICE_ACC(c, LES_S_V2_FLUC, (v[o] - vb[k]) * (v[o] - vb[k]));
+    SLICE_ACC(c, LES_S_UV_FLUC, (u[o] - ub[k]) * (v[o] - vb[k]));
+    SLICE_ACC(c, LES_S_U2W_FLUC, ARG7(o, k) * ARG7(o, k) * w[o]);
+    SLICE_ACC(c, LES_S_V2W_FLUC, ARG8(o, k) * ARG8(o, k) * w[o]);
+    /* dissip = -|S|^3 Cs_opt2 l^2 on w-nodes (106_sgs_stag.f90:212) */
+    SLICE_ACC(c, LES_S_DISSIP, -(MAGS(o) * MAGS(o) * MAGS(o)) * cs[o] * l2);
+    SLICE_ACC(c, LES_S_UTXZ_FLUC, ARG7(o, k) * txz[o]);
+    SLICE_ACC(c, LES_S_VTYZ_FLUC, ARG8(o, k) * tyz[o]);
+    SLICE_ACC(c, LES_S_WTZZ, w[o] * tzz[o]);
+    SLICE_ACC(c, LES_S_CS, sqrt(cs[o]));
+#undef ARG1
+#undef ARG2
+#undef ARG3
+#undef ARG7
+#undef ARG8
+#undef MAGS
+    free(po); free(ub); free(vb);
+    if (jt % c->p_count == 0) close_window(c, 0, LES_N_FLOW_SLICES);   /* :186-241 */
+}
+
+/* theta_slice, 011_scalars2.f90:246-318 */
+static void theta_slice(les_ctx* c, int jt) {
+    const int nz = c->nz;
+    const double *th = F(c, LES_F_THETA), *w = F(c, LES_F_W), *dtdz = F(c, LES_F_DTDZ);
+    const double *t3 = F(c, LES_F_SGS_T3), *nut = F(c, LES_F_NU_T);
+    double* tb = (double*)malloc(sizeof(double) * c->nlev);
+    for (int k = 0; k <= nz - 1; ++k) tb[k] = orc_hor_avg(c, LEV(c, th, k));
+    const size_t P = c->plane;
+    SLICE_ACC(c, LES_S_THETA, th[o]);
+    SLICE_ACC(c, LES_S_THETA2, th[o] * th[o]);
+    SLICE_ACC(c, LES_S_T2_FLUC, (th[o] - tb[k]) * (th[o] - tb[k]));
+    SLICE_ACC(c, LES_S_DTDZ, dtdz[o]);
+    SLICE_ACC(c, LES_S_SGS_T3, t3[o]);
+    SLICE_ACC(c, LES_S_WT, w[o] * ((th[o] + th[o - P]) / 2.0));
+    SLICE_ACC(c, LES_S_NU_T, nut[o]);
+    free(tb);
+    if (jt % c->p_count == 0) close_window(c, LES_N_FLOW_SLICES, LES_N_SLICES);   /* :303-315 */
+}
+
+/* energy (103_flosol.f90:393-429) and d(ke)/dx_i added to dudt (:196-213):
+ * ke = (u^2 + v^2 + w_uvp^2)/2 at 1..nz-1, 0 at the wall level; the top
+ * ghost ke(nz) and dkedz(1) are not set by the reference (0 here). */
+static void diag_ke(les_ctx* c) {
+    const int nz = c->nz;
+    const size_t np = c->plane, nl = c->nlev;
+    const double *u = F(c, LES_F_U), *v = F(c, LES_F_V), *w = F(c, LES_F_W);
+    memset(c->ke, 0, sizeof(double) * np * nl);
+    for (int k = 1; k <= nz - 1; ++k)
+        for (size_t q = 0; q < np; ++q) {
+            const size_t o = (size_t)k * np + q;
+            const double tw = .5 * (w[o] + w[o + np]);
+            c->ke[o] = .5 * (u[o] * u[o] + v[o] * v[o] + tw * tw);
+        }
+    memcpy(c->ke_temp, c->ke, sizeof(double) * np * nl);
+    filter_arr(c, c->ke_temp);
+    ddx_arr(c, c->dke, c->ke_temp);
+    for (int k = 1; k <= nz - 1; ++k)
+        for (size_t q = 0; q < np; ++q) c->dudt[(size_t)k * np + q] += c->dke[(size_t)k * np + q];
+    ddy_arr(c, c->dke, c->ke_temp);
+    for (int k = 1; k <= nz - 1; ++k)
+        for (size_t q = 0; q < np; ++q) c->dvdt[(size_t)k * np + q] += c->dke[(size_t)k * np + q];
+    memset(c->dke, 0, sizeof(double) * np * nl);
+    ddz_uv_arr(c, c->dke, c->ke);
+    for (int k = 1; k <= nz - 1; ++k)
+        for (size_t q = 0; q < np; ++q) {
+            const size_t o = (size_t)k * np + q;
+            c->dwdt[o] += (k == 1) ? 0.0 : c->dke[o];
+        }
+}
+
 static double max_lev(les_ctx* c, const double* f) {
     double m = -INFINITY;
     for (int k = 1; k <= c->nz - 1; ++k)
@@ -1269,6 +1463,19 @@ int orc_step(les_ctx* c, int jt) {
             }
             if (damp) RZ[o] = RZ[o] - c->sponge[k] * w[o];
         }
+    const int st = stats_step(c, jt);
+    if (st)                                                      /* dudt, :119-156 */
+        for (int k = 1; k <= nz - 1; ++k)
+            for (size_t q = 0; q < np; ++q) {
+                size_t o = (size_t)k * np + q;
+                c->dudt[o] = -dx[o];
+                c->dvdt[o] = -dy[o];
+                c->dwdt[o] = -dzt[o];
+                if (P->coriolis_forcing) {
+                    c->dudt[o] = c->dudt[o] + P->coriol * (v[o] - P->vg);
+                    c->dvdt[o] = c->dvdt[o] + P->coriol * (P->ug - u[o]);
+                }
+            }
     if (jt == 1 && !P->restart_vel) {                            /* :331-338 */
         memcpy(RXf, RX, sizeof(double) * np * nl);
         memcpy(RYf, RY, sizeof(double) * np * nl);
@@ -1292,11 +1499,21 @@ int orc_step(les_ctx* c, int jt) {
             RX[o] = RX[o] - dpdx[o];
             RY[o] = RY[o] - dpdy[o];
             RZ[o] = RZ[o] - dpdz[o];
+            if (st) {                                            /* :183-185 */
+                c->dudt[o] = c->dudt[o] - dpdx[o];
+                c->dvdt[o] = c->dvdt[o] - dpdy[o];
+                c->dwdt[o] = c->dwdt[o] - dpdz[o];
+            }
         }
     orc_project(c);                                              /* :188-189 */
+    if (st) diag_ke(c);                                          /* :192-213 */
     orc_filter_data(c, LES_F_U);                                 /* :227-229 */
     orc_filter_data(c, LES_F_V);
     orc_filter_data(c, LES_F_W);
+    if (st) {                                                    /* :231-236 */
+        flow_slice(c, jt);
+        if (P->theta_flag && jt >= P->theta_init_time) theta_slice(c, jt);
+    }
     return LES_OK;
 }
 

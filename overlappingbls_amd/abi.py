@@ -45,6 +45,18 @@ N_FIELDS3D = len(_FIELDS3D)
 SURFACE = {"USTAR_SFC": 100, "PSI_M_SFC": 101, "PHI_M_SFC": 102, "PSI_H_SFC": 103, "PHI_H_SFC": 104}
 FIELD.update(SURFACE)
 
+# time-averaged slices, enum les_slice (flow_slice then theta_slice)
+FLOW_SLICES = [
+    "U", "V", "W", "P", "DUDT", "DVDT", "DWDT", "DUDZ", "DVDZ",
+    "U2", "V2", "W2", "P2", "UV", "UW", "VW", "WP",
+    "TXX", "TXY", "TXZ", "TYY", "TYZ", "TZZ",
+    "U3", "V3", "W3",
+    "U2_FLUC", "V2_FLUC", "UV_FLUC", "U2W_FLUC", "V2W_FLUC", "DISSIP",
+    "UTXZ_FLUC", "VTYZ_FLUC", "WTZZ", "CS",
+]
+THETA_SLICES = ["THETA", "THETA2", "T2_FLUC", "DTDZ", "SGS_T3", "WT", "NU_T"]
+SLICE = {name: i for i, name in enumerate(FLOW_SLICES + THETA_SLICES)}
+
 
 class LesParams(C.Structure):
     """les_params of include/les_gpu.h (field order and types must match)."""
@@ -122,6 +134,8 @@ def bind(lib: C.CDLL, prefix: str):
         "get_cfl": [P, C.POINTER(C.c_double)],
         "sync": [P],
         "time_kernel": [P, i, i, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+        "set_avgslice": [P, i, i, i],
+        "get_avgslice": [P, i, i, _PD],
     }
     fns = {}
     for name, argtypes in sig.items():
@@ -299,6 +313,16 @@ class LesContext:
         st = self._f["run"](self._h, int(jt0), int(nsteps), C.byref(failed))
         if st != LES_OK:
             raise LesError(f"{self.label}:run (step {failed.value})", st)
+
+    def set_avgslice(self, use_avgslice, c_count, p_count):
+        """use_avgslice, c_count, p_count (001_param.f90:73,140-141)."""
+        self._call("set_avgslice", self._h, int(bool(use_avgslice)), int(c_count), int(p_count))
+
+    def avgslice(self, name, window=0):
+        """(nzt-1, nxt) y-averaged slice LES_S_<name>; window 0 running, 1 last closed."""
+        out = np.empty((self.nz - 1, self.nx))
+        self._call("get_avgslice", self._h, SLICE[name], int(window), _pd(out))
+        return out
 
     def time_kernel(self, kernel, reps=20):
         """(ms per launch, algorithmic bytes per launch) of one kernel family."""

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "liblesgpu.so")
-SOURCES = ["k_lines.hip", "k_lines_ct.hip", "k_tdma.hip", "k_point.hip", "les_api.hip"]
+SOURCES = ["k_lines.hip", "k_lines_rg.hip", "k_tdma.hip", "k_point.hip", "k_stats.hip", "les_api.hip"]
 ARCH = os.environ.get("LES_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
@@ -18,6 +18,8 @@ FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # no packed-FP32 instructions anywhere in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # every product rounded, as the reference's arithmetic (no fused multiply-adds)
+    "-ffp-contract=off",
     "-Wno-unused-result",
 ]
 

@@ -123,6 +123,11 @@ def workload(name: str) -> Namelist:
                         coriolis_forcing=True, ug_dim=1.0, **base)
     if name == "small":
         return Namelist(nxt=32, nyt=32, nzt=17, cs_count=3, dyn_init=3, wt_s=-0.005 * 300.0, **base)
+    if name == "wide":        # production line lengths 256 / 384, few levels
+        return Namelist(nxt=256, nyt=256, nzt=5, cs_count=2, dyn_init=2, wt_s=-0.005 * 300.0, **base)
+    if name == "wide_rect":   # 512 / 768 along x, 128 / 192 along y
+        return Namelist(nxt=512, nyt=128, nzt=5, cs_count=2, dyn_init=2, wt_s=0.01 * 300.0,
+                        coriolis_forcing=True, ug_dim=1.0, **base)
     raise KeyError(name)
 
 

@@ -1,0 +1,694 @@
+// Line-transform kernels for the production line lengths (64..768) built on
+// the register-radix FFT of fft_rg.hpp.  Same operations as the generic
+// runtime-length kernels of k_lines.hip:
+//   x-lines  : real rows packed two to a complex line
+//   y-lines  : real column pairs packed into one complex line (column tiles,
+//              unit-stride row segments in global memory)
+//   r2c/c2r  : x half-spectrum transforms of fftn_mpi / ifftn_mpi
+//   yspec    : complex y-lines of the half spectrum
+// The first stage of each transform reads global memory directly, the last
+// stage writes it directly; stages exchange through LDS.
+#include "fft_rg.hpp"
+#include "les_internal.hpp"
+
+namespace les {
+
+namespace {
+
+constexpr int round64(int n) { return ((n + 63) / 64) * 64; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+// ---- one Stockham stage: butterflies distributed over the block --------------
+// Butterfly (line, b), b < N/R: inputs at b + r N/R, twiddles W_{NS R}^{r (b mod NS)},
+// outputs at (b / NS) NS R + b mod NS + r NS.  YM: lines vary fastest across
+// threads (y-lines: neighbouring threads read neighbouring columns).
+template <typename T, int N, int S, int SIGN, int NL, int NTH, bool YM>
+struct Stage {
+    static constexpr int R = rg_r(N, S, SIGN);
+    static constexpr int NS = rg_ns(N, S, SIGN);
+    static constexpr int NB = N / R;
+    static constexpr int TOT = NL * NB;
+    static constexpr int PB = (TOT + NTH - 1) / NTH;
+    cx<T> v[PB][R];
+
+    __device__ __forceinline__ static bool bf(int q, int& line, int& b) {
+        const int jb = (int)threadIdx.x + q * NTH;
+        if (TOT % NTH != 0 && jb >= TOT) return false;
+        if constexpr (YM) {
+            line = jb % NL;
+            b = jb / NL;
+        } else {
+            line = jb / NB;
+            b = jb - line * NB;
+        }
+        return true;
+    }
+
+    // inputs from LDS (line stride LS, padded index)
+    template <int LS>
+    __device__ __forceinline__ void read(const cx<T>* buf) {
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            int line, b;
+            if (!bf(q, line, b)) continue;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = buf[line * LS + rg_pad(b + r * NB)];
+        }
+    }
+
+    // inputs from a functor ld(line, position)
+    template <typename LD>
+    __device__ __forceinline__ void load(const LD& ld) {
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            int line, b;
+            if (!bf(q, line, b)) continue;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = ld(line, b + r * NB);
+        }
+    }
+
+    __device__ __forceinline__ void compute(const cx<T>* tw) {
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            int line, b;
+            if (!bf(q, line, b)) continue;
+            if constexpr (NS > 1) {
+                constexpr int tstep = N / (NS * R);
+                const int k = b % NS;
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    cx<T> w = tw[r * k * tstep];
+                    if (SIGN > 0) w.y = -w.y;
+                    v[q][r] = cmul(v[q][r], w);
+                }
+            }
+            dft<T, R, SIGN>(v[q]);
+        }
+    }
+
+    template <int LS>
+    __device__ __forceinline__ void write(cx<T>* buf) const {
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            int line, b;
+            if (!bf(q, line, b)) continue;
+            const int base = (b / NS) * NS * R + b % NS;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[line * LS + rg_pad(base + r * NS)] = v[q][r];
+        }
+    }
+
+    // last stage: outputs at b + r NB, to a functor st(line, position, value)
+    template <typename ST>
+    __device__ __forceinline__ void store(const ST& st) const {
+        static_assert(NS * R == N, "store() is for the last stage");
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            int line, b;
+            if (!bf(q, line, b)) continue;
+#pragma unroll
+            for (int r = 0; r < R; ++r) st(line, b + r * NB, v[q][r]);
+        }
+    }
+};
+
+// Stages 1 .. NST-2 (LDS to LDS) of a transform whose stage 0 output is in LDS.
+template <typename T, int N, int SIGN, int NL, int NTH, bool YM, int LS>
+__device__ __forceinline__ void middle_stages(cx<T>* buf, const cx<T>* tw) {
+    if constexpr (rg_nst(N) == 3) {
+        Stage<T, N, 1, SIGN, NL, NTH, YM> s1;
+        s1.template read<LS>(buf);
+        s1.compute(tw);
+        __syncthreads();
+        s1.template write<LS>(buf);
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void load_twr(cx<T>* lds, const cx<T>* g, int n) {
+    for (int m = threadIdx.x; m < n; m += blockDim.x) lds[m] = g[m];
+}
+
+// spectral multiplier of the same-length ops at output position k of NIN
+template <typename T, int N, int OP>
+__device__ __forceinline__ cx<T> same_len_op(cx<T> z, int k, T kfac) {
+    if (2 * k == N) return {T(0), T(0)};
+    if constexpr (OP == OP_DERIV) {
+        const int m = (2 * k < N) ? k : k - N;
+        const T f = (T)m * kfac / (T)N;
+        return {-f * z.y, f * z.x};
+    } else {
+        const T s = T(1) / (T)N;
+        return {z.x * s, z.y * s};
+    }
+}
+
+// source position in the NIN spectrum of output position k of NOUT (pad /
+// unpad), or -1 for a zero
+template <int NIN, int NOUT, int OP>
+__device__ __forceinline__ int resize_src(int k) {
+    if constexpr (OP == OP_PAD) {
+        constexpr int h = NIN / 2;
+        if (k < h) return k;
+        if (k > NOUT - h) return k - NOUT + NIN;
+        return -1;
+    } else {
+        constexpr int h = NOUT / 2;
+        if (k < h) return k;
+        if (k > h) return k - NOUT + NIN;
+        return -1;
+    }
+}
+
+// FFT(NIN) -> op -> IFFT(NOUT) on NL lines; ld/st are the global accessors.
+template <typename T, int NIN, int NOUT, int OP, int NL, int NTH, bool YM, typename LD, typename ST>
+__device__ __forceinline__ void line_pipeline(const LD& ld, const ST& st, cx<T>* buf, const cx<T>* twi,
+                                              const cx<T>* two, T kfac) {
+    constexpr int LS = rg_ls(cmax(NIN, NOUT));
+    constexpr int SF = rg_nst(NIN);
+    constexpr int SI = rg_nst(NOUT);
+    // forward
+    Stage<T, NIN, 0, -1, NL, NTH, YM> f0;
+    f0.load(ld);
+    f0.compute(twi);
+    f0.template write<LS>(buf);
+    __syncthreads();
+    middle_stages<T, NIN, -1, NL, NTH, YM, LS>(buf, twi);
+    Stage<T, NIN, SF - 1, -1, NL, NTH, YM> fl;
+    fl.template read<LS>(buf);
+    fl.compute(twi);
+    // op and first inverse stage
+    Stage<T, NOUT, 0, +1, NL, NTH, YM> i0;
+    if constexpr (NIN == NOUT) {
+        static_assert(decltype(fl)::R == decltype(i0)::R, "reversed plans share the butterfly layout");
+#pragma unroll
+        for (int q = 0; q < decltype(i0)::PB; ++q) {
+            int line, b;
+            if (!decltype(i0)::bf(q, line, b)) continue;
+#pragma unroll
+            for (int r = 0; r < decltype(i0)::R; ++r)
+                i0.v[q][r] = same_len_op<T, NIN, OP>(fl.v[q][r], b + r * decltype(i0)::NB, kfac);
+        }
+        i0.compute(two);
+        __syncthreads();
+    } else {
+        __syncthreads();
+        fl.store([&](int line, int p, cx<T> z) { buf[line * LS + rg_pad(p)] = z; });
+        __syncthreads();
+        const T s = T(1) / (T)NIN;
+        i0.load([&](int line, int p) {
+            const int src = resize_src<NIN, NOUT, OP>(p);
+            if (src < 0) return cx<T>{T(0), T(0)};
+            const cx<T> z = buf[line * LS + rg_pad(src)];
+            return cx<T>{z.x * s, z.y * s};
+        });
+        i0.compute(two);
+        __syncthreads();
+    }
+    i0.template write<LS>(buf);
+    __syncthreads();
+    middle_stages<T, NOUT, +1, NL, NTH, YM, LS>(buf, two);
+    Stage<T, NOUT, SI - 1, +1, NL, NTH, YM> il;
+    il.template read<LS>(buf);
+    il.compute(two);
+    il.store(st);
+}
+
+// A derivative computed in a lower precision than its input is taken of the
+// line minus its first element (the derivative of a constant is zero), so the
+// conversion rounds the fluctuation about the line, not the mean flow.
+template <typename T, typename TI, int OP>
+constexpr bool SHIFT_IN = (OP == OP_DERIV) && (sizeof(TI) > sizeof(T));
+
+template <typename T>
+struct pair_t;
+template <>
+struct pair_t<double> {
+    using type = double2;
+};
+template <>
+struct pair_t<float> {
+    using type = float2;
+};
+
+// lines per block and threads per block of a line kernel: y-lines take
+// 128-byte row segments (8 FP64 or 16 FP32 column pairs), x-lines enough lines
+// for 128 threads; halved while the block would exceed 512 threads or 64 KB
+// of LDS (line buffers and twiddle tables).
+template <typename T, int NIN, int NOUT, bool YM>
+constexpr int rg_lines() {
+    constexpr int mb = cmax(rg_maxbf(NIN), rg_maxbf(NOUT));
+    int nl = YM ? (sizeof(T) == 8 ? 8 : 16) : cmax(1, 128 / mb);
+    while (nl > 1 && (nl * mb > 512 ||
+                      (nl * rg_ls(cmax(NIN, NOUT)) + NIN + NOUT) * (int)sizeof(cx<T>) > 65536))
+        nl /= 2;
+    return nl;
+}
+template <typename T, int NIN, int NOUT, bool YM>
+constexpr int rg_threads() {
+    return round64(rg_lines<T, NIN, NOUT, YM>() * cmax(rg_maxbf(NIN), rg_maxbf(NOUT)));
+}
+
+template <typename T, typename TI, int NIN, int NOUT, int OP>
+__global__ __launch_bounds__((rg_threads<T, NIN, NOUT, false>())) void kr_xline(
+    const TI* __restrict__ in, T* __restrict__ out, long nrows, int ld_in, int ld_out,
+    const cx<T>* __restrict__ twin_g, const cx<T>* __restrict__ twout_g, T kfac) {
+    constexpr int NL = rg_lines<T, NIN, NOUT, false>();
+    constexpr int NTH = rg_threads<T, NIN, NOUT, false>();
+    constexpr int LS = rg_ls(cmax(NIN, NOUT));
+    __shared__ cx<T> buf[NL * LS];
+    __shared__ cx<T> twi[NIN];
+    __shared__ cx<T> two[NIN == NOUT ? 1 : NOUT];
+    cx<T>* twop = (NIN == NOUT) ? twi : two;
+    load_twr(twi, twin_g, NIN);
+    if (NIN != NOUT) load_twr(twop, twout_g, NOUT);
+    const long row0 = (long)blockIdx.x * 2 * NL;
+    auto ld = [&](int line, int p) {
+        const long ra = row0 + 2 * line, rb = ra + 1;
+        T a = T(0), b = T(0);
+        if (ra < nrows) {
+            if constexpr (SHIFT_IN<T, TI, OP>) a = (T)(in[ra * ld_in + p] - in[ra * ld_in]);
+            else a = (T)in[ra * ld_in + p];
+        }
+        if (rb < nrows) {
+            if constexpr (SHIFT_IN<T, TI, OP>) b = (T)(in[rb * ld_in + p] - in[rb * ld_in]);
+            else b = (T)in[rb * ld_in + p];
+        }
+        return cx<T>{a, b};
+    };
+    auto st = [&](int line, int p, cx<T> z) {
+        const long ra = row0 + 2 * line, rb = ra + 1;
+        if (ra < nrows) out[ra * ld_out + p] = z.x;
+        if (rb < nrows) out[rb * ld_out + p] = z.y;
+    };
+    __syncthreads();
+    line_pipeline<T, NIN, NOUT, OP, NL, NTH, false>(ld, st, buf, twi, twop, kfac);
+}
+
+template <typename T, typename TI, int NIN, int NOUT, int OP>
+__global__ __launch_bounds__((rg_threads<T, NIN, NOUT, true>())) void kr_yline(
+    const TI* __restrict__ in, T* __restrict__ out, int ncols, int ld_in, int ld_out, long ps_in, long ps_out,
+    const cx<T>* __restrict__ twin_g, const cx<T>* __restrict__ twout_g, T kfac) {
+    constexpr int NL = rg_lines<T, NIN, NOUT, true>();
+    constexpr int NTH = rg_threads<T, NIN, NOUT, true>();
+    constexpr int LS = rg_ls(cmax(NIN, NOUT));
+    __shared__ cx<T> buf[NL * LS];
+    __shared__ cx<T> twi[NIN];
+    __shared__ cx<T> two[NIN == NOUT ? 1 : NOUT];
+    cx<T>* twop = (NIN == NOUT) ? twi : two;
+    load_twr(twi, twin_g, NIN);
+    if (NIN != NOUT) load_twr(twop, twout_g, NOUT);
+    const int x0 = blockIdx.x * 2 * NL;
+    const TI* src = in + (long)blockIdx.y * ps_in;
+    T* dst = out + (long)blockIdx.y * ps_out;
+    using PI = typename pair_t<TI>::type;
+    using PO = typename pair_t<T>::type;
+    auto ld = [&](int line, int p) {
+        const int x = x0 + 2 * line;
+        if (x >= ncols) return cx<T>{T(0), T(0)};
+        const PI v = *reinterpret_cast<const PI*>(src + (long)p * ld_in + x);
+        if constexpr (SHIFT_IN<T, TI, OP>) {
+            const PI v0 = *reinterpret_cast<const PI*>(src + x);
+            return cx<T>{(T)(v.x - v0.x), (T)(v.y - v0.y)};
+        } else {
+            return cx<T>{(T)v.x, (T)v.y};
+        }
+    };
+    auto st = [&](int line, int p, cx<T> z) {
+        const int x = x0 + 2 * line;
+        if (x < ncols) *reinterpret_cast<PO*>(dst + (long)p * ld_out + x) = PO{z.x, z.y};
+    };
+    __syncthreads();
+    line_pipeline<T, NIN, NOUT, OP, NL, NTH, true>(ld, st, buf, twi, twop, kfac);
+}
+
+// x r2c of fftn_mpi (FFTW r2c: k = 0..N/2; the x Nyquist is zeroed as in
+// fftn_mpi), two real rows per complex line, split by Hermitian symmetry.
+template <typename T, typename TI, int N>
+__global__ __launch_bounds__((rg_threads<T, N, N, false>())) void kr_x_r2c(const TI* __restrict__ in,
+                                                                         cx<T>* __restrict__ out, long nrows,
+                                                                         int ld_in, int ldc,
+                                                                         const cx<T>* __restrict__ tw_g, T scale) {
+    constexpr int NL = rg_lines<T, N, N, false>();
+    constexpr int NTH = rg_threads<T, N, N, false>();
+    constexpr int LS = rg_ls(N);
+    constexpr int LH = N / 2 + 1;
+    constexpr int SF = rg_nst(N);
+    __shared__ cx<T> buf[NL * LS];
+    __shared__ cx<T> tw[N];
+    load_twr(tw, tw_g, N);
+    const long row0 = (long)blockIdx.x * 2 * NL;
+    __syncthreads();
+    Stage<T, N, 0, -1, NL, NTH, false> f0;
+    f0.load([&](int line, int p) {
+        const long ra = row0 + 2 * line, rb = ra + 1;
+        return cx<T>{(ra < nrows) ? (T)in[ra * ld_in + p] : T(0), (rb < nrows) ? (T)in[rb * ld_in + p] : T(0)};
+    });
+    f0.compute(tw);
+    f0.template write<LS>(buf);
+    __syncthreads();
+    middle_stages<T, N, -1, NL, NTH, false, LS>(buf, tw);
+    Stage<T, N, SF - 1, -1, NL, NTH, false> fl;
+    fl.template read<LS>(buf);
+    fl.compute(tw);
+    __syncthreads();
+    fl.store([&](int line, int p, cx<T> z) { buf[line * LS + rg_pad(p)] = z; });
+    __syncthreads();
+    for (int e = threadIdx.x; e < 2 * NL * LH; e += NTH) {
+        const int rl = e / LH, k = e - rl * LH;
+        const long gr = row0 + rl;
+        if (gr >= nrows) continue;
+        cx<T> r = {T(0), T(0)};
+        if (2 * k != N) {
+            const cx<T> zk = buf[(rl >> 1) * LS + rg_pad(k)];
+            const cx<T> zc = buf[(rl >> 1) * LS + rg_pad((N - k) % N)];
+            if ((rl & 1) == 0) r = {(zk.x + zc.x) * T(0.5), (zk.y - zc.y) * T(0.5)};
+            else r = {(zk.y + zc.y) * T(0.5), -(zk.x - zc.x) * T(0.5)};
+            r.x *= scale;
+            r.y *= scale;
+        }
+        out[gr * ldc + k] = r;
+    }
+}
+
+// x c2r of ifftn_mpi (FFTW c2r: imaginary parts of the DC and Nyquist
+// coefficients ignored); DERIV applies i kx (kx Nyquist = 0) first.
+template <typename T, int N, int DERIV>
+__global__ __launch_bounds__((rg_threads<T, N, N, false>())) void kr_x_c2r(const cx<T>* __restrict__ in,
+                                                                         T* __restrict__ out, long nrows, int ldc,
+                                                                         int ld_out, const cx<T>* __restrict__ tw_g,
+                                                                         T kfac) {
+    constexpr int NL = rg_lines<T, N, N, false>();
+    constexpr int NTH = rg_threads<T, N, N, false>();
+    constexpr int LS = rg_ls(N);
+    constexpr int SI = rg_nst(N);
+    __shared__ cx<T> buf[NL * LS];
+    __shared__ cx<T> tw[N];
+    load_twr(tw, tw_g, N);
+    const long row0 = (long)blockIdx.x * 2 * NL;
+    __syncthreads();
+    Stage<T, N, 0, +1, NL, NTH, false> i0;
+    i0.load([&](int line, int k) {
+        const int kk = (2 * k <= N) ? k : N - k;
+        const long ga = row0 + 2 * line, gb = ga + 1;
+        cx<T> a = (ga < nrows) ? in[ga * ldc + kk] : cx<T>{T(0), T(0)};
+        cx<T> b = (gb < nrows) ? in[gb * ldc + kk] : cx<T>{T(0), T(0)};
+        if (kk == 0 || 2 * kk == N) {
+            a.y = T(0);
+            b.y = T(0);
+        }
+        if constexpr (DERIV) {
+            const T f = (2 * kk == N) ? T(0) : (T)kk * kfac;
+            a = {-f * a.y, f * a.x};
+            b = {-f * b.y, f * b.x};
+        }
+        if (2 * k > N) {
+            a.y = -a.y;
+            b.y = -b.y;
+        }
+        return cx<T>{a.x - b.y, a.y + b.x};
+    });
+    i0.compute(tw);
+    i0.template write<LS>(buf);
+    __syncthreads();
+    middle_stages<T, N, +1, NL, NTH, false, LS>(buf, tw);
+    Stage<T, N, SI - 1, +1, NL, NTH, false> il;
+    il.template read<LS>(buf);
+    il.compute(tw);
+    il.store([&](int line, int p, cx<T> z) {
+        const long ra = row0 + 2 * line, rb = ra + 1;
+        if (ra < nrows) out[ra * ld_out + p] = z.x;
+        if (rb < nrows) out[rb * ld_out + p] = z.y;
+    });
+}
+
+// complex y-lines of the half spectrum (lhx columns, N rows per plane)
+template <typename T, int N, int OP>
+__global__ __launch_bounds__((rg_threads<T, N, N, true>())) void kr_yspec(const cx<T>* __restrict__ in,
+                                                                        cx<T>* __restrict__ out, int lhx, long ps,
+                                                                        const cx<T>* __restrict__ tw_g,
+                                                                        const double* __restrict__ G) {
+    // one complex column per line: 8 (FP64) / 16 (FP32) columns = 128 bytes
+    constexpr int NL = rg_lines<T, N, N, true>();
+    constexpr int NTH = rg_threads<T, N, N, true>();
+    constexpr int LS = rg_ls(N);
+    constexpr int S = rg_nst(N);
+    __shared__ cx<T> buf[NL * LS];
+    __shared__ cx<T> tw[N];
+    load_twr(tw, tw_g, N);
+    const int x0 = blockIdx.x * NL;
+    const cx<T>* src = in + (long)blockIdx.y * ps;
+    cx<T>* dst = out + (long)blockIdx.y * ps;
+    const double* Gp = G;
+    auto gload = [&](int line, int p) {
+        const int x = x0 + line;
+        return (x < lhx) ? src[(long)p * lhx + x] : cx<T>{T(0), T(0)};
+    };
+    auto gstore = [&](int line, int p, cx<T> z) {
+        const int x = x0 + line;
+        if (x < lhx) dst[(long)p * lhx + x] = z;
+    };
+    __syncthreads();
+    if constexpr (OP == SP_FWD || OP == SP_FILT) {
+        Stage<T, N, 0, -1, NL, NTH, true> f0;
+        f0.load(gload);
+        f0.compute(tw);
+        f0.template write<LS>(buf);
+        __syncthreads();
+        middle_stages<T, N, -1, NL, NTH, true, LS>(buf, tw);
+        Stage<T, N, S - 1, -1, NL, NTH, true> fl;
+        fl.template read<LS>(buf);
+        fl.compute(tw);
+        if constexpr (OP == SP_FWD) {
+            fl.store([&](int line, int p, cx<T> z) { gstore(line, p, (2 * p == N) ? cx<T>{T(0), T(0)} : z); });
+            return;
+        } else {
+            // Nyquist row zeroed, times G, then the inverse from registers
+            Stage<T, N, 0, +1, NL, NTH, true> i0;
+#pragma unroll
+            for (int q = 0; q < decltype(i0)::PB; ++q) {
+                int line, b;
+                if (!decltype(i0)::bf(q, line, b)) continue;
+                const int x = x0 + line;
+#pragma unroll
+                for (int r = 0; r < decltype(i0)::R; ++r) {
+                    const int p = b + r * decltype(i0)::NB;
+                    const T g = (x < lhx && 2 * p != N) ? (T)Gp[(long)p * lhx + x] : T(0);
+                    i0.v[q][r] = {fl.v[q][r].x * g, fl.v[q][r].y * g};
+                }
+            }
+            i0.compute(tw);
+            __syncthreads();
+            i0.template write<LS>(buf);
+        }
+    } else {
+        Stage<T, N, 0, +1, NL, NTH, true> i0;
+        if constexpr (OP == SP_MULINV) {
+            i0.load([&](int line, int p) {
+                const int x = x0 + line;
+                if (x >= lhx) return cx<T>{T(0), T(0)};
+                const cx<T> z = src[(long)p * lhx + x];
+                const T g = (T)Gp[(long)p * lhx + x];
+                return cx<T>{z.x * g, z.y * g};
+            });
+        } else {
+            i0.load(gload);
+        }
+        i0.compute(tw);
+        i0.template write<LS>(buf);
+    }
+    __syncthreads();
+    middle_stages<T, N, +1, NL, NTH, true, LS>(buf, tw);
+    Stage<T, N, S - 1, +1, NL, NTH, true> il;
+    il.template read<LS>(buf);
+    il.compute(tw);
+    il.store(gstore);
+}
+
+template <typename T>
+const cx<T>* twp(const Twiddle& t);
+template <>
+const cx<double>* twp<double>(const Twiddle& t) { return t.d; }
+template <>
+const cx<float>* twp<float>(const Twiddle& t) { return t.f; }
+
+// ---- dispatch --------------------------------------------------------------
+template <typename T, typename TI, int NIN, int NOUT, int OP>
+void launch_x(hipStream_t s, const TI* in, T* out, long nrows, int ld_in, int ld_out, const Twiddle& tin,
+              const Twiddle& tout, double kfac) {
+    constexpr int NL = rg_lines<T, NIN, NOUT, false>();
+    constexpr int NTH = rg_threads<T, NIN, NOUT, false>();
+    const long nb = (nrows + 2 * NL - 1) / (2 * NL);
+    hipLaunchKernelGGL((kr_xline<T, TI, NIN, NOUT, OP>), dim3((unsigned)nb), dim3(NTH), 0, s, in, out, nrows, ld_in,
+                       ld_out, twp<T>(tin), twp<T>(tout), (T)kfac);
+}
+
+template <typename T, typename TI, int NIN, int NOUT, int OP>
+void launch_y(hipStream_t s, const TI* in, T* out, int nplanes, int ncols, int ld_in, int ld_out, long ps_in,
+              long ps_out, const Twiddle& tin, const Twiddle& tout, double kfac) {
+    constexpr int NL = rg_lines<T, NIN, NOUT, true>();
+    constexpr int NTH = rg_threads<T, NIN, NOUT, true>();
+    const int nbx = (ncols + 2 * NL - 1) / (2 * NL);
+    hipLaunchKernelGGL((kr_yline<T, TI, NIN, NOUT, OP>), dim3(nbx, nplanes), dim3(NTH), 0, s, in, out, ncols, ld_in,
+                       ld_out, ps_in, ps_out, twp<T>(tin), twp<T>(tout), (T)kfac);
+}
+
+#define LES_SIZES(X) X(64) X(128) X(256) X(512)
+
+// the y-kernels read column pairs as one aligned vector: even column count,
+// row stride and plane stride
+template <typename TI, typename T>
+bool pairs_aligned(const TI* in, const T* out, int ncols, int ld_in, int ld_out, long ps_in, long ps_out) {
+    return ncols % 2 == 0 && ld_in % 2 == 0 && ld_out % 2 == 0 && ps_in % 2 == 0 && ps_out % 2 == 0 &&
+           (uintptr_t)in % (2 * sizeof(TI)) == 0 && (uintptr_t)out % (2 * sizeof(T)) == 0;
+}
+
+template <typename T, typename TI>
+bool line_dispatch_x(hipStream_t s, const TI* in, T* out, long nrows, int ld_in, int ld_out, const Twiddle& tin,
+                     const Twiddle& tout, LineOp op, double kfac) {
+    const int a = tin.n, b = tout.n;
+#define XCASE(n)                                                                                    \
+    if (a == n && b == n && op == OP_DERIV) { launch_x<T, TI, n, n, OP_DERIV>(s, in, out, nrows, ld_in, ld_out, tin, tout, kfac); return true; } \
+    if (a == n && b == n && op == OP_NYQ) { launch_x<T, TI, n, n, OP_NYQ>(s, in, out, nrows, ld_in, ld_out, tin, tout, kfac); return true; }     \
+    if (a == n && b == 3 * n / 2 && op == OP_PAD) { launch_x<T, TI, n, 3 * n / 2, OP_PAD>(s, in, out, nrows, ld_in, ld_out, tin, tout, kfac); return true; } \
+    if (a == 3 * n / 2 && b == n && op == OP_UNPAD) { launch_x<T, TI, 3 * n / 2, n, OP_UNPAD>(s, in, out, nrows, ld_in, ld_out, tin, tout, kfac); return true; }
+    LES_SIZES(XCASE)
+#undef XCASE
+    return false;
+}
+
+template <typename T, typename TI>
+bool line_dispatch_y(hipStream_t s, const TI* in, T* out, int nplanes, int ncols, int ld_in, int ld_out, long ps_in,
+                     long ps_out, const Twiddle& tin, const Twiddle& tout, LineOp op, double kfac) {
+    if (!pairs_aligned(in, out, ncols, ld_in, ld_out, ps_in, ps_out)) return false;
+    const int a = tin.n, b = tout.n;
+#define YCASE(n)                                                                                    \
+    if (a == n && b == n && op == OP_DERIV) { launch_y<T, TI, n, n, OP_DERIV>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, kfac); return true; } \
+    if (a == n && b == n && op == OP_NYQ) { launch_y<T, TI, n, n, OP_NYQ>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, kfac); return true; }     \
+    if (a == n && b == 3 * n / 2 && op == OP_PAD) { launch_y<T, TI, n, 3 * n / 2, OP_PAD>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, kfac); return true; } \
+    if (a == 3 * n / 2 && b == n && op == OP_UNPAD) { launch_y<T, TI, 3 * n / 2, n, OP_UNPAD>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, kfac); return true; }
+    LES_SIZES(YCASE)
+#undef YCASE
+    return false;
+}
+
+template <typename T, typename TI, int N>
+void launch_r2c(hipStream_t s, const TI* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
+                double scale) {
+    constexpr int NL = rg_lines<T, N, N, false>();
+    constexpr int NTH = rg_threads<T, N, N, false>();
+    const long nb = (nrows + 2 * NL - 1) / (2 * NL);
+    hipLaunchKernelGGL((kr_x_r2c<T, TI, N>), dim3((unsigned)nb), dim3(NTH), 0, s, in, out, nrows, ld_in, ldc,
+                       twp<T>(tw), (T)scale);
+}
+
+template <typename T, int N, int D>
+void launch_c2r(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int ld_out, const Twiddle& tw,
+                double kfac) {
+    constexpr int NL = rg_lines<T, N, N, false>();
+    constexpr int NTH = rg_threads<T, N, N, false>();
+    const long nb = (nrows + 2 * NL - 1) / (2 * NL);
+    hipLaunchKernelGGL((kr_x_c2r<T, N, D>), dim3((unsigned)nb), dim3(NTH), 0, s, in, out, nrows, ldc, ld_out,
+                       twp<T>(tw), (T)kfac);
+}
+
+template <typename T, int N, int OP>
+void launch_spec(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lhx, long ps, const Twiddle& tw,
+                 const double* G) {
+    constexpr int NL = rg_lines<T, N, N, true>();
+    constexpr int NTH = rg_threads<T, N, N, true>();
+    const int nbx = (lhx + NL - 1) / NL;
+    hipLaunchKernelGGL((kr_yspec<T, N, OP>), dim3(nbx, nplanes), dim3(NTH), 0, s, in, out, lhx, ps, twp<T>(tw), G);
+}
+
+template <typename T, typename TI>
+bool r2c_dispatch(hipStream_t s, const TI* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
+                  double scale) {
+#define RCASE(NN_) \
+    if (tw.n == NN_) { launch_r2c<T, TI, NN_>(s, in, out, nrows, ld_in, ldc, tw, scale); return true; }
+    LES_SIZES(RCASE)
+#undef RCASE
+    return false;
+}
+
+template <typename T>
+bool c2r_dispatch(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int ld_out, const Twiddle& tw,
+                  int deriv, double kfac) {
+#define CCASE(NN_)                                                                               \
+    if (tw.n == NN_) {                                                                           \
+        if (deriv) launch_c2r<T, NN_, 1>(s, in, out, nrows, ldc, ld_out, tw, kfac);              \
+        else launch_c2r<T, NN_, 0>(s, in, out, nrows, ldc, ld_out, tw, kfac);                    \
+        return true;                                                                           \
+    }
+    LES_SIZES(CCASE)
+#undef CCASE
+    return false;
+}
+
+template <typename T>
+bool spec_dispatch(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lhx, long ps, const Twiddle& tw,
+                   SpecOp op, const double* G) {
+#define SCASE(NN_)                                                                                          \
+    if (tw.n == NN_) {                                                                                      \
+        switch (op) {                                                                                     \
+            case SP_FWD: launch_spec<T, NN_, SP_FWD>(s, in, out, nplanes, lhx, ps, tw, G); break;           \
+            case SP_INV: launch_spec<T, NN_, SP_INV>(s, in, out, nplanes, lhx, ps, tw, G); break;           \
+            case SP_FILT: launch_spec<T, NN_, SP_FILT>(s, in, out, nplanes, lhx, ps, tw, G); break;         \
+            default: launch_spec<T, NN_, SP_MULINV>(s, in, out, nplanes, lhx, ps, tw, G); break;            \
+        }                                                                                                 \
+        return true;                                                                                      \
+    }
+    LES_SIZES(SCASE)
+#undef SCASE
+    return false;
+}
+
+}  // namespace
+
+// exported to k_lines.hip
+template <typename T, typename TI>
+bool xline_real_ct(hipStream_t s, const TI* in, T* out, long nrows, int ld_in, int ld_out, const Twiddle& tin,
+                   const Twiddle& tout, LineOp op, double kfac) {
+    return line_dispatch_x<T, TI>(s, in, out, nrows, ld_in, ld_out, tin, tout, op, kfac);
+}
+template <typename T, typename TI>
+bool yline_real_ct(hipStream_t s, const TI* in, T* out, int nplanes, int ncols, int ld_in, int ld_out, long ps_in,
+                   long ps_out, const Twiddle& tin, const Twiddle& tout, LineOp op, double kfac) {
+    return line_dispatch_y<T, TI>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, op, kfac);
+}
+template <typename T, typename TI>
+bool x_r2c_ct(hipStream_t s, const TI* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
+              double scale) {
+    return r2c_dispatch<T, TI>(s, in, out, nrows, ld_in, ldc, tw, scale);
+}
+template <typename T>
+bool x_c2r_ct(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int ld_out, const Twiddle& tw, int deriv,
+              double kfac) {
+    return c2r_dispatch<T>(s, in, out, nrows, ldc, ld_out, tw, deriv, kfac);
+}
+template <typename T>
+bool yline_spec_ct(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lhx, long ps, const Twiddle& tw,
+                   SpecOp op, const double* G) {
+    return spec_dispatch<T>(s, in, out, nplanes, lhx, ps, tw, op, G);
+}
+
+#define INST_IO(T, TI)                                                                                       \
+    template bool xline_real_ct<T, TI>(hipStream_t, const TI*, T*, long, int, int, const Twiddle&, const Twiddle&, \
+                                       LineOp, double);                                                          \
+    template bool yline_real_ct<T, TI>(hipStream_t, const TI*, T*, int, int, int, int, long, long, const Twiddle&, \
+                                       const Twiddle&, LineOp, double);                                          \
+    template bool x_r2c_ct<T, TI>(hipStream_t, const TI*, cx<T>*, long, int, int, const Twiddle&, double);
+INST_IO(double, double)
+INST_IO(float, float)
+INST_IO(float, double)
+#undef INST_IO
+template bool x_c2r_ct<double>(hipStream_t, const cx<double>*, double*, long, int, int, const Twiddle&, int, double);
+template bool x_c2r_ct<float>(hipStream_t, const cx<float>*, float*, long, int, int, const Twiddle&, int, double);
+template bool yline_spec_ct<double>(hipStream_t, const cx<double>*, cx<double>*, int, int, long, const Twiddle&,
+                                    SpecOp, const double*);
+template bool yline_spec_ct<float>(hipStream_t, const cx<float>*, cx<float>*, int, int, long, const Twiddle&,
+                                   SpecOp, const double*);
+
+}  // namespace les

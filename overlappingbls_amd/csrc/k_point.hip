@@ -3,6 +3,7 @@
 // 0..nzt; the bottom z-slab's rules at levels 0/1, the top z-slab's at
 // nzt-1/nzt).  Streaming kernels are HBM-bound: one read of each input and
 // one write of each output per point, grid-stride over (level, plane).
+#include "fft_rg.hpp"
 #include "les_internal.hpp"
 
 namespace les {
@@ -86,28 +87,85 @@ void level_means(hipStream_t s, const double* f, size_t plane, int lev0, int nle
     hipLaunchKernelGGL(k_level_means, dim3(nlev), dim3(256), 0, s, f + (size_t)lev0 * plane, (long)plane, nn, out);
 }
 
-// obukhov sums at the wall level: [0] sum sqrt(u^2+v^2), [1] sum psi_m,
-// [2] sum sgs_t3(1), [3] sum theta(1)
-__global__ __launch_bounds__(256) void k_obukhov_sums(const double* __restrict__ u1, const double* __restrict__ v1,
-                                                       const double* __restrict__ psi, const double* __restrict__ t3,
-                                                       const double* __restrict__ th1, long plane,
-                                                       double* __restrict__ out) {
-    __shared__ double sh[256];
+// ---- sums in the reference's order ---------------------------------------
+// calc_hor_avg and the obukhov plane averages (202_parallel.f90:993-1059,
+// 010_scalars.f90:758-775) are Fortran sums over a plane, x fastest: one
+// running sum, each element added in index order.  A sequential sum rounds
+// differently from a tree sum (the sum of a uniform plane, as psi_m, is not
+// n times the value), and the difference is amplified where the mean feeds
+// a well-resolved quantity (ustar, psi_m, the wall stress) or a cancellation
+// (theta - mean(theta) in calc_beta and the fluctuation statistics), so these
+// sums keep the reference's order.  One wavefront per sum: its lanes load
+// stages of SEQ_STAGE values (coalesced) into LDS while the running sum walks
+// the previous stage, one broadcast LDS read and one FP64 add per value.
+constexpr int SEQ_STAGE = 1024;  // values per stage, 16 per lane
+
+template <typename F>
+__device__ __forceinline__ double wave_chain(long n, const F& val, double* lds) {
+    const int lane = threadIdx.x & 63;
+    constexpr int PER = SEQ_STAGE / 64;
+    double r[PER];
+    auto gload = [&](long b) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const long idx = b + j * 64 + lane;
+            r[j] = val(idx < n ? idx : n - 1);   // in bounds; values past n are never added
+        }
+    };
     double s = 0.0;
-    const int which = blockIdx.x;
-    for (long i = threadIdx.x; i < plane; i += 256) {
-        if (which == 0) s += sqrt(u1[i] * u1[i] + v1[i] * v1[i]);
-        else if (which == 1) s += psi[i];
-        else if (which == 2) s += t3 ? t3[i] : 0.0;
-        else s += th1 ? th1[i] : 0.0;
-    }
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    gload(0);
+    int buf = 0;
+    for (long base = 0; base < n; base += SEQ_STAGE) {
+        double* L = lds + buf * SEQ_STAGE;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) L[j * 64 + lane] = r[j];
+        if (base + SEQ_STAGE < n) gload(base + SEQ_STAGE);   // next stage in flight during the adds
         __syncthreads();
+        const int lim = (n - base < SEQ_STAGE) ? (int)(n - base) : SEQ_STAGE;
+        int j = 0;
+        for (; j + 32 <= lim; j += 32) {
+            double t[32];
+#pragma unroll
+            for (int q = 0; q < 32; ++q) t[q] = L[j + q];
+            // all 32 broadcast reads are issued before the first add consumes one
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int q = 0; q < 32; ++q) s += t[q];
+        }
+        for (; j < lim; ++j) s += L[j];
+        buf ^= 1;
     }
-    if (threadIdx.x == 0) out[which] = sh[0];
+    return s;
+}
+
+// plane means, one wavefront (block) per plane
+__global__ __launch_bounds__(64) void k_level_means_seq(const double* __restrict__ f, long plane, double nn,
+                                                         double* __restrict__ out) {
+    __shared__ double lds[2 * SEQ_STAGE];
+    const double* p = f + (long)blockIdx.x * plane;
+    const double s = wave_chain(plane, [&](long i) { return p[i]; }, lds);
+    if (threadIdx.x == 0) out[blockIdx.x] = s / nn;
+}
+
+void level_means_seq(hipStream_t s, const double* f, size_t plane, int lev0, int nlev, double nn, double* out) {
+    hipLaunchKernelGGL(k_level_means_seq, dim3(nlev), dim3(64), 0, s, f + (size_t)lev0 * plane, (long)plane, nn,
+                       out);
+}
+
+// obukhov sums at the wall level, in the reference's order, one wavefront
+// per sum: [0] sum sqrt(u^2+v^2), [1] sum psi_m, [2] sum sgs_t3(1), [3] sum theta(1)
+__global__ __launch_bounds__(64) void k_obukhov_sums(const double* __restrict__ u1, const double* __restrict__ v1,
+                                                      const double* __restrict__ psi, const double* __restrict__ t3,
+                                                      const double* __restrict__ th1, long plane,
+                                                      double* __restrict__ out) {
+    __shared__ double lds[2 * SEQ_STAGE];
+    const int which = blockIdx.x;
+    double s;
+    if (which == 0) s = wave_chain(plane, [&](long i) { return sqrt(u1[i] * u1[i] + v1[i] * v1[i]); }, lds);
+    else if (which == 1) s = wave_chain(plane, [&](long i) { return psi[i]; }, lds);
+    else if (which == 2) s = wave_chain(plane, [&](long i) { return t3[i]; }, lds);
+    else s = wave_chain(plane, [&](long i) { return th1[i]; }, lds);
+    if (threadIdx.x == 0) out[which] = s;
 }
 
 // neutral / theta-off branch, 010_scalars.f90:718-738
@@ -154,27 +212,37 @@ __global__ void k_obukhov_active(double* ustar, double* psi_m, double* phi_m, do
     }
 }
 
-void obukhov_dev(les_ctx* c, int jt) {
+// obukhov's plane sums (and the fills that precede them) on stream s
+void obukhov_sums_dev(les_ctx* c, int jt, hipStream_t s) {
     const les_params& P = c->prm;
     const long np = (long)c->plane;
-    const double nn = (double)(c->nx * c->ny);
     const double* u1 = c->f3[LES_F_U] + np;
     const double* v1 = c->f3[LES_F_V] + np;
-    hipStream_t s = c->stream;
     if (!P.theta_flag || jt < P.theta_init_time) {
         if (!P.restart_theta) fill_levels(s, c->psi_m, c->plane, 0, 1, 0.0);
-        hipLaunchKernelGGL(k_obukhov_sums, dim3(2), dim3(256), 0, s, u1, v1, c->psi_m, (const double*)nullptr,
+        hipLaunchKernelGGL(k_obukhov_sums, dim3(2), dim3(64), 0, s, u1, v1, c->psi_m, (const double*)nullptr,
                            (const double*)nullptr, np, c->red);
-        hipLaunchKernelGGL(k_obukhov_neutral, grid_for(np), dim3(256), 0, s, c->ustar, c->phi_m, c->psi_h,
-                           c->phi_h, (const double*)c->red, np, nn, (int)(P.walltype == LES_WALL_SMOOTH), P.zo,
-                           c->dz, P.nu_molec / P.u_scale / P.z_i);
         return;
     }
     double* t3 = c->f3[LES_F_SGS_T3] + np;
     if (jt == P.theta_init_time && !P.restart_theta)
         fill_levels(s, t3, c->plane, 0, 1, P.wt_s / P.u_scale / P.T_scale);
-    hipLaunchKernelGGL(k_obukhov_sums, dim3(4), dim3(256), 0, s, u1, v1, c->psi_m, (const double*)t3,
+    hipLaunchKernelGGL(k_obukhov_sums, dim3(4), dim3(64), 0, s, u1, v1, c->psi_m, (const double*)t3,
                        (const double*)(c->f3[LES_F_THETA] + np), np, c->red);
+}
+
+// the rest of obukhov from the sums, on the context's stream
+void obukhov_finish_dev(les_ctx* c, int jt) {
+    const les_params& P = c->prm;
+    const long np = (long)c->plane;
+    const double nn = (double)(c->nx * c->ny);
+    hipStream_t s = c->stream;
+    if (!P.theta_flag || jt < P.theta_init_time) {
+        hipLaunchKernelGGL(k_obukhov_neutral, grid_for(np), dim3(256), 0, s, c->ustar, c->phi_m, c->psi_h,
+                           c->phi_h, (const double*)c->red, np, nn, (int)(P.walltype == LES_WALL_SMOOTH), P.zo,
+                           c->dz, P.nu_molec / P.u_scale / P.z_i);
+        return;
+    }
     // zo_ = exp(mean(log(zo))) of a homogeneous zo, summed in order
     double sl = 0.0;
     const double lz = log(P.zo);
@@ -183,6 +251,11 @@ void obukhov_dev(les_ctx* c, int jt) {
     const double g_ = GRAV / (P.u_scale * P.u_scale / P.z_i);
     hipLaunchKernelGGL(k_obukhov_active, grid_for(np), dim3(256), 0, s, c->ustar, c->psi_m, c->phi_m, c->psi_h,
                        c->phi_h, (const double*)c->red, np, nn, zo_, c->dz, g_);
+}
+
+void obukhov_dev(les_ctx* c, int jt) {
+    obukhov_sums_dev(c, jt, c->stream);
+    obukhov_finish_dev(c, jt);
 }
 
 // ---- wall stress, 104_wallstress.f90:98-260 ------------------------------
@@ -944,6 +1017,7 @@ struct RhsArgs {
     double *rx, *ry, *rz, *rxf, *ryf, *rzf;
     const W *cx, *cy, *cz, *dx, *dy, *dz, *beta;
     const double *u, *v, *w, *sponge;
+    double *du, *dv, *dw;  // dudt diagnostics (103_flosol.f90:119-156), or null
 };
 
 template <typename W>
@@ -963,6 +1037,14 @@ __global__ void k_rhs(RhsArgs<W> a, KP p, int buoy, int cori, double coriol, dou
         }
         if (damp) rz = rz - a.sponge[k] * a.w[i];
         a.rx[i] = rx; a.ry[i] = ry; a.rz[i] = rz;
+        if (a.du) {
+            double du = -(double)a.dx[i], dv = -(double)a.dy[i];
+            if (cori) {
+                du = du + coriol * (a.v[i] - vg);
+                dv = dv + coriol * (ug - a.u[i]);
+            }
+            a.du[i] = du; a.dv[i] = dv; a.dw[i] = -(double)a.dz[i];
+        }
         if (euler) { a.rxf[i] = rx; a.ryf[i] = ry; a.rzf[i] = rz; }
     }
 }
@@ -977,6 +1059,10 @@ void rhs_assembly_dev(les_ctx* c, int jt) {
     a.cx = WF<W>(c, LES_F_CX); a.cy = WF<W>(c, LES_F_CY); a.cz = WF<W>(c, LES_F_CZ);
     a.dx = WF<W>(c, LES_F_DIVTX); a.dy = WF<W>(c, LES_F_DIVTY); a.dz = WF<W>(c, LES_F_DIVTZ);
     a.beta = WF<W>(c, LES_F_BETA_SCAL);
+    const bool st = stats_step(c, jt);
+    a.du = st ? c->st_diag[0] : nullptr;
+    a.dv = st ? c->st_diag[1] : nullptr;
+    a.dw = st ? c->st_diag[2] : nullptr;
     a.u = c->f3[LES_F_U]; a.v = c->f3[LES_F_V]; a.w = c->f3[LES_F_W]; a.sponge = c->sponge;
     const int damp = (P.ubc_mom == LES_BC_STRESS_FREE && P.ubc == 1 && P.damping_method == 1);
     hipLaunchKernelGGL(k_rhs<W>, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, a, p,
@@ -1010,24 +1096,33 @@ void ab2_dev(les_ctx* c) {
                        P.use_mean_p_force ? P.mean_p_force : 0.0);
 }
 
+// RHS -= grad p (103_flosol.f90:179-181); du.. (null or the dudt
+// diagnostics) likewise (:183-185)
 template <typename W>
 __global__ void k_rhs_gradp(double* rx, double* ry, double* rz, const W* __restrict__ px, const W* __restrict__ py,
-                            const W* __restrict__ pz, KP p) {
+                            const W* __restrict__ pz, KP p, double* du, double* dv, double* dw) {
     const long n = p.plane * (p.nz - 1);
     GRID_LOOP(t, n) {
         const long i = p.plane + t;
         rx[i] = rx[i] - px[i];
         ry[i] = ry[i] - py[i];
         rz[i] = rz[i] - pz[i];
+        if (du) {
+            du[i] = du[i] - px[i];
+            dv[i] = dv[i] - py[i];
+            dw[i] = dw[i] - pz[i];
+        }
     }
 }
 
 template <typename W>
-void rhs_minus_gradp_dev(les_ctx* c) {
+void rhs_minus_gradp_dev(les_ctx* c, int jt) {
     KP p = kp(c);
+    const bool st = stats_step(c, jt);
     hipLaunchKernelGGL(k_rhs_gradp<W>, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, c->f3[LES_F_RHSX],
                        c->f3[LES_F_RHSY], c->f3[LES_F_RHSZ], (const W*)WF<W>(c, LES_F_DPDX),
-                       (const W*)WF<W>(c, LES_F_DPDY), (const W*)WF<W>(c, LES_F_DPDZ), p);
+                       (const W*)WF<W>(c, LES_F_DPDY), (const W*)WF<W>(c, LES_F_DPDZ), p,
+                       st ? c->st_diag[0] : nullptr, st ? c->st_diag[1] : nullptr, st ? c->st_diag[2] : nullptr);
 }
 
 // project (110_project.f90:16-40) and the boundary rows that follow
@@ -1156,7 +1251,7 @@ void max3_dev(les_ctx* c, double* out) {
     template void press_hz_sums_dev<W>(les_ctx*, double*);                                                  \
     template void dpdz_dev<W>(les_ctx*);                                                                    \
     template void rhs_assembly_dev<W>(les_ctx*, int);                                                       \
-    template void rhs_minus_gradp_dev<W>(les_ctx*);                                                         \
+    template void rhs_minus_gradp_dev<W>(les_ctx*, int);                                                         \
     template void project_dev<W>(les_ctx*);
 LES_INST_POINT(double)
 LES_INST_POINT(float)

@@ -4,6 +4,7 @@
 // stream per context; entry points synchronise before returning.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <utility>
 #include <vector>
@@ -180,6 +181,7 @@ double* surf(les_ctx* c, int id) {
 }
 
 int finish(les_ctx* c) {
+    les_note(c, hipStreamSynchronize(c->aux));
     hipError_t e = hipStreamSynchronize(c->stream);
     les_note(c, e);
     les_note(c, hipGetLastError());
@@ -347,6 +349,9 @@ void theta_all_in_one(les_ctx* c, int jt) {
     W* big[4];
     for (int i = 0; i < 4; ++i) big[i] = scr<W>(c->big3[i]);
     filter_f<double>(c, th, 0, (int)c->nlev);
+    // calc_beta's plane means of theta (reference order) overlap the transforms
+    fork(c);
+    level_means_seq(c->aux, th, np, 0, nz, (double)(c->nx * c->ny), c->red);   // calc_hor_avg order
     ddx_f<W, double>(c, dtdx, th, 0, (int)c->nlev);
     ddy_f<W, double>(c, dtdy, th, 0, (int)c->nlev);
     ddz_uv_dev<W, double>(c, dtdz, th);
@@ -368,7 +373,7 @@ void theta_all_in_one(les_ctx* c, int jt) {
     scalar_flux_dev<W>(c, temp, 2);
     ddz_w_dev<W, W>(c, dtemp, temp);
     scalar_accum_dev<W>(c, dtemp, (const W*)nullptr);
-    level_means(c->stream, th, np, 0, nz, (double)(c->nx * c->ny), c->red);
+    join(c);
     calc_beta_dev<W>(c, c->red);
     if (jt == P.theta_init_time && !P.restart_theta)
         les_note(c, hipMemcpyAsync(F(c, LES_F_RHST_F), F(c, LES_F_RHST), np * c->nlev * 8, hipMemcpyDeviceToDevice,
@@ -411,13 +416,50 @@ void press_stag_array(les_ctx* c) {
     dpdz_dev<W>(c);
 }
 
+// energy and d(ke)/dx_i added to the dudt diagnostics (103_flosol.f90:192-213,
+// :393-429); dkedz at the wall level is not set by the reference (0 here)
+void diagnostics_ke(les_ctx* c) {
+    const int nl = (int)c->nlev;
+    ke_dev(c, c->st_ke);
+    les_note(c, hipMemcpyAsync(c->st_tmp, c->st_ke, c->plane * c->nlev * 8, hipMemcpyDeviceToDevice, c->stream));
+    filter_f<double>(c, c->st_tmp, 0, nl);
+    ddx_f<double, double>(c, c->st_dke, c->st_tmp, 0, nl);
+    add_levels_dev(c, c->st_diag[0], c->st_dke, 0);
+    ddy_f<double, double>(c, c->st_dke, c->st_tmp, 0, nl);
+    add_levels_dev(c, c->st_diag[1], c->st_dke, 0);
+    ddz_uv_dev<double, double>(c, c->st_dke, c->st_ke);
+    add_levels_dev(c, c->st_diag[2], c->st_dke, 1);
+}
+
+// copy the running sums of slices [s0, s1) to the window set, zero them
+void close_window(les_ctx* c, int s0, int s1) {
+    const size_t L = (size_t)c->nx * (c->nz - 1);
+    les_note(c, hipMemcpyAsync(c->st_win + s0 * L, c->st_run + s0 * L, sizeof(double) * L * (s1 - s0),
+                               hipMemcpyDeviceToDevice, c->stream));
+    fill_levels(c->stream, c->st_run + s0 * L, L, 0, s1 - s0, 0.0);
+}
+
+// flow_slice and theta_slice (011_scalars2.f90:28-318) at the end of flosol
+template <typename W>
+void avg_slices(les_ctx* c, int jt) {
+    const les_params& P = c->prm;
+    flow_slices_dev<W>(c);
+    if (jt % c->st_p == 0) close_window(c, 0, LES_N_FLOW_SLICES);
+    if (P.theta_flag && jt >= P.theta_init_time) {
+        theta_slices_dev<W>(c);
+        if (jt % c->st_p == 0) close_window(c, LES_N_FLOW_SLICES, LES_N_SLICES);
+    }
+}
+
 template <typename W>
 void step_body(les_ctx* c, int jt) {
     // newstep: RHS_f = RHS (103_flosol.f90:311-313) as a pointer exchange
     std::swap(c->f3[LES_F_RHSX], c->f3[LES_F_RHSX_F]);
     std::swap(c->f3[LES_F_RHSY], c->f3[LES_F_RHSY_F]);
     std::swap(c->f3[LES_F_RHSZ], c->f3[LES_F_RHSZ_F]);
-    obukhov_dev(c, jt);
+    // obukhov's ordered plane sums overlap the horizontal derivatives
+    fork(c);
+    obukhov_sums_dev(c, jt, c->aux);
     const int nl = (int)c->nlev;
     const double *u = F(c, LES_F_U), *v = F(c, LES_F_V), *w = F(c, LES_F_W);
     ddx_f<W, double>(c, WF<W>(c, LES_F_DUDX), u, 0, nl);
@@ -429,6 +471,8 @@ void step_body(les_ctx* c, int jt) {
     ddz_uv_dev<W, double>(c, WF<W>(c, LES_F_DUDZ), u);
     ddz_uv_dev<W, double>(c, WF<W>(c, LES_F_DVDZ), v);
     ddz_w_dev<W, double>(c, WF<W>(c, LES_F_DWDZ), w);
+    join(c);
+    obukhov_finish_dev(c, jt);
     wallstress<W>(c, 0);
     wallstress<W>(c, 1);
     sgs_stag<W>(c, jt);
@@ -442,11 +486,14 @@ void step_body(les_ctx* c, int jt) {
     rhs_assembly_dev<W>(c, jt);
     ab2_dev(c);
     press_stag_array<W>(c);
-    rhs_minus_gradp_dev<W>(c);
+    rhs_minus_gradp_dev<W>(c, jt);
     project_dev<W>(c);
+    const bool st = stats_step(c, jt);
+    if (st) diagnostics_ke(c);
     filter_f<double>(c, F(c, LES_F_U), 0, nl);
     filter_f<double>(c, F(c, LES_F_V), 0, nl);
     filter_f<double>(c, F(c, LES_F_W), 0, nl);
+    if (st) avg_slices<W>(c, jt);
 }
 
 bool mixed(const les_ctx* c) { return c->precision == LES_MIXED; }
@@ -524,6 +571,8 @@ int les_create(const les_params* prm, les_ctx** out) {
     if (3 * prm->nxt / 2 > 3072 || 3 * prm->nyt / 2 > 3072) return LES_E_BADARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return LES_E_NODEV;
+    // LES_LINE_KERNELS=generic forces the runtime-length line kernels (process-wide)
+    if (const char* lk = getenv("LES_LINE_KERNELS")) g_use_ct = strcmp(lk, "generic") != 0;
     les_ctx* c = new les_ctx();
     c->prm = *prm;
     c->nx = prm->nxt; c->ny = prm->nyt; c->nz = prm->nzt;
@@ -540,6 +589,11 @@ int les_create(const les_params* prm, les_ctx** out) {
     if (hipGetDevice(&c->dev) != hipSuccess) { delete c; return LES_E_NODEV; }
     c->err = hipSuccess;
     if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return LES_E_HIP; }
+    if (hipStreamCreate(&c->aux) != hipSuccess || hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        les_destroy(c);
+        return LES_E_HIP;
+    }
     const size_t F3 = c->plane * c->nlev;
     hipError_t e = hipSuccess;
     for (int f = 0; f < LES_N_FIELDS3D && e == hipSuccess; ++f) e = dalloc(c, &c->f3[f], F3);
@@ -597,8 +651,12 @@ int les_create(const les_params* prm, les_ctx** out) {
 int les_destroy(les_ctx* c) {
     if (!c) return LES_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->aux) (void)hipStreamSynchronize(c->aux);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->cfl_host) (void)hipHostFree(c->cfl_host);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return LES_OK;
@@ -851,6 +909,44 @@ int les_run(les_ctx* c, int jt0, int nsteps, int* failed_jt) {
         done += chunk;
     }
     return LES_OK;
+}
+
+int les_set_avgslice(les_ctx* c, int use_avgslice, int c_count, int p_count) {
+    if (!c || c_count < 1 || p_count < 1) return LES_E_BADARG;
+    const size_t F3 = c->plane * c->nlev;
+    const size_t S = (size_t)LES_N_SLICES * (size_t)c->nx * (c->nz - 1);
+    if (!c->st_run) {
+        hipError_t e = hipSuccess;
+        for (int i = 0; i < 3 && e == hipSuccess; ++i) e = dalloc(c, &c->st_diag[i], F3);
+        if (e == hipSuccess) e = dalloc(c, &c->st_ke, F3);
+        if (e == hipSuccess) e = dalloc(c, &c->st_tmp, F3);
+        if (e == hipSuccess) e = dalloc(c, &c->st_dke, F3);
+        if (e == hipSuccess) e = dalloc(c, &c->st_mean, 3 * c->nlev);
+        double *run = nullptr, *win = nullptr;
+        if (e == hipSuccess) e = dalloc(c, &run, S);
+        if (e == hipSuccess) e = dalloc(c, &win, S);
+        if (e != hipSuccess) return LES_E_NOMEM;
+        c->st_run = run;
+        c->st_win = win;
+        for (int i = 0; i < 3; ++i) fill_levels(c->stream, c->st_diag[i], F3, 0, 1, 0.0);
+        fill_levels(c->stream, c->st_dke, F3, 0, 1, 0.0);
+    }
+    fill_levels(c->stream, c->st_run, S, 0, 1, 0.0);
+    fill_levels(c->stream, c->st_win, S, 0, 1, 0.0);
+    c->st_use = use_avgslice;
+    c->st_c = c_count;
+    c->st_p = p_count;
+    c->st_fr = (1.0 / (double)p_count) * (double)c_count;   // 102_initial.f90:170
+    return finish(c);
+}
+
+int les_get_avgslice(les_ctx* c, int slice, int window, double* host) {
+    if (!c || !host || !c->st_run || slice < 0 || slice >= LES_N_SLICES || (window != 0 && window != 1))
+        return LES_E_BADARG;
+    const size_t L = (size_t)c->nx * (c->nz - 1);
+    les_note(c, hipMemcpyAsync(host, (window ? c->st_win : c->st_run) + slice * L, sizeof(double) * L,
+                               hipMemcpyDeviceToHost, c->stream));
+    return finish(c);
 }
 
 int les_get_cfl(les_ctx* c, double* cfl) {

@@ -52,6 +52,8 @@ struct les_ctx {
     int precision;
     int dev;
     hipStream_t stream;
+    hipStream_t aux;                // second stream (default priority) for the ordered plane sums
+    hipEvent_t ev_fork, ev_join;
     // device fields (FP64), nlev planes each
     double* f3[LES_N_FIELDS3D];
     // work fields in the precision of the current mode (double: f3[id];
@@ -61,6 +63,13 @@ struct les_ctx {
     les::cx<float>* phf;            // FP32 pressure spectrum of the mixed mode
     double* hzm;                    // FP64 plane sums of the z pressure RHS (nlev)
     double* lmean;                  // plane means of u and v (2 nlev), mixed-mode lagrange_Sdep
+    // time-averaged slices (flow_slice / theta_slice) and their diagnostics
+    int st_use, st_c, st_p;
+    double st_fr;
+    double* st_diag[3];             // dudt, dvdt, dwdt (FP64 fields)
+    double *st_ke, *st_tmp, *st_dke;  // ke, filtered ke, one derivative of it
+    double* st_mean;                // plane means of u, v, theta (3 nlev)
+    double *st_run, *st_win;        // LES_N_SLICES slices of nxt x (nzt-1)
     double *ustar, *psi_m, *phi_m, *psi_h, *phi_h;  // one plane each
     double *u_big, *v_big, *w_big;                  // nlev planes of nx2*ny2
     // wavenumber tables (half spectrum, 002_fft.f90:143-167)
@@ -122,7 +131,8 @@ template <typename T>
 void yline_spec(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lhx, long ps,
                 const Twiddle& tw, SpecOp op, const double* G);
 
-// compile-time specialised versions (k_lines_ct.hip); return false when the
+// register-radix versions for the production line lengths (k_lines_rg.hip);
+// return false when the
 // line length has no specialisation
 template <typename T, typename TI>
 bool xline_real_ct(hipStream_t s, const TI* in, T* out, long nrows, int ld_in, int ld_out, const Twiddle& tin,
@@ -158,7 +168,20 @@ void fill_levels(hipStream_t s, T* f, size_t plane, int lev0, int nlev, double v
 template <typename TO, typename TF>
 void convert(hipStream_t s, TO* out, const TF* in, long n);
 void level_means(hipStream_t s, const double* f, size_t plane, int lev0, int nlev, double inv_n, double* out);
+void level_means_seq(hipStream_t s, const double* f, size_t plane, int lev0, int nlev, double nn, double* out);
 void obukhov_dev(les_ctx* c, int jt);
+void obukhov_sums_dev(les_ctx* c, int jt, hipStream_t s);
+void obukhov_finish_dev(les_ctx* c, int jt);
+// run work on the auxiliary stream after everything enqueued so far on the
+// context's stream; join() makes the context's stream wait for it
+inline void fork(les_ctx* c) {
+    les_note(c, hipEventRecord(c->ev_fork, c->stream));
+    les_note(c, hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+}
+inline void join(les_ctx* c) {
+    les_note(c, hipEventRecord(c->ev_join, c->aux));
+    les_note(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+}
 template <typename W> void wallstress_dev(les_ctx* c, int top, const double* u1, const double* v1);
 template <typename W> void strain_dev(les_ctx* c);
 void lag_accumulate_dev(les_ctx* c);
@@ -184,9 +207,18 @@ template <typename W> void press_hz_sums_dev(les_ctx* c, double* out);
 template <typename W> void dpdz_dev(les_ctx* c);
 template <typename W> void rhs_assembly_dev(les_ctx* c, int jt);
 void ab2_dev(les_ctx* c);
-template <typename W> void rhs_minus_gradp_dev(les_ctx* c);
+template <typename W> void rhs_minus_gradp_dev(les_ctx* c, int jt);
 template <typename W> void project_dev(les_ctx* c);
 void max3_dev(les_ctx* c, double* out);
+
+// ---- time-averaged slices (k_stats.hip) -----------------------------------
+// a step samples the slices (and computes the dudt diagnostics) when
+// use_avgslice and mod(jt, c_count) == 0 (103_flosol.f90:231)
+inline bool stats_step(const les_ctx* c, int jt) { return c->st_use && c->st_run && (jt % c->st_c) == 0; }
+template <typename W> void flow_slices_dev(les_ctx* c);
+template <typename W> void theta_slices_dev(les_ctx* c);
+void ke_dev(les_ctx* c, double* ke);
+void add_levels_dev(les_ctx* c, double* d, const double* a, int skip_level1);
 
 }  // namespace les
 

@@ -29,6 +29,17 @@ integer(c_int), parameter, public :: LES_F_U = 0, LES_F_V = 1, LES_F_W = 2, LES_
     LES_F_NU_T = 39, LES_F_U_LAG = 40, LES_F_V_LAG = 41, LES_F_W_LAG = 42, &
     LES_F_SGS_T3 = 56, LES_F_PSI_M_SFC = 101
 
+! time-averaged slice identifiers (enum les_slice): flow_slice then theta_slice
+integer(c_int), parameter, public :: LES_S_U = 0, LES_S_V = 1, LES_S_W = 2, LES_S_P = 3, &
+    LES_S_DUDT = 4, LES_S_DVDT = 5, LES_S_DWDT = 6, LES_S_DUDZ = 7, LES_S_DVDZ = 8, &
+    LES_S_U2 = 9, LES_S_V2 = 10, LES_S_W2 = 11, LES_S_P2 = 12, LES_S_UV = 13, LES_S_UW = 14, &
+    LES_S_VW = 15, LES_S_WP = 16, LES_S_TXX = 17, LES_S_TXY = 18, LES_S_TXZ = 19, LES_S_TYY = 20, &
+    LES_S_TYZ = 21, LES_S_TZZ = 22, LES_S_U3 = 23, LES_S_V3 = 24, LES_S_W3 = 25, &
+    LES_S_U2_FLUC = 26, LES_S_V2_FLUC = 27, LES_S_UV_FLUC = 28, LES_S_U2W_FLUC = 29, &
+    LES_S_V2W_FLUC = 30, LES_S_DISSIP = 31, LES_S_UTXZ_FLUC = 32, LES_S_VTYZ_FLUC = 33, &
+    LES_S_WTZZ = 34, LES_S_CS = 35, LES_S_THETA = 36, LES_S_THETA2 = 37, LES_S_T2_FLUC = 38, &
+    LES_S_DTDZ = 39, LES_S_SGS_T3 = 40, LES_S_WT = 41, LES_S_NU_T = 42, LES_N_SLICES = 43
+
 type, bind(C), public :: les_params
     integer(c_int32_t) :: nxt, nyt, nzt, model, cs_count, dyn_init, inilag, restart_vel, sgs, molec, &
                           walltype, lbc_mom, ubc_mom, ubc, damping_method, coriolis_forcing, &
@@ -43,7 +54,8 @@ public :: les_create, les_destroy, les_set_field, les_get_field, les_step, les_r
           les_sgs_stag, les_convec, les_theta_all_in_one, les_project, les_set_precision, &
           les_fftn, les_ifftn, les_zero_padding, les_zero_unpadding, les_test_filter, &
           les_obukhov, les_wallstress_bottom, les_wallstress_top, les_divstress_uv, &
-          les_divstress_w, les_set_stokes, les_get_cfl, les_check
+          les_divstress_w, les_set_stokes, les_get_cfl, les_set_avgslice, les_get_avgslice, &
+          les_check
 
 interface
     integer(c_int) function les_create(prm, ctx) bind(C, name="les_create")
@@ -196,6 +208,18 @@ interface
         import :: c_int, c_ptr, c_double
         type(c_ptr), value :: ctx
         real(c_double), intent(out) :: cfl
+    end function
+    integer(c_int) function les_set_avgslice(ctx, use_avgslice, c_count, p_count) &
+            bind(C, name="les_set_avgslice")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: use_avgslice, c_count, p_count
+    end function
+    integer(c_int) function les_get_avgslice(ctx, slice, window, host) bind(C, name="les_get_avgslice")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: slice, window
+        real(c_double), intent(out) :: host(*)
     end function
 end interface
 

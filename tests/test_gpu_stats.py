@@ -1,0 +1,80 @@
+"""GPU time-averaged slices against the oracle (flow_slice / theta_slice,
+011_scalars2.f90:28-318), including the dudt diagnostics they read
+(103_flosol.f90:119-213).  Both sides start from the same fields, sample
+every step (c_count = 1) and close a window every 2 steps (p_count = 2);
+after 3 steps the closed window holds steps 1-2 and the running sums step 3.
+
+Bar: FP64 relative L2 per slice <= 1e-12, widened only by two measured
+allowances, as for the state fields: NOISE times the spread of the oracle's
+rounding variants (fused multiply-adds; radix-2 transforms), and
+100 eps kappa, kappa the slice's relative response to a relative
+perturbation of the initial u, v, w, theta (the fluctuation slices subtract
+a plane mean from fields ~1e6 times their fluctuation: theta - mean(theta)).
+Mixed precision: 1e-5.
+"""
+import numpy as np
+import pytest
+
+from overlappingbls_amd import config
+from overlappingbls_amd.abi import LES_MIXED, SLICE
+
+from test_gpu_parity import NOISE, TOL, noise_ctx, pair, rel
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["tiny_ragged", "small", "mid", "wide"]
+
+
+def run_both(case, oracle_mod, gpu_lib, mixed=False):
+    nl, o, g = pair(case, oracle_mod, gpu_lib)
+    of = noise_ctx(oracle_mod, nl.params())
+    of.load(config.initial_fields(nl, seed=1234))
+    if mixed:
+        g.set_precision(LES_MIXED)
+    for ctx in (o, g, of):
+        ctx.set_avgslice(True, 1, 2)
+        for jt in (1, 2, 3):
+            ctx.step(jt)
+    return o, g, of
+
+
+def perturbed_oracle(case, oracle_mod, eps_in):
+    nl = config.workload(case)
+    op = oracle_mod.create(nl.params())
+    f = config.initial_fields(nl, seed=1234)
+    for k, n in enumerate(("U", "V", "W", "THETA")):
+        f[n] = f[n] * (1.0 + eps_in * config.uniform(77, k, f[n].shape))
+    config.load_state(op, f)
+    op.set_avgslice(True, 1, 2)
+    for jt in (1, 2, 3):
+        op.step(jt)
+    return op
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_slices_fp64(case, oracle_mod, gpu_lib):
+    o, g, of = run_both(case, oracle_mod, gpu_lib)
+    eps_in = 1e-10
+    op = perturbed_oracle(case, oracle_mod, eps_in)
+    bad = {}
+    for name in SLICE:
+        for win in (0, 1):
+            ref = o.avgslice(name, win)
+            r = rel(g.avgslice(name, win), ref)
+            kappa = rel(op.avgslice(name, win), ref) / eps_in
+            bound = max(TOL, NOISE * of.spread(lambda c: c.avgslice(name, win), ref), 100 * 2.2e-16 * kappa)
+            if not r < bound:
+                bad[(name, win)] = (r, bound)
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("case", ["tiny_ragged", "mid"])
+def test_slices_mixed(case, oracle_mod, gpu_lib):
+    o, g, _ = run_both(case, oracle_mod, gpu_lib, mixed=True)
+    errs = {}
+    for name in SLICE:
+        ref = o.avgslice(name, 1)
+        errs[name] = rel(g.avgslice(name, 1), ref)
+    print(case, {k: float("%.2g" % v) for k, v in errs.items()})
+    # P and P2 carry the pressure of an FP32 projection; d/dt the FP32 terms
+    assert all(v < 1e-5 for v in errs.values()), errs
