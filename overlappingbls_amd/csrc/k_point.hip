@@ -227,8 +227,10 @@ void obukhov_sums_dev(les_ctx* c, int jt, hipStream_t s) {
     double* t3 = c->f3[LES_F_SGS_T3] + np;
     if (jt == P.theta_init_time && !P.restart_theta)
         fill_levels(s, t3, c->plane, 0, 1, P.wt_s / P.u_scale / P.T_scale);
+    // theta(1) as the previous step left it: a copy taken on the context's
+    // stream, since the step filters theta while these sums may still run
     hipLaunchKernelGGL(k_obukhov_sums, dim3(4), dim3(64), 0, s, u1, v1, c->psi_m, (const double*)t3,
-                       (const double*)(c->f3[LES_F_THETA] + np), np, c->red);
+                       (const double*)c->th1_copy, np, c->red);
 }
 
 // the rest of obukhov from the sums, on the context's stream
@@ -253,7 +255,13 @@ void obukhov_finish_dev(les_ctx* c, int jt) {
                        c->phi_h, (const double*)c->red, np, nn, zo_, c->dz, g_);
 }
 
+void obukhov_theta_copy(les_ctx* c) {
+    les_note(c, hipMemcpyAsync(c->th1_copy, c->f3[LES_F_THETA] + c->plane, c->plane * sizeof(double),
+                               hipMemcpyDeviceToDevice, c->stream));
+}
+
 void obukhov_dev(les_ctx* c, int jt) {
+    obukhov_theta_copy(c);
     obukhov_sums_dev(c, jt, c->stream);
     obukhov_finish_dev(c, jt);
 }

@@ -333,35 +333,53 @@ void convec(les_ctx* c) {
     fill_levels(c->stream, cyf, np, nz, 1, BOGUS);
 }
 
+// theta_all_in_one (010_scalars.f90:111-671) in two parts.  The first needs
+// only theta, so the step runs it before the wall model to overlap the
+// ordered plane sums; the second needs Nu_t (sgs_stag) and the padded
+// velocity (convec).
+bool theta_active(const les_ctx* c, int jt) {
+    return c->prm.theta_flag && jt >= c->prm.theta_init_time;
+}
+
+// filter_data(theta), dTdx, dTdy, dTdz, the theta RHS history (:111-140) and
+// the padded gradients (:466-470); calc_beta's plane means of theta start on
+// the auxiliary stream
 template <typename W>
-void theta_all_in_one(les_ctx* c, int jt) {
-    const les_params& P = c->prm;
+void theta_pre(les_ctx* c, int jt) {
+    if (!theta_active(c, jt)) return;
     const int nz = c->nz;
     const size_t np = c->plane, np2 = c->plane2;
-    if (!(P.theta_flag && jt >= P.theta_init_time)) {
-        fill_levels(c->stream, WF<W>(c, LES_F_BETA_SCAL), np, 0, (int)c->nlev, 0.0);
-        return;
-    }
     double* th = F(c, LES_F_THETA);
     W* dtdx = WF<W>(c, LES_F_DTDX);
     W* dtdy = WF<W>(c, LES_F_DTDY);
     W* dtdz = WF<W>(c, LES_F_DTDZ);
-    W* big[4];
-    for (int i = 0; i < 4; ++i) big[i] = scr<W>(c->big3[i]);
     filter_f<double>(c, th, 0, (int)c->nlev);
-    // calc_beta's plane means of theta (reference order) overlap the transforms
     fork(c);
-    level_means_seq(c->aux, th, np, 0, nz, (double)(c->nx * c->ny), c->red);   // calc_hor_avg order
+    level_means_seq(c->aux, th, np, 0, nz, (double)(c->nx * c->ny), c->th_mean);   // calc_hor_avg order
+    les_note(c, hipEventRecord(c->ev_th, c->aux));
     ddx_f<W, double>(c, dtdx, th, 0, (int)c->nlev);
     ddy_f<W, double>(c, dtdy, th, 0, (int)c->nlev);
     ddz_uv_dev<W, double>(c, dtdz, th);
     theta_prep_dev<W>(c);
-    pad_f<W, W>(c, dtdx + np, big[0] + np2, nz - 1);
-    pad_f<W, W>(c, dtdy + np, big[1] + np2, nz - 1);
-    pad_f<W, W>(c, dtdz + 2 * np, big[2] + 2 * np2, nz - 1);
-    scalar_adv_dev<W>(c, big[0], big[1], big[2], big[3]);
+    pad_f<W, W>(c, dtdx + np, scr<W>(c->tbig[0]) + np2, nz - 1);
+    pad_f<W, W>(c, dtdy + np, scr<W>(c->tbig[1]) + np2, nz - 1);
+    pad_f<W, W>(c, dtdz + 2 * np, scr<W>(c->tbig[2]) + 2 * np2, nz - 1);
+}
+
+// advection, SGS fluxes, calc_beta and step_scalar (:466-671)
+template <typename W>
+void theta_post(les_ctx* c, int jt) {
+    const les_params& P = c->prm;
+    const int nz = c->nz;
+    const size_t np = c->plane, np2 = c->plane2;
+    if (!theta_active(c, jt)) {
+        fill_levels(c->stream, WF<W>(c, LES_F_BETA_SCAL), np, 0, (int)c->nlev, 0.0);
+        return;
+    }
+    W* rm = scr<W>(c->big3[3]);
+    scalar_adv_dev<W>(c, scr<W>(c->tbig[0]), scr<W>(c->tbig[1]), scr<W>(c->tbig[2]), rm);
     W* adv = scr<W>(c->work3[2]);
-    unpad_f<W>(c, big[3] + np2, adv + np, nz - 1);
+    unpad_f<W>(c, rm + np2, adv + np, nz - 1);
     W* temp = scr<W>(c->work3[0]);
     W* dtemp = scr<W>(c->work3[1]);
     scalar_flux_dev<W>(c, temp, 0);
@@ -373,12 +391,18 @@ void theta_all_in_one(les_ctx* c, int jt) {
     scalar_flux_dev<W>(c, temp, 2);
     ddz_w_dev<W, W>(c, dtemp, temp);
     scalar_accum_dev<W>(c, dtemp, (const W*)nullptr);
-    join(c);
-    calc_beta_dev<W>(c, c->red);
+    les_note(c, hipStreamWaitEvent(c->stream, c->ev_th, 0));
+    calc_beta_dev<W>(c, c->th_mean);
     if (jt == P.theta_init_time && !P.restart_theta)
         les_note(c, hipMemcpyAsync(F(c, LES_F_RHST_F), F(c, LES_F_RHST), np * c->nlev * 8, hipMemcpyDeviceToDevice,
                                    c->stream));
     step_scalar_dev(c);
+}
+
+template <typename W>
+void theta_all_in_one(les_ctx* c, int jt) {
+    theta_pre<W>(c, jt);
+    theta_post<W>(c, jt);
 }
 
 // FP64: H spectra in spec3[0..2], p_hat (and d') in spec3[3].  Mixed: FP32
@@ -457,9 +481,12 @@ void step_body(les_ctx* c, int jt) {
     std::swap(c->f3[LES_F_RHSX], c->f3[LES_F_RHSX_F]);
     std::swap(c->f3[LES_F_RHSY], c->f3[LES_F_RHSY_F]);
     std::swap(c->f3[LES_F_RHSZ], c->f3[LES_F_RHSZ_F]);
-    // obukhov's ordered plane sums overlap the horizontal derivatives
+    // obukhov's ordered plane sums overlap the horizontal derivatives and
+    // the theta gradients
+    obukhov_theta_copy(c);
     fork(c);
     obukhov_sums_dev(c, jt, c->aux);
+    les_note(c, hipEventRecord(c->ev_obk, c->aux));
     const int nl = (int)c->nlev;
     const double *u = F(c, LES_F_U), *v = F(c, LES_F_V), *w = F(c, LES_F_W);
     ddx_f<W, double>(c, WF<W>(c, LES_F_DUDX), u, 0, nl);
@@ -471,7 +498,8 @@ void step_body(les_ctx* c, int jt) {
     ddz_uv_dev<W, double>(c, WF<W>(c, LES_F_DUDZ), u);
     ddz_uv_dev<W, double>(c, WF<W>(c, LES_F_DVDZ), v);
     ddz_w_dev<W, double>(c, WF<W>(c, LES_F_DWDZ), w);
-    join(c);
+    theta_pre<W>(c, jt);
+    les_note(c, hipStreamWaitEvent(c->stream, c->ev_obk, 0));
     obukhov_finish_dev(c, jt);
     wallstress<W>(c, 0);
     wallstress<W>(c, 1);
@@ -482,7 +510,7 @@ void step_body(les_ctx* c, int jt) {
     divstress<W>(c, WF<W>(c, LES_F_DIVTY), txy, tyy, tyz, 0);
     divstress<W>(c, WF<W>(c, LES_F_DIVTZ), txz, tyz, tzz, 1);
     convec<W>(c);
-    theta_all_in_one<W>(c, jt);
+    theta_post<W>(c, jt);
     rhs_assembly_dev<W>(c, jt);
     ab2_dev(c);
     press_stag_array<W>(c);
@@ -590,7 +618,9 @@ int les_create(const les_params* prm, les_ctx** out) {
     c->err = hipSuccess;
     if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return LES_E_HIP; }
     if (hipStreamCreate(&c->aux) != hipSuccess || hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_obk, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_th, hipEventDisableTiming) != hipSuccess) {
         les_destroy(c);
         return LES_E_HIP;
     }
@@ -604,6 +634,9 @@ int les_create(const les_params* prm, les_ctx** out) {
     if (e == hipSuccess) e = dalloc(c, &c->v_big, B3);
     if (e == hipSuccess) e = dalloc(c, &c->w_big, B3);
     for (int i = 0; i < 6 && e == hipSuccess; ++i) e = dalloc(c, &c->big3[i], B3);
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = dalloc(c, &c->tbig[i], B3);
+    if (e == hipSuccess) e = dalloc(c, &c->th_mean, c->nlev);
+    if (e == hipSuccess) e = dalloc(c, &c->th1_copy, c->plane);
     for (int i = 0; i < 8 && e == hipSuccess; ++i) e = dalloc(c, &c->work3[i], i == 6 ? (F3 * 3) / 2 + c->plane2 : F3);
     for (int i = 0; i < 4 && e == hipSuccess; ++i) e = dalloc(c, &c->spec3[i], c->ns * c->nlev);
     const size_t np21 = (size_t)21 * c->nz;
@@ -656,6 +689,8 @@ int les_destroy(les_ctx* c) {
     if (c->cfl_host) (void)hipHostFree(c->cfl_host);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->ev_obk) (void)hipEventDestroy(c->ev_obk);
+    if (c->ev_th) (void)hipEventDestroy(c->ev_th);
     if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

@@ -54,6 +54,7 @@ struct les_ctx {
     hipStream_t stream;
     hipStream_t aux;                // second stream (default priority) for the ordered plane sums
     hipEvent_t ev_fork, ev_join;
+    hipEvent_t ev_obk, ev_th;       // obukhov sums / plane means of theta done on aux
     // device fields (FP64), nlev planes each
     double* f3[LES_N_FIELDS3D];
     // work fields in the precision of the current mode (double: f3[id];
@@ -81,6 +82,9 @@ struct les_ctx {
     // work memory
     double* work3[8];          // 8 extra nlev real fields
     double* big3[6];           // 6 padded fields (nlev planes of plane2)
+    double* tbig[3];           // padded theta gradients
+    double* th_mean;           // plane means of theta for calc_beta (nlev)
+    double* th1_copy;          // theta(1) for the obukhov sums (one plane)
     les::cx<double>* spec3[4]; // half-spectrum fields (nlev planes of ns): Hx, Hy, Hz, p_hat
     double* lag_buf;           // lagrange_Sdep: 21 fields x 3 (raw, 2d, 4d)
     les::cx<double>* lag_spec; // 21 spectra
@@ -170,6 +174,7 @@ void convert(hipStream_t s, TO* out, const TF* in, long n);
 void level_means(hipStream_t s, const double* f, size_t plane, int lev0, int nlev, double inv_n, double* out);
 void level_means_seq(hipStream_t s, const double* f, size_t plane, int lev0, int nlev, double nn, double* out);
 void obukhov_dev(les_ctx* c, int jt);
+void obukhov_theta_copy(les_ctx* c);   // theta(1) -> th1_copy on the context's stream
 void obukhov_sums_dev(les_ctx* c, int jt, hipStream_t s);
 void obukhov_finish_dev(les_ctx* c, int jt);
 // run work on the auxiliary stream after everything enqueued so far on the
