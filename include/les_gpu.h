@@ -241,6 +241,53 @@ int les_set_avgslice(les_ctx* ctx, int use_avgslice, int c_count, int p_count);
 /* window 0: running sums; 1: the last closed window. */
 int les_get_avgslice(les_ctx* ctx, int slice, int window, double* host);
 
+/* ---- tangent-linear model and adjoint -------------------------------------- */
+/* No reference counterpart: the reference has no derivative code.  These
+ * differentiate the FP64 step of les_step (flosol, 103_flosol.f90:4-290) with
+ * the Lagrangian SGS coefficients Cs_opt2 and F_LM..F_NN held at their
+ * forward values ("frozen SGS"; Nu_t = Cs_opt2 Delta^2 |S| still varies with
+ * the strain).  The wall model, obukhov's stability functions, advection,
+ * the pressure projection and the scalar with calc_beta are all linearised.
+ * Supported: LES_FP64, theta_flag = 1, lbc_mom = wall, ubc_mom = stress
+ * free, steps jt >= 2 with jt > theta_init_time; else LES_E_BADARG.
+ *
+ * The linear state ("lin") is one tangent or adjoint of the step's state:
+ * LES_F_U, LES_F_V, LES_F_W, LES_F_THETA, LES_F_RHSX, LES_F_RHSY,
+ * LES_F_RHSZ, LES_F_RHST (levels 1..nzt-1; other levels are ignored and
+ * read back as 0) and LES_F_PSI_M_SFC (one plane).  The objective J of the
+ * windows is the mean over the plane and over the last obj_steps steps of
+ * ustar^2 (the surface momentum flux |tau_w|). */
+int les_lin_set(les_ctx* ctx, int field, const double* host, int ld, int lev0, int nlev);
+int les_lin_get(les_ctx* ctx, int field, double* host, int ld, int lev0, int nlev);
+int les_lin_zero(les_ctx* ctx);
+/* Hold Cs_opt2 and F_* (skip lagrange_Sdep/interpolag_Sdep and the
+ * Lagrangian accumulation) in every forward step, so that finite
+ * differences of the forward step see the model the derivative is of. */
+int les_set_sgs_frozen(les_ctx* ctx, int frozen);
+/* Tangent-linear window: the state advances nsteps from jt0 and lin (the
+ * tangent of the state at jt0) becomes the tangent at jt0 + nsteps; *dJ
+ * (may be null) receives the tangent of J. */
+int les_tlm_window(les_ctx* ctx, int jt0, int nsteps, int obj_steps, double* dJ);
+int les_step_tlm(les_ctx* ctx, int jt);
+/* Adjoint window: on entry the context holds the state at jt0 and lin the
+ * adjoint seed y of the state at jt0 + nsteps (zero for J alone).  The
+ * forward runs nsteps steps saving a checkpoint every k_ckpt steps; then,
+ * segment by segment from the last, it recomputes the segment's steps with
+ * tapes and runs their adjoint steps backwards.  On return the state is at
+ * jt0 + nsteps, lin holds d(<y, x(jt0+nsteps)> + J)/dx(jt0) and *J (may be
+ * null) the objective.  Device memory: nsteps/k_ckpt + 1 checkpoints (17
+ * fields each) and k_ckpt tapes (12 fields + 9 padded fields each). */
+int les_adjoint_window(les_ctx* ctx, int jt0, int nsteps, int k_ckpt, int obj_steps, double* J);
+int les_step_adj(les_ctx* ctx, int jt);
+/* Dot-product test of stage `stage` of the linearised step (0 ..
+ * les_lin_nstages()-1) or of the whole step (-1), about forward step jt from
+ * the context's state (which advances one step): lhs = <M dx, y>, rhs =
+ * <dx, M^T y> for seeded random dx, y on the stage's input and output
+ * levels.  The stage's linear buffers are left zero. */
+int les_lin_check(les_ctx* ctx, int stage, int jt, unsigned long long seed, double* lhs, double* rhs);
+int les_lin_nstages(void);
+const char* les_lin_stage_name(int stage);
+
 /* ---- timing support ------------------------------------------------------ */
 /* Kernel families timed by les_time_kernel (no reference counterpart: the
  * reference has no timers; these back the roofline figures). */

@@ -59,6 +59,7 @@ struct les_ctx {
     double *ustar_avg, *psi_m, *phi_m, *psi_h, *phi_h;  /* surface planes */
     double *u_big, *v_big, *w_big;     /* padded velocity kept for theta (convec) */
     int f_lm_mm_init, f_qn_nn_init;
+    int sgs_frozen;
     double cfl;
     int nthreads;
     /* time-averaged slices (011_scalars2.f90) and the flosol diagnostics

@@ -256,6 +256,7 @@ int orc_get_field(les_ctx* c, int id, double* h, int ld, int lev0, int nlev) {
     return LES_OK;
 }
 
+int orc_set_sgs_frozen(les_ctx* c, int frozen) { c->sgs_frozen = frozen ? 1 : 0; return LES_OK; }
 int orc_set_sgs_init_flags(les_ctx* c, int a, int b) { c->f_lm_mm_init = a; c->f_qn_nn_init = b; return LES_OK; }
 int orc_get_sgs_init_flags(les_ctx* c, int* a, int* b) { *a = c->f_lm_mm_init; *b = c->f_qn_nn_init; return LES_OK; }
 int orc_get_cfl(les_ctx* c, double* cfl) { *cfl = c->cfl; return LES_OK; }
@@ -770,8 +771,9 @@ int orc_sgs_stag(les_ctx* c, int jt) {
             S11[o] = ux; S12[o] = 0.5 * (uy + vx); S13[o] = 0.5 * (uz + wx);
             S22[o] = vy; S23[o] = 0.5 * (vz + wy); S33[o] = wz;
         }
-    /* u_lag += u (:151-155) */
-    {
+    /* u_lag += u (:151-155); skipped with the SGS frozen (no reference
+     * counterpart: holds Cs_opt2 and F_* for derivative checks) */
+    if (!c->sgs_frozen) {
         double *ul = F(c, LES_F_U_LAG), *vl = F(c, LES_F_V_LAG), *wl = F(c, LES_F_W_LAG);
         const double *u = F(c, LES_F_U), *v = F(c, LES_F_V), *w = F(c, LES_F_W);
         for (size_t q = 0; q < np * nl; ++q) { ul[q] = ul[q] + u[q]; vl[q] = vl[q] + v[q]; wl[q] = wl[q] + w[q]; }
@@ -780,7 +782,7 @@ int orc_sgs_stag(les_ctx* c, int jt) {
     double* Cs = F(c, LES_F_CS_OPT2);
     if (jt == 1 && P->inilag) {
         for (size_t q = np; q < np * nl; ++q) Cs[q] = 0.03;
-    } else if ((jt >= P->dyn_init || P->restart_vel) && (jt % P->cs_count) == 0) {
+    } else if (!c->sgs_frozen && (jt >= P->dyn_init || P->restart_vel) && (jt % P->cs_count) == 0) {
         lagrange_sdep(c, jt);
     }
     /* |S| and Nu_t (:206-216), l = delta */

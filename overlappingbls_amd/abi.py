@@ -136,6 +136,16 @@ def bind(lib: C.CDLL, prefix: str):
         "time_kernel": [P, i, i, C.POINTER(C.c_double), C.POINTER(C.c_double)],
         "set_avgslice": [P, i, i, i],
         "get_avgslice": [P, i, i, _PD],
+        "lin_set": [P, i, _PD, i, i, i],
+        "lin_get": [P, i, _PD, i, i, i],
+        "lin_zero": [P],
+        "set_sgs_frozen": [P, i],
+        "tlm_window": [P, i, i, i, C.POINTER(C.c_double)],
+        "step_tlm": [P, i],
+        "adjoint_window": [P, i, i, i, i, C.POINTER(C.c_double)],
+        "step_adj": [P, i],
+        "lin_check": [P, i, i, C.c_ulonglong, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+        "lin_nstages": [],
     }
     fns = {}
     for name, argtypes in sig.items():
@@ -145,6 +155,11 @@ def bind(lib: C.CDLL, prefix: str):
         f.argtypes = argtypes
         f.restype = C.c_int
         fns[name] = f
+    f = getattr(lib, prefix + "lin_stage_name", None)
+    if f is not None:
+        f.argtypes = [i]
+        f.restype = C.c_char_p
+        fns["lin_stage_name"] = f
     return fns
 
 
@@ -333,3 +348,54 @@ class LesContext:
     def sync(self):
         if "sync" in self._f:
             self._call("sync", self._h)
+
+    # ---- tangent-linear model and adjoint (FP64, frozen SGS) -------------
+    def lin_set(self, name, arr, lev0=0):
+        """Set the linear state (tangent or adjoint seed) of U, V, W, THETA,
+        RHSX, RHSY, RHSZ, RHST (levels) or PSI_M_SFC (one plane)."""
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if name in SURFACE:
+            a = a.reshape(1, self.ny, self.nx)
+            self._call("lin_set", self._h, FIELD[name], _pd(a), self.nx, 0, 1)
+            return
+        a = a.reshape(-1, self.ny, self.nx)
+        self._call("lin_set", self._h, FIELD[name], _pd(a), self.nx, lev0, a.shape[0])
+
+    def lin_get(self, name, lev0=0, nlev=None):
+        if name in SURFACE:
+            out = np.empty((self.ny, self.nx))
+            self._call("lin_get", self._h, FIELD[name], _pd(out), self.nx, 0, 1)
+            return out
+        if nlev is None:
+            nlev = self.nlev - lev0
+        out = np.empty((nlev, self.ny, self.nx))
+        self._call("lin_get", self._h, FIELD[name], _pd(out), self.nx, lev0, nlev)
+        return out
+
+    def lin_zero(self):
+        self._call("lin_zero", self._h)
+
+    def set_sgs_frozen(self, frozen=True):
+        self._call("set_sgs_frozen", self._h, int(bool(frozen)))
+
+    def tlm_window(self, jt0, nsteps, obj_steps=0):
+        """Advance the state and the tangent nsteps; returns the tangent of J."""
+        dj = C.c_double(0.0)
+        self._call("tlm_window", self._h, int(jt0), int(nsteps), int(obj_steps), C.byref(dj))
+        return dj.value
+
+    def adjoint_window(self, jt0, nsteps, k_ckpt, obj_steps=0):
+        """Adjoint of the window (checkpoint every k_ckpt); returns J."""
+        j = C.c_double(0.0)
+        self._call("adjoint_window", self._h, int(jt0), int(nsteps), int(k_ckpt), int(obj_steps), C.byref(j))
+        return j.value
+
+    def lin_check(self, stage, jt, seed=1):
+        """(<M dx, y>, <dx, M^T y>) of one stage (-1: the whole step)."""
+        a, b = C.c_double(), C.c_double()
+        self._call("lin_check", self._h, int(stage), int(jt), C.c_ulonglong(seed), C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def lin_stages(self):
+        n = self._f["lin_nstages"]()
+        return [self._f["lin_stage_name"](k).decode() for k in range(n)]

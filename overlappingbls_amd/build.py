@@ -10,7 +10,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "liblesgpu.so")
-SOURCES = ["k_lines.hip", "k_lines_rg.hip", "k_tdma.hip", "k_point.hip", "k_stats.hip", "les_api.hip"]
+SOURCES = [
+    "k_lines.hip", "k_lines_rg.hip", "k_tdma.hip", "k_point.hip", "k_stats.hip", "k_lin.hip", "les_api.hip",
+    "les_lin.hip",
+]
 ARCH = os.environ.get("LES_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 

@@ -962,12 +962,17 @@ __global__ void k_press_prep(W* rx, W* ry, W* rz, const double* __restrict__ u, 
 }
 
 template <typename W>
-void press_rhs_prep_dev(les_ctx* c, W* rx, W* ry, W* rz) {
+void press_rhs_prep_on(les_ctx* c, W* rx, W* ry, W* rz, const double* u, const double* v, const double* w,
+                       const W* divtz) {
     KP p = kp(c);
-    hipLaunchKernelGGL(k_press_prep<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, rx, ry, rz,
-                       (const double*)c->f3[LES_F_U], (const double*)c->f3[LES_F_V], (const double*)c->f3[LES_F_W],
-                       (const W*)WF<W>(c, LES_F_DIVTZ), p, 1.0 / (double)(c->nx * c->ny),
-                       (int)(sizeof(W) < sizeof(double)));
+    hipLaunchKernelGGL(k_press_prep<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, rx, ry, rz, u, v, w,
+                       divtz, p, 1.0 / (double)(c->nx * c->ny), (int)(sizeof(W) < sizeof(double)));
+}
+
+template <typename W>
+void press_rhs_prep_dev(les_ctx* c, W* rx, W* ry, W* rz) {
+    press_rhs_prep_on<W>(c, rx, ry, rz, c->f3[LES_F_U], c->f3[LES_F_V], c->f3[LES_F_W],
+                         (const W*)WF<W>(c, LES_F_DIVTZ));
 }
 
 // Plane sums of the z right-hand side of the pressure equation in FP64, one
@@ -1013,10 +1018,14 @@ __global__ void k_dpdz(W* dpdz, const W* __restrict__ pr, KP p) {
 }
 
 template <typename W>
-void dpdz_dev(les_ctx* c) {
+void dpdz_on(les_ctx* c, W* dpdz, const W* pr) {
     KP p = kp(c);
-    hipLaunchKernelGGL(k_dpdz<W>, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, WF<W>(c, LES_F_DPDZ),
-                       (const W*)WF<W>(c, LES_F_P), p);
+    hipLaunchKernelGGL(k_dpdz<W>, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, dpdz, pr, p);
+}
+
+template <typename W>
+void dpdz_dev(les_ctx* c) {
+    dpdz_on<W>(c, WF<W>(c, LES_F_DPDZ), (const W*)WF<W>(c, LES_F_P));
 }
 
 // ---- RHS assembly and AB2, 103_flosol.f90:115-162, 319-389 -----------------
@@ -1256,6 +1265,9 @@ void max3_dev(les_ctx* c, double* out) {
     template void scalar_accum_dev<W>(les_ctx*, const W*, const W*);                                        \
     template void calc_beta_dev<W>(les_ctx*, const double*);                                                \
     template void press_rhs_prep_dev<W>(les_ctx*, W*, W*, W*);                                              \
+    template void press_rhs_prep_on<W>(les_ctx*, W*, W*, W*, const double*, const double*, const double*,   \
+                                       const W*);                                                           \
+    template void dpdz_on<W>(les_ctx*, W*, const W*);                                                       \
     template void press_hz_sums_dev<W>(les_ctx*, double*);                                                  \
     template void dpdz_dev<W>(les_ctx*);                                                                    \
     template void rhs_assembly_dev<W>(les_ctx*, int);                                                       \

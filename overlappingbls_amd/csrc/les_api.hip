@@ -8,9 +8,53 @@
 #include <cstring>
 #include <utility>
 #include <vector>
-#include "les_internal.hpp"
+#include "les_ops.hpp"
 
 using namespace les;
+
+namespace les {
+
+// press_stag_array (109_press_stag_array.f90:104-238) on explicit fields:
+// (u, v, w, divtz(1), divtz(nz)) -> p (levels 0..nz-1), dp/dx, dp/dy, dp/dz
+// (levels 1..nz-1).  Linear in its inputs; the linearised step calls it on
+// tangent fields.
+template <typename W>
+void press_solve(les_ctx* c, const double* u, const double* v, const double* w, const W* divtz, W* pf, W* dpdx,
+                 W* dpdy, W* dpdz) {
+    const int nz = c->nz;
+    const size_t np = c->plane, ns = c->ns;
+    W *rx = scr<W>(c->work3[0]), *ry = scr<W>(c->work3[1]), *rz = scr<W>(c->work3[2]);
+    press_rhs_prep_on<W>(c, rx, ry, rz, u, v, w, divtz);
+    cx<W>*Hx = scr<W>(c->spec3[0]), *Hy = scr<W>(c->spec3[1]), *Hz = scr<W>(c->spec3[2]);
+    const long rows = (long)(nz + 1) * c->ny;
+    x_r2c<W>(c->stream, rx, Hx, rows, c->nx, c->lhx, c->tw_x, 1.0);
+    x_r2c<W>(c->stream, ry, Hy, rows, c->nx, c->lhx, c->tw_x, 1.0);
+    x_r2c<W>(c->stream, rz, Hz, rows, c->nx, c->lhx, c->tw_x, 1.0);
+    yline_spec<W>(c->stream, Hx, Hx, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
+    yline_spec<W>(c->stream, Hy, Hy, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
+    yline_spec<W>(c->stream, Hz, Hz, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
+    cx<W>* ph;
+    const double* hzm = nullptr;
+    if (sizeof(W) == sizeof(double)) {
+        ph = scr<W>(c->spec3[3]);
+    } else {
+        ph = reinterpret_cast<cx<W>*>(c->phf);
+        press_hz_sums_dev<W>(c, c->hzm);
+        hzm = c->hzm;
+    }
+    tdma_press<W>(c, Hx, Hy, Hz, ph, reinterpret_cast<double*>(c->spec3[3]), hzm);
+    yline_spec<W>(c->stream, ph, ph, nz, c->lhx, (long)ns, c->tw_y, SP_INV, nullptr);
+    x_c2r<W>(c->stream, ph, pf, (long)nz * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
+    x_c2r<W>(c->stream, ph + ns, dpdx + np, (long)(nz - 1) * c->ny, c->lhx, c->nx, c->tw_x, 1, kfx(c));
+    ddy_f<W, W>(c, dpdy, pf, 1, nz - 1);
+    dpdz_on<W>(c, dpdz, pf);
+}
+template void press_solve<double>(les_ctx*, const double*, const double*, const double*, const double*, double*,
+                                  double*, double*, double*);
+template void press_solve<float>(les_ctx*, const double*, const double*, const double*, const float*, float*, float*,
+                                 float*, float*);
+
+}  // namespace les
 
 namespace {
 
@@ -140,8 +184,6 @@ std::vector<double> sponge_profile(const les_ctx* c) {
     return sp;
 }
 
-double* F(les_ctx* c, int id) { return c->f3[id]; }
-
 // Work fields: recomputed inside every step, stored in the precision of the
 // mode (FP32 in LES_MIXED).  Every other 3-D field is FP64 state.
 bool is_work_field(int id) {
@@ -159,16 +201,6 @@ FieldRef field_ref(les_ctx* c, int id) {
     return {c->f3[id], nullptr};
 }
 
-// scratch memory (allocated for FP64) viewed in precision W
-template <typename W>
-W* scr(double* p) {
-    return reinterpret_cast<W*>(p);
-}
-template <typename W>
-cx<W>* scr(cx<double>* p) {
-    return reinterpret_cast<cx<W>*>(p);
-}
-
 double* surf(les_ctx* c, int id) {
     switch (id) {
         case LES_F_USTAR_SFC: return c->ustar;
@@ -180,63 +212,8 @@ double* surf(les_ctx* c, int id) {
     }
 }
 
-int finish(les_ctx* c) {
-    les_note(c, hipStreamSynchronize(c->aux));
-    hipError_t e = hipStreamSynchronize(c->stream);
-    les_note(c, e);
-    les_note(c, hipGetLastError());
-    const hipError_t first = c->err;
-    c->err = hipSuccess;
-    return first == hipSuccess ? LES_OK : LES_E_HIP;
-}
 
 bool valid_field(int f) { return f >= 0 && f < LES_N_FIELDS3D; }
-
-// ---- transform building blocks on device fields -------------------------------
-double kfx(const les_ctx* c) { return 2.0 * PI / c->prm.lx_tot; }
-double kfy(const les_ctx* c) { return 2.0 * PI / c->prm.ly_tot; }
-
-template <typename W, typename TI>
-void ddx_f(les_ctx* c, W* d, const TI* f, int lev0, int nlev) {
-    xline_real<W, TI>(c->stream, f + (size_t)lev0 * c->plane, d + (size_t)lev0 * c->plane, (long)nlev * c->ny, c->nx,
-                      c->nx, c->tw_x, c->tw_x, OP_DERIV, kfx(c));
-}
-template <typename W, typename TI>
-void ddy_f(les_ctx* c, W* d, const TI* f, int lev0, int nlev) {
-    yline_real<W, TI>(c->stream, f + (size_t)lev0 * c->plane, d + (size_t)lev0 * c->plane, nlev, c->nx, c->nx, c->nx,
-                      (long)c->plane, (long)c->plane, c->tw_y, c->tw_y, OP_DERIV, kfy(c));
-}
-template <typename T>
-void filter_f(les_ctx* c, T* f, int lev0, int nlev) {
-    T* p = f + (size_t)lev0 * c->plane;
-    xline_real<T>(c->stream, p, p, (long)nlev * c->ny, c->nx, c->nx, c->tw_x, c->tw_x, OP_NYQ, 0.0);
-    yline_real<T>(c->stream, p, p, nlev, c->nx, c->nx, c->nx, (long)c->plane, (long)c->plane, c->tw_y, c->tw_y, OP_NYQ,
-                  0.0);
-}
-// zero_padding of nlev planes: src (nx x ny planes) -> dst (nx2 x ny2 planes)
-template <typename W, typename TI>
-void pad_f(les_ctx* c, const TI* src, W* dst, int nlev) {
-    W* tmp = scr<W>(c->work3[6]);  // >= nlev * ny * nx2
-    xline_real<W, TI>(c->stream, src, tmp, (long)nlev * c->ny, c->nx, c->nx2, c->tw_x, c->tw_x2, OP_PAD, 0.0);
-    yline_real<W>(c->stream, tmp, dst, nlev, c->nx2, c->nx2, c->nx2, (long)c->ny * c->nx2, (long)c->plane2, c->tw_y,
-                  c->tw_y2, OP_PAD, 0.0);
-}
-// zero_unpadding of nlev planes: src (nx2 x ny2) -> dst (nx x ny)
-template <typename W>
-void unpad_f(les_ctx* c, const W* src, W* dst, int nlev) {
-    W* tmp = scr<W>(c->work3[6]);  // >= nlev * ny2 * nx
-    xline_real<W>(c->stream, src, tmp, (long)nlev * c->ny2, c->nx2, c->nx, c->tw_x2, c->tw_x, OP_UNPAD, 0.0);
-    yline_real<W>(c->stream, tmp, dst, nlev, c->nx, c->nx, c->nx, (long)c->ny2 * c->nx, (long)c->plane, c->tw_y2,
-                  c->tw_y, OP_UNPAD, 0.0);
-}
-// test_filter of nplanes consecutive planes in place
-template <typename T>
-void test_filter_f(les_ctx* c, T* p, int nplanes, const double* G) {
-    cx<T>* sp = scr<T>(c->spec3[2]);  // >= nplanes planes of ns
-    x_r2c<T>(c->stream, p, sp, (long)nplanes * c->ny, c->nx, c->lhx, c->tw_x, 1.0);
-    yline_spec<T>(c->stream, sp, sp, nplanes, c->lhx, (long)c->ns, c->tw_y, SP_FILT, G);
-    x_c2r<T>(c->stream, sp, p, (long)nplanes * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
-}
 
 // ---- routines ---------------------------------------------------------------
 // The wall model's filtered first-level velocity is FP64 in both modes.
@@ -287,9 +264,9 @@ template <typename W>
 void sgs_stag(les_ctx* c, int jt) {
     const les_params& P = c->prm;
     strain_dev<W>(c);
-    lag_accumulate_dev(c);
+    if (!c->sgs_frozen) lag_accumulate_dev(c);
     if (jt == 1 && P.inilag) cs_init_dev(c);
-    else if ((jt >= P.dyn_init || P.restart_vel) && (jt % P.cs_count) == 0) lagrange<W>(c, jt);
+    else if (!c->sgs_frozen && (jt >= P.dyn_init || P.restart_vel) && (jt % P.cs_count) == 0) lagrange<W>(c, jt);
     nu_t_stress_dev<W>(c);
 }
 
@@ -368,7 +345,7 @@ void theta_pre(les_ctx* c, int jt) {
 
 // advection, SGS fluxes, calc_beta and step_scalar (:466-671)
 template <typename W>
-void theta_post(les_ctx* c, int jt) {
+void theta_post(les_ctx* c, int jt, Tape* tape = nullptr) {
     const les_params& P = c->prm;
     const int nz = c->nz;
     const size_t np = c->plane, np2 = c->plane2;
@@ -393,6 +370,7 @@ void theta_post(les_ctx* c, int jt) {
     scalar_accum_dev<W>(c, dtemp, (const W*)nullptr);
     les_note(c, hipStreamWaitEvent(c->stream, c->ev_th, 0));
     calc_beta_dev<W>(c, c->th_mean);
+    if (tape) tape_save(c, tape, TAPE_THETA);
     if (jt == P.theta_init_time && !P.restart_theta)
         les_note(c, hipMemcpyAsync(F(c, LES_F_RHST_F), F(c, LES_F_RHST), np * c->nlev * 8, hipMemcpyDeviceToDevice,
                                    c->stream));
@@ -410,34 +388,8 @@ void theta_all_in_one(les_ctx* c, int jt) {
 // mean mode from FP64 plane sums.
 template <typename W>
 void press_stag_array(les_ctx* c) {
-    const int nz = c->nz;
-    const size_t np = c->plane, ns = c->ns;
-    W *rx = scr<W>(c->work3[0]), *ry = scr<W>(c->work3[1]), *rz = scr<W>(c->work3[2]);
-    press_rhs_prep_dev<W>(c, rx, ry, rz);
-    cx<W>*Hx = scr<W>(c->spec3[0]), *Hy = scr<W>(c->spec3[1]), *Hz = scr<W>(c->spec3[2]);
-    const long rows = (long)(nz + 1) * c->ny;
-    x_r2c<W>(c->stream, rx, Hx, rows, c->nx, c->lhx, c->tw_x, 1.0);
-    x_r2c<W>(c->stream, ry, Hy, rows, c->nx, c->lhx, c->tw_x, 1.0);
-    x_r2c<W>(c->stream, rz, Hz, rows, c->nx, c->lhx, c->tw_x, 1.0);
-    yline_spec<W>(c->stream, Hx, Hx, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
-    yline_spec<W>(c->stream, Hy, Hy, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
-    yline_spec<W>(c->stream, Hz, Hz, nz + 1, c->lhx, (long)ns, c->tw_y, SP_FWD, nullptr);
-    cx<W>* ph;
-    const double* hzm = nullptr;
-    if (sizeof(W) == sizeof(double)) {
-        ph = scr<W>(c->spec3[3]);
-    } else {
-        ph = reinterpret_cast<cx<W>*>(c->phf);
-        press_hz_sums_dev<W>(c, c->hzm);
-        hzm = c->hzm;
-    }
-    tdma_press<W>(c, Hx, Hy, Hz, ph, reinterpret_cast<double*>(c->spec3[3]), hzm);
-    W* pf = WF<W>(c, LES_F_P);
-    yline_spec<W>(c->stream, ph, ph, nz, c->lhx, (long)ns, c->tw_y, SP_INV, nullptr);
-    x_c2r<W>(c->stream, ph, pf, (long)nz * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
-    x_c2r<W>(c->stream, ph + ns, WF<W>(c, LES_F_DPDX) + np, (long)(nz - 1) * c->ny, c->lhx, c->nx, c->tw_x, 1, kfx(c));
-    ddy_f<W, W>(c, WF<W>(c, LES_F_DPDY), pf, 1, nz - 1);
-    dpdz_dev<W>(c);
+    press_solve<W>(c, F(c, LES_F_U), F(c, LES_F_V), F(c, LES_F_W), WF<W>(c, LES_F_DIVTZ), WF<W>(c, LES_F_P),
+                   WF<W>(c, LES_F_DPDX), WF<W>(c, LES_F_DPDY), WF<W>(c, LES_F_DPDZ));
 }
 
 // energy and d(ke)/dx_i added to the dudt diagnostics (103_flosol.f90:192-213,
@@ -476,7 +428,7 @@ void avg_slices(les_ctx* c, int jt) {
 }
 
 template <typename W>
-void step_body(les_ctx* c, int jt) {
+void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     // newstep: RHS_f = RHS (103_flosol.f90:311-313) as a pointer exchange
     std::swap(c->f3[LES_F_RHSX], c->f3[LES_F_RHSX_F]);
     std::swap(c->f3[LES_F_RHSY], c->f3[LES_F_RHSY_F]);
@@ -501,7 +453,9 @@ void step_body(les_ctx* c, int jt) {
     theta_pre<W>(c, jt);
     les_note(c, hipStreamWaitEvent(c->stream, c->ev_obk, 0));
     obukhov_finish_dev(c, jt);
+    if (tape) tape_save(c, tape, TAPE_OBUKHOV);
     wallstress<W>(c, 0);
+    if (tape) tape_save(c, tape, TAPE_WALL);
     wallstress<W>(c, 1);
     sgs_stag<W>(c, jt);
     W *txx = WF<W>(c, LES_F_TXX), *txy = WF<W>(c, LES_F_TXY), *txz = WF<W>(c, LES_F_TXZ);
@@ -510,18 +464,19 @@ void step_body(les_ctx* c, int jt) {
     divstress<W>(c, WF<W>(c, LES_F_DIVTY), txy, tyy, tyz, 0);
     divstress<W>(c, WF<W>(c, LES_F_DIVTZ), txz, tyz, tzz, 1);
     convec<W>(c);
-    theta_post<W>(c, jt);
+    theta_post<W>(c, jt, tape);
     rhs_assembly_dev<W>(c, jt);
     ab2_dev(c);
     press_stag_array<W>(c);
     rhs_minus_gradp_dev<W>(c, jt);
     project_dev<W>(c);
-    const bool st = stats_step(c, jt);
+    const bool st = stats && stats_step(c, jt);
     if (st) diagnostics_ke(c);
     filter_f<double>(c, F(c, LES_F_U), 0, nl);
     filter_f<double>(c, F(c, LES_F_V), 0, nl);
     filter_f<double>(c, F(c, LES_F_W), 0, nl);
     if (st) avg_slices<W>(c, jt);
+    if (tape) tape_save(c, tape, TAPE_END);
 }
 
 bool mixed(const les_ctx* c) { return c->precision == LES_MIXED; }
@@ -584,6 +539,18 @@ int divstress_entry(les_ctx* c, int divt, int tx, int ty, int tz, int wver) {
 }
 
 }  // namespace
+
+void les::step_fp64(les_ctx* c, int jt, Tape* tape, bool stats) { step_body<double>(c, jt, tape, stats); }
+
+int les::finish(les_ctx* c) {
+    les_note(c, hipStreamSynchronize(c->aux));
+    hipError_t e = hipStreamSynchronize(c->stream);
+    les_note(c, e);
+    les_note(c, hipGetLastError());
+    const hipError_t first = c->err;
+    c->err = hipSuccess;
+    return first == hipSuccess ? LES_OK : LES_E_HIP;
+}
 
 // ============================================================================
 // C-ABI
@@ -685,6 +652,7 @@ int les_destroy(les_ctx* c) {
     if (!c) return LES_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->aux) (void)hipStreamSynchronize(c->aux);
+    lin_destroy(c);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->cfl_host) (void)hipHostFree(c->cfl_host);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);

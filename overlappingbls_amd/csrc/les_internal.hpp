@@ -42,6 +42,22 @@ struct Twiddle {
 
 struct Scalars;  // device-side per-step scalars (obukhov)
 
+// Forward values of one step that its tangent-linear and adjoint read (FP64).
+struct Tape {
+    double* obk;                 // obukhov plane sums [4]
+    double *u1, *v1;             // u, v at level 1 as the step found them
+    double* uv1f;                // test-filtered u, v at level 1 (2 planes)
+    double *thf, *tmean;         // filtered theta (before step_scalar), its plane means
+    double* S[6];                // S11 S12 S13 S22 S23 S33
+    double *nu, *cs;             // Nu_t, Cs_opt2
+    double* dT[3];               // dTdx, dTdy, dTdz
+    double *ub, *vb, *wb;        // padded u, v, w
+    double* ob[3];               // padded vorticity
+    double* tb[3];               // padded theta gradients
+};
+enum TapePoint : int { TAPE_OBUKHOV = 0, TAPE_WALL = 1, TAPE_THETA = 2, TAPE_END = 3 };
+struct LinCtx;
+
 }  // namespace les
 
 struct les_ctx {
@@ -95,6 +111,8 @@ struct les_ctx {
     double* cfl_host;          // pinned
     int cfl_cap;
     int f_lm_mm_init, f_qn_nn_init;
+    int sgs_frozen;            // skip the Lagrangian SGS update (Cs_opt2, F_* held)
+    les::LinCtx* lin;          // linearised-step buffers (les_lin.hip), or null
     double cfl;
     hipError_t err;            // first asynchronous HIP error since the last finish()
     std::vector<void*> allocs;
@@ -208,6 +226,10 @@ template <typename W> void scalar_accum_dev(les_ctx* c, const W* dtemp, const W*
 template <typename W> void calc_beta_dev(les_ctx* c, const double* means);
 void step_scalar_dev(les_ctx* c);
 template <typename W> void press_rhs_prep_dev(les_ctx* c, W* rx, W* ry, W* rz);
+template <typename W>
+void press_rhs_prep_on(les_ctx* c, W* rx, W* ry, W* rz, const double* u, const double* v, const double* w,
+                       const W* divtz);
+template <typename W> void dpdz_on(les_ctx* c, W* dpdz, const W* p);
 template <typename W> void press_hz_sums_dev(les_ctx* c, double* out);
 template <typename W> void dpdz_dev(les_ctx* c);
 template <typename W> void rhs_assembly_dev(les_ctx* c, int jt);
@@ -215,6 +237,16 @@ void ab2_dev(les_ctx* c);
 template <typename W> void rhs_minus_gradp_dev(les_ctx* c, int jt);
 template <typename W> void project_dev(les_ctx* c);
 void max3_dev(les_ctx* c, double* out);
+
+// ---- forward step and the linearised step (les_api.hip, les_lin.hip) ---------
+template <typename W>
+void press_solve(les_ctx* c, const double* u, const double* v, const double* w, const W* divtz, W* pf, W* dpdx,
+                 W* dpdy, W* dpdz);
+// one FP64 flosol step; tape != null records the values the linearised step
+// reads; stats = false skips the time-averaged slices
+void step_fp64(les_ctx* c, int jt, Tape* tape, bool stats);
+void tape_save(les_ctx* c, Tape* t, int point);
+void lin_destroy(les_ctx* c);
 
 // ---- time-averaged slices (k_stats.hip) -----------------------------------
 // a step samples the slices (and computes the dudt diagnostics) when
