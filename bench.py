@@ -17,7 +17,8 @@ roofline: the dominant kernel family (by device time in the step) timed live
 with HIP events on the library's stream; `achieved` = algorithmic bytes per
 launch / launch time.  cpu_baseline: the FP64 CPU restatement (oracle) on the
 host cores, two steps of the same 256^3 workload (one plain, one dynamic-SGS
-update) weighted 4:1 as in a cs_count = 5 cycle.
+update) weighted 4:1 as in a cs_count = 5 cycle.  adjoint: the discrete
+adjoint of the FP64 step at 128^3 (see adjoint_leg).
 """
 from __future__ import annotations
 
@@ -43,7 +44,129 @@ def parse():
     ap.add_argument("--profile-only", action="store_true", help="run the timed steps only (for rocprofv3)")
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both",
                     help="which step(s) to time (profile runs usually pick one)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint leg's timed windows (for rocprofv3)")
     return ap.parse_args()
+
+
+ADJ_STATE = ("U", "V", "W", "THETA", "RHSX", "RHSY", "RHSZ", "RHST")
+
+
+def adjoint_leg(workload="sbl128", nsteps=20, k_ckpt=5, obj_steps=5, ndir=8, eps=1e-6, timed_only=False):
+    """Discrete adjoint at 128^3 (FP64, SGS frozen): a 20-step window with a
+    checkpoint every 5 steps, J = mean surface momentum flux ustar^2 over the
+    last 5 steps, gradient with respect to the whole initial state.  Reports
+    the window times against the plain forward window, the tangent/adjoint
+    dot-product test over the window, and central differences of J
+    (eps = 1e-6 relative to each field's rms) along 8 seeded Gaussian
+    directions against <grad J, d>."""
+    import numpy as np
+    from overlappingbls_amd import config, lib
+    nl = config.workload(workload)
+    p = nl.params()
+    nz, jt0 = p.nzt, 3
+    g = lib.create(p)
+    config.load_state(g, config.initial_fields(nl, seed=2025))
+    g.run(1, jt0 - 1)
+    g.set_sgs_frozen(True)
+    x0 = {n: g.get(n) for n in config.STATE_FIELDS}
+    x0["PSI_M_SFC"] = g.get("PSI_M_SFC")
+    flags = g.sgs_init_flags()
+
+    def reset(d=None, e=0.0):
+        for n, a in x0.items():
+            if d is not None and n in d:
+                a = a.copy()
+                if n == "PSI_M_SFC":
+                    a = a + e * d[n]
+                else:
+                    a[1:nz] += e * d[n]
+            g.set(n, a)
+        g.set_sgs_init_flags(*flags)
+
+    def direction(seed, scale):
+        d = {n: scale[n] * config.gaussian(seed, 60 + i, (nz - 1, p.nyt, p.nxt)) for i, n in enumerate(ADJ_STATE)}
+        d["PSI_M_SFC"] = scale["PSI_M_SFC"] * np.ones((p.nyt, p.nxt)) * float(config.gaussian(seed, 70, (1,))[0])
+        return d
+
+    def lin_load(d):
+        g.lin_zero()
+        for n, a in d.items():
+            if n == "PSI_M_SFC":
+                g.lin_set(n, a)
+            else:
+                g.lin_set(n, a, lev0=1)
+
+    def lin_read():
+        out = {n: g.lin_get(n, lev0=1, nlev=nz - 1) for n in ADJ_STATE}
+        out["PSI_M_SFC"] = g.lin_get("PSI_M_SFC")
+        return out
+
+    def inner(a, b):
+        return sum(float(np.sum(a[n] * b[n])) for n in a)
+
+    def timed(fn):
+        g.sync()
+        t0 = time.perf_counter()
+        r = fn()
+        g.sync()
+        return time.perf_counter() - t0, r
+
+    unit = {n: 1.0 for n in ADJ_STATE}
+    unit["PSI_M_SFC"] = 0.1
+    # warm: allocates the linear buffers, tapes and checkpoints
+    reset(); lin_load(direction(1, unit)); g.adjoint_window(jt0, nsteps, k_ckpt, obj_steps)
+    reset(); t_fwd, _ = timed(lambda: g.run(jt0, nsteps))
+    dx, y = direction(2, unit), direction(3, unit)
+    reset(); lin_load(dx)
+    t_tlm, dJ = timed(lambda: g.tlm_window(jt0, nsteps, obj_steps))
+    tl = lin_read()
+    reset(); lin_load(y)
+    t_adj, J = timed(lambda: g.adjoint_window(jt0, nsteps, k_ckpt, obj_steps))
+    grad_y = lin_read()
+    res = {
+        "workload": f"{workload}: {p.nxt}x{p.nyt}x{nz - 1}, FP64, SGS frozen, window {nsteps} steps from jt={jt0}, "
+                    f"checkpoint every {k_ckpt}, J = mean ustar^2 over the last {obj_steps} steps",
+        "ms_forward_window": 1e3 * t_fwd,
+        "ms_tlm_window": 1e3 * t_tlm,
+        "ms_adjoint_window": 1e3 * t_adj,
+        "ms_forward_step": 1e3 * t_fwd / nsteps,
+        # per step: the checkpointed forward, the taped recomputation and the adjoint sweep
+        "ratio_window": t_adj / t_fwd,
+        # per step: taped recomputation + adjoint sweep, against one forward step
+        "ratio_step": (t_adj - t_fwd) / t_fwd,
+        "J": J,
+    }
+    if timed_only:
+        return res
+    lhs = inner(tl, y) + dJ
+    rhs = inner(dx, grad_y)
+    res["dot_test"] = {"lhs": lhs, "rhs": rhs, "rel": abs(lhs - rhs) / max(abs(lhs), abs(rhs))}
+    # gradient of J alone, and central differences along Gaussian directions
+    reset(); g.lin_zero()
+    g.adjoint_window(jt0, nsteps, k_ckpt, obj_steps)
+    gJ = lin_read()
+    scale = {n: float(np.sqrt(np.mean(x0[n][1:nz] ** 2))) for n in ADJ_STATE}
+    scale["PSI_M_SFC"] = max(abs(float(x0["PSI_M_SFC"].mean())), 1e-3)
+
+    def J_at(d, e):
+        reset(d, e)
+        s = 0.0
+        for i in range(nsteps):
+            g.step(jt0 + i)
+            if i >= nsteps - obj_steps:
+                s += float(np.mean(g.get("USTAR_SFC") ** 2))
+        return s / obj_steps
+
+    fd = []
+    for s in range(ndir):
+        d = direction(100 + s, scale)
+        a = inner(d, gJ)
+        f = (J_at(d, eps) - J_at(d, -eps)) / (2 * eps)
+        fd.append({"adjoint": a, "fd": f, "rel": abs(a - f) / max(abs(a), abs(f))})
+    res["fd_check"] = {"eps": eps, "directions": ndir, "max_rel": max(x["rel"] for x in fd), "cases": fd}
+    return res
 
 
 def traffic_from_profiles(kernel_name: str):
@@ -114,6 +237,9 @@ def main():
     from overlappingbls_amd import config, lib
     from overlappingbls_amd.abi import K_TDMA, K_XDERIV, K_YDERIV, K_PAD, K_COPY, LES_MIXED
 
+    if args.adjoint_only:
+        print(json.dumps(adjoint_leg(timed_only=True)))
+        return
     nl = config.workload(args.workload)
     p = nl.params()
     n = p.nxt * p.nyt * (p.nzt - 1)
@@ -179,6 +305,8 @@ def main():
     }
     if mixed is not None:
         out["mixed"] = mixed
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(out))
