@@ -87,7 +87,7 @@ def adjoint_leg(workload="sbl128", nsteps=20, k_ckpt=5, obj_steps=5, ndir=8, eps
 
     def direction(seed, scale):
         d = {n: scale[n] * config.gaussian(seed, 60 + i, (nz - 1, p.nyt, p.nxt)) for i, n in enumerate(ADJ_STATE)}
-        d["PSI_M_SFC"] = scale["PSI_M_SFC"] * np.ones((p.nyt, p.nxt)) * float(config.gaussian(seed, 70, (1,))[0])
+        d["PSI_M_SFC"] = scale["PSI_M_SFC"] * np.ones((p.nyt, p.nxt)) * float(config.gaussian(seed, 70, (1, 1, 1)).flat[0])
         return d
 
     def lin_load(d):
@@ -138,6 +138,10 @@ def adjoint_leg(workload="sbl128", nsteps=20, k_ckpt=5, obj_steps=5, ndir=8, eps
         "ratio_step": (t_adj - t_fwd) / t_fwd,
         "J": J,
     }
+    reset()
+    times = g.lin_stage_times(jt0)
+    res["stage_ms"] = {k: [round(a, 4), round(b, 4)] for k, (a, b) in times.items()}
+    res["stage_ms_total"] = [round(sum(a for a, _ in times.values()), 4), round(sum(b for _, b in times.values()), 4)]
     if timed_only:
         return res
     lhs = inner(tl, y) + dJ

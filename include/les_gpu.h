@@ -287,6 +287,10 @@ int les_step_adj(les_ctx* ctx, int jt);
 int les_lin_check(les_ctx* ctx, int stage, int jt, unsigned long long seed, double* lhs, double* rhs);
 int les_lin_nstages(void);
 const char* les_lin_stage_name(int stage);
+/* Device time (ms, HIP events) of each stage's tangent and adjoint about
+ * forward step jt from the context's state (which advances one step);
+ * arrays of les_lin_nstages() entries. */
+int les_lin_stage_times(les_ctx* ctx, int jt, double* tl_ms, double* adj_ms);
 
 /* ---- timing support ------------------------------------------------------ */
 /* Kernel families timed by les_time_kernel (no reference counterpart: the

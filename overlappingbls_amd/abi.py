@@ -146,6 +146,7 @@ def bind(lib: C.CDLL, prefix: str):
         "step_adj": [P, i],
         "lin_check": [P, i, i, C.c_ulonglong, C.POINTER(C.c_double), C.POINTER(C.c_double)],
         "lin_nstages": [],
+        "lin_stage_times": [P, i, _PD, _PD],
     }
     fns = {}
     for name, argtypes in sig.items():
@@ -395,6 +396,13 @@ class LesContext:
         a, b = C.c_double(), C.c_double()
         self._call("lin_check", self._h, int(stage), int(jt), C.c_ulonglong(seed), C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def lin_stage_times(self, jt):
+        """{stage: (tangent ms, adjoint ms)} about forward step jt."""
+        names = self.lin_stages()
+        a = np.zeros(len(names)); b = np.zeros(len(names))
+        self._call("lin_stage_times", self._h, int(jt), _pd(a), _pd(b))
+        return {n: (float(x), float(y)) for n, x, y in zip(names, a, b)}
 
     def lin_stages(self):
         n = self._f["lin_nstages"]()

@@ -1229,6 +1229,8 @@ __global__ void k_press_adj_p(double* lp, const double* __restrict__ lP, const d
     }
 }
 
+constexpr int LTD = 8;  // levels per load batch of the transposed TDMA
+
 // Transposed TDMA on the spectrum G = F(lp) (levels 0..nz-1): lambda of Hx,
 // Hy (levels 1..nz-1; 0 and nz zero) and Hz (0..nz).  The wave-mode matrix
 // is symmetric, so its factorisation solves the transposed system.
@@ -1257,24 +1259,55 @@ __global__ __launch_bounds__(128) void k_tdma_adj(const double* __restrict__ G, 
     if (k2 > 0.0) {
         const double idz = 1.0 / dz;
         const double sgn = comp ? 1.0 : -1.0;
+        // loads issued LTD levels ahead of each sweep's dependency chain
         double dprev = 0.0;
-        for (int k = 1; k <= nz - 1; ++k) {
-            const double a = (k == 1) ? 0.0 : idz * idz;
-            const double g = G[k * s2 + t] + ((k == 1) ? g0 : 0.0);
-            const double dv = (g - a * dprev) / cb[k * ns + q];
-            dsc[k * s2 + t] = dv;
-            dprev = dv;
+        for (int k0 = 1; k0 <= nz - 1; k0 += LTD) {
+            double g[LTD], bb[LTD];
+#pragma unroll
+            for (int d = 0; d < LTD; ++d) {
+                const int k = k0 + d;
+                if (k <= nz - 1) {
+                    g[d] = G[k * s2 + t];
+                    bb[d] = cb[k * ns + q];
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < LTD; ++d) {
+                const int k = k0 + d;
+                if (k <= nz - 1) {
+                    const double a = (k == 1) ? 0.0 : idz * idz;
+                    const double gk = g[d] + ((k == 1) ? g0 : 0.0);
+                    const double dv = (gk - a * dprev) / bb[d];
+                    dsc[k * s2 + t] = dv;
+                    dprev = dv;
+                }
+            }
         }
         double rn = dprev;          // lambda_r at level nz-1
         lHz[nz * s2 + t] = rn * idz;
         lHx[(nz - 1) * s2 + oxy] = sgn * kx * rn;
         lHy[(nz - 1) * s2 + oxy] = sgn * ky * rn;
-        for (int k = nz - 2; k >= 1; --k) {
-            const double r = dsc[k * s2 + t] - cc[k * ns + q] * rn;
-            lHz[(k + 1) * s2 + t] = (r - rn) * idz;
-            lHx[k * s2 + oxy] = sgn * kx * r;
-            lHy[k * s2 + oxy] = sgn * ky * r;
-            rn = r;
+        for (int k0 = nz - 2; k0 >= 1; k0 -= LTD) {
+            double dv[LTD], cv[LTD];
+#pragma unroll
+            for (int d = 0; d < LTD; ++d) {
+                const int k = k0 - d;
+                if (k >= 1) {
+                    dv[d] = dsc[k * s2 + t];
+                    cv[d] = cc[k * ns + q];
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < LTD; ++d) {
+                const int k = k0 - d;
+                if (k >= 1) {
+                    const double r = dv[d] - cv[d] * rn;
+                    lHz[(k + 1) * s2 + t] = (r - rn) * idz;
+                    lHx[k * s2 + oxy] = sgn * kx * r;
+                    lHy[k * s2 + oxy] = sgn * ky * r;
+                    rn = r;
+                }
+            }
         }
         lHz[s2 + t] = -rn * idz;                  // level 1: -lambda_r(1)/dz
         lHz[t] = rn * idz - dz * g0;              // level 0: lambda_r(1)/dz - dz g0

@@ -654,7 +654,7 @@ void lin_destroy(les_ctx* c) {
 }
 
 void tape_save(les_ctx* c, Tape* t, int point) {
-    const size_t F3 = c->plane * c->nlev, B3 = c->plane2 * c->nlev, np = c->plane;
+    const size_t F3 = c->plane * c->nlev, np = c->plane;
     switch (point) {
         case TAPE_OBUKHOV:
             dcopy(c, t->obk, c->red, 4);
@@ -668,19 +668,27 @@ void tape_save(les_ctx* c, Tape* t, int point) {
             dcopy(c, t->thf, c->f3[LES_F_THETA], F3);
             dcopy(c, t->tmean, c->th_mean, c->nlev);
             break;
-        default:
-            for (int m = 0; m < 6; ++m) dcopy(c, t->S[m], c->f3[LES_F_S11 + m], F3);
-            dcopy(c, t->nu, c->f3[LES_F_NU_T], F3);
-            dcopy(c, t->cs, c->f3[LES_F_CS_OPT2], F3);
-            dcopy(c, t->dT[0], c->f3[LES_F_DTDX], F3);
-            dcopy(c, t->dT[1], c->f3[LES_F_DTDY], F3);
-            dcopy(c, t->dT[2], c->f3[LES_F_DTDZ], F3);
-            dcopy(c, t->ub, c->u_big, B3);
-            dcopy(c, t->vb, c->v_big, B3);
-            dcopy(c, t->wb, c->w_big, B3);
-            for (int m = 0; m < 3; ++m) dcopy(c, t->ob[m], c->big3[m], B3);
-            for (int m = 0; m < 3; ++m) dcopy(c, t->tb[m], c->tbig[m], B3);
+        default: {
+            // Work fields and padded scratch the next step rewrites before
+            // reading: the tape takes the buffers the step wrote and hands
+            // its own to the context (FP64 mode: a work field's wk is its f3)
+            auto take = [&](int id, double*& tp) {
+                std::swap(c->f3[id], tp);
+                c->wk[id] = c->f3[id];
+            };
+            for (int m = 0; m < 6; ++m) take(LES_F_S11 + m, t->S[m]);
+            take(LES_F_NU_T, t->nu);
+            take(LES_F_DTDX, t->dT[0]);
+            take(LES_F_DTDY, t->dT[1]);
+            take(LES_F_DTDZ, t->dT[2]);
+            std::swap(c->u_big, t->ub);
+            std::swap(c->v_big, t->vb);
+            std::swap(c->w_big, t->wb);
+            for (int m = 0; m < 3; ++m) std::swap(c->big3[m], t->ob[m]);
+            for (int m = 0; m < 3; ++m) std::swap(c->tbig[m], t->tb[m]);
+            dcopy(c, t->cs, c->f3[LES_F_CS_OPT2], F3);   // state: copied
             break;
+        }
     }
 }
 
@@ -762,7 +770,8 @@ int les_step_tlm(les_ctx* c, int jt) { return les_tlm_window(c, jt, 1, 0, nullpt
 // return the state is at jt0 + nsteps, lin holds the gradient with respect
 // to the state at jt0 of <lin seed, x(jt0 + nsteps)> + J, and *J the
 // objective (mean of ustar^2 over the plane and the last obj_steps steps).
-// Checkpoints every k_ckpt steps; each segment is recomputed once with tapes.
+// Checkpoints every k_ckpt steps; the last segment is taped during the
+// forward sweep, every earlier one recomputed once with tapes.
 int les_adjoint_window(les_ctx* c, int jt0, int nsteps, int k_ckpt, int obj_steps, double* J) {
     if (!c || k_ckpt < 1 || obj_steps < 0 || obj_steps > nsteps) return LES_E_BADARG;
     int st = lin_check_supported(c, jt0, nsteps);
@@ -778,10 +787,11 @@ int les_adjoint_window(les_ctx* c, int jt0, int nsteps, int k_ckpt, int obj_step
     fill_levels(c->stream, b.acc, 8, 0, 1, 0.0);
     const double w = obj_steps > 0 ? 1.0 / ((double)obj_steps * (double)(c->nx * c->ny)) : 0.0;
     auto wobj_of = [&](int i) { return (i >= nsteps - obj_steps) ? w : 0.0; };
-    // forward sweep with checkpoints
+    // forward sweep with checkpoints; the last segment is taped on the way
+    const int ilast = (nseg - 1) * k_ckpt;
     for (int i = 0; i < nsteps; ++i) {
         if (i % k_ckpt == 0) ckpt_save(c, L->ckpts[i / k_ckpt]);
-        step_fp64(c, jt0 + i, nullptr, false);
+        step_fp64(c, jt0 + i, i >= ilast ? &L->tapes[i - ilast] : nullptr, false);
         if (wobj_of(i) != 0.0) obj_accumulate(c, wobj_of(i));
     }
     ckpt_save(c, L->ckpts[nseg]);
@@ -792,8 +802,10 @@ int les_adjoint_window(les_ctx* c, int jt0, int nsteps, int k_ckpt, int obj_step
     for (int sgi = nseg - 1; sgi >= 0; --sgi) {
         const int i0 = sgi * k_ckpt;
         const int m = (nsteps - i0) < k_ckpt ? (nsteps - i0) : k_ckpt;
-        ckpt_load(c, L->ckpts[sgi]);
-        for (int j = 0; j < m; ++j) step_fp64(c, jt0 + i0 + j, &L->tapes[j], false);
+        if (sgi != nseg - 1) {
+            ckpt_load(c, L->ckpts[sgi]);
+            for (int j = 0; j < m; ++j) step_fp64(c, jt0 + i0 + j, &L->tapes[j], false);
+        }
         for (int j = m - 1; j >= 0; --j) {
             adj_step(c, L->tapes[j], wobj_of(i0 + j));
             clear_state_ghosts(c);
@@ -808,6 +820,47 @@ int les_adjoint_window(les_ctx* c, int jt0, int nsteps, int k_ckpt, int obj_step
 }
 
 int les_step_adj(les_ctx* c, int jt) { return les_adjoint_window(c, jt, 1, 1, 0, nullptr); }
+
+// Device time of each stage's tangent and adjoint (ms, HIP events on the
+// context's stream) about forward step jt from the context's state (which
+// advances one step); arrays of les_lin_nstages() entries.
+int les_lin_stage_times(les_ctx* c, int jt, double* tl_ms, double* adj_ms) {
+    if (!c || !tl_ms || !adj_ms) return LES_E_BADARG;
+    int st = lin_check_supported(c, jt, 1);
+    if (st != LES_OK) return st;
+    if (lin_alloc(c) != hipSuccess || ensure_tapes(c, 1) != hipSuccess) return LES_E_NOMEM;
+    const std::vector<Stage> all = stages(c->nz);
+    const int n = (int)all.size();
+    std::vector<hipEvent_t> ev(2 * (n + 1));
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) return LES_E_HIP;
+    LinCtx* L = c->lin;
+    step_fp64(c, jt, &L->tapes[0], false);
+    clear_lin(c);
+    Run r = make_run(c, L->tapes[0], 0.0, nullptr);
+    les_note(c, hipEventRecord(ev[0], c->stream));
+    for (int i = 0; i < n; ++i) {
+        all[i].tl(r);
+        les_note(c, hipEventRecord(ev[i + 1], c->stream));
+    }
+    les_note(c, hipEventRecord(ev[n + 1], c->stream));
+    for (int i = n - 1; i >= 0; --i) {
+        all[i].adj(r);
+        les_note(c, hipEventRecord(ev[n + 1 + (n - i)], c->stream));
+    }
+    st = finish(c);
+    for (int i = 0; i < n && st == LES_OK; ++i) {
+        float a = 0.f, b = 0.f;
+        if (hipEventElapsedTime(&a, ev[i], ev[i + 1]) != hipSuccess) st = LES_E_HIP;
+        if (hipEventElapsedTime(&b, ev[n + 1 + (n - 1 - i)], ev[n + 1 + (n - i)]) != hipSuccess) st = LES_E_HIP;
+        tl_ms[i] = a;
+        adj_ms[i] = b;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    clear_lin(c);
+    const int st2 = finish(c);
+    return st != LES_OK ? st : st2;
+}
 
 int les_lin_nstages(void) { return (int)stages(8).size(); }
 
