@@ -221,6 +221,51 @@ interface
         integer(c_int), value :: slice, window
         real(c_double), intent(out) :: host(*)
     end function
+    ! ---- tangent-linear model and adjoint (no reference counterpart) ----
+    integer(c_int) function les_lin_set(ctx, f, h, ld, lev0, nlev) bind(C, name="les_lin_set")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: f, ld, lev0, nlev
+        real(c_double), intent(in) :: h(*)
+    end function
+    integer(c_int) function les_lin_get(ctx, f, h, ld, lev0, nlev) bind(C, name="les_lin_get")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: f, ld, lev0, nlev
+        real(c_double), intent(out) :: h(*)
+    end function
+    integer(c_int) function les_lin_zero(ctx) bind(C, name="les_lin_zero")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function les_set_sgs_frozen(ctx, frozen) bind(C, name="les_set_sgs_frozen")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: frozen
+    end function
+    integer(c_int) function les_tlm_window(ctx, jt0, nsteps, obj_steps, dJ) bind(C, name="les_tlm_window")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: jt0, nsteps, obj_steps
+        real(c_double), intent(out) :: dJ
+    end function
+    integer(c_int) function les_adjoint_window(ctx, jt0, nsteps, k_ckpt, obj_steps, J) &
+            bind(C, name="les_adjoint_window")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: jt0, nsteps, k_ckpt, obj_steps
+        real(c_double), intent(out) :: J
+    end function
+    integer(c_int) function les_step_tlm(ctx, jt) bind(C, name="les_step_tlm")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: jt
+    end function
+    integer(c_int) function les_step_adj(ctx, jt) bind(C, name="les_step_adj")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: jt
+    end function
 end interface
 
 contains
