@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint leg's timed windows (for rocprofv3)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="run only the kernel-family timing loops, FP64 (for rocprofv3 --pmc)")
     return ap.parse_args()
 
 
@@ -243,6 +245,13 @@ def main():
 
     if args.adjoint_only:
         print(json.dumps(adjoint_leg(timed_only=True)))
+        return
+    if args.kernels_only:
+        nl = config.workload(args.workload)
+        ctx = lib.create(nl.params())
+        config.load_state(ctx, config.initial_fields(nl, seed=2024))
+        fams = {"tdma": K_TDMA, "xline_deriv": K_XDERIV, "yline_deriv": K_YDERIV, "pad32": K_PAD, "copy": K_COPY}
+        print(json.dumps(kernel_table(ctx, fams)))
         return
     nl = config.workload(args.workload)
     p = nl.params()

@@ -975,18 +975,24 @@ void press_rhs_prep_dev(les_ctx* c, W* rx, W* ry, W* rz) {
                          (const W*)WF<W>(c, LES_F_DIVTZ));
 }
 
-// Plane sums of the z right-hand side of the pressure equation in FP64, one
-// block per level 0..nz (the kx = ky = 0 coefficient of its forward
-// transform, formed from the FP64 velocity rather than from a reduced-
-// precision transform).
+// Plane sums of the z right-hand side of the pressure equation in FP64 (the
+// kx = ky = 0 coefficient of its forward transform, formed from the FP64
+// velocity rather than from a reduced-precision transform), levels 0..nz.
+// HZ_SPLIT blocks per level write partial sums; one thread per level adds
+// them in order.
+constexpr int HZ_SPLIT = 8;
+
 template <typename W>
-__global__ __launch_bounds__(256) void k_hz_sums(const double* __restrict__ w, const W* __restrict__ divtz, KP p,
-                                                  double cst, double* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_hz_part(const double* __restrict__ w, const W* __restrict__ divtz, KP p,
+                                                  double cst, double* __restrict__ part) {
     __shared__ double sh[256];
-    const int k = blockIdx.x;
+    const int k = blockIdx.x / HZ_SPLIT, part_id = blockIdx.x % HZ_SPLIT;
     const double tadv1 = 1.5;
+    const long chunk = (p.plane + HZ_SPLIT - 1) / HZ_SPLIT;
+    const long q0 = (long)part_id * chunk;
+    const long q1 = (q0 + chunk < p.plane) ? q0 + chunk : p.plane;
     double s = 0.0;
-    for (long q = threadIdx.x; q < p.plane; q += 256) {
+    for (long q = q0 + threadIdx.x; q < q1; q += 256) {
         const long i = (long)k * p.plane + q;
         if (k == 0) s += cst * divtz[i + p.plane];
         else if (k == p.nz) s += cst * divtz[i];
@@ -998,14 +1004,26 @@ __global__ __launch_bounds__(256) void k_hz_sums(const double* __restrict__ w, c
         if (threadIdx.x < m) sh[threadIdx.x] += sh[threadIdx.x + m];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[k] = sh[0];
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_hz_fold(const double* __restrict__ part, int nlev, double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nlev) return;
+    double s = 0.0;
+    for (int j = 0; j < HZ_SPLIT; ++j) s += part[k * HZ_SPLIT + j];
+    out[k] = s;
 }
 
 template <typename W>
 void press_hz_sums_dev(les_ctx* c, double* out) {
     KP p = kp(c);
-    hipLaunchKernelGGL(k_hz_sums<W>, dim3(p.nz + 1), dim3(256), 0, c->stream, (const double*)c->f3[LES_F_W],
-                       (const W*)WF<W>(c, LES_F_DIVTZ), p, 1.0 / (double)(c->nx * c->ny), out);
+    double* part = c->work3[5];  // (nz+1) * HZ_SPLIT partial sums
+    hipLaunchKernelGGL(k_hz_part<W>, dim3((p.nz + 1) * HZ_SPLIT), dim3(256), 0, c->stream,
+                       (const double*)c->f3[LES_F_W], (const W*)WF<W>(c, LES_F_DIVTZ), p,
+                       1.0 / (double)(c->nx * c->ny), part);
+    hipLaunchKernelGGL(k_hz_fold, dim3((p.nz + 1 + 63) / 64), dim3(64), 0, c->stream, (const double*)part, p.nz + 1,
+                       out);
 }
 
 template <typename W>
