@@ -395,18 +395,41 @@ __global__ void k_grad_adj(double* lf, const double* __restrict__ tx, const doub
         lf[i] = lf[i] + a;
     }
 }
-__global__ void k_clear_levels(double* f, long lo, long hi) {
-    LGRID(t, hi - lo) f[lo + t] = 0.0;
+// Clears are queued per stage and issued as one launch (segment s of the
+// batch in blockIdx.y): an adjoint stage clears the adjoints of its outputs
+// after the kernels that read them, and no kernel of the same stage writes
+// them again.
+__global__ void k_clear_batch(ClearBatch cb) {
+    const int s = blockIdx.y;
+    double* p = cb.p[s];
+    const long n = cb.n[s];
+    LGRID(i, n) p[i] = 0.0;
 }
-void clear_levels(les_ctx* c, double* f, int l0, int l1) {
+void lin_flush(les_ctx* c, LinBufs& b) {
+    if (b.clr.count == 0) return;
+    long mx = 0;
+    for (int i = 0; i < b.clr.count; ++i) mx = b.clr.n[i] > mx ? b.clr.n[i] : mx;
+    long gx = (mx + 255) / 256;
+    if (gx > 2048) gx = 2048;
+    hipLaunchKernelGGL(k_clear_batch, dim3((unsigned)gx, (unsigned)b.clr.count), dim3(256), 0, c->stream, b.clr);
+    b.clr.count = 0;
+}
+void lin_clear(les_ctx* c, LinBufs& b, double* p, long n) {
+    if (n <= 0) return;
+    if (b.clr.count == LIN_MAXCLR) lin_flush(c, b);
+    b.clr.p[b.clr.count] = p;
+    b.clr.n[b.clr.count] = n;
+    ++b.clr.count;
+}
+void clear_levels(les_ctx* c, LinBufs& b, double* f, int l0, int l1) {
     if (l1 < l0) return;
     const long lo = (long)l0 * (long)c->plane, hi = (long)(l1 + 1) * (long)c->plane;
-    hipLaunchKernelGGL(k_clear_levels, lgrid(hi - lo), dim3(256), 0, c->stream, f, lo, hi);
+    lin_clear(c, b, f + lo, hi - lo);
 }
-void clear_levels2(les_ctx* c, double* f, int l0, int l1) {
+void clear_levels2(les_ctx* c, LinBufs& b, double* f, int l0, int l1) {
     if (l1 < l0) return;
     const long lo = (long)l0 * (long)c->plane2, hi = (long)(l1 + 1) * (long)c->plane2;
-    hipLaunchKernelGGL(k_clear_levels, lgrid(hi - lo), dim3(256), 0, c->stream, f, lo, hi);
+    lin_clear(c, b, f + lo, hi - lo);
 }
 void grad_adj(les_ctx* c, double* lf, const double* tx, const double* ty, double* lz, int wform) {
     const LP p = lin_params(c);
@@ -488,7 +511,7 @@ void strain_tl(les_ctx* c, LinBufs& b) {
 void strain_adj(les_ctx* c, LinBufs& b) {
     const LP p = lin_params(c);
     hipLaunchKernelGGL(k_strain_adj, lgrid(p.plane * p.nz), dim3(256), 0, c->stream, lin_grad(b), lin_strain(b), p);
-    for (int m = 0; m < 6; ++m) clear_levels(c, b.v[L_S11 + m], 1, p.nz);
+    for (int m = 0; m < 6; ++m) clear_levels(c, b, b.v[L_S11 + m], 1, p.nz);
 }
 
 // ---- Nu_t and tau_ij (106_sgs_stag.f90:206-315), Cs_opt2 frozen ------------------
@@ -641,11 +664,11 @@ void stress_adj(les_ctx* c, const Tape& t, LinBufs& b) {
     const LP p = lin_params(c);
     hipLaunchKernelGGL(k_stress_adj, lgrid(p.plane * p.nz), dim3(256), 0, c->stream, tape_s(t), lin_strain(b),
                        (const double*)b.v[L_NU], lin_tau(b), p);
-    clear_levels(c, b.v[L_NU], 1, p.nz);
+    clear_levels(c, b, b.v[L_NU], 1, p.nz);
     const int ids[4] = {L_TXX, L_TXY, L_TYY, L_TZZ};
-    for (int m = 0; m < 4; ++m) clear_levels(c, b.v[ids[m]], 1, p.nz - 1);
-    clear_levels(c, b.v[L_TXZ], 2, p.nz - 1);
-    clear_levels(c, b.v[L_TYZ], 2, p.nz - 1);
+    for (int m = 0; m < 4; ++m) clear_levels(c, b, b.v[ids[m]], 1, p.nz - 1);
+    clear_levels(c, b, b.v[L_TXZ], 2, p.nz - 1);
+    clear_levels(c, b, b.v[L_TYZ], 2, p.nz - 1);
 }
 
 // ---- divstress (107_divstress.f90:36-102) ------------------------------------------
@@ -704,9 +727,9 @@ void div_adj(les_ctx* c, LinBufs& b, double* const* t) {
     hipLaunchKernelGGL(k_div_adj, lgrid(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, lin_tau(b),
                        (const double*)t[0], (const double*)t[1], (const double*)t[2], (const double*)t[3],
                        (const double*)t[4], (const double*)t[5], b.v[L_DVX], b.v[L_DVY], b.v[L_DVZ], p);
-    clear_levels(c, b.v[L_DVX], 1, p.nz - 1);
-    clear_levels(c, b.v[L_DVY], 1, p.nz - 1);
-    clear_levels(c, b.v[L_DVZ], 1, p.nz);
+    clear_levels(c, b, b.v[L_DVX], 1, p.nz - 1);
+    clear_levels(c, b, b.v[L_DVY], 1, p.nz - 1);
+    clear_levels(c, b, b.v[L_DVZ], 1, p.nz);
 }
 
 // ---- vorticity (108_convec.f90:44-69) ------------------------------------------------
@@ -758,7 +781,7 @@ void vort_adj(les_ctx* c, LinBufs& b) {
     const LP p = lin_params(c);
     hipLaunchKernelGGL(k_vort_adj, lgrid(p.plane * p.nz), dim3(256), 0, c->stream, b.v[L_O1], b.v[L_O2], b.v[L_O3],
                        lin_grad(b), p);
-    for (int m = 0; m < 3; ++m) clear_levels(c, b.v[L_O1 + m], 1, p.nz);
+    for (int m = 0; m < 3; ++m) clear_levels(c, b, b.v[L_O1 + m], 1, p.nz);
 }
 
 // ---- cross product on the 3/2 grid (108_convec.f90:103-202) ---------------------------
@@ -829,7 +852,7 @@ void cross_adj(les_ctx* c, const Tape& t, LinBufs& b) {
     const LP p = lin_params(c);
     hipLaunchKernelGGL(k_cross_adj, lgrid(p.plane2 * p.nz), dim3(256), 0, c->stream, tape_big(t),
                        (const double*)c->ust, (const double*)c->vst, lin_big(b), p);
-    for (int m = 0; m < 3; ++m) clear_levels2(c, b.v[L_CB1 + m], 1, p.nz - 1);
+    for (int m = 0; m < 3; ++m) clear_levels2(c, b, b.v[L_CB1 + m], 1, p.nz - 1);
 }
 
 // ---- scalar advection on the 3/2 grid (010_scalars.f90:466-484) ------------------------
@@ -877,7 +900,7 @@ void sadv_adj(les_ctx* c, const Tape& t, LinBufs& b) {
     const LP p = lin_params(c);
     hipLaunchKernelGGL(k_sadv_adj, lgrid(p.plane2 * (p.nz - 1)), dim3(256), 0, c->stream, tape_big(t),
                        (const double*)c->ust, (const double*)c->vst, lin_big(b), p);
-    clear_levels2(c, b.v[L_RM], 1, p.nz - 1);
+    clear_levels2(c, b, b.v[L_RM], 1, p.nz - 1);
 }
 
 // ---- SGS scalar fluxes (010_scalars.f90:513-594) -------------------------------------
@@ -942,9 +965,9 @@ void sflux_adj(les_ctx* c, const Tape& t, LinBufs& b) {
                        (const double*)t.dT[0], (const double*)t.dT[1], (const double*)t.dT[2], b.v[L_NU], b.v[L_TX],
                        b.v[L_TY], b.v[L_TZ], (const double*)b.v[L_FX], (const double*)b.v[L_FY],
                        (const double*)b.v[L_FZ], p, 1.0 / c->prm.Pr);
-    clear_levels(c, b.v[L_FX], 1, p.nz - 1);
-    clear_levels(c, b.v[L_FY], 1, p.nz - 1);
-    clear_levels(c, b.v[L_FZ], 2, p.nz);
+    clear_levels(c, b, b.v[L_FX], 1, p.nz - 1);
+    clear_levels(c, b, b.v[L_FY], 1, p.nz - 1);
+    clear_levels(c, b, b.v[L_FZ], 2, p.nz);
 }
 
 // RT(k) = -adv + t1 + t2 + (fz(k+1) - fz(k))/dz on 1..nz-1; zero elsewhere
@@ -981,7 +1004,7 @@ void sdiv_adj(les_ctx* c, LinBufs& b, const double* t1, const double* t2) {
     const LP p = lin_params(c);
     hipLaunchKernelGGL(k_sdiv_adj, lgrid(p.plane * p.nz), dim3(256), 0, c->stream, b.v[L_RT], b.v[L_ADV], b.v[L_FX],
                        b.v[L_FY], b.v[L_FZ], t1, t2, p);
-    clear_levels(c, b.v[L_RT], 1, p.nz - 1);
+    clear_levels(c, b, b.v[L_RT], 1, p.nz - 1);
 }
 
 // ---- calc_beta (010_scalars.f90:214-258) ----------------------------------------------
@@ -1044,7 +1067,7 @@ void beta_adj(les_ctx* c, const Tape& t, LinBufs& b, double g) {
     hipLaunchKernelGGL(k_beta_adj_pt, lgrid(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, b.v[L_TH],
                        b.v[L_BETA], (const double*)t.tmean, (const double*)b.small, p, g,
                        (double)(c->nx * c->ny));
-    clear_levels(c, b.v[L_BETA], 2, p.nz - 1);
+    clear_levels(c, b, b.v[L_BETA], 2, p.nz - 1);
 }
 
 // ---- AB2 updates, RHS assembly, projection ----------------------------------------------
@@ -1378,10 +1401,10 @@ void press_adj(les_ctx* c, LinBufs& b) {
     hipLaunchKernelGGL(k_press_adj_out, lgrid(p.plane * (nz + 1)), dim3(256), 0, c->stream, (const double*)rx,
                        (const double*)ry, (const double*)rz, b.v[L_U], b.v[L_V], b.v[L_W], b.v[L_DVZ], p,
                        cst / (1.5 * p.dt), cst);
-    clear_levels(c, b.v[L_DPX], 1, nz - 1);
-    clear_levels(c, b.v[L_DPY], 1, nz - 1);
-    clear_levels(c, b.v[L_DPZ], 1, nz - 1);
-    clear_levels(c, b.v[L_P], 0, nz - 1);
+    clear_levels(c, b, b.v[L_DPX], 1, nz - 1);
+    clear_levels(c, b, b.v[L_DPY], 1, nz - 1);
+    clear_levels(c, b, b.v[L_DPZ], 1, nz - 1);
+    clear_levels(c, b, b.v[L_P], 0, nz - 1);
 }
 
 }  // namespace les

@@ -148,10 +148,15 @@ __global__ __launch_bounds__(NT) void k_yline_real(const TI* __restrict__ in, T*
     fft_lines<T, EPT>(buf, pin, C, nin, twin, -1);
     apply_line_op<T>(buf, C, nin, nout, op, kfac);
     fft_lines<T, EPT>(buf, pout, C, nout, twout, +1);
+    // pad / unpad with kfac != 0 accumulate: out += kfac * result
+    const bool acc = (op == OP_PAD || op == OP_UNPAD) && kfac != T(0);
     for (int e = threadIdx.x; e < nout * w2; e += NT) {
         const int y = e / w2, cl = e - y * w2;
         const int x = x0 + cl;
-        if (x < ncols) dst[(long)y * ld_out + x] = reinterpret_cast<const T*>(&buf[(cl >> 1) * nout + y])[cl & 1];
+        if (x >= ncols) continue;
+        const T v = reinterpret_cast<const T*>(&buf[(cl >> 1) * nout + y])[cl & 1];
+        T* o = dst + (long)y * ld_out + x;
+        *o = acc ? *o + kfac * v : v;
     }
 }
 

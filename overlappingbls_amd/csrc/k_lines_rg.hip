@@ -316,9 +316,20 @@ __global__ __launch_bounds__((rg_threads<T, NIN, NOUT, true>())) void kr_yline(
             return cx<T>{(T)v.x, (T)v.y};
         }
     };
+    // pad / unpad with kfac != 0 accumulate: out += kfac * result (the
+    // transposes of the 3/2-grid maps in the adjoint step)
     auto st = [&](int line, int p, cx<T> z) {
         const int x = x0 + 2 * line;
-        if (x < ncols) *reinterpret_cast<PO*>(dst + (long)p * ld_out + x) = PO{z.x, z.y};
+        if (x >= ncols) return;
+        PO* o = reinterpret_cast<PO*>(dst + (long)p * ld_out + x);
+        if constexpr (OP == OP_PAD || OP == OP_UNPAD) {
+            if (kfac != T(0)) {
+                const PO a = *o;
+                *o = PO{a.x + kfac * z.x, a.y + kfac * z.y};
+                return;
+            }
+        }
+        *o = PO{z.x, z.y};
     };
     __syncthreads();
     line_pipeline<T, NIN, NOUT, OP, NL, NTH, true>(ld, st, buf, twi, twop, kfac);

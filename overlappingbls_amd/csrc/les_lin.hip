@@ -209,17 +209,13 @@ double* W3(Run& r, int i) { return r.c->work3[i]; }
 // transpose of a pad, a = 9/4) or += a * pad(...) (transpose of an unpad, a = 4/9)
 void add_unpad(Run& r, double* dst, double* src, int l0, int l1, double a) {
     if (l1 < l0) return;
-    double* tmp = W3(r, 0);
-    unpad_f<double>(r.c, src + (size_t)l0 * NP2(r), tmp, l1 - l0 + 1);
-    lin_axpy(r.c, dst + (size_t)l0 * NP(r), tmp, a, (long)(l1 - l0 + 1) * (long)NP(r));
+    unpad_acc_f(r.c, src + (size_t)l0 * NP2(r), dst + (size_t)l0 * NP(r), l1 - l0 + 1, a);
 }
 void add_pad(Run& r, double* dst, double* src, int l0, int l1, double a) {
     if (l1 < l0) return;
-    double* tmp = r.c->big3[3];
-    pad_f<double, double>(r.c, src + (size_t)l0 * NP(r), tmp, l1 - l0 + 1);
-    lin_axpy(r.c, dst + (size_t)l0 * NP2(r), tmp, a, (long)(l1 - l0 + 1) * (long)NP2(r));
+    pad_acc_f(r.c, src + (size_t)l0 * NP(r), dst + (size_t)l0 * NP2(r), l1 - l0 + 1, a);
 }
-void clear_all(Run& r, int v) { fill_levels(r.c->stream, V(r, v), lv_size(r.c, v), 0, 1, 0.0); }
+void clear_all(Run& r, int v) { lin_clear(r.c, r.b, V(r, v), (long)lv_size(r.c, v)); }
 
 // ---- stages ---------------------------------------------------------------------------
 // 0: obukhov (010_scalars.f90:758-843): u(1), v(1), psi_m, theta(1) -> psi_m, phi_m
@@ -557,7 +553,10 @@ void tlm_step(les_ctx* c, const Tape& t, double wobj) {
     for (int m = 0; m < 3; ++m) std::swap(b.v[L_RX + m], b.v[L_RXF + m]);
     Run r = make_run(c, t, wobj, wobj != 0.0 ? b.us2 : nullptr);
     const std::vector<Stage> st = stages(c->nz);
-    for (const Stage& s : st) s.tl(r);
+    for (const Stage& s : st) {
+        s.tl(r);
+        lin_flush(c, b);
+    }
     if (wobj != 0.0) {
         lin_dot_part(c, b.us2, nullptr, (long)c->plane, 0, 0, 0, 2, b.part);
         lin_acc(c, b.part, wobj, b.acc);
@@ -569,7 +568,10 @@ void adj_step(les_ctx* c, const Tape& t, double wobj) {
     LinBufs& b = c->lin->b;
     Run r = make_run(c, t, wobj, nullptr);
     const std::vector<Stage> st = stages(c->nz);
-    for (int i = (int)st.size() - 1; i >= 0; --i) st[i].adj(r);
+    for (int i = (int)st.size() - 1; i >= 0; --i) {
+        st[i].adj(r);
+        lin_flush(c, b);
+    }
     for (int m = 0; m < 3; ++m) std::swap(b.v[L_RX + m], b.v[L_RXF + m]);
 }
 
@@ -577,9 +579,10 @@ void adj_step(les_ctx* c, const Tape& t, double wobj) {
 void clear_state_ghosts(les_ctx* c) {
     LinBufs& b = c->lin->b;
     for (int v = L_U; v <= L_RT; ++v) {
-        clear_levels(c, b.v[v], 0, 0);
-        clear_levels(c, b.v[v], c->nz, c->nz);
+        clear_levels(c, b, b.v[v], 0, 0);
+        clear_levels(c, b, b.v[v], c->nz, c->nz);
     }
+    lin_flush(c, b);
 }
 void clear_intermediates(les_ctx* c) {
     LinBufs& b = c->lin->b;
@@ -841,11 +844,13 @@ int les_lin_stage_times(les_ctx* c, int jt, double* tl_ms, double* adj_ms) {
     les_note(c, hipEventRecord(ev[0], c->stream));
     for (int i = 0; i < n; ++i) {
         all[i].tl(r);
+        lin_flush(c, L->b);
         les_note(c, hipEventRecord(ev[i + 1], c->stream));
     }
     les_note(c, hipEventRecord(ev[n + 1], c->stream));
     for (int i = n - 1; i >= 0; --i) {
         all[i].adj(r);
+        lin_flush(c, L->b);
         les_note(c, hipEventRecord(ev[n + 1 + (n - i)], c->stream));
     }
     st = finish(c);
@@ -894,6 +899,7 @@ int les_lin_check(les_ctx* c, int stage, int jt, unsigned long long seed, double
     } else {
         Run r = make_run(c, t, 0.0, nullptr);
         all[stage].tl(r);
+        lin_flush(c, L->b);
     }
     double l = 0.0;
     for (const Reg& g : out) l += dot_region(c, g, sy);
@@ -905,6 +911,7 @@ int les_lin_check(les_ctx* c, int stage, int jt, unsigned long long seed, double
     } else {
         Run r = make_run(c, t, 0.0, nullptr);
         all[stage].adj(r);
+        lin_flush(c, L->b);
     }
     double rr = 0.0;
     for (const Reg& g : in) rr += dot_region(c, g, sx);

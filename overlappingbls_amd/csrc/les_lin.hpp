@@ -28,8 +28,16 @@ inline bool lin_plane_only(int v) { return v == L_PSI || v == L_PHI; }
 inline bool lin_padded(int v) { return v >= L_UB; }
 
 constexpr int LIN_NPART = 256;  // partial sums per reduction
+constexpr int LIN_MAXCLR = 32;  // clears per batched launch
+
+struct ClearBatch {
+    double* p[LIN_MAXCLR];
+    long n[LIN_MAXCLR];
+    int count;
+};
 
 struct LinBufs {
+    ClearBatch clr;  // queued clears (host side)
     double* v[L_COUNT];
     double* uv1f;    // tangent / adjoint of the test-filtered u, v at level 1 (2 planes)
     double* us2;     // 2 ustar d(ustar) (one plane)
@@ -106,8 +114,10 @@ void lin_rand(les_ctx* c, double* f, long n, unsigned long long seed, unsigned l
 void lin_dot_part(les_ctx* c, const double* a, const double* b, long n, unsigned long long seed,
                   unsigned long long salt, long off, int sq, double* part);
 void lin_acc(les_ctx* c, const double* part, double w, double* acc);
-void clear_levels(les_ctx* c, double* f, int l0, int l1);
-void clear_levels2(les_ctx* c, double* f, int l0, int l1);
+void lin_clear(les_ctx* c, LinBufs& b, double* p, long n);
+void lin_flush(les_ctx* c, LinBufs& b);
+void clear_levels(les_ctx* c, LinBufs& b, double* f, int l0, int l1);
+void clear_levels2(les_ctx* c, LinBufs& b, double* f, int l0, int l1);
 
 void obk_tl(les_ctx* c, const Tape& t, LinBufs& b, const ObkArgs& a);
 void obk_adj(les_ctx* c, const Tape& t, LinBufs& b, const ObkArgs& a);

@@ -56,6 +56,20 @@ inline void unpad_f(les_ctx* c, const W* src, W* dst, int nlev) {
     yline_real<W>(c->stream, tmp, dst, nlev, c->nx, c->nx, c->nx, (long)c->ny2 * c->nx, (long)c->plane, c->tw_y2,
                   c->tw_y, OP_UNPAD, 0.0);
 }
+// dst += acc * zero_padding(src) / zero_unpadding(src) (acc != 0): the
+// transposes of unpad / pad in the adjoint step, accumulated by the y pass
+inline void pad_acc_f(les_ctx* c, const double* src, double* dst, int nlev, double acc) {
+    double* tmp = c->work3[6];
+    xline_real<double>(c->stream, src, tmp, (long)nlev * c->ny, c->nx, c->nx2, c->tw_x, c->tw_x2, OP_PAD, 0.0);
+    yline_real<double>(c->stream, tmp, dst, nlev, c->nx2, c->nx2, c->nx2, (long)c->ny * c->nx2, (long)c->plane2,
+                       c->tw_y, c->tw_y2, OP_PAD, acc);
+}
+inline void unpad_acc_f(les_ctx* c, const double* src, double* dst, int nlev, double acc) {
+    double* tmp = c->work3[6];
+    xline_real<double>(c->stream, src, tmp, (long)nlev * c->ny2, c->nx2, c->nx, c->tw_x2, c->tw_x, OP_UNPAD, 0.0);
+    yline_real<double>(c->stream, tmp, dst, nlev, c->nx, c->nx, c->nx, (long)c->ny2 * c->nx, (long)c->plane,
+                       c->tw_y2, c->tw_y, OP_UNPAD, acc);
+}
 // test_filter of nplanes consecutive planes in place
 template <typename T>
 inline void test_filter_f(les_ctx* c, T* p, int nplanes, const double* G) {
