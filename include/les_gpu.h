@@ -133,6 +133,15 @@ int les_set_stokes(les_ctx* ctx, const double* ust, const double* vst);
  * for damping_method 1, overridable here. */
 int les_set_sponge(les_ctx* ctx, const double* sponge);
 
+/* ---- registration (the reference's module arrays stay the caller's) ------- */
+/* Register a caller array for field `field` (levels lev0..lev0+nlev-1, row
+ * stride ld); les_upload copies every registered array to the device,
+ * les_download copies the device fields back (the sync points around a
+ * device-resident les_run).  The arrays must stay valid while registered. */
+int les_bind_field(les_ctx* ctx, int field, double* host, int ld, int lev0, int nlev);
+int les_upload(les_ctx* ctx);
+int les_download(les_ctx* ctx);
+
 /* ---- field transfer -------------------------------------------------------- */
 /* Copy levels lev0..lev0+nlev-1 of a field between host and device.  The
  * host block is nlev planes of nyt rows with row stride ld (doubles). */
@@ -199,6 +208,15 @@ int les_press_stag_array(les_ctx* ctx, int jt);
 /* project, 110_project.f90:4-62, followed by data_exchange ghost rules
  * (202_parallel.f90:722-733). */
 int les_project(les_ctx* ctx);
+
+/* Admas_Bashforth, 103_flosol.f90:319-389: u, v, w += dt (1.5 RHS - 0.5 RHS_f
+ * + force), Euler (RHS_f <- RHS) at jt = 1 unless restart_vel, level 0 <- BOGUS. */
+int les_ab2(les_ctx* ctx, int jt);
+/* tdmaSolver_pipelined with press_stag_array's right-hand side
+ * (109_press_stag_array.f90:201-319) on host half spectra: Hx, Hy, Hz hold
+ * levels 0..nzt of (nxt/2+1) x nyt complex values (re, im interleaved, kx
+ * fastest); p_hat receives levels 0..nzt-1 in the same layout. */
+int les_tdma_solve(les_ctx* ctx, const double* hx, const double* hy, const double* hz, double* p_hat);
 
 /* ---- whole step --------------------------------------------------------- */
 /* flosol, 103_flosol.f90:4-290, minus file I/O (the ke/dudt diagnostics are

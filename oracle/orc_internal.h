@@ -60,6 +60,9 @@ struct les_ctx {
     double *u_big, *v_big, *w_big;     /* padded velocity kept for theta (convec) */
     int f_lm_mm_init, f_qn_nn_init;
     int sgs_frozen;
+    /* les_bind_field registrations (host arrays the caller owns) */
+    int nbind;
+    struct { int id; double* h; int ld, lev0, nlev; } bind[128];
     double cfl;
     int nthreads;
     /* time-averaged slices (011_scalars2.f90) and the flosol diagnostics

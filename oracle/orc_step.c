@@ -256,6 +256,35 @@ int orc_get_field(les_ctx* c, int id, double* h, int ld, int lev0, int nlev) {
     return LES_OK;
 }
 
+/* les_bind_field / les_upload / les_download: the caller's arrays are
+ * registered once and synchronised in bulk (no reference counterpart; the
+ * reference's module arrays are these host arrays) */
+int orc_bind_field(les_ctx* c, int id, double* h, int ld, int lev0, int nlev) {
+    if (!h || c->nbind >= 128) return LES_E_BADARG;
+    double* probe = (id >= 0 && id < LES_N_FIELDS3D) ? c->f3[id] : surf_field(c, id);
+    if (!probe || ld < c->nx) return LES_E_BADARG;
+    if (id >= 0 && id < LES_N_FIELDS3D ? (lev0 < 0 || nlev < 0 || lev0 + nlev > (int)c->nlev) : (lev0 != 0 || nlev != 1))
+        return LES_E_BADARG;
+    c->bind[c->nbind].id = id; c->bind[c->nbind].h = h; c->bind[c->nbind].ld = ld;
+    c->bind[c->nbind].lev0 = lev0; c->bind[c->nbind].nlev = nlev;
+    ++c->nbind;
+    return LES_OK;
+}
+int orc_upload(les_ctx* c) {
+    for (int i = 0; i < c->nbind; ++i) {
+        int st = orc_set_field(c, c->bind[i].id, c->bind[i].h, c->bind[i].ld, c->bind[i].lev0, c->bind[i].nlev);
+        if (st != LES_OK) return st;
+    }
+    return LES_OK;
+}
+int orc_download(les_ctx* c) {
+    for (int i = 0; i < c->nbind; ++i) {
+        int st = orc_get_field(c, c->bind[i].id, c->bind[i].h, c->bind[i].ld, c->bind[i].lev0, c->bind[i].nlev);
+        if (st != LES_OK) return st;
+    }
+    return LES_OK;
+}
+
 int orc_set_sgs_frozen(les_ctx* c, int frozen) { c->sgs_frozen = frozen ? 1 : 0; return LES_OK; }
 int orc_set_sgs_init_flags(les_ctx* c, int a, int b) { c->f_lm_mm_init = a; c->f_qn_nn_init = b; return LES_OK; }
 int orc_get_sgs_init_flags(les_ctx* c, int* a, int* b) { *a = c->f_lm_mm_init; *b = c->f_qn_nn_init; return LES_OK; }
@@ -1094,6 +1123,48 @@ int orc_theta_all_in_one(les_ctx* c, int jt) {
 }
 
 /* --------------------------------------------------------------------------
+ * press_stag_array's right-hand side (109_press_stag_array.f90:201-225) and
+ * tdmaSolver_pipelined (:244-319): half spectra Hx, Hy, Hz (levels 0..nz)
+ * -> p_hat (levels 0..nz-1)
+ * ------------------------------------------------------------------------ */
+static void tdma_solve(les_ctx* c, const cplx* Hx, const cplx* Hy, const cplx* Hz, cplx* ph) {
+    const int nz = c->nz;
+    const size_t ns = (size_t)c->lhx * c->ny;
+    const double dz = c->dz;
+    cplx* rhs = (cplx*)calloc(ns * nz, sizeof(cplx));           /* RHS_col(0:nz-1) */
+    for (size_t q = 0; q < ns; ++q) {
+        const double k2 = c->k2[q], kx = c->kx2d[q], ky = c->ky2d[q];
+        if (k2 > 0)                                              /* :203-209 */
+            rhs[ns + q] = I * (kx * Hx[ns + q] + ky * Hy[ns + q]) + (Hz[2 * ns + q] - Hz[ns + q] + Hz[q]) / dz;
+        rhs[q] = Hz[q];
+        for (int k = 2; k <= nz - 1; ++k) {                      /* :213-223 */
+            size_t o = (size_t)k * ns + q;
+            if (k2 > 0) rhs[o] = I * (kx * Hx[o] + ky * Hy[o]) + (Hz[o + ns] - Hz[o]) / dz;
+            if (k2 == 0 && c->notnyq[q]) rhs[o] = Hz[o];
+        }
+    }
+    for (size_t q = 0; q < ns; ++q) {
+        if (!c->notnyq[q]) continue;                             /* Nyquist stays 0 */
+        for (int k = 1; k <= nz - 1; ++k) {                      /* forward :256-266 */
+            size_t o = (size_t)k * ns + q;
+            rhs[o] = (rhs[o] - c->coef_a[o] * rhs[o - ns]) / c->coef_b[o];
+        }
+        ph[(size_t)(nz - 1) * ns + q] = rhs[(size_t)(nz - 1) * ns + q];   /* c'(nz-1) = 0 */
+        for (int k = nz - 2; k >= 1; --k) {                      /* backward :290-297 */
+            size_t o = (size_t)k * ns + q;
+            ph[o] = rhs[o] - c->coef_c[o] * ph[o + ns];
+        }
+        ph[q] = ph[ns + q] - dz * rhs[q];                        /* :313-315 */
+    }
+    free(rhs);
+}
+
+int orc_tdma_solve(les_ctx* c, const double* hx, const double* hy, const double* hz, double* ph) {
+    tdma_solve(c, (const cplx*)hx, (const cplx*)hy, (const cplx*)hz, (cplx*)ph);
+    return LES_OK;
+}
+
+/* --------------------------------------------------------------------------
  * press_stag_array + tdmaSolver_pipelined, 109_press_stag_array.f90:107-319,
  * then dpdz (103_flosol.f90:174-175)
  * ------------------------------------------------------------------------ */
@@ -1128,32 +1199,8 @@ int orc_press_stag_array(les_ctx* c, int jt) {
         orc_fftn_plane(c, r, Hz + (size_t)nz * ns);
         free(r);
     }
-    cplx* rhs = (cplx*)calloc(ns * nz, sizeof(cplx));           /* RHS_col(0:nz-1) */
-    for (size_t q = 0; q < ns; ++q) {
-        const double k2 = c->k2[q], kx = c->kx2d[q], ky = c->ky2d[q];
-        if (k2 > 0)                                              /* :203-209 */
-            rhs[ns + q] = I * (kx * Hx[ns + q] + ky * Hy[ns + q]) + (Hz[2 * ns + q] - Hz[ns + q] + Hz[q]) / dz;
-        rhs[q] = Hz[q];
-        for (int k = 2; k <= nz - 1; ++k) {                      /* :213-223 */
-            size_t o = (size_t)k * ns + q;
-            if (k2 > 0) rhs[o] = I * (kx * Hx[o] + ky * Hy[o]) + (Hz[o + ns] - Hz[o]) / dz;
-            if (k2 == 0 && c->notnyq[q]) rhs[o] = Hz[o];
-        }
-    }
     cplx* ph = (cplx*)calloc(ns * nz, sizeof(cplx));            /* p_hat(0:nz-1) */
-    for (size_t q = 0; q < ns; ++q) {
-        if (!c->notnyq[q]) continue;                             /* Nyquist stays 0 */
-        for (int k = 1; k <= nz - 1; ++k) {                      /* forward :256-266 */
-            size_t o = (size_t)k * ns + q;
-            rhs[o] = (rhs[o] - c->coef_a[o] * rhs[o - ns]) / c->coef_b[o];
-        }
-        ph[(size_t)(nz - 1) * ns + q] = rhs[(size_t)(nz - 1) * ns + q];   /* c'(nz-1) = 0 */
-        for (int k = nz - 2; k >= 1; --k) {                      /* backward :290-297 */
-            size_t o = (size_t)k * ns + q;
-            ph[o] = rhs[o] - c->coef_c[o] * ph[o + ns];
-        }
-        ph[q] = ph[ns + q] - dz * rhs[q];                        /* :313-315 */
-    }
+    tdma_solve(c, Hx, Hy, Hz, ph);
     double* p = F(c, LES_F_P);
     double *dpdx = F(c, LES_F_DPDX), *dpdy = F(c, LES_F_DPDY), *dpdz = F(c, LES_F_DPDZ);
     orc_ifftn_plane(c, ph, p);                                    /* :231 */
@@ -1173,7 +1220,7 @@ int orc_press_stag_array(les_ctx* c, int jt) {
             size_t o = (size_t)k * np + q;
             dpdz[o] = (p[o] - p[o - np]) / dz;
         }
-    free(Hx); free(Hy); free(Hz); free(rhs); free(ph);
+    free(Hx); free(Hy); free(Hz); free(ph);
     return LES_OK;
 }
 
@@ -1419,6 +1466,33 @@ static double max_lev(les_ctx* c, const double* f) {
     return m;
 }
 
+/* Admas_Bashforth, 103_flosol.f90:319-389: Euler at jt = 1 unless
+ * restart_vel (:331-338), level 0 <- BOGUS (:385-387) */
+int orc_ab2(les_ctx* c, int jt) {
+    const les_params* P = &c->prm;
+    const int nz = c->nz;
+    const size_t np = c->plane, nl = c->nlev;
+    double *RX = F(c, LES_F_RHSX), *RY = F(c, LES_F_RHSY), *RZ = F(c, LES_F_RHSZ);
+    double *RXf = F(c, LES_F_RHSX_F), *RYf = F(c, LES_F_RHSY_F), *RZf = F(c, LES_F_RHSZ_F);
+    double *u = F(c, LES_F_U), *v = F(c, LES_F_V), *w = F(c, LES_F_W);
+    if (jt == 1 && !P->restart_vel) {
+        memcpy(RXf, RX, sizeof(double) * np * nl);
+        memcpy(RYf, RY, sizeof(double) * np * nl);
+        memcpy(RZf, RZ, sizeof(double) * np * nl);
+    }
+    const double force = P->use_mean_p_force ? P->mean_p_force : 0.0;
+    const double tadv1 = 1.5, tadv2 = 1.0 - tadv1;
+    for (int k = 1; k <= nz - 1; ++k)                            /* :369-377 */
+        for (size_t q = 0; q < np; ++q) {
+            size_t o = (size_t)k * np + q;
+            u[o] = u[o] + P->dt * (tadv1 * RX[o] + tadv2 * RXf[o] + 0.0 + force);
+            v[o] = v[o] + P->dt * (tadv1 * RY[o] + tadv2 * RYf[o] + 0.0);
+            w[o] = w[o] + P->dt * (tadv1 * RZ[o] + tadv2 * RZf[o] + 0.0);
+        }
+    for (size_t q = 0; q < np; ++q) u[q] = v[q] = w[q] = ORC_BOGUS;
+    return LES_OK;
+}
+
 int orc_step(les_ctx* c, int jt) {
     const les_params* P = &c->prm;
     const int nz = c->nz;
@@ -1450,7 +1524,6 @@ int orc_step(les_ctx* c, int jt) {
     const double *dx = F(c, LES_F_DIVTX), *dy = F(c, LES_F_DIVTY), *dzt = F(c, LES_F_DIVTZ);
     const double* beta = F(c, LES_F_BETA_SCAL);
     double *u = F(c, LES_F_U), *v = F(c, LES_F_V), *w = F(c, LES_F_W);
-    double *RXf = F(c, LES_F_RHSX_F), *RYf = F(c, LES_F_RHSY_F), *RZf = F(c, LES_F_RHSZ_F);
     const int damp = (P->ubc_mom == LES_BC_STRESS_FREE && P->ubc == 1 && P->damping_method == 1);
     for (int k = 1; k <= nz - 1; ++k)
         for (size_t q = 0; q < np; ++q) {
@@ -1478,21 +1551,7 @@ int orc_step(les_ctx* c, int jt) {
                     c->dvdt[o] = c->dvdt[o] + P->coriol * (P->ug - u[o]);
                 }
             }
-    if (jt == 1 && !P->restart_vel) {                            /* :331-338 */
-        memcpy(RXf, RX, sizeof(double) * np * nl);
-        memcpy(RYf, RY, sizeof(double) * np * nl);
-        memcpy(RZf, RZ, sizeof(double) * np * nl);
-    }
-    const double force = P->use_mean_p_force ? P->mean_p_force : 0.0;
-    const double tadv1 = 1.5, tadv2 = 1.0 - tadv1;
-    for (int k = 1; k <= nz - 1; ++k)                            /* :369-377 */
-        for (size_t q = 0; q < np; ++q) {
-            size_t o = (size_t)k * np + q;
-            u[o] = u[o] + P->dt * (tadv1 * RX[o] + tadv2 * RXf[o] + 0.0 + force);
-            v[o] = v[o] + P->dt * (tadv1 * RY[o] + tadv2 * RYf[o] + 0.0);
-            w[o] = w[o] + P->dt * (tadv1 * RZ[o] + tadv2 * RZf[o] + 0.0);
-        }
-    for (size_t q = 0; q < np; ++q) u[q] = v[q] = w[q] = ORC_BOGUS;   /* :385-387 */
+    orc_ab2(c, jt);                                              /* :319-389 */
     orc_press_stag_array(c, jt);                                 /* :168-175 */
     const double *dpdx = F(c, LES_F_DPDX), *dpdy = F(c, LES_F_DPDY), *dpdz = F(c, LES_F_DPDZ);
     for (int k = 1; k <= nz - 1; ++k)                            /* :179-181 */

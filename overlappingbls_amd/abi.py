@@ -136,6 +136,11 @@ def bind(lib: C.CDLL, prefix: str):
         "time_kernel": [P, i, i, C.POINTER(C.c_double), C.POINTER(C.c_double)],
         "set_avgslice": [P, i, i, i],
         "get_avgslice": [P, i, i, _PD],
+        "ab2": [P, i],
+        "tdma_solve": [P, _PD, _PD, _PD, _PD],
+        "bind_field": [P, i, _PD, i, i, i],
+        "upload": [P],
+        "download": [P],
         "lin_set": [P, i, _PD, i, i, i],
         "lin_get": [P, i, _PD, i, i, i],
         "lin_zero": [P],
@@ -316,6 +321,33 @@ class LesContext:
 
     def press_stag_array(self, jt):
         self._call("press_stag_array", self._h, int(jt))
+
+    def ab2(self, jt):
+        """Admas_Bashforth (103_flosol.f90:319-389) on the context's fields."""
+        self._call("ab2", self._h, int(jt))
+
+    def tdma_solve(self, hx, hy, hz):
+        """p_hat (nzt, nyt, nxt/2+1) complex from Hx, Hy, Hz (nzt+1, nyt, nxt/2+1)."""
+        a = [np.ascontiguousarray(x, dtype=np.complex128) for x in (hx, hy, hz)]
+        out = np.zeros((self.nz, self.ny, self.nx // 2 + 1), dtype=np.complex128)
+        self._call("tdma_solve", self._h, *[x.ctypes.data_as(_PD) for x in a], out.ctypes.data_as(_PD))
+        return out
+
+    def bind(self, name, arr, lev0=0):
+        """Register a caller array (kept alive by this object) for upload/download."""
+        if not hasattr(self, "_bound"):
+            self._bound = []
+        a = arr if name in SURFACE else arr.reshape(-1, self.ny, self.nx)
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+        nlev = 1 if name in SURFACE else a.shape[0]
+        self._call("bind_field", self._h, FIELD[name], _pd(a), self.nx, 0 if name in SURFACE else lev0, nlev)
+        self._bound.append(a)
+
+    def upload(self):
+        self._call("upload", self._h)
+
+    def download(self):
+        self._call("download", self._h)
 
     def project(self):
         self._call("project", self._h)

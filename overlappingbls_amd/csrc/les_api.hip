@@ -752,6 +752,36 @@ int les_get_field(les_ctx* c, int id, double* h, int ld, int lev0, int nlev) {
     return finish(c);
 }
 
+// Registration of the caller's arrays (the reference's module arrays) for
+// bulk synchronisation; the arrays must outlive the context or the next
+// les_bind_field of the same field.
+int les_bind_field(les_ctx* c, int id, double* h, int ld, int lev0, int nlev) {
+    if (!c || !h || ld < c->nx) return LES_E_BADARG;
+    if (valid_field(id)) {
+        if (lev0 < 0 || nlev < 0 || lev0 + nlev > (int)c->nlev) return LES_E_BADARG;
+    } else if (!surf(c, id) || lev0 != 0 || nlev != 1) {
+        return LES_E_BADARG;
+    }
+    c->binds.push_back({id, h, ld, lev0, nlev});
+    return LES_OK;
+}
+int les_upload(les_ctx* c) {
+    if (!c) return LES_E_BADARG;
+    for (const Binding& b : c->binds) {
+        const int st = les_set_field(c, b.id, b.h, b.ld, b.lev0, b.nlev);
+        if (st != LES_OK) return st;
+    }
+    return LES_OK;
+}
+int les_download(les_ctx* c) {
+    if (!c) return LES_E_BADARG;
+    for (const Binding& b : c->binds) {
+        const int st = les_get_field(c, b.id, b.h, b.ld, b.lev0, b.nlev);
+        if (st != LES_OK) return st;
+    }
+    return LES_OK;
+}
+
 int les_set_sgs_init_flags(les_ctx* c, int a, int b) {
     if (!c) return LES_E_BADARG;
     c->f_lm_mm_init = a;
@@ -876,6 +906,33 @@ int les_press_stag_array(les_ctx* c, int jt) {
     (void)jt;
     if (!c) return LES_E_BADARG;
     LES_DISPATCH(press_stag_array, c);
+    return finish(c);
+}
+// Admas_Bashforth (103_flosol.f90:319-389) on the device fields: Euler at
+// jt = 1 unless restart_vel (RHS_f <- RHS, :331-338), level 0 <- BOGUS
+int les_ab2(les_ctx* c, int jt) {
+    if (!c) return LES_E_BADARG;
+    if (jt == 1 && !c->prm.restart_vel) {
+        const size_t n = c->plane * c->nlev * 8;
+        for (int m = 0; m < 3; ++m)
+            les_note(c, hipMemcpyAsync(c->f3[LES_F_RHSX_F + m], c->f3[LES_F_RHSX + m], n, hipMemcpyDeviceToDevice,
+                                       c->stream));
+    }
+    ab2_dev(c);
+    return finish(c);
+}
+// The pressure TDMA alone (109_press_stag_array.f90:201-319): host half
+// spectra Hx, Hy, Hz (levels 0..nzt, complex interleaved, kx fastest, lhx x
+// nyt per level) -> p_hat (levels 0..nzt-1).  FP64.
+int les_tdma_solve(les_ctx* c, const double* hx, const double* hy, const double* hz, double* ph) {
+    if (!c || !hx || !hy || !hz || !ph) return LES_E_BADARG;
+    const size_t bytes = c->ns * 16;
+    les_note(c, hipMemcpyAsync(c->spec3[0], hx, bytes * c->nlev, hipMemcpyHostToDevice, c->stream));
+    les_note(c, hipMemcpyAsync(c->spec3[1], hy, bytes * c->nlev, hipMemcpyHostToDevice, c->stream));
+    les_note(c, hipMemcpyAsync(c->spec3[2], hz, bytes * c->nlev, hipMemcpyHostToDevice, c->stream));
+    tdma_press<double>(c, c->spec3[0], c->spec3[1], c->spec3[2], c->spec3[3],
+                       reinterpret_cast<double*>(c->spec3[3]), nullptr);
+    les_note(c, hipMemcpyAsync(ph, c->spec3[3], bytes * c->nz, hipMemcpyDeviceToHost, c->stream));
     return finish(c);
 }
 int les_project(les_ctx* c) {

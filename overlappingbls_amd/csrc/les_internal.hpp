@@ -57,6 +57,12 @@ struct Tape {
 };
 enum TapePoint : int { TAPE_OBUKHOV = 0, TAPE_WALL = 1, TAPE_THETA = 2, TAPE_END = 3 };
 struct LinCtx;
+// a caller array registered with les_bind_field
+struct Binding {
+    int id;
+    double* h;
+    int ld, lev0, nlev;
+};
 
 }  // namespace les
 
@@ -116,6 +122,7 @@ struct les_ctx {
     double cfl;
     hipError_t err;            // first asynchronous HIP error since the last finish()
     std::vector<void*> allocs;
+    std::vector<les::Binding> binds;
 };
 
 // Record the first failing HIP runtime call of a context.

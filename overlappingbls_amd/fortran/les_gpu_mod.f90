@@ -221,6 +221,31 @@ interface
         integer(c_int), value :: slice, window
         real(c_double), intent(out) :: host(*)
     end function
+    integer(c_int) function les_ab2(ctx, jt) bind(C, name="les_ab2")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: jt
+    end function
+    integer(c_int) function les_tdma_solve(ctx, hx, hy, hz, ph) bind(C, name="les_tdma_solve")
+        import :: c_ptr, c_int, c_double
+        type(c_ptr), value :: ctx
+        real(c_double), intent(in) :: hx(*), hy(*), hz(*)
+        real(c_double), intent(out) :: ph(*)
+    end function
+    integer(c_int) function les_bind_field(ctx, f, h, ld, lev0, nlev) bind(C, name="les_bind_field")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: f, ld, lev0, nlev
+        real(c_double), intent(inout), target :: h(*)
+    end function
+    integer(c_int) function les_upload(ctx) bind(C, name="les_upload")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function les_download(ctx) bind(C, name="les_download")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+    end function
     ! ---- tangent-linear model and adjoint (no reference counterpart) ----
     integer(c_int) function les_lin_set(ctx, f, h, ld, lev0, nlev) bind(C, name="les_lin_set")
         import :: c_int, c_ptr, c_double

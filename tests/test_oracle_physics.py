@@ -58,9 +58,38 @@ def dense_pressure(P, u, v, w, divtz):
         Hz[k] = fftn(cst / 1.5 * (w[k] / P.dt))
     Hz[0] = fftn(cst * divtz[1])
     Hz[nz] = fftn(cst * divtz[nz])
+    ph = dense_tdma(P, Hx, Hy, Hz)
+    kx, ky = wavenumbers(P)
+
+    def ifftn(s):
+        t = np.fft.ifft(s, axis=0) * ny
+        t[:, 0] = t[:, 0].real; t[:, -1] = t[:, -1].real
+        return np.fft.irfft(t, n=nx, axis=1) * nx
+
+    KX = np.broadcast_to(kx[None, :], (ny, lhx)).copy(); KX[ny // 2] = 0
+    KY = np.broadcast_to(ky[:, None], (ny, lhx)).copy(); KY[:, -1] = 0; KY[ny // 2] = 0
+    p = np.array([ifftn(ph[k]) for k in range(nz)])
+    dpdx = np.array([ifftn(1j * KX * ph[k]) for k in range(nz)])
+    dpdy = np.array([ifftn(1j * KY * ph[k]) for k in range(nz)])
+    return p, dpdx, dpdy, ph
+
+
+def wavenumbers(P):
+    nx, ny = P.nxt, P.nyt
+    lhx = nx // 2 + 1
     kx = np.array([i * 2 * math.pi / P.lx_tot for i in range(lhx)]); kx[-1] = 0
     m = np.array([((j + ny // 2) % ny) - ny // 2 for j in range(ny)])
     ky = m * 2 * math.pi / P.ly_tot
+    return kx, ky
+
+
+def dense_tdma(P, Hx, Hy, Hz):
+    """p_hat from half spectra Hx, Hy, Hz by dense solves of the systems of
+    init_coef_press (109:4-68) with press_stag_array's right-hand side."""
+    nx, ny, nz = P.nxt, P.nyt, P.nzt
+    dz = P.lz_tot / (nz - 1)
+    lhx = nx // 2 + 1
+    kx, ky = wavenumbers(P)
     ph = np.zeros((nz, ny, lhx), complex)
     for j in range(ny):
         for i in range(lhx):
@@ -90,18 +119,53 @@ def dense_pressure(P, u, v, w, divtz):
             sol = np.linalg.solve(A, r)
             ph[1:, j, i] = sol
             ph[0, j, i] = sol[0] - dz * Hz[0, j, i]
+    return ph
 
-    def ifftn(s):
-        t = np.fft.ifft(s, axis=0) * ny
-        t[:, 0] = t[:, 0].real; t[:, -1] = t[:, -1].real
-        return np.fft.irfft(t, n=nx, axis=1) * nx
 
-    KX = np.broadcast_to(kx[None, :], (ny, lhx)).copy(); KX[ny // 2] = 0
-    KY = np.broadcast_to(ky[:, None], (ny, lhx)).copy(); KY[:, -1] = 0; KY[ny // 2] = 0
-    p = np.array([ifftn(ph[k]) for k in range(nz)])
-    dpdx = np.array([ifftn(1j * KX * ph[k]) for k in range(nz)])
-    dpdy = np.array([ifftn(1j * KY * ph[k]) for k in range(nz)])
-    return p, dpdx, dpdy, ph
+def random_spectra(nz, ny, lhx, seed):
+    sh = (nz + 1, ny, lhx)
+    return [config.uniform(seed, 2 * i, sh) + 1j * config.uniform(seed, 2 * i + 1, sh) for i in range(3)]
+
+
+@pytest.mark.parametrize("nx,ny,nzt", [(16, 12, 8), (12, 16, 5)])
+def test_tdma_solve_matches_dense_solve(oracle_mod, nx, ny, nzt):
+    nl, P, o = mk(oracle_mod, nx, ny, nzt)
+    Hx, Hy, Hz = random_spectra(nzt, ny, nx // 2 + 1, 21)
+    ph = o.tdma_solve(Hx, Hy, Hz)
+    ref = dense_tdma(P, Hx, Hy, Hz)
+    assert np.abs(ph - ref).max() < 1e-12 * np.abs(ref).max()
+    # Nyquist column and row stay zero
+    assert not ph[:, :, -1].any() and not ph[:, ny // 2, :].any()
+
+
+def test_ab2_closed_form(oracle_mod):
+    nl, P, o = mk(oracle_mod, 12, 8, 6)
+    sh = (7, 8, 12)
+    u, r, rf = (config.uniform(31, i, sh) for i in range(3))
+    for n, a in (("U", u), ("V", u), ("W", u), ("RHSX", r), ("RHSY", r), ("RHSZ", r),
+                 ("RHSX_F", rf), ("RHSY_F", rf), ("RHSZ_F", rf)):
+        o.set(n, a)
+    o.ab2(2)
+    exp = u + P.dt * (1.5 * r - 0.5 * rf)
+    assert np.allclose(o.get("U")[1:6], exp[1:6], rtol=0, atol=1e-15)
+    assert (o.get("U")[0] == -1234567890.0).all()
+    o.set("U", u)
+    o.ab2(1)     # Euler start: RHS_f <- RHS
+    assert np.allclose(o.get("U")[1:6], (u + P.dt * r)[1:6], rtol=0, atol=1e-15)
+    assert np.array_equal(o.get("RHSX_F"), o.get("RHSX"))
+
+
+def test_bind_upload_download(oracle_mod):
+    nl, P, o = mk(oracle_mod, 12, 8, 6)
+    u = np.ascontiguousarray(config.uniform(41, 0, (7, 8, 12)))
+    ps = np.ascontiguousarray(config.uniform(41, 1, (1, 8, 12))[0])
+    o.bind("U", u)
+    o.bind("PSI_M_SFC", ps)
+    o.upload()
+    assert np.array_equal(o.get("U"), u) and np.array_equal(o.get("PSI_M_SFC"), ps)
+    o.set("U", 2.0 * u)
+    o.download()
+    assert np.array_equal(u, 2.0 * config.uniform(41, 0, (7, 8, 12)))
 
 
 @pytest.mark.parametrize("nx,ny,nzt", [(16, 12, 8), (12, 16, 5)])
