@@ -55,6 +55,9 @@ public :: les_create, les_destroy, les_set_field, les_get_field, les_step, les_r
           les_fftn, les_ifftn, les_zero_padding, les_zero_unpadding, les_test_filter, &
           les_obukhov, les_wallstress_bottom, les_wallstress_top, les_divstress_uv, &
           les_divstress_w, les_set_stokes, les_get_cfl, les_set_avgslice, les_get_avgslice, &
+          les_ab2, les_tdma_solve, les_bind_field, les_upload, les_download, &
+          les_lin_set, les_lin_get, les_lin_zero, les_set_sgs_frozen, les_tlm_window, &
+          les_adjoint_window, les_step_tlm, les_step_adj, &
           les_check
 
 interface
