@@ -1051,41 +1051,59 @@ template <typename W>
 struct RhsArgs {
     double *rx, *ry, *rz, *rxf, *ryf, *rzf;
     const W *cx, *cy, *cz, *dx, *dy, *dz, *beta;
-    const double *u, *v, *w, *sponge;
+    const double* sponge;
     double *du, *dv, *dw;  // dudt diagnostics (103_flosol.f90:119-156), or null
 };
 
+// RHS assembly (103_flosol.f90:115-157) fused with Admas_Bashforth
+// (:319-389): each point forms RHS from the same-point fields, stores it (and
+// RHS_f at an Euler start, the dudt diagnostics on sampling steps), then
+// advances u, v, w with it; level 0 of u, v, w <- BOGUS.  The arithmetic and
+// its order are those of the two loops.
 template <typename W>
-__global__ void k_rhs(RhsArgs<W> a, KP p, int buoy, int cori, double coriol, double ug, double vg, int damp,
-                      int euler) {
-    const long n = p.plane * (p.nz - 1);
-    GRID_LOOP(t, n) {
-        const long i = p.plane + t;
+__global__ void k_rhs_ab2(RhsArgs<W> a, double* u, double* v, double* w, KP p, int buoy, int cori, double coriol,
+                          double ug, double vg, int damp, int euler, double force) {
+    const long n = p.plane * p.nz;  // levels 0..nz-1
+    const double tadv1 = 1.5, tadv2 = 1.0 - tadv1;
+    GRID_LOOP(i, n) {
+        if (i < p.plane) {
+            u[i] = BOGUS; v[i] = BOGUS; w[i] = BOGUS;
+            continue;
+        }
         const int k = (int)(i / p.plane);
         double rx = -(double)a.cx[i] - (double)a.dx[i];
         double ry = -(double)a.cy[i] - (double)a.dy[i];
         double rz = -(double)a.cz[i] - (double)a.dz[i];
         if (buoy) rz = rz + a.beta[i];
         if (cori) {
-            rx = rx + coriol * (a.v[i] - vg);
-            ry = ry + coriol * (ug - a.u[i]);
+            rx = rx + coriol * (v[i] - vg);
+            ry = ry + coriol * (ug - u[i]);
         }
-        if (damp) rz = rz - a.sponge[k] * a.w[i];
+        if (damp) rz = rz - a.sponge[k] * w[i];
         a.rx[i] = rx; a.ry[i] = ry; a.rz[i] = rz;
         if (a.du) {
             double du = -(double)a.dx[i], dv = -(double)a.dy[i];
             if (cori) {
-                du = du + coriol * (a.v[i] - vg);
-                dv = dv + coriol * (ug - a.u[i]);
+                du = du + coriol * (v[i] - vg);
+                dv = dv + coriol * (ug - u[i]);
             }
             a.du[i] = du; a.dv[i] = dv; a.dw[i] = -(double)a.dz[i];
         }
-        if (euler) { a.rxf[i] = rx; a.ryf[i] = ry; a.rzf[i] = rz; }
+        double rxf, ryf, rzf;
+        if (euler) {
+            a.rxf[i] = rx; a.ryf[i] = ry; a.rzf[i] = rz;
+            rxf = rx; ryf = ry; rzf = rz;
+        } else {
+            rxf = a.rxf[i]; ryf = a.ryf[i]; rzf = a.rzf[i];
+        }
+        u[i] = u[i] + p.dt * (tadv1 * rx + tadv2 * rxf + 0.0 + force);
+        v[i] = v[i] + p.dt * (tadv1 * ry + tadv2 * ryf + 0.0);
+        w[i] = w[i] + p.dt * (tadv1 * rz + tadv2 * rzf + 0.0);
     }
 }
 
 template <typename W>
-void rhs_assembly_dev(les_ctx* c, int jt) {
+void rhs_ab2_dev(les_ctx* c, int jt) {
     KP p = kp(c);
     const les_params& P = c->prm;
     RhsArgs<W> a;
@@ -1098,11 +1116,12 @@ void rhs_assembly_dev(les_ctx* c, int jt) {
     a.du = st ? c->st_diag[0] : nullptr;
     a.dv = st ? c->st_diag[1] : nullptr;
     a.dw = st ? c->st_diag[2] : nullptr;
-    a.u = c->f3[LES_F_U]; a.v = c->f3[LES_F_V]; a.w = c->f3[LES_F_W]; a.sponge = c->sponge;
+    a.sponge = c->sponge;
     const int damp = (P.ubc_mom == LES_BC_STRESS_FREE && P.ubc == 1 && P.damping_method == 1);
-    hipLaunchKernelGGL(k_rhs<W>, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, a, p,
-                       (int)(P.theta_flag && !P.passive_scalar), P.coriolis_forcing, P.coriol, P.ug, P.vg, damp,
-                       (int)(jt == 1 && !P.restart_vel));
+    hipLaunchKernelGGL(k_rhs_ab2<W>, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, a, c->f3[LES_F_U],
+                       c->f3[LES_F_V], c->f3[LES_F_W], p, (int)(P.theta_flag && !P.passive_scalar),
+                       P.coriolis_forcing, P.coriol, P.ug, P.vg, damp, (int)(jt == 1 && !P.restart_vel),
+                       P.use_mean_p_force ? P.mean_p_force : 0.0);
 }
 
 __global__ void k_ab2(double* u, double* v, double* w, const double* __restrict__ rx,
@@ -1131,33 +1150,31 @@ void ab2_dev(les_ctx* c) {
                        P.use_mean_p_force ? P.mean_p_force : 0.0);
 }
 
-// RHS -= grad p (103_flosol.f90:179-181); du.. (null or the dudt
-// diagnostics) likewise (:183-185)
+// RHS -= grad p (103_flosol.f90:179-181; the dudt diagnostics likewise,
+// :183-185, when du != null) fused with project (110_project.f90:16-40):
+// u, v on levels 1..nz-1, w from level 2
 template <typename W>
-__global__ void k_rhs_gradp(double* rx, double* ry, double* rz, const W* __restrict__ px, const W* __restrict__ py,
-                            const W* __restrict__ pz, KP p, double* du, double* dv, double* dw) {
+__global__ void k_gradp_project(double* rx, double* ry, double* rz, double* u, double* v, double* w,
+                                const W* __restrict__ px, const W* __restrict__ py, const W* __restrict__ pz, KP p,
+                                double* du, double* dv, double* dw) {
     const long n = p.plane * (p.nz - 1);
+    const double tadv1 = 1.5;
     GRID_LOOP(t, n) {
         const long i = p.plane + t;
-        rx[i] = rx[i] - px[i];
-        ry[i] = ry[i] - py[i];
-        rz[i] = rz[i] - pz[i];
+        const int k = (int)(i / p.plane);
+        const double gx = px[i], gy = py[i], gz = pz[i];
+        rx[i] = rx[i] - gx;
+        ry[i] = ry[i] - gy;
+        rz[i] = rz[i] - gz;
         if (du) {
-            du[i] = du[i] - px[i];
-            dv[i] = dv[i] - py[i];
-            dw[i] = dw[i] - pz[i];
+            du[i] = du[i] - gx;
+            dv[i] = dv[i] - gy;
+            dw[i] = dw[i] - gz;
         }
+        u[i] = u[i] + p.dt * (-tadv1 * gx);
+        v[i] = v[i] + p.dt * (-tadv1 * gy);
+        if (k >= 2) w[i] = w[i] + p.dt * (-tadv1 * gz);
     }
-}
-
-template <typename W>
-void rhs_minus_gradp_dev(les_ctx* c, int jt) {
-    KP p = kp(c);
-    const bool st = stats_step(c, jt);
-    hipLaunchKernelGGL(k_rhs_gradp<W>, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, c->f3[LES_F_RHSX],
-                       c->f3[LES_F_RHSY], c->f3[LES_F_RHSZ], (const W*)WF<W>(c, LES_F_DPDX),
-                       (const W*)WF<W>(c, LES_F_DPDY), (const W*)WF<W>(c, LES_F_DPDZ), p,
-                       st ? c->st_diag[0] : nullptr, st ? c->st_diag[1] : nullptr, st ? c->st_diag[2] : nullptr);
 }
 
 // project (110_project.f90:16-40) and the boundary rows that follow
@@ -1193,6 +1210,21 @@ __global__ void k_project_bc(double* u, double* v, double* w, double* th, KP p, 
         u[top] = BOGUS; v[top] = BOGUS;
         if (theta) { th[q] = BOGUS; th[top] = BOGUS; }
     }
+}
+
+template <typename W>
+void gradp_project_dev(les_ctx* c, int jt) {
+    KP p = kp(c);
+    const les_params& P = c->prm;
+    const bool st = stats_step(c, jt);
+    hipLaunchKernelGGL(k_gradp_project<W>, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream,
+                       c->f3[LES_F_RHSX], c->f3[LES_F_RHSY], c->f3[LES_F_RHSZ], c->f3[LES_F_U], c->f3[LES_F_V],
+                       c->f3[LES_F_W], (const W*)WF<W>(c, LES_F_DPDX), (const W*)WF<W>(c, LES_F_DPDY),
+                       (const W*)WF<W>(c, LES_F_DPDZ), p, st ? c->st_diag[0] : nullptr,
+                       st ? c->st_diag[1] : nullptr, st ? c->st_diag[2] : nullptr);
+    hipLaunchKernelGGL(k_project_bc, grid_for(p.plane), dim3(256), 0, c->stream, c->f3[LES_F_U], c->f3[LES_F_V],
+                       c->f3[LES_F_W], c->f3[LES_F_THETA], p, (int)(P.ubc_mom == LES_BC_STRESS_FREE),
+                       P.theta_flag);
 }
 
 template <typename W>
@@ -1288,8 +1320,8 @@ void max3_dev(les_ctx* c, double* out) {
     template void dpdz_on<W>(les_ctx*, W*, const W*);                                                       \
     template void press_hz_sums_dev<W>(les_ctx*, double*);                                                  \
     template void dpdz_dev<W>(les_ctx*);                                                                    \
-    template void rhs_assembly_dev<W>(les_ctx*, int);                                                       \
-    template void rhs_minus_gradp_dev<W>(les_ctx*, int);                                                         \
+    template void rhs_ab2_dev<W>(les_ctx*, int);                                                            \
+    template void gradp_project_dev<W>(les_ctx*, int);                                                      \
     template void project_dev<W>(les_ctx*);
 LES_INST_POINT(double)
 LES_INST_POINT(float)

@@ -465,11 +465,9 @@ void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     divstress<W>(c, WF<W>(c, LES_F_DIVTZ), txz, tyz, tzz, 1);
     convec<W>(c);
     theta_post<W>(c, jt, tape);
-    rhs_assembly_dev<W>(c, jt);
-    ab2_dev(c);
+    rhs_ab2_dev<W>(c, jt);
     press_stag_array<W>(c);
-    rhs_minus_gradp_dev<W>(c, jt);
-    project_dev<W>(c);
+    gradp_project_dev<W>(c, jt);
     const bool st = stats && stats_step(c, jt);
     if (st) diagnostics_ke(c);
     filter_f<double>(c, F(c, LES_F_U), 0, nl);

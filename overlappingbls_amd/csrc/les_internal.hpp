@@ -239,9 +239,10 @@ void press_rhs_prep_on(les_ctx* c, W* rx, W* ry, W* rz, const double* u, const d
 template <typename W> void dpdz_on(les_ctx* c, W* dpdz, const W* p);
 template <typename W> void press_hz_sums_dev(les_ctx* c, double* out);
 template <typename W> void dpdz_dev(les_ctx* c);
-template <typename W> void rhs_assembly_dev(les_ctx* c, int jt);
+// RHS assembly + Admas_Bashforth, and RHS -= grad p + project, each one pass
+template <typename W> void rhs_ab2_dev(les_ctx* c, int jt);
 void ab2_dev(les_ctx* c);
-template <typename W> void rhs_minus_gradp_dev(les_ctx* c, int jt);
+template <typename W> void gradp_project_dev(les_ctx* c, int jt);
 template <typename W> void project_dev(les_ctx* c);
 void max3_dev(les_ctx* c, double* out);
 
