@@ -265,15 +265,24 @@ __global__ __launch_bounds__((rg_threads<T, NIN, NOUT, false>())) void kr_xline(
     load_twr(twi, twin_g, NIN);
     if (NIN != NOUT) load_twr(twop, twout_g, NOUT);
     const long row0 = (long)blockIdx.x * 2 * NL;
+    // first element of each of the block's rows (the shift of a reduced-
+    // precision derivative), staged once instead of reloaded per element
+    __shared__ TI first[SHIFT_IN<T, TI, OP> ? 2 * NL : 1];
+    if constexpr (SHIFT_IN<T, TI, OP>) {
+        for (int t = threadIdx.x; t < 2 * NL; t += NTH) {
+            const long r = row0 + t;
+            first[t] = (r < nrows) ? in[r * ld_in] : TI(0);
+        }
+    }
     auto ld = [&](int line, int p) {
         const long ra = row0 + 2 * line, rb = ra + 1;
         T a = T(0), b = T(0);
         if (ra < nrows) {
-            if constexpr (SHIFT_IN<T, TI, OP>) a = (T)(in[ra * ld_in + p] - in[ra * ld_in]);
+            if constexpr (SHIFT_IN<T, TI, OP>) a = (T)(in[ra * ld_in + p] - first[2 * line]);
             else a = (T)in[ra * ld_in + p];
         }
         if (rb < nrows) {
-            if constexpr (SHIFT_IN<T, TI, OP>) b = (T)(in[rb * ld_in + p] - in[rb * ld_in]);
+            if constexpr (SHIFT_IN<T, TI, OP>) b = (T)(in[rb * ld_in + p] - first[2 * line + 1]);
             else b = (T)in[rb * ld_in + p];
         }
         return cx<T>{a, b};
