@@ -165,10 +165,10 @@ __global__ __launch_bounds__(NT) void k_yline_real(const TI* __restrict__ in, T*
 template <typename T, typename TI>
 __global__ __launch_bounds__(NT) void k_x_r2c(const TI* __restrict__ in, cx<T>* __restrict__ out, long nrows,
                                               int ld_in, int ldc, FftPlan pl, const cx<T>* __restrict__ tw_g,
-                                              T scale, int P) {
+                                              T scale, int P, int kst) {
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
-    const int n = pl.n, lh = n / 2 + 1;
+    const int n = pl.n, lh = kst;
     cx<T>* tw = buf + P * n;
     load_tw(tw, tw_g, n);
     const long row0 = (long)blockIdx.x * 2 * P;
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(NT) void k_x_r2c(const TI* __restrict__ in, cx<T>* 
 template <typename T>
 __global__ __launch_bounds__(NT) void k_x_c2r(const cx<T>* __restrict__ in, T* __restrict__ out, long nrows,
                                               int ldc, int ld_out, FftPlan pl, const cx<T>* __restrict__ tw_g,
-                                              int deriv, T kfac, int P) {
+                                              int deriv, T kfac, int P, int kst) {
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int n = pl.n;
@@ -213,8 +213,8 @@ __global__ __launch_bounds__(NT) void k_x_c2r(const cx<T>* __restrict__ in, T* _
         const int line = e / n, k = e - line * n;
         const int kk = (2 * k <= n) ? k : n - k;
         const long ga = row0 + 2 * line, gb = ga + 1;
-        cx<T> a = (ga < nrows) ? in[ga * ldc + kk] : cx<T>{T(0), T(0)};
-        cx<T> b = (gb < nrows) ? in[gb * ldc + kk] : cx<T>{T(0), T(0)};
+        cx<T> a = (ga < nrows && kk < kst) ? in[ga * ldc + kk] : cx<T>{T(0), T(0)};
+        cx<T> b = (gb < nrows && kk < kst) ? in[gb * ldc + kk] : cx<T>{T(0), T(0)};
         if (kk == 0 || 2 * kk == n) { a.y = T(0); b.y = T(0); }
         if (deriv) {
             const T f = (2 * kk == n) ? T(0) : (T)kk * kfac;
@@ -235,8 +235,9 @@ __global__ __launch_bounds__(NT) void k_x_c2r(const cx<T>* __restrict__ in, T* _
 
 // complex y-lines of the half spectrum (203_fftn_mpi.f90:29-33, :52-54).
 template <typename T>
-__global__ __launch_bounds__(NT) void k_yline_spec(const cx<T>* __restrict__ in, cx<T>* __restrict__ out, int lhx,
-                                                   long ps, FftPlan pl, const cx<T>* __restrict__ tw_g, int op,
+__global__ __launch_bounds__(NT) void k_yline_spec(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
+                                                   int ncols, int ld, int ldG, int mc, long ps, FftPlan pl,
+                                                   const cx<T>* __restrict__ tw_g, int op,
                                                    const double* __restrict__ G, int C) {
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
@@ -246,10 +247,12 @@ __global__ __launch_bounds__(NT) void k_yline_spec(const cx<T>* __restrict__ in,
     const int x0 = blockIdx.x * C;
     const cx<T>* src = in + (long)blockIdx.y * ps;
     cx<T>* dst = out + (long)blockIdx.y * ps;
+    auto keep = [&](int y) { return y < mc || y > ny - mc; };
     for (int e = threadIdx.x; e < ny * C; e += NT) {
         const int y = e / C, cl = e - y * C;
         const int x = x0 + cl;
-        buf[cl * ny + y] = (x < lhx) ? src[(long)y * lhx + x] : cx<T>{T(0), T(0)};
+        const bool rd = x < ncols && (op != SP_MULINV || keep(y));
+        buf[cl * ny + y] = rd ? src[(long)y * ld + x] : cx<T>{T(0), T(0)};
     }
     __syncthreads();
     if (op == SP_FWD || op == SP_FILT) {
@@ -261,7 +264,7 @@ __global__ __launch_bounds__(NT) void k_yline_spec(const cx<T>* __restrict__ in,
         for (int e = threadIdx.x; e < ny * C; e += NT) {
             const int cl = e / ny, y = e - cl * ny;
             const int x = x0 + cl;
-            const T g = (x < lhx) ? (T)G[(long)y * lhx + x] : T(0);
+            const T g = (x < ncols) ? (T)G[(long)y * ldG + x] : T(0);
             buf[e].x *= g;
             buf[e].y *= g;
         }
@@ -271,7 +274,7 @@ __global__ __launch_bounds__(NT) void k_yline_spec(const cx<T>* __restrict__ in,
     for (int e = threadIdx.x; e < ny * C; e += NT) {
         const int y = e / C, cl = e - y * C;
         const int x = x0 + cl;
-        if (x < lhx) dst[(long)y * lhx + x] = buf[cl * ny + y];
+        if (x < ncols && (op != SP_FWD || keep(y))) dst[(long)y * ld + x] = buf[cl * ny + y];
     }
 }
 
@@ -316,35 +319,40 @@ void yline_real(hipStream_t s, const TI* in, T* out, int nplanes, int ncols, int
 
 template <typename T, typename TI>
 void x_r2c(hipStream_t s, const TI* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
-           double scale) {
-    if (g_use_ct && x_r2c_ct<T, TI>(s, in, out, nrows, ld_in, ldc, tw, scale)) return;
+           double scale, int kst) {
+    if (kst < 0) kst = tw.n / 2 + 1;
+    if (g_use_ct && x_r2c_ct<T, TI>(s, in, out, nrows, ld_in, ldc, tw, scale, kst)) return;
     const int P = lines_per_block(tw.n);
     const long nb = (nrows + 2 * P - 1) / (2 * P);
     const size_t lds = sizeof(cx<T>) * ((size_t)P * tw.n + tw.n);
     hipLaunchKernelGGL((k_x_r2c<T, TI>), dim3((unsigned)nb), dim3(NT), lds, s, in, out, nrows, ld_in, ldc, tw.plan,
-                       twp<T>(tw), (T)scale, P);
+                       twp<T>(tw), (T)scale, P, kst);
 }
 
 template <typename T>
 void x_c2r(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int ld_out, const Twiddle& tw, int deriv,
-           double kfac) {
-    if (g_use_ct && x_c2r_ct<T>(s, in, out, nrows, ldc, ld_out, tw, deriv, kfac)) return;
+           double kfac, int kst) {
+    if (kst < 0) kst = tw.n / 2 + 1;
+    if (g_use_ct && x_c2r_ct<T>(s, in, out, nrows, ldc, ld_out, tw, deriv, kfac, kst)) return;
     const int P = lines_per_block(tw.n);
     const long nb = (nrows + 2 * P - 1) / (2 * P);
     const size_t lds = sizeof(cx<T>) * ((size_t)P * tw.n + tw.n);
     hipLaunchKernelGGL(k_x_c2r<T>, dim3((unsigned)nb), dim3(NT), lds, s, in, out, nrows, ldc, ld_out, tw.plan,
-                       twp<T>(tw), deriv, (T)kfac, P);
+                       twp<T>(tw), deriv, (T)kfac, P, kst);
 }
 
 template <typename T>
-void yline_spec(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lhx, long ps, const Twiddle& tw,
-                SpecOp op, const double* G) {
-    if (g_use_ct && yline_spec_ct<T>(s, in, out, nplanes, lhx, ps, tw, op, G)) return;
+void yline_spec(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int ncols, long ps, const Twiddle& tw,
+                SpecOp op, const double* G, int ld, int ldG, int mc) {
+    if (ld < 0) ld = ncols;
+    if (ldG < 0) ldG = ld;
+    if (mc < 0) mc = tw.n;
+    if (g_use_ct && yline_spec_ct<T>(s, in, out, nplanes, ncols, ps, tw, op, G, ld, ldG, mc)) return;
     const int C = lines_per_block(tw.n);
-    const int nbx = (lhx + C - 1) / C;
+    const int nbx = (ncols + C - 1) / C;
     const size_t lds = sizeof(cx<T>) * ((size_t)C * tw.n + tw.n);
-    hipLaunchKernelGGL(k_yline_spec<T>, dim3(nbx, nplanes), dim3(NT), lds, s, in, out, lhx, ps, tw.plan,
-                       twp<T>(tw), (int)op, G, C);
+    hipLaunchKernelGGL(k_yline_spec<T>, dim3(nbx, nplanes), dim3(NT), lds, s, in, out, ncols, ld, ldG, mc, ps,
+                       tw.plan, twp<T>(tw), (int)op, G, C);
 }
 
 #define LES_INST_LINES(T, TI)                                                                                   \
@@ -352,16 +360,17 @@ void yline_spec(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lhx
                                     LineOp, double);                                                               \
     template void yline_real<T, TI>(hipStream_t, const TI*, T*, int, int, int, int, long, long, const Twiddle&,      \
                                     const Twiddle&, LineOp, double);                                               \
-    template void x_r2c<T, TI>(hipStream_t, const TI*, cx<T>*, long, int, int, const Twiddle&, double);
+    template void x_r2c<T, TI>(hipStream_t, const TI*, cx<T>*, long, int, int, const Twiddle&, double, int);
 LES_INST_LINES(double, double)
 LES_INST_LINES(float, float)
 LES_INST_LINES(float, double)
 #undef LES_INST_LINES
-template void x_c2r<double>(hipStream_t, const cx<double>*, double*, long, int, int, const Twiddle&, int, double);
-template void x_c2r<float>(hipStream_t, const cx<float>*, float*, long, int, int, const Twiddle&, int, double);
+template void x_c2r<double>(hipStream_t, const cx<double>*, double*, long, int, int, const Twiddle&, int, double,
+                            int);
+template void x_c2r<float>(hipStream_t, const cx<float>*, float*, long, int, int, const Twiddle&, int, double, int);
 template void yline_spec<double>(hipStream_t, const cx<double>*, cx<double>*, int, int, long, const Twiddle&,
-                                 SpecOp, const double*);
+                                 SpecOp, const double*, int, int, int);
 template void yline_spec<float>(hipStream_t, const cx<float>*, cx<float>*, int, int, long, const Twiddle&,
-                                SpecOp, const double*);
+                                SpecOp, const double*, int, int, int);
 
 }  // namespace les

@@ -18,6 +18,21 @@ namespace {
 constexpr int round64(int n) { return ((n + 63) / 64) * 64; }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
+// A first-stage accessor in two parts: raw(line, position) reads global memory
+// in the input precision, cv(line, value) converts to the transform input.
+// Stage::load issues every raw read before the first conversion; fused into
+// one accessor, the compiler converts each value as it arrives and waits on
+// each load in turn, which halves the bandwidth of FP64-input kernels.
+template <typename RAW, typename CV>
+struct SplitLoad {
+    RAW raw;
+    CV cv;
+};
+template <typename RAW, typename CV>
+__device__ __forceinline__ SplitLoad<RAW, CV> split_load(const RAW& raw, const CV& cv) {
+    return {raw, cv};
+}
+
 // ---- one Stockham stage: butterflies distributed over the block --------------
 // Butterfly (line, b), b < N/R: inputs at b + r N/R, twiddles W_{NS R}^{r (b mod NS)},
 // outputs at (b / NS) NS R + b mod NS + r NS.  YM: lines vary fastest across
@@ -65,6 +80,30 @@ struct Stage {
             if (!bf(q, line, b)) continue;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[q][r] = ld(line, b + r * NB);
+        }
+    }
+
+    // inputs from a split accessor: all reads first, then the conversions
+    template <typename RAW, typename CV>
+    __device__ __forceinline__ void load(const SplitLoad<RAW, CV>& ld) {
+        using P = decltype(ld.raw(0, 0));
+        P t[PB][R];
+        // threads without a butterfly in this pass read line 0 (no branch
+        // around the reads) and discard it
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            int line = 0, b = 0;
+            if (!bf(q, line, b)) line = b = 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) t[q][r] = ld.raw(line, b + r * NB);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            int line, b;
+            if (!bf(q, line, b)) continue;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = ld.cv(line, t[q][r]);
         }
     }
 
@@ -124,6 +163,15 @@ __device__ __forceinline__ void middle_stages(cx<T>* buf, const cx<T>* tw) {
         s1.template write<LS>(buf);
         __syncthreads();
     }
+}
+
+// The pad kernels read their twiddle tables through the vector cache instead
+// of staging them in LDS: the freed LDS admits more blocks per CU (measured
+// at 256 -> 384: FP32 y pass 78 -> 55 us, FP64 x pass 99 -> 89 us).  The
+// unpad and same-length kernels are faster with the tables in LDS.
+template <int NIN, int NOUT>
+constexpr bool tw_global() {
+    return NIN < NOUT;
 }
 
 template <typename T>
@@ -259,11 +307,17 @@ __global__ __launch_bounds__((rg_threads<T, NIN, NOUT, false>())) void kr_xline(
     constexpr int NTH = rg_threads<T, NIN, NOUT, false>();
     constexpr int LS = rg_ls(cmax(NIN, NOUT));
     __shared__ cx<T> buf[NL * LS];
-    __shared__ cx<T> twi[NIN];
-    __shared__ cx<T> two[NIN == NOUT ? 1 : NOUT];
-    cx<T>* twop = (NIN == NOUT) ? twi : two;
-    load_twr(twi, twin_g, NIN);
-    if (NIN != NOUT) load_twr(twop, twout_g, NOUT);
+    constexpr bool TWG = tw_global<NIN, NOUT>();
+    __shared__ cx<T> twl[TWG ? 1 : NIN];
+    __shared__ cx<T> twl_o[(TWG || NIN == NOUT) ? 1 : NOUT];
+    const cx<T>* twi = twin_g;
+    const cx<T>* twop = twout_g;
+    if constexpr (!TWG) {
+        load_twr(twl, twin_g, NIN);
+        if (NIN != NOUT) load_twr(twl_o, twout_g, NOUT);
+        twi = twl;
+        twop = (NIN == NOUT) ? twl : twl_o;
+    }
     const long row0 = (long)blockIdx.x * 2 * NL;
     // first element of each of the block's rows (the shift of a reduced-
     // precision derivative), staged once instead of reloaded per element
@@ -274,18 +328,20 @@ __global__ __launch_bounds__((rg_threads<T, NIN, NOUT, false>())) void kr_xline(
             first[t] = (r < nrows) ? in[r * ld_in] : TI(0);
         }
     }
-    auto ld = [&](int line, int p) {
+    using PI = typename pair_t<TI>::type;
+    // rows 2 line, 2 line + 1 at position p (zero past the last row).  The
+    // reads are unconditional, from the last row when past it, so that no
+    // branch separates them.
+    auto raw = [&](int line, int p) {
         const long ra = row0 + 2 * line, rb = ra + 1;
-        T a = T(0), b = T(0);
-        if (ra < nrows) {
-            if constexpr (SHIFT_IN<T, TI, OP>) a = (T)(in[ra * ld_in + p] - first[2 * line]);
-            else a = (T)in[ra * ld_in + p];
-        }
-        if (rb < nrows) {
-            if constexpr (SHIFT_IN<T, TI, OP>) b = (T)(in[rb * ld_in + p] - first[2 * line + 1]);
-            else b = (T)in[rb * ld_in + p];
-        }
-        return cx<T>{a, b};
+        const TI a = in[(ra < nrows ? ra : nrows - 1) * ld_in + p];
+        const TI b = in[(rb < nrows ? rb : nrows - 1) * ld_in + p];
+        return PI{(ra < nrows) ? a : TI(0), (rb < nrows) ? b : TI(0)};
+    };
+    // rows past the last have first[] = 0, so they stay zero
+    auto cv = [&](int line, PI w) {
+        if constexpr (SHIFT_IN<T, TI, OP>) return cx<T>{(T)(w.x - first[2 * line]), (T)(w.y - first[2 * line + 1])};
+        else return cx<T>{(T)w.x, (T)w.y};
     };
     auto st = [&](int line, int p, cx<T> z) {
         const long ra = row0 + 2 * line, rb = ra + 1;
@@ -293,7 +349,12 @@ __global__ __launch_bounds__((rg_threads<T, NIN, NOUT, false>())) void kr_xline(
         if (rb < nrows) out[rb * ld_out + p] = z.y;
     };
     __syncthreads();
-    line_pipeline<T, NIN, NOUT, OP, NL, NTH, false>(ld, st, buf, twi, twop, kfac);
+    if constexpr (sizeof(TI) > sizeof(T)) {
+        line_pipeline<T, NIN, NOUT, OP, NL, NTH, false>(split_load(raw, cv), st, buf, twi, twop, kfac);
+    } else {
+        auto ld = [&](int line, int p) { return cv(line, raw(line, p)); };
+        line_pipeline<T, NIN, NOUT, OP, NL, NTH, false>(ld, st, buf, twi, twop, kfac);
+    }
 }
 
 template <typename T, typename TI, int NIN, int NOUT, int OP>
@@ -304,22 +365,41 @@ __global__ __launch_bounds__((rg_threads<T, NIN, NOUT, true>())) void kr_yline(
     constexpr int NTH = rg_threads<T, NIN, NOUT, true>();
     constexpr int LS = rg_ls(cmax(NIN, NOUT));
     __shared__ cx<T> buf[NL * LS];
-    __shared__ cx<T> twi[NIN];
-    __shared__ cx<T> two[NIN == NOUT ? 1 : NOUT];
-    cx<T>* twop = (NIN == NOUT) ? twi : two;
-    load_twr(twi, twin_g, NIN);
-    if (NIN != NOUT) load_twr(twop, twout_g, NOUT);
+    constexpr bool TWG = tw_global<NIN, NOUT>();
+    __shared__ cx<T> twl[TWG ? 1 : NIN];
+    __shared__ cx<T> twl_o[(TWG || NIN == NOUT) ? 1 : NOUT];
+    const cx<T>* twi = twin_g;
+    const cx<T>* twop = twout_g;
+    if constexpr (!TWG) {
+        load_twr(twl, twin_g, NIN);
+        if (NIN != NOUT) load_twr(twl_o, twout_g, NOUT);
+        twi = twl;
+        twop = (NIN == NOUT) ? twl : twl_o;
+    }
     const int x0 = blockIdx.x * 2 * NL;
     const TI* src = in + (long)blockIdx.y * ps_in;
     T* dst = out + (long)blockIdx.y * ps_out;
     using PI = typename pair_t<TI>::type;
     using PO = typename pair_t<T>::type;
-    auto ld = [&](int line, int p) {
+    // row 0 of each column pair (the shift of a reduced-precision derivative)
+    __shared__ PI first[SHIFT_IN<T, TI, OP> ? NL : 1];
+    if constexpr (SHIFT_IN<T, TI, OP>) {
+        for (int t = threadIdx.x; t < NL; t += NTH) {
+            const int x = x0 + 2 * t;
+            first[t] = (x < ncols) ? *reinterpret_cast<const PI*>(src + x) : PI{TI(0), TI(0)};
+        }
+    }
+    // column pair x, x + 1 at row p (zero past the last column; the read is
+    // unconditional, from column 0 when past it, so that no branch separates
+    // the reads)
+    auto raw = [&](int line, int p) {
         const int x = x0 + 2 * line;
-        if (x >= ncols) return cx<T>{T(0), T(0)};
-        const PI v = *reinterpret_cast<const PI*>(src + (long)p * ld_in + x);
+        const PI v = *reinterpret_cast<const PI*>(src + (long)p * ld_in + (x < ncols ? x : 0));
+        return (x < ncols) ? v : PI{TI(0), TI(0)};
+    };
+    auto cv = [&](int line, PI v) {
         if constexpr (SHIFT_IN<T, TI, OP>) {
-            const PI v0 = *reinterpret_cast<const PI*>(src + x);
+            const PI v0 = first[line];
             return cx<T>{(T)(v.x - v0.x), (T)(v.y - v0.y)};
         } else {
             return cx<T>{(T)v.x, (T)v.y};
@@ -341,7 +421,12 @@ __global__ __launch_bounds__((rg_threads<T, NIN, NOUT, true>())) void kr_yline(
         *o = PO{z.x, z.y};
     };
     __syncthreads();
-    line_pipeline<T, NIN, NOUT, OP, NL, NTH, true>(ld, st, buf, twi, twop, kfac);
+    if constexpr (sizeof(TI) > sizeof(T)) {
+        line_pipeline<T, NIN, NOUT, OP, NL, NTH, true>(split_load(raw, cv), st, buf, twi, twop, kfac);
+    } else {
+        auto ld = [&](int line, int p) { return cv(line, raw(line, p)); };
+        line_pipeline<T, NIN, NOUT, OP, NL, NTH, true>(ld, st, buf, twi, twop, kfac);
+    }
 }
 
 // x r2c of fftn_mpi (FFTW r2c: k = 0..N/2; the x Nyquist is zeroed as in
@@ -349,13 +434,13 @@ __global__ __launch_bounds__((rg_threads<T, NIN, NOUT, true>())) void kr_yline(
 template <typename T, typename TI, int N>
 __global__ __launch_bounds__((rg_threads<T, N, N, false>())) void kr_x_r2c(const TI* __restrict__ in,
                                                                          cx<T>* __restrict__ out, long nrows,
-                                                                         int ld_in, int ldc,
+                                                                         int ld_in, int ldc, int kst,
                                                                          const cx<T>* __restrict__ tw_g, T scale) {
     constexpr int NL = rg_lines<T, N, N, false>();
     constexpr int NTH = rg_threads<T, N, N, false>();
     constexpr int LS = rg_ls(N);
-    constexpr int LH = N / 2 + 1;
     constexpr int SF = rg_nst(N);
+    const int LH = kst;  // stored columns (at most N / 2 + 1)
     __shared__ cx<T> buf[NL * LS];
     __shared__ cx<T> tw[N];
     load_twr(tw, tw_g, N);
@@ -398,8 +483,8 @@ __global__ __launch_bounds__((rg_threads<T, N, N, false>())) void kr_x_r2c(const
 template <typename T, int N, int DERIV>
 __global__ __launch_bounds__((rg_threads<T, N, N, false>())) void kr_x_c2r(const cx<T>* __restrict__ in,
                                                                          T* __restrict__ out, long nrows, int ldc,
-                                                                         int ld_out, const cx<T>* __restrict__ tw_g,
-                                                                         T kfac) {
+                                                                         int kst, int ld_out,
+                                                                         const cx<T>* __restrict__ tw_g, T kfac) {
     constexpr int NL = rg_lines<T, N, N, false>();
     constexpr int NTH = rg_threads<T, N, N, false>();
     constexpr int LS = rg_ls(N);
@@ -413,8 +498,8 @@ __global__ __launch_bounds__((rg_threads<T, N, N, false>())) void kr_x_c2r(const
     i0.load([&](int line, int k) {
         const int kk = (2 * k <= N) ? k : N - k;
         const long ga = row0 + 2 * line, gb = ga + 1;
-        cx<T> a = (ga < nrows) ? in[ga * ldc + kk] : cx<T>{T(0), T(0)};
-        cx<T> b = (gb < nrows) ? in[gb * ldc + kk] : cx<T>{T(0), T(0)};
+        cx<T> a = (ga < nrows && kk < kst) ? in[ga * ldc + kk] : cx<T>{T(0), T(0)};
+        cx<T> b = (gb < nrows && kk < kst) ? in[gb * ldc + kk] : cx<T>{T(0), T(0)};
         if (kk == 0 || 2 * kk == N) {
             a.y = T(0);
             b.y = T(0);
@@ -444,10 +529,13 @@ __global__ __launch_bounds__((rg_threads<T, N, N, false>())) void kr_x_c2r(const
     });
 }
 
-// complex y-lines of the half spectrum (lhx columns, N rows per plane)
+// complex y-lines of the half spectrum (ncols columns of row stride ld, N
+// rows per plane); SP_FWD stores and SP_MULINV reads only the rows with
+// min(p, N - p) < mc
 template <typename T, int N, int OP>
 __global__ __launch_bounds__((rg_threads<T, N, N, true>())) void kr_yspec(const cx<T>* __restrict__ in,
-                                                                        cx<T>* __restrict__ out, int lhx, long ps,
+                                                                        cx<T>* __restrict__ out, int ncols, int ld,
+                                                                        int ldG, int mc, long ps,
                                                                         const cx<T>* __restrict__ tw_g,
                                                                         const double* __restrict__ G) {
     // one complex column per line: 8 (FP64) / 16 (FP32) columns = 128 bytes
@@ -462,13 +550,14 @@ __global__ __launch_bounds__((rg_threads<T, N, N, true>())) void kr_yspec(const 
     const cx<T>* src = in + (long)blockIdx.y * ps;
     cx<T>* dst = out + (long)blockIdx.y * ps;
     const double* Gp = G;
+    auto keep = [&](int p) { return p < mc || p > N - mc; };
     auto gload = [&](int line, int p) {
         const int x = x0 + line;
-        return (x < lhx) ? src[(long)p * lhx + x] : cx<T>{T(0), T(0)};
+        return (x < ncols) ? src[(long)p * ld + x] : cx<T>{T(0), T(0)};
     };
     auto gstore = [&](int line, int p, cx<T> z) {
         const int x = x0 + line;
-        if (x < lhx) dst[(long)p * lhx + x] = z;
+        if (x < ncols) dst[(long)p * ld + x] = z;
     };
     __syncthreads();
     if constexpr (OP == SP_FWD || OP == SP_FILT) {
@@ -482,7 +571,9 @@ __global__ __launch_bounds__((rg_threads<T, N, N, true>())) void kr_yspec(const 
         fl.template read<LS>(buf);
         fl.compute(tw);
         if constexpr (OP == SP_FWD) {
-            fl.store([&](int line, int p, cx<T> z) { gstore(line, p, (2 * p == N) ? cx<T>{T(0), T(0)} : z); });
+            fl.store([&](int line, int p, cx<T> z) {
+                if (keep(p)) gstore(line, p, (2 * p == N) ? cx<T>{T(0), T(0)} : z);
+            });
             return;
         } else {
             // Nyquist row zeroed, times G, then the inverse from registers
@@ -495,7 +586,7 @@ __global__ __launch_bounds__((rg_threads<T, N, N, true>())) void kr_yspec(const 
 #pragma unroll
                 for (int r = 0; r < decltype(i0)::R; ++r) {
                     const int p = b + r * decltype(i0)::NB;
-                    const T g = (x < lhx && 2 * p != N) ? (T)Gp[(long)p * lhx + x] : T(0);
+                    const T g = (x < ncols && 2 * p != N) ? (T)Gp[(long)p * ldG + x] : T(0);
                     i0.v[q][r] = {fl.v[q][r].x * g, fl.v[q][r].y * g};
                 }
             }
@@ -508,9 +599,9 @@ __global__ __launch_bounds__((rg_threads<T, N, N, true>())) void kr_yspec(const 
         if constexpr (OP == SP_MULINV) {
             i0.load([&](int line, int p) {
                 const int x = x0 + line;
-                if (x >= lhx) return cx<T>{T(0), T(0)};
-                const cx<T> z = src[(long)p * lhx + x];
-                const T g = (T)Gp[(long)p * lhx + x];
+                if (x >= ncols || !keep(p)) return cx<T>{T(0), T(0)};
+                const cx<T> z = src[(long)p * ld + x];
+                const T g = (T)Gp[(long)p * ldG + x];
                 return cx<T>{z.x * g, z.y * g};
             });
         } else {
@@ -596,38 +687,39 @@ bool line_dispatch_y(hipStream_t s, const TI* in, T* out, int nplanes, int ncols
 
 template <typename T, typename TI, int N>
 void launch_r2c(hipStream_t s, const TI* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
-                double scale) {
+                double scale, int kst) {
     constexpr int NL = rg_lines<T, N, N, false>();
     constexpr int NTH = rg_threads<T, N, N, false>();
     const long nb = (nrows + 2 * NL - 1) / (2 * NL);
-    hipLaunchKernelGGL((kr_x_r2c<T, TI, N>), dim3((unsigned)nb), dim3(NTH), 0, s, in, out, nrows, ld_in, ldc,
+    hipLaunchKernelGGL((kr_x_r2c<T, TI, N>), dim3((unsigned)nb), dim3(NTH), 0, s, in, out, nrows, ld_in, ldc, kst,
                        twp<T>(tw), (T)scale);
 }
 
 template <typename T, int N, int D>
 void launch_c2r(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int ld_out, const Twiddle& tw,
-                double kfac) {
+                double kfac, int kst) {
     constexpr int NL = rg_lines<T, N, N, false>();
     constexpr int NTH = rg_threads<T, N, N, false>();
     const long nb = (nrows + 2 * NL - 1) / (2 * NL);
-    hipLaunchKernelGGL((kr_x_c2r<T, N, D>), dim3((unsigned)nb), dim3(NTH), 0, s, in, out, nrows, ldc, ld_out,
+    hipLaunchKernelGGL((kr_x_c2r<T, N, D>), dim3((unsigned)nb), dim3(NTH), 0, s, in, out, nrows, ldc, kst, ld_out,
                        twp<T>(tw), (T)kfac);
 }
 
 template <typename T, int N, int OP>
-void launch_spec(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lhx, long ps, const Twiddle& tw,
-                 const double* G) {
+void launch_spec(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int ncols, long ps, const Twiddle& tw,
+                 const double* G, int ld, int ldG, int mc) {
     constexpr int NL = rg_lines<T, N, N, true>();
     constexpr int NTH = rg_threads<T, N, N, true>();
-    const int nbx = (lhx + NL - 1) / NL;
-    hipLaunchKernelGGL((kr_yspec<T, N, OP>), dim3(nbx, nplanes), dim3(NTH), 0, s, in, out, lhx, ps, twp<T>(tw), G);
+    const int nbx = (ncols + NL - 1) / NL;
+    hipLaunchKernelGGL((kr_yspec<T, N, OP>), dim3(nbx, nplanes), dim3(NTH), 0, s, in, out, ncols, ld, ldG, mc, ps,
+                       twp<T>(tw), G);
 }
 
 template <typename T, typename TI>
 bool r2c_dispatch(hipStream_t s, const TI* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
-                  double scale) {
+                  double scale, int kst) {
 #define RCASE(NN_) \
-    if (tw.n == NN_) { launch_r2c<T, TI, NN_>(s, in, out, nrows, ld_in, ldc, tw, scale); return true; }
+    if (tw.n == NN_) { launch_r2c<T, TI, NN_>(s, in, out, nrows, ld_in, ldc, tw, scale, kst); return true; }
     LES_SIZES(RCASE)
 #undef RCASE
     return false;
@@ -635,11 +727,11 @@ bool r2c_dispatch(hipStream_t s, const TI* in, cx<T>* out, long nrows, int ld_in
 
 template <typename T>
 bool c2r_dispatch(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int ld_out, const Twiddle& tw,
-                  int deriv, double kfac) {
+                  int deriv, double kfac, int kst) {
 #define CCASE(NN_)                                                                               \
     if (tw.n == NN_) {                                                                           \
-        if (deriv) launch_c2r<T, NN_, 1>(s, in, out, nrows, ldc, ld_out, tw, kfac);              \
-        else launch_c2r<T, NN_, 0>(s, in, out, nrows, ldc, ld_out, tw, kfac);                    \
+        if (deriv) launch_c2r<T, NN_, 1>(s, in, out, nrows, ldc, ld_out, tw, kfac, kst);         \
+        else launch_c2r<T, NN_, 0>(s, in, out, nrows, ldc, ld_out, tw, kfac, kst);               \
         return true;                                                                           \
     }
     LES_SIZES(CCASE)
@@ -648,15 +740,15 @@ bool c2r_dispatch(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, i
 }
 
 template <typename T>
-bool spec_dispatch(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lhx, long ps, const Twiddle& tw,
-                   SpecOp op, const double* G) {
+bool spec_dispatch(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int ncols, long ps, const Twiddle& tw,
+                   SpecOp op, const double* G, int ld, int ldG, int mc) {
 #define SCASE(NN_)                                                                                          \
     if (tw.n == NN_) {                                                                                      \
         switch (op) {                                                                                     \
-            case SP_FWD: launch_spec<T, NN_, SP_FWD>(s, in, out, nplanes, lhx, ps, tw, G); break;           \
-            case SP_INV: launch_spec<T, NN_, SP_INV>(s, in, out, nplanes, lhx, ps, tw, G); break;           \
-            case SP_FILT: launch_spec<T, NN_, SP_FILT>(s, in, out, nplanes, lhx, ps, tw, G); break;         \
-            default: launch_spec<T, NN_, SP_MULINV>(s, in, out, nplanes, lhx, ps, tw, G); break;            \
+            case SP_FWD: launch_spec<T, NN_, SP_FWD>(s, in, out, nplanes, ncols, ps, tw, G, ld, ldG, mc); break;  \
+            case SP_INV: launch_spec<T, NN_, SP_INV>(s, in, out, nplanes, ncols, ps, tw, G, ld, ldG, mc); break;  \
+            case SP_FILT: launch_spec<T, NN_, SP_FILT>(s, in, out, nplanes, ncols, ps, tw, G, ld, ldG, mc); break; \
+            default: launch_spec<T, NN_, SP_MULINV>(s, in, out, nplanes, ncols, ps, tw, G, ld, ldG, mc); break;   \
         }                                                                                                 \
         return true;                                                                                      \
     }
@@ -680,18 +772,18 @@ bool yline_real_ct(hipStream_t s, const TI* in, T* out, int nplanes, int ncols, 
 }
 template <typename T, typename TI>
 bool x_r2c_ct(hipStream_t s, const TI* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
-              double scale) {
-    return r2c_dispatch<T, TI>(s, in, out, nrows, ld_in, ldc, tw, scale);
+              double scale, int kst) {
+    return r2c_dispatch<T, TI>(s, in, out, nrows, ld_in, ldc, tw, scale, kst);
 }
 template <typename T>
 bool x_c2r_ct(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int ld_out, const Twiddle& tw, int deriv,
-              double kfac) {
-    return c2r_dispatch<T>(s, in, out, nrows, ldc, ld_out, tw, deriv, kfac);
+              double kfac, int kst) {
+    return c2r_dispatch<T>(s, in, out, nrows, ldc, ld_out, tw, deriv, kfac, kst);
 }
 template <typename T>
-bool yline_spec_ct(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lhx, long ps, const Twiddle& tw,
-                   SpecOp op, const double* G) {
-    return spec_dispatch<T>(s, in, out, nplanes, lhx, ps, tw, op, G);
+bool yline_spec_ct(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int ncols, long ps, const Twiddle& tw,
+                   SpecOp op, const double* G, int ld, int ldG, int mc) {
+    return spec_dispatch<T>(s, in, out, nplanes, ncols, ps, tw, op, G, ld, ldG, mc);
 }
 
 #define INST_IO(T, TI)                                                                                       \
@@ -699,16 +791,18 @@ bool yline_spec_ct(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int 
                                        LineOp, double);                                                          \
     template bool yline_real_ct<T, TI>(hipStream_t, const TI*, T*, int, int, int, int, long, long, const Twiddle&, \
                                        const Twiddle&, LineOp, double);                                          \
-    template bool x_r2c_ct<T, TI>(hipStream_t, const TI*, cx<T>*, long, int, int, const Twiddle&, double);
+    template bool x_r2c_ct<T, TI>(hipStream_t, const TI*, cx<T>*, long, int, int, const Twiddle&, double, int);
 INST_IO(double, double)
 INST_IO(float, float)
 INST_IO(float, double)
 #undef INST_IO
-template bool x_c2r_ct<double>(hipStream_t, const cx<double>*, double*, long, int, int, const Twiddle&, int, double);
-template bool x_c2r_ct<float>(hipStream_t, const cx<float>*, float*, long, int, int, const Twiddle&, int, double);
+template bool x_c2r_ct<double>(hipStream_t, const cx<double>*, double*, long, int, int, const Twiddle&, int, double,
+                               int);
+template bool x_c2r_ct<float>(hipStream_t, const cx<float>*, float*, long, int, int, const Twiddle&, int, double,
+                              int);
 template bool yline_spec_ct<double>(hipStream_t, const cx<double>*, cx<double>*, int, int, long, const Twiddle&,
-                                    SpecOp, const double*);
+                                    SpecOp, const double*, int, int, int);
 template bool yline_spec_ct<float>(hipStream_t, const cx<float>*, cx<float>*, int, int, long, const Twiddle&,
-                                   SpecOp, const double*);
+                                   SpecOp, const double*, int, int, int);
 
 }  // namespace les

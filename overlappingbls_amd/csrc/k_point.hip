@@ -608,8 +608,10 @@ __device__ __forceinline__ void lag_relax(double& F1, double& F2, double A, doub
     F1 = fmax(CLIP, F1);
 }
 
+// Two waves per SIMD: the FP64 contraction (42 filtered values and three
+// pow calls per point) spills to scratch under the default four.
 template <typename W>
-__global__ void k_lag_contract(const W* __restrict__ b2, const W* __restrict__ b4, double* FLM,
+__global__ __launch_bounds__(256, 2) void k_lag_contract(const W* __restrict__ b2, const W* __restrict__ b4, double* FLM,
                                double* FMM, double* FQN, double* FNN, double* Cs, KP p, int init_lm, int init_qn,
                                double lagran_dt, double bexp) {
     const long n = p.plane * p.nz;

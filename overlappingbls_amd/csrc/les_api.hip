@@ -2,6 +2,7 @@
 // lifecycle, field transfer, per-routine entry points and the flosol
 // orchestration (103_flosol.f90:4-290).  All work is enqueued on one HIP
 // stream per context; entry points synchronise before returning.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -129,6 +130,22 @@ hipError_t init_spectral(les_ctx* c) {
         const double kc2 = (PI / delta) * (PI / delta);
         for (size_t q = 0; q < ns; ++q) (*Gs[a])[q] = (k2[q] >= kc2) ? 0.0 : 1.0 / (double)(c->nx * c->ny);
     }
+    // filter support for the pruned spectra of lagrange_Sdep
+    int kc[2] = {1, 1}, mc = 1;
+    for (int j = 0; j < ny; ++j) {
+        if (2 * j == ny) continue;
+        for (int i = 0; i + 1 < lhx; ++i) {
+            const size_t q = (size_t)j * lhx + i;
+            for (int a = 0; a < 2; ++a) {
+                if ((*Gs[a])[q] == 0.0) continue;
+                kc[a] = std::max(kc[a], i + 1);
+                mc = std::max(mc, std::min(j, ny - j) + 1);
+            }
+        }
+    }
+    c->lag_kc = std::max(kc[0], kc[1]);
+    c->lag_kc4 = kc[1];
+    c->lag_mc = mc;
     // TDMA coefficients with the wall row at level 1 and the top row at
     // nzt-1, factorised (109_press_stag_array.f90:4-103)
     const int nz = c->nz;
@@ -244,15 +261,20 @@ void lagrange(les_ctx* c, int jt) {
     W* raw = scr<W>(c->lag_buf);
     W* b2 = raw + (size_t)np21 * c->plane;
     W* b4 = b2 + (size_t)np21 * c->plane;
+    // Spectra pruned to the filters' support: kc columns per row, rows
+    // min(y, ny - y) < mc.  Modes outside it are zero after either filter,
+    // so they are neither stored nor transformed back.
+    const int kc = c->lag_kc, kc4 = c->lag_kc4, mc = c->lag_mc;
+    const long psp = (long)kc * c->ny;
     cx<W>* spf = scr<W>(c->lag_spec);
-    cx<W>* spt = spf + (size_t)np21 * c->ns;
+    cx<W>* spt = spf + (size_t)np21 * psp;
     lagrange_prep_dev<W>(c, raw);
-    x_r2c<W>(c->stream, raw, spf, np21 * c->ny, c->nx, c->lhx, c->tw_x, 1.0);
-    yline_spec<W>(c->stream, spf, spf, (int)np21, c->lhx, (long)c->ns, c->tw_y, SP_FWD, nullptr);
-    yline_spec<W>(c->stream, spf, spt, (int)np21, c->lhx, (long)c->ns, c->tw_y, SP_MULINV, c->Gtest);
-    x_c2r<W>(c->stream, spt, b2, np21 * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
-    yline_spec<W>(c->stream, spf, spt, (int)np21, c->lhx, (long)c->ns, c->tw_y, SP_MULINV, c->Gtest2);
-    x_c2r<W>(c->stream, spt, b4, np21 * c->ny, c->lhx, c->nx, c->tw_x, 0, 0.0);
+    x_r2c<W>(c->stream, raw, spf, np21 * c->ny, c->nx, kc, c->tw_x, 1.0, kc);
+    yline_spec<W>(c->stream, spf, spf, (int)np21, kc, psp, c->tw_y, SP_FWD, nullptr, kc, -1, mc);
+    yline_spec<W>(c->stream, spf, spt, (int)np21, kc, psp, c->tw_y, SP_MULINV, c->Gtest, kc, c->lhx, mc);
+    x_c2r<W>(c->stream, spt, b2, np21 * c->ny, kc, c->nx, c->tw_x, 0, 0.0, kc);
+    yline_spec<W>(c->stream, spf, spt, (int)np21, kc4, psp, c->tw_y, SP_MULINV, c->Gtest2, kc, c->lhx, mc);
+    x_c2r<W>(c->stream, spt, b4, np21 * c->ny, kc, c->nx, c->tw_x, 0, 0.0, kc4);
     const int init_lm = P.inilag && !c->f_lm_mm_init && (jt == P.cs_count || jt == P.dyn_init);
     const int init_qn = P.inilag && !c->f_qn_nn_init && (jt == P.cs_count || jt == P.dyn_init);
     lagrange_contract_dev<W>(c, b2, b4, init_lm, init_qn);

@@ -97,6 +97,10 @@ struct les_ctx {
     double *u_big, *v_big, *w_big;                  // nlev planes of nx2*ny2
     // wavenumber tables (half spectrum, 002_fft.f90:143-167)
     double *kx2d, *ky2d, *k2, *Gtest, *Gtest2;
+    // support of the two test filters (Nyquist column and row excluded, which
+    // the forward transforms zero): columns kx < lag_kc (lag_kc4 for the 4
+    // delta filter), rows with min(y, ny - y) < lag_mc
+    int lag_kc, lag_kc4, lag_mc;
     unsigned char* notnyq;
     double *coef_b, *coef_c;  // factorised TDMA, nz planes of ns
     double *ust, *vst, *sponge;
@@ -147,18 +151,25 @@ void xline_real(hipStream_t s, const TI* in, T* out, long nrows, int ld_in, int 
 template <typename T, typename TI = T>
 void yline_real(hipStream_t s, const TI* in, T* out, int nplanes, int ncols, int ld_in, int ld_out,
                 long ps_in, long ps_out, const Twiddle& tin, const Twiddle& tout, LineOp op, double kfac);
-// x r2c (fftn_mpi first half): nrows real rows (length n) -> half-spectrum rows (lhx).
+// Pruned half spectra (the test-filtered fields of lagrange_Sdep): only the
+// columns kx < kst are stored, and only the rows y with min(y, n - y) < mc.
+// x r2c (fftn_mpi first half): nrows real rows (length n) -> half-spectrum
+// rows of stride ldc; columns kx < kst are stored (kst < 0: all n/2 + 1).
 template <typename T, typename TI = T>
 void x_r2c(hipStream_t s, const TI* in, cx<T>* out, long nrows, int ld_in, int ldc,
-           const Twiddle& tw, double scale);
-// x c2r (ifftn_mpi second half); deriv != 0 applies i*kx (kx Nyquist = 0) first.
+           const Twiddle& tw, double scale, int kst = -1);
+// x c2r (ifftn_mpi second half); deriv != 0 applies i*kx (kx Nyquist = 0)
+// first; modes kx >= kst are taken as zero and not read (kst < 0: none).
 template <typename T>
 void x_c2r(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int ld_out,
-           const Twiddle& tw, int deriv, double kfac);
-// complex y-lines over nplanes half-spectrum planes (lhx columns, ny rows).
+           const Twiddle& tw, int deriv, double kfac, int kst = -1);
+// complex y-lines over nplanes half-spectrum planes: ncols columns of row
+// stride ld (< 0: ncols), G of row stride ldG (< 0: ld).  With mc >= 0,
+// SP_FWD stores and SP_MULINV reads only the rows with min(y, n - y) < mc
+// (the rows outside a filter's support); the other ops take every row.
 template <typename T>
-void yline_spec(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lhx, long ps,
-                const Twiddle& tw, SpecOp op, const double* G);
+void yline_spec(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int ncols, long ps,
+                const Twiddle& tw, SpecOp op, const double* G, int ld = -1, int ldG = -1, int mc = -1);
 
 // register-radix versions for the production line lengths (k_lines_rg.hip);
 // return false when the
@@ -171,13 +182,13 @@ bool yline_real_ct(hipStream_t s, const TI* in, T* out, int nplanes, int ncols, 
                    long ps_out, const Twiddle& tin, const Twiddle& tout, LineOp op, double kfac);
 template <typename T, typename TI>
 bool x_r2c_ct(hipStream_t s, const TI* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
-              double scale);
+              double scale, int kst);
 template <typename T>
 bool x_c2r_ct(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int ld_out, const Twiddle& tw, int deriv,
-              double kfac);
+              double kfac, int kst);
 template <typename T>
-bool yline_spec_ct(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int lhx, long ps, const Twiddle& tw,
-                   SpecOp op, const double* G);
+bool yline_spec_ct(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int ncols, long ps, const Twiddle& tw,
+                   SpecOp op, const double* G, int ld, int ldG, int mc);
 // process-wide switch: 0 forces the generic runtime-length kernels
 extern int g_use_ct;
 
