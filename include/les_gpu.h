@@ -291,10 +291,13 @@ int les_step_tlm(les_ctx* ctx, int jt);
  * adjoint seed y of the state at jt0 + nsteps (zero for J alone).  The
  * forward runs nsteps steps saving a checkpoint every k_ckpt steps; then,
  * segment by segment from the last, it recomputes the segment's steps with
- * tapes and runs their adjoint steps backwards.  On return the state is at
- * jt0 + nsteps, lin holds d(<y, x(jt0+nsteps)> + J)/dx(jt0) and *J (may be
- * null) the objective.  Device memory: nsteps/k_ckpt + 1 checkpoints (17
- * fields each) and k_ckpt tapes (12 fields + 9 padded fields each). */
+ * tapes and runs their adjoint steps backwards (the last segment is taped
+ * during the forward sweep).  k_ckpt = 0 chooses the spacing from free
+ * device memory, up to the whole window taped at once.  On return the state
+ * is at jt0 + nsteps, lin holds d(<y, x(jt0+nsteps)> + J)/dx(jt0) and *J
+ * (may be null) the objective.  Device memory: nsteps/k_ckpt + 1
+ * checkpoints (17 fields each) and k_ckpt tapes (12 fields + 9 padded
+ * fields each; 550 MB at 128^3, 4.4 GB at 256^3). */
 int les_adjoint_window(les_ctx* ctx, int jt0, int nsteps, int k_ckpt, int obj_steps, double* J);
 int les_step_adj(les_ctx* ctx, int jt);
 /* Dot-product test of stage `stage` of the linearised step (0 ..

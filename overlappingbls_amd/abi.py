@@ -418,7 +418,8 @@ class LesContext:
         return dj.value
 
     def adjoint_window(self, jt0, nsteps, k_ckpt, obj_steps=0):
-        """Adjoint of the window (checkpoint every k_ckpt); returns J."""
+        """Adjoint of the window (checkpoint every k_ckpt; 0: spacing chosen
+        from free device memory, up to the whole window taped); returns J."""
         j = C.c_double(0.0)
         self._call("adjoint_window", self._h, int(jt0), int(nsteps), int(k_ckpt), int(obj_steps), C.byref(j))
         return j.value

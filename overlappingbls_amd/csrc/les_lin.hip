@@ -12,6 +12,7 @@
 // checkpoints the state every k steps, then for each segment, last first,
 // reruns the segment's forward steps into k tapes and sweeps its adjoint
 // steps backwards.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -116,6 +117,23 @@ hipError_t ensure_tapes(les_ctx* c, int n) {
         L->tapes.push_back(t);
     }
     return hipSuccess;
+}
+
+// device bytes of one tape (tape_alloc)
+size_t tape_bytes(const les_ctx* c) {
+    const size_t F3 = c->plane * c->nlev, B3 = c->plane2 * c->nlev, np = c->plane;
+    return 8 * (8 + 4 * np + 12 * F3 + c->nlev + 9 * B3);
+}
+
+// Checkpoint spacing for k_ckpt = 0: tapes already held plus those that fit
+// in the free device memory less a sixteenth of the device, within 1..nsteps.
+int auto_ckpt(les_ctx* c, int nsteps) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 1;
+    const size_t have = c->lin ? c->lin->tapes.size() : 0;
+    const size_t margin = tot / 16;
+    const size_t fit = have + (fr > margin ? (fr - margin) / tape_bytes(c) : 0);
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)nsteps, fit));
 }
 
 hipError_t ensure_ckpts(les_ctx* c, int n) {
@@ -774,11 +792,15 @@ int les_step_tlm(les_ctx* c, int jt) { return les_tlm_window(c, jt, 1, 0, nullpt
 // to the state at jt0 of <lin seed, x(jt0 + nsteps)> + J, and *J the
 // objective (mean of ustar^2 over the plane and the last obj_steps steps).
 // Checkpoints every k_ckpt steps; the last segment is taped during the
-// forward sweep, every earlier one recomputed once with tapes.
+// forward sweep, every earlier one recomputed once with tapes.  k_ckpt = 0
+// picks the spacing from free device memory: as many tapes as fit with a
+// sixteenth of the device left free, up to the whole window (then nothing
+// is recomputed).
 int les_adjoint_window(les_ctx* c, int jt0, int nsteps, int k_ckpt, int obj_steps, double* J) {
-    if (!c || k_ckpt < 1 || obj_steps < 0 || obj_steps > nsteps) return LES_E_BADARG;
+    if (!c || k_ckpt < 0 || nsteps < 1 || obj_steps < 0 || obj_steps > nsteps) return LES_E_BADARG;
     int st = lin_check_supported(c, jt0, nsteps);
     if (st != LES_OK) return st;
+    if (k_ckpt == 0) k_ckpt = auto_ckpt(c, nsteps);
     const int nseg = (nsteps + k_ckpt - 1) / k_ckpt;
     const int kt = nsteps < k_ckpt ? nsteps : k_ckpt;
     if (lin_alloc(c) != hipSuccess || ensure_tapes(c, kt > 0 ? kt : 1) != hipSuccess ||
