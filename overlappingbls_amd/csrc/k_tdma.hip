@@ -15,8 +15,8 @@
 // The mean mode (k^2 = 0, not Nyquist) has a = -1/dz, b = 1/dz, c = 0 above
 // the wall, i.e. p_1 = 0, p_k = p_{k-1} + dz H_z(k).  Nyquist modes are zero.
 //
-// The column walk is a dependency chain, so loads are issued D levels at a
-// time ahead of the arithmetic (one memory latency per D levels).
+// The column walk is a dependency chain, so loads are issued a batch of
+// levels ahead of the arithmetic.
 //
 // The right-hand sides H may be FP32 (mixed precision); the recurrences, the
 // forward-sweep store d' and the mean mode are FP64 in both precisions.  With
@@ -26,18 +26,55 @@
 
 namespace les {
 
-constexpr int TD = 8;  // levels per load batch
+constexpr int TDF = 24;  // forward-sweep levels per load batch
+constexpr int TDB = 32;  // backward-sweep levels per load batch
+
+// Coefficients of one system (109_press_stag_array.f90:4-103 as
+// init_spectral forms them) and its right-hand-side factors.
+struct TdmaCoef {
+    int kt;
+    bool wave;
+    double sgn, kx, ky, idz, tab1, b_edge, b_int, m_a, m_b, h0;
+};
+
+// Level k of the forward recurrence: b' = b - a c'_{k-1}, c' = c / b'
+// (init_spectral's factorisation, bitwise), d' = (r - a d'_{k-1}) / b'.
+// Returns d'_k; the carried values are scalars of the caller.
+__device__ __forceinline__ double tdma_level(const TdmaCoef C, int k, double hxv, double hyv, double hzv,
+                                             double& hzk, double& dprev, double& cprev) {
+    const bool first = (k == 1), top = (k == C.kt);
+    const double at = first ? 0.0 : (C.wave ? C.tab1 : C.m_a);
+    const double bt = C.wave ? ((first || top) ? C.b_edge : C.b_int) : C.m_b;
+    const double ct = (C.wave && (first || !top)) ? C.tab1 : 0.0;
+    const double bp = bt - at * cprev;
+    cprev = ct / bp;
+    const double dh = first ? (hzv - hzk) + C.h0 : hzv - hzk;
+    const double rw = C.sgn * (C.kx * hxv + C.ky * hyv) + dh * C.idz;
+    const double r = C.wave ? rw : (first ? 0.0 : hzk);
+    const double a = first ? 0.0 : (C.wave ? C.idz * C.idz : -C.idz);
+    const double dv = (r - a * dprev) / bp;
+    dprev = dv;
+    hzk = hzv;
+    return dv;
+}
 
 // dsc holds d' (FP64); out receives p.  For H of type double both may be the
 // same buffer: each element is read by its own lane before it is overwritten.
+//
+// With one lane per system there are only 2 ns lanes (about four waves per
+// CU at 256^2), so a sweep waits on one memory latency per load batch: the
+// batches are long, and every load of a batch is unconditional (level
+// indices clamped to the column) so that no branch splits them.  The
+// factorised diagonal b' is formed in the forward sweep from the same
+// operations as init_spectral's table (bitwise equal), which saves reading
+// that table; c' is read from its table in the backward sweep.
 template <typename TH>
 __global__ __launch_bounds__(128) void k_tdma(const TH* __restrict__ Hx, const TH* __restrict__ Hy,
                                                const TH* __restrict__ Hz, double* dsc, TH* out,
                                                const double* __restrict__ hzm,
                                                const double* __restrict__ kx2d, const double* __restrict__ ky2d,
                                                const double* __restrict__ k2a, const unsigned char* __restrict__ nn,
-                                               const double* __restrict__ cb, const double* __restrict__ cc,
-                                               long ns, int nz, double dz) {
+                                               const double* __restrict__ cc, long ns, int nz, double dz) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 2 * ns) return;
     const long q = t >> 1;
@@ -47,70 +84,89 @@ __global__ __launch_bounds__(128) void k_tdma(const TH* __restrict__ Hx, const T
         for (int k = 0; k <= nz - 1; ++k) out[k * s2 + t] = (TH)0.0;
         return;
     }
+    const int kt = nz - 1;             // top row of the system
     const double k2 = k2a[q], kx = kx2d[q], ky = ky2d[q];
     const bool wave = k2 > 0.0;
     const double idz = 1.0 / dz;
+    // coefficients as init_spectral forms them (109_press_stag_array.f90:4-103)
+    const double tab1 = 1.0 / (dz * dz), tab2 = 2.0 / (dz * dz);
+    const double b_edge = -(k2 + tab1), b_int = -(k2 + tab2);
+    const double m_a = -1.0 / dz, m_b = 1.0 / dz;
     // i (kx Hx + ky Hy): real part -(kx Hx.im + ky Hy.im), imag part kx Hx.re + ky Hy.re
     const long oxy = t ^ 1;            // the other component of the same element
     const double sgn = comp ? 1.0 : -1.0;
     const bool m64 = (hzm != nullptr) && (t == 0);
     const double h0 = m64 ? hzm[0] : (double)Hz[t];
     double hzk = m64 ? hzm[1] : (double)Hz[s2 + t];
-    double dprev = h0;
-    // ---- forward sweep ----
-    for (int k0 = 1; k0 <= nz - 1; k0 += TD) {
-        double hx[TD], hy[TD], hzn[TD], b[TD];
+    double dprev = h0, cprev = 0.0;
+    const TdmaCoef C{kt, wave, sgn, kx, ky, idz, tab1, b_edge, b_int, m_a, m_b, h0};
+    // ---- forward sweep: full batches, then one partial batch ----
+    int k0 = 1;
+    for (; k0 + TDF - 1 <= kt; k0 += TDF) {
+        TH hx[TDF], hy[TDF];
+        double hz[TDF];
 #pragma unroll
-        for (int d = 0; d < TD; ++d) {
-            const int k = k0 + d;
-            if (k <= nz - 1) {
-                const long o = k * s2;
-                hx[d] = Hx[o + oxy];
-                hy[d] = Hy[o + oxy];
-                hzn[d] = m64 ? hzm[k + 1] : (double)Hz[o + s2 + t];
-                b[d] = cb[k * ns + q];
-            }
+        for (int d = 0; d < TDF; ++d) {
+            const long o = (long)(k0 + d) * s2;
+            hx[d] = Hx[o + oxy];
+            hy[d] = Hy[o + oxy];
+            hz[d] = (double)Hz[o + s2 + t];
+        }
+        if (m64) {  // the FP64 mean-mode column (one lane)
+#pragma unroll
+            for (int d = 0; d < TDF; ++d) hz[d] = hzm[k0 + d + 1];
         }
 #pragma unroll
-        for (int d = 0; d < TD; ++d) {
-            const int k = k0 + d;
-            if (k <= nz - 1) {
-                double r, a;
-                if (wave) {
-                    double dh = hzn[d] - hzk;
-                    if (k == 1) dh += h0;
-                    r = sgn * (kx * hx[d] + ky * hy[d]) + dh * idz;
-                    a = (k == 1) ? 0.0 : idz * idz;
-                } else {
-                    r = (k == 1) ? 0.0 : hzk;
-                    a = (k == 1) ? 0.0 : -idz;
-                }
-                const double dv = (r - a * dprev) / b[d];
-                dsc[k * s2 + t] = dv;
-                dprev = dv;
-                hzk = hzn[d];
-            }
+        for (int d = 0; d < TDF; ++d) dsc[(k0 + d) * s2 + t] = tdma_level(C, k0 + d, (double)hx[d], (double)hy[d], hz[d], hzk, dprev, cprev);
+    }
+    if (k0 <= kt) {
+        TH hx[TDF], hy[TDF];
+        double hz[TDF];
+#pragma unroll
+        for (int d = 0; d < TDF; ++d) {
+            const long o = (long)min(k0 + d, kt) * s2;
+            hx[d] = Hx[o + oxy];
+            hy[d] = Hy[o + oxy];
+            hz[d] = (double)Hz[o + s2 + t];
         }
+        if (m64) {
+#pragma unroll
+            for (int d = 0; d < TDF; ++d) hz[d] = hzm[min(k0 + d, kt) + 1];
+        }
+#pragma unroll
+        for (int d = 0; d < TDF; ++d)
+            if (k0 + d <= kt) dsc[(k0 + d) * s2 + t] = tdma_level(C, k0 + d, (double)hx[d], (double)hy[d], hz[d], hzk, dprev, cprev);
     }
     // ---- backward sweep ----
     double pn = dprev;  // level nz-1 (c'(nz-1) = 0)
-    if (nz - 1 >= 1) out[(nz - 1) * s2 + t] = (TH)pn;
-    for (int k0 = nz - 2; k0 >= 1; k0 -= TD) {
-        double dv[TD], c[TD];
+    if (kt >= 1) out[kt * s2 + t] = (TH)pn;
+    k0 = kt - 1;
+    for (; k0 - TDB + 1 >= 1; k0 -= TDB) {
+        double dv[TDB], c[TDB];
 #pragma unroll
-        for (int d = 0; d < TD; ++d) {
-            const int k = k0 - d;
-            if (k >= 1) {
-                dv[d] = dsc[k * s2 + t];
-                c[d] = cc[k * ns + q];
-            }
+        for (int d = 0; d < TDB; ++d) {
+            dv[d] = dsc[(k0 - d) * s2 + t];
+            c[d] = cc[(k0 - d) * ns + q];
         }
 #pragma unroll
-        for (int d = 0; d < TD; ++d) {
-            const int k = k0 - d;
-            if (k >= 1) {
+        for (int d = 0; d < TDB; ++d) {
+            pn = dv[d] - c[d] * pn;
+            out[(k0 - d) * s2 + t] = (TH)pn;
+        }
+    }
+    if (k0 >= 1) {
+        double dv[TDB], c[TDB];
+#pragma unroll
+        for (int d = 0; d < TDB; ++d) {
+            const int k = max(k0 - d, 1);
+            dv[d] = dsc[k * s2 + t];
+            c[d] = cc[k * ns + q];
+        }
+#pragma unroll
+        for (int d = 0; d < TDB; ++d) {
+            if (k0 - d >= 1) {
                 pn = dv[d] - c[d] * pn;
-                out[k * s2 + t] = (TH)pn;
+                out[(k0 - d) * s2 + t] = (TH)pn;
             }
         }
     }
@@ -124,7 +180,7 @@ void tdma_press(les_ctx* c, const cx<T>* Hx, const cx<T>* Hy, const cx<T>* Hz, c
     const int nb = (int)((2 * ns + 127) / 128);
     hipLaunchKernelGGL(k_tdma<T>, dim3(nb), dim3(128), 0, c->stream, reinterpret_cast<const T*>(Hx),
                        reinterpret_cast<const T*>(Hy), reinterpret_cast<const T*>(Hz), dsc,
-                       reinterpret_cast<T*>(ph), hzm, c->kx2d, c->ky2d, c->k2, c->notnyq, c->coef_b, c->coef_c, ns,
+                       reinterpret_cast<T*>(ph), hzm, c->kx2d, c->ky2d, c->k2, c->notnyq, c->coef_c, ns,
                        c->nz, c->dz);
 }
 template void tdma_press<double>(les_ctx*, const cx<double>*, const cx<double>*, const cx<double>*, cx<double>*,
