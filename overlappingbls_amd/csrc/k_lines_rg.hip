@@ -211,7 +211,10 @@ __device__ __forceinline__ int resize_src(int k) {
 }
 
 // FFT(NIN) -> op -> IFFT(NOUT) on NL lines; ld/st are the global accessors.
-template <typename T, int NIN, int NOUT, int OP, int NL, int NTH, bool YM, typename LD, typename ST>
+// LDS_IN: ld reads the line buffer itself (inputs staged there), so the
+// first stage's reads complete before it overwrites the buffer.
+template <typename T, int NIN, int NOUT, int OP, int NL, int NTH, bool YM, bool LDS_IN = false, typename LD,
+          typename ST>
 __device__ __forceinline__ void line_pipeline(const LD& ld, const ST& st, cx<T>* buf, const cx<T>* twi,
                                               const cx<T>* two, T kfac) {
     constexpr int LS = rg_ls(cmax(NIN, NOUT));
@@ -221,6 +224,7 @@ __device__ __forceinline__ void line_pipeline(const LD& ld, const ST& st, cx<T>*
     Stage<T, NIN, 0, -1, NL, NTH, YM> f0;
     f0.load(ld);
     f0.compute(twi);
+    if constexpr (LDS_IN) __syncthreads();
     f0.template write<LS>(buf);
     __syncthreads();
     middle_stages<T, NIN, -1, NL, NTH, YM, LS>(buf, twi);
@@ -618,6 +622,133 @@ __global__ __launch_bounds__((rg_threads<T, N, N, true>())) void kr_yspec(const 
     il.store(gstore);
 }
 
+// ---- first passes of pads / unpads whose input is a pointwise function ----
+// The 3/2-grid products of convec (cross_product, 108_convec.f90:103-202)
+// and of the scalar advection (010_scalars.f90:466-484) feed an unpad; the
+// vorticity (108_convec.f90:44-69) feeds a pad.  Each fused kernel evaluates
+// the function for its block's lines straight into the line buffer (every
+// value as the separate kernel k_cross / k_scalar_adv / k_vort computes and
+// rounds it), then runs the x pass from LDS: the product or vorticity field
+// is never written to and re-read from memory.
+//
+// A block takes one component (blockIdx.x % NC; consecutive blocks take the
+// components of the same rows, so the shared inputs are re-read from cache)
+// and NL row pairs: line l is rows 2 rp and 2 rp + 1, rp = (blockIdx.x / NC)
+// NL + l, row 0 being the first row of level 1.  The evaluation is branch
+// free (every input loaded, the wall-level formula chosen by a select), so
+// the loads of all of a thread's elements can be in flight together.
+template <typename T, int COMP>
+__device__ __forceinline__ double fused_value(const ConvIn<T>& a, int k, long i, long P2, int) {
+    if constexpr (COMP == 3) {  // scalar advection
+        const T ub = a.ub[i], vb = a.vb[i], wb = a.wb[i], wp = a.wb[i + P2];
+        const T sx = a.b1[i], sy = a.b2[i], sz = a.b3[i], szp = a.b3[i + P2];
+        const double ust = a.ust[k], vst = a.vst[k];
+        const double w1 = (ub + ust) * sx + (vb + vst) * sy + (0.5 * wp) * szp;
+        const double wg = (ub + ust) * sx + (vb + vst) * sy + ((double)wb * sz + (double)wp * szp) / 2.0;
+        return (k == 1) ? w1 : wg;
+    } else if constexpr (COMP == 0) {
+        const T vb = a.vb[i], b3 = a.b3[i], wp = a.wb[i + P2], b2p = a.b2[i + P2], wb = a.wb[i], b2 = a.b2[i];
+        const double vst = a.vst[k];
+        const double w1 = (vb + vst) * (-b3) + 0.5 * wp * b2p;
+        const double wg = (vb + vst) * (-b3) + 0.5 * (wp * (b2p) + wb * (b2));
+        return (k == 1) ? w1 : wg;
+    } else if constexpr (COMP == 1) {
+        const T ub = a.ub[i], b3 = a.b3[i], wp = a.wb[i + P2], b1p = a.b1[i + P2], wb = a.wb[i], b1 = a.b1[i];
+        const double ust = a.ust[k];
+        const double w1 = (ub + ust) * (b3) + 0.5 * wp * (-b1p);
+        const double wg = (ub + ust) * (b3) + 0.5 * (wp * (-b1p) + wb * (-b1));
+        return (k == 1) ? w1 : wg;
+    } else {
+        const T ub = a.ub[i], um = a.ub[i - P2], b2 = a.b2[i], vb = a.vb[i], vm = a.vb[i - P2], b1 = a.b1[i];
+        const double wg = 0.5 * ((ub + um + a.ust[k] + a.ust[k - 1]) * (-b2) + (vb + vm + a.vst[k] + a.vst[k - 1]) * (b1));
+        return (k == 1) ? 0.0 : wg;
+    }
+}
+template <typename T, int COMP>
+__device__ __forceinline__ double fused_value(const VortIn<T>& g, int k, long i, long P, int nz) {
+    if constexpr (COMP == 2) {
+        return (double)g.vx[i] - (double)g.uy[i];
+    } else {
+        // neighbours clamped into the column (levels 1..nz are evaluated;
+        // level nz + 1 does not exist)
+        const long ip = (k < nz) ? i + P : i, im = i - P;
+        if constexpr (COMP == 0) {
+            const double wy = g.wy[i], wyp = g.wy[ip], wym = g.wy[im], vz = g.vz[i];
+            const double w1 = 0.5 * (wy + wyp) - vz, wn = 0.5 * (wy + wym) - vz, wg = wy - vz;
+            return (k == 1) ? w1 : ((k == nz) ? wn : wg);
+        } else {
+            const double wx = g.wx[i], wxp = g.wx[ip], wxm = g.wx[im], uz = g.uz[i];
+            const double w1 = uz - 0.5 * (wx + wxp), wn = uz - 0.5 * (wx + wxm), wg = uz - wx;
+            return (k == 1) ? w1 : ((k == nz) ? wn : wg);
+        }
+    }
+}
+
+// inputs of a block's NL lines into the line buffer (component COMP)
+template <typename T, int NIN, int NL, int NTH, int LS, int COMP, typename IN>
+__device__ __forceinline__ void fused_inputs(const IN& a, cx<T>* buf, long rp0, long nrp, int ny_in, long plane_in,
+                                             int nz) {
+    static_assert(NIN % 64 == 0 && NTH % 64 == 0, "a wave's run of positions lies in one line");
+    constexpr int PER = (NL * NIN + NTH - 1) / NTH;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int e = (int)threadIdx.x + j * NTH;
+        if (NL * NIN % NTH != 0 && e >= NL * NIN) break;
+        // a wave's run of positions lies in one line: the line, its rows and
+        // level are wave-uniform (scalar arithmetic, 32-bit row numbers)
+        const int l = __builtin_amdgcn_readfirstlane(e / NIN), p = e - l * NIN;
+        const long rp = rp0 + l;
+        const int rpc = (int)((rp < nrp) ? rp : nrp - 1);  // rows past the end read the last pair
+        const int ra = 2 * rpc;
+        const int km = ra / ny_in;
+        const int k = 1 + km;
+        const long ia = (long)k * plane_in + (long)(ra - km * ny_in) * NIN + p;
+        const double va = fused_value<T, COMP>(a, k, ia, plane_in, nz);
+        const double vb = fused_value<T, COMP>(a, k, ia + NIN, plane_in, nz);
+        buf[l * LS + rg_pad(p)] = (rp < nrp) ? cx<T>{(T)va, (T)vb} : cx<T>{T(0), T(0)};
+    }
+}
+
+// NC components per row pair; comp0 is the first component's index (3 for
+// the scalar advection).  ny_in rows per level, plane_in values per plane.
+template <typename T, int NIN, int NOUT, int OP, int NC, typename IN>
+__global__ __launch_bounds__((rg_threads<T, NIN, NOUT, false>())) void kr_fused_x(
+    IN a, int comp0, T* o0, T* o1, T* o2, long nrp, int ny_in, long plane_in, int nz,
+    const cx<T>* __restrict__ twin_g, const cx<T>* __restrict__ twout_g) {
+    constexpr int NL = rg_lines<T, NIN, NOUT, false>();
+    constexpr int NTH = rg_threads<T, NIN, NOUT, false>();
+    constexpr int LS = rg_ls(cmax(NIN, NOUT));
+    constexpr bool TWG = tw_global<NIN, NOUT>();
+    __shared__ cx<T> buf[NL * LS];
+    __shared__ cx<T> twl[TWG ? 1 : NIN];
+    __shared__ cx<T> twl_o[(TWG || NIN == NOUT) ? 1 : NOUT];
+    const cx<T>* twi = twin_g;
+    const cx<T>* twop = twout_g;
+    if constexpr (!TWG) {
+        load_twr(twl, twin_g, NIN);
+        if (NIN != NOUT) load_twr(twl_o, twout_g, NOUT);
+        twi = twl;
+        twop = (NIN == NOUT) ? twl : twl_o;
+    }
+    const int cb = (int)(blockIdx.x % NC);
+    const long rp0 = (long)(blockIdx.x / NC) * NL;
+    const int comp = comp0 + cb;
+    if (comp == 0) fused_inputs<T, NIN, NL, NTH, LS, 0>(a, buf, rp0, nrp, ny_in, plane_in, nz);
+    else if (comp == 1) fused_inputs<T, NIN, NL, NTH, LS, 1>(a, buf, rp0, nrp, ny_in, plane_in, nz);
+    else if (comp == 2) fused_inputs<T, NIN, NL, NTH, LS, 2>(a, buf, rp0, nrp, ny_in, plane_in, nz);
+    else fused_inputs<T, NIN, NL, NTH, LS, 3>(a, buf, rp0, nrp, ny_in, plane_in, nz);
+    __syncthreads();
+    T* o = (cb == 0) ? o0 : ((cb == 1) ? o1 : o2);
+    auto ld = [&](int line, int p) { return buf[line * LS + rg_pad(p)]; };
+    auto st = [&](int line, int p, cx<T> z) {
+        const long rp = rp0 + line;
+        if (rp >= nrp) return;
+        o[(2 * rp) * NOUT + p] = z.x;
+        o[(2 * rp + 1) * NOUT + p] = z.y;
+    };
+    line_pipeline<T, NIN, NOUT, OP, NL, NTH, false, true>(ld, st, buf, twi, twop, T(0));
+}
+
 template <typename T>
 const cx<T>* twp(const Twiddle& t);
 template <>
@@ -785,6 +916,57 @@ bool yline_spec_ct(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int 
                    SpecOp op, const double* G, int ld, int ldG, int mc) {
     return spec_dispatch<T>(s, in, out, nplanes, ncols, ps, tw, op, G, ld, ldG, mc);
 }
+
+template <typename T, int NIN, int NOUT, int OP, int NC, typename IN>
+void launch_fused(hipStream_t s, const IN& a, int comp0, T* o0, T* o1, T* o2, long nrp, int ny_in, long plane_in,
+                  int nz, const Twiddle& tin, const Twiddle& tout) {
+    constexpr int NL = rg_lines<T, NIN, NOUT, false>();
+    constexpr int NTH = rg_threads<T, NIN, NOUT, false>();
+    const long nb = NC * ((nrp + NL - 1) / NL);
+    hipLaunchKernelGGL((kr_fused_x<T, NIN, NOUT, OP, NC, IN>), dim3((unsigned)nb), dim3(NTH), 0, s, a, comp0, o0, o1,
+                       o2, nrp, ny_in, plane_in, nz, twp<T>(tin), twp<T>(tout));
+}
+
+template <typename T>
+bool cross_unpad_x(hipStream_t s, const ConvIn<T>& a, int scalar, T* o0, T* o1, T* o2, int nz, int ny2,
+                   const Twiddle& tin, const Twiddle& tout) {
+    const long plane2 = (long)tin.n * ny2;
+    const long nrp = (long)(nz - 1) * ny2 / 2;
+#define FCASE(NN_)                                                                                          \
+    if (tin.n == 3 * NN_ / 2 && tout.n == NN_ && ny2 % 2 == 0) {                                              \
+        if (scalar)                                                                                       \
+            launch_fused<T, 3 * NN_ / 2, NN_, OP_UNPAD, 1>(s, a, 3, o0, o1, o2, nrp, ny2, plane2, nz, tin, tout); \
+        else                                                                                              \
+            launch_fused<T, 3 * NN_ / 2, NN_, OP_UNPAD, 3>(s, a, 0, o0, o1, o2, nrp, ny2, plane2, nz, tin, tout); \
+        return true;                                                                                      \
+    }
+    FCASE(128) FCASE(256) FCASE(512)  // (96-point rows would split waves across lines)
+#undef FCASE
+    return false;
+}
+
+template <typename T>
+bool vort_pad_x(hipStream_t s, const VortIn<T>& g, T* o0, T* o1, T* o2, int nz, int ny, const Twiddle& tin,
+                const Twiddle& tout) {
+    const long plane = (long)tin.n * ny;
+    const long nrp = (long)nz * ny / 2;
+#define VCASE(NN_)                                                                                           \
+    if (tin.n == NN_ && tout.n == 3 * NN_ / 2 && ny % 2 == 0) {                                                \
+        launch_fused<T, NN_, 3 * NN_ / 2, OP_PAD, 3>(s, g, 0, o0, o1, o2, nrp, ny, plane, nz, tin, tout);  \
+        return true;                                                                                       \
+    }
+    LES_SIZES(VCASE)
+#undef VCASE
+    return false;
+}
+template bool cross_unpad_x<double>(hipStream_t, const ConvIn<double>&, int, double*, double*, double*, int, int,
+                                    const Twiddle&, const Twiddle&);
+template bool cross_unpad_x<float>(hipStream_t, const ConvIn<float>&, int, float*, float*, float*, int, int,
+                                   const Twiddle&, const Twiddle&);
+template bool vort_pad_x<double>(hipStream_t, const VortIn<double>&, double*, double*, double*, int, int,
+                                 const Twiddle&, const Twiddle&);
+template bool vort_pad_x<float>(hipStream_t, const VortIn<float>&, float*, float*, float*, int, int, const Twiddle&,
+                                const Twiddle&);
 
 #define INST_IO(T, TI)                                                                                       \
     template bool xline_real_ct<T, TI>(hipStream_t, const TI*, T*, long, int, int, const Twiddle&, const Twiddle&, \

@@ -313,18 +313,38 @@ void convec(les_ctx* c) {
     pad_f<W, double>(c, F(c, LES_F_U), ub, (int)c->nlev);
     pad_f<W, double>(c, F(c, LES_F_V), vb, (int)c->nlev);
     pad_f<W, double>(c, F(c, LES_F_W), wb, (int)c->nlev);
-    W *v1 = scr<W>(c->work3[2]), *v2 = scr<W>(c->work3[3]), *v3 = scr<W>(c->work3[4]);
-    vorticity_dev<W>(c, v1, v2, v3);
-    pad_f<W, W>(c, v1 + np, big[0] + np2, nz);
-    pad_f<W, W>(c, v2 + np, big[1] + np2, nz);
-    pad_f<W, W>(c, v3 + np, big[2] + np2, nz);
-    cross_product_dev<W>(c, big[0], big[1], big[2], big[3], big[4], big[5]);
+    // vorticity and the x pass of its pads in one kernel (x-padded rows in
+    // big[3..5]), then the y passes into big[0..2]
+    const VortIn<W> vg{WF<W>(c, LES_F_DUDY), WF<W>(c, LES_F_DUDZ), WF<W>(c, LES_F_DVDX),
+                       WF<W>(c, LES_F_DVDZ), WF<W>(c, LES_F_DWDX), WF<W>(c, LES_F_DWDY)};
+    if (vort_pad_x<W>(c->stream, vg, big[3], big[4], big[5], nz, c->ny, c->tw_x, c->tw_x2)) {
+        for (int m = 0; m < 3; ++m)
+            yline_real<W>(c->stream, big[3 + m], big[m] + np2, nz, c->nx2, c->nx2, c->nx2, (long)c->ny * c->nx2,
+                          (long)np2, c->tw_y, c->tw_y2, OP_PAD, 0.0);
+    } else {
+        W *v1 = scr<W>(c->work3[2]), *v2 = scr<W>(c->work3[3]), *v3 = scr<W>(c->work3[4]);
+        vorticity_dev<W>(c, v1, v2, v3);
+        pad_f<W, W>(c, v1 + np, big[0] + np2, nz);
+        pad_f<W, W>(c, v2 + np, big[1] + np2, nz);
+        pad_f<W, W>(c, v3 + np, big[2] + np2, nz);
+    }
     W* cxf = WF<W>(c, LES_F_CX);
     W* cyf = WF<W>(c, LES_F_CY);
     W* czf = WF<W>(c, LES_F_CZ);
-    unpad_f<W>(c, big[3] + np2, cxf + np, nz - 1);
-    unpad_f<W>(c, big[4] + np2, cyf + np, nz - 1);
-    unpad_f<W>(c, big[5] + np2, czf + np, nz - 1);
+    // cross product and the x pass of its unpads in one kernel (x-unpadded
+    // rows in big[3..5]), then the y passes
+    const ConvIn<W> ci{ub, vb, wb, big[0], big[1], big[2], c->ust, c->vst};
+    if (cross_unpad_x<W>(c->stream, ci, 0, big[3], big[4], big[5], nz, c->ny2, c->tw_x2, c->tw_x)) {
+        W* dst[3] = {cxf, cyf, czf};
+        for (int m = 0; m < 3; ++m)
+            yline_real<W>(c->stream, big[3 + m], dst[m] + np, nz - 1, c->nx, c->nx, c->nx, (long)c->ny2 * c->nx,
+                          (long)np, c->tw_y2, c->tw_y, OP_UNPAD, 0.0);
+    } else {
+        cross_product_dev<W>(c, big[0], big[1], big[2], big[3], big[4], big[5]);
+        unpad_f<W>(c, big[3] + np2, cxf + np, nz - 1);
+        unpad_f<W>(c, big[4] + np2, cyf + np, nz - 1);
+        unpad_f<W>(c, big[5] + np2, czf + np, nz - 1);
+    }
     fill_levels(c->stream, cxf, np, 0, 1, BOGUS);
     fill_levels(c->stream, cyf, np, 0, 1, BOGUS);
     fill_levels(c->stream, czf, np, 0, 1, BOGUS);
@@ -376,9 +396,18 @@ void theta_post(les_ctx* c, int jt, Tape* tape = nullptr) {
         return;
     }
     W* rm = scr<W>(c->big3[3]);
-    scalar_adv_dev<W>(c, scr<W>(c->tbig[0]), scr<W>(c->tbig[1]), scr<W>(c->tbig[2]), rm);
     W* adv = scr<W>(c->work3[2]);
-    unpad_f<W>(c, rm + np2, adv + np, nz - 1);
+    // advection and the x pass of its unpad in one kernel (x-unpadded rows in
+    // big3[3]), then the y pass
+    const ConvIn<W> si{scr<W>(c->u_big), scr<W>(c->v_big), scr<W>(c->w_big), scr<W>(c->tbig[0]),
+                       scr<W>(c->tbig[1]), scr<W>(c->tbig[2]), c->ust, c->vst};
+    if (cross_unpad_x<W>(c->stream, si, 1, rm, rm, rm, nz, c->ny2, c->tw_x2, c->tw_x)) {
+        yline_real<W>(c->stream, rm, adv + np, nz - 1, c->nx, c->nx, c->nx, (long)c->ny2 * c->nx, (long)np,
+                      c->tw_y2, c->tw_y, OP_UNPAD, 0.0);
+    } else {
+        scalar_adv_dev<W>(c, scr<W>(c->tbig[0]), scr<W>(c->tbig[1]), scr<W>(c->tbig[2]), rm);
+        unpad_f<W>(c, rm + np2, adv + np, nz - 1);
+    }
     W* temp = scr<W>(c->work3[0]);
     W* dtemp = scr<W>(c->work3[1]);
     scalar_flux_dev<W>(c, temp, 0);

@@ -189,6 +189,32 @@ bool x_c2r_ct(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int l
 template <typename T>
 bool yline_spec_ct(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int ncols, long ps, const Twiddle& tw,
                    SpecOp op, const double* G, int ld, int ldG, int mc);
+// First x passes of a pad / unpad fused with the pointwise function that
+// produces their input (k_lines_rg.hip); false when the line lengths have no
+// register-radix specialisation (the caller then runs the separate kernels).
+// ConvIn: padded u, v, w and b1..b3 = padded vorticity (cross product) or
+// padded theta gradients (scalar advection); ust, vst the Galilean shift.
+template <typename T>
+struct ConvIn {
+    const T *ub, *vb, *wb, *b1, *b2, *b3;
+    const double *ust, *vst;
+};
+// VortIn: the velocity gradients of the vorticity
+template <typename T>
+struct VortIn {
+    const T *uy, *uz, *vx, *vz, *wx, *wy;
+};
+// cross product (scalar = 0: components to o0, o1, o2) or scalar advection
+// (scalar = 1: to o0) on levels 1..nz-1 of the 3/2 grid, then the x unpad:
+// o* hold (nz - 1) * ny2 rows of length tout.n
+template <typename T>
+bool cross_unpad_x(hipStream_t s, const ConvIn<T>& a, int scalar, T* o0, T* o1, T* o2, int nz, int ny2,
+                   const Twiddle& tin, const Twiddle& tout);
+// vorticity on levels 1..nz, then the x pad: o0..o2 hold nz * ny rows of
+// length tout.n
+template <typename T>
+bool vort_pad_x(hipStream_t s, const VortIn<T>& g, T* o0, T* o1, T* o2, int nz, int ny, const Twiddle& tin,
+                const Twiddle& tout);
 // process-wide switch: 0 forces the generic runtime-length kernels
 extern int g_use_ct;
 
