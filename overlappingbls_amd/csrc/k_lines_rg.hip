@@ -433,29 +433,18 @@ __global__ __launch_bounds__((rg_threads<T, NIN, NOUT, true>())) void kr_yline(
     }
 }
 
-// x r2c of fftn_mpi (FFTW r2c: k = 0..N/2; the x Nyquist is zeroed as in
-// fftn_mpi), two real rows per complex line, split by Hermitian symmetry.
-template <typename T, typename TI, int N>
-__global__ __launch_bounds__((rg_threads<T, N, N, false>())) void kr_x_r2c(const TI* __restrict__ in,
-                                                                         cx<T>* __restrict__ out, long nrows,
-                                                                         int ld_in, int ldc, int kst,
-                                                                         const cx<T>* __restrict__ tw_g, T scale) {
-    constexpr int NL = rg_lines<T, N, N, false>();
-    constexpr int NTH = rg_threads<T, N, N, false>();
+// The x r2c of NL lines (two real rows each): ld(line, p) gives the packed
+// inputs (LDS_IN: read from buf itself), row(rl) the output row of local row
+// rl (null: none); the first LH coefficients are stored, times scale.
+template <typename T, int N, int NL, int NTH, bool LDS_IN, typename LD, typename ROW>
+__device__ __forceinline__ void r2c_body(const LD& ld, const ROW& row, int LH, cx<T>* buf, const cx<T>* tw,
+                                         T scale) {
     constexpr int LS = rg_ls(N);
     constexpr int SF = rg_nst(N);
-    const int LH = kst;  // stored columns (at most N / 2 + 1)
-    __shared__ cx<T> buf[NL * LS];
-    __shared__ cx<T> tw[N];
-    load_twr(tw, tw_g, N);
-    const long row0 = (long)blockIdx.x * 2 * NL;
-    __syncthreads();
     Stage<T, N, 0, -1, NL, NTH, false> f0;
-    f0.load([&](int line, int p) {
-        const long ra = row0 + 2 * line, rb = ra + 1;
-        return cx<T>{(ra < nrows) ? (T)in[ra * ld_in + p] : T(0), (rb < nrows) ? (T)in[rb * ld_in + p] : T(0)};
-    });
+    f0.load(ld);
     f0.compute(tw);
+    if constexpr (LDS_IN) __syncthreads();
     f0.template write<LS>(buf);
     __syncthreads();
     middle_stages<T, N, -1, NL, NTH, false, LS>(buf, tw);
@@ -467,8 +456,8 @@ __global__ __launch_bounds__((rg_threads<T, N, N, false>())) void kr_x_r2c(const
     __syncthreads();
     for (int e = threadIdx.x; e < 2 * NL * LH; e += NTH) {
         const int rl = e / LH, k = e - rl * LH;
-        const long gr = row0 + rl;
-        if (gr >= nrows) continue;
+        cx<T>* o = row(rl);
+        if (!o) continue;
         cx<T> r = {T(0), T(0)};
         if (2 * k != N) {
             const cx<T> zk = buf[(rl >> 1) * LS + rg_pad(k)];
@@ -478,8 +467,32 @@ __global__ __launch_bounds__((rg_threads<T, N, N, false>())) void kr_x_r2c(const
             r.x *= scale;
             r.y *= scale;
         }
-        out[gr * ldc + k] = r;
+        o[k] = r;
     }
+}
+
+// x r2c of fftn_mpi (FFTW r2c: k = 0..N/2; the x Nyquist is zeroed as in
+// fftn_mpi), two real rows per complex line, split by Hermitian symmetry.
+template <typename T, typename TI, int N>
+__global__ __launch_bounds__((rg_threads<T, N, N, false>())) void kr_x_r2c(const TI* __restrict__ in,
+                                                                         cx<T>* __restrict__ out, long nrows,
+                                                                         int ld_in, int ldc, int kst,
+                                                                         const cx<T>* __restrict__ tw_g, T scale) {
+    constexpr int NL = rg_lines<T, N, N, false>();
+    constexpr int NTH = rg_threads<T, N, N, false>();
+    constexpr int LS = rg_ls(N);
+    const int LH = kst;  // stored columns (at most N / 2 + 1)
+    __shared__ cx<T> buf[NL * LS];
+    __shared__ cx<T> tw[N];
+    load_twr(tw, tw_g, N);
+    const long row0 = (long)blockIdx.x * 2 * NL;
+    __syncthreads();
+    r2c_body<T, N, NL, NTH, false>(
+        [&](int line, int p) {
+            const long ra = row0 + 2 * line, rb = ra + 1;
+            return cx<T>{(ra < nrows) ? (T)in[ra * ld_in + p] : T(0), (rb < nrows) ? (T)in[rb * ld_in + p] : T(0)};
+        },
+        [&](int rl) { return (row0 + rl < nrows) ? out + (row0 + rl) * ldc : (cx<T>*)nullptr; }, LH, buf, tw, scale);
 }
 
 // x c2r of ifftn_mpi (FFTW c2r: imaginary parts of the DC and Nyquist
@@ -622,6 +635,24 @@ __global__ __launch_bounds__((rg_threads<T, N, N, true>())) void kr_yspec(const 
     il.store(gstore);
 }
 
+// Workgroups are dealt to the 8 XCDs round robin (workgroup b to XCD b % 8),
+// each XCD with its own L2.  Kernels whose NC consecutive work items read the
+// same inputs place them on one XCD: work item (group rg, item c) runs as
+// workgroup (rg % 8) + 8 (c + NC (rg / 8)), so the shared inputs come from
+// that XCD's L2.  Launch NC * 8 * ceil(ngroups / 8) workgroups; those past
+// the last group return at once.
+constexpr int N_XCD = 8;
+template <int NC>
+__device__ __forceinline__ void xcd_item(long& rg, int& c) {
+    const long b = blockIdx.x, q = b / N_XCD;
+    c = (int)(q % NC);
+    rg = (q / NC) * N_XCD + b % N_XCD;
+}
+template <int NC>
+inline long xcd_blocks(long ngroups) {
+    return (long)NC * N_XCD * ((ngroups + N_XCD - 1) / N_XCD);
+}
+
 // ---- first passes of pads / unpads whose input is a pointwise function ----
 // The 3/2-grid products of convec (cross_product, 108_convec.f90:103-202)
 // and of the scalar advection (010_scalars.f90:466-484) feed an unpad; the
@@ -730,8 +761,11 @@ __global__ __launch_bounds__((rg_threads<T, NIN, NOUT, false>())) void kr_fused_
         twi = twl;
         twop = (NIN == NOUT) ? twl : twl_o;
     }
-    const int cb = (int)(blockIdx.x % NC);
-    const long rp0 = (long)(blockIdx.x / NC) * NL;
+    long rg;
+    int cb;
+    xcd_item<NC>(rg, cb);
+    const long rp0 = rg * NL;
+    if (rp0 >= nrp) return;  // whole workgroup: past the last group
     const int comp = comp0 + cb;
     if (comp == 0) fused_inputs<T, NIN, NL, NTH, LS, 0>(a, buf, rp0, nrp, ny_in, plane_in, nz);
     else if (comp == 1) fused_inputs<T, NIN, NL, NTH, LS, 1>(a, buf, rp0, nrp, ny_in, plane_in, nz);
@@ -922,7 +956,7 @@ void launch_fused(hipStream_t s, const IN& a, int comp0, T* o0, T* o1, T* o2, lo
                   int nz, const Twiddle& tin, const Twiddle& tout) {
     constexpr int NL = rg_lines<T, NIN, NOUT, false>();
     constexpr int NTH = rg_threads<T, NIN, NOUT, false>();
-    const long nb = NC * ((nrp + NL - 1) / NL);
+    const long nb = xcd_blocks<NC>((nrp + NL - 1) / NL);
     hipLaunchKernelGGL((kr_fused_x<T, NIN, NOUT, OP, NC, IN>), dim3((unsigned)nb), dim3(NTH), 0, s, a, comp0, o0, o1,
                        o2, nrp, ny_in, plane_in, nz, twp<T>(tin), twp<T>(tout));
 }

@@ -3,6 +3,7 @@
 // 0..nzt; the bottom z-slab's rules at levels 0/1, the top z-slab's at
 // nzt-1/nzt).  Streaming kernels are HBM-bound: one read of each input and
 // one write of each output per point, grid-stride over (level, plane).
+#include <utility>
 #include "fft_rg.hpp"
 #include "les_internal.hpp"
 
@@ -522,10 +523,14 @@ __global__ void k_interpolag(const double* __restrict__ ul, const double* __rest
 
 void interpolag_dev(les_ctx* c) {
     KP p = kp(c);
-    const size_t bytes = sizeof(double) * c->plane * c->nlev;
     const int ids[4] = {LES_F_F_LM, LES_F_F_MM, LES_F_F_QN, LES_F_F_NN};
-    for (int m = 0; m < 4; ++m)
-        les_note(c, hipMemcpyAsync(c->fold[m], c->f3[ids[m]], bytes, hipMemcpyDeviceToDevice, c->stream));
+    // the old F_* become the interpolation source by a buffer exchange; the
+    // kernel writes levels 1..nz of the new ones, level 0 is carried over
+    for (int m = 0; m < 4; ++m) {
+        std::swap(c->f3[ids[m]], c->fold[m]);
+        les_note(c, hipMemcpyAsync(c->f3[ids[m]], c->fold[m], sizeof(double) * c->plane, hipMemcpyDeviceToDevice,
+                                   c->stream));
+    }
     hipLaunchKernelGGL(k_interpolag, grid_for(p.plane * p.nz), dim3(256), 0, c->stream,
                        (const double*)c->f3[LES_F_U_LAG], (const double*)c->f3[LES_F_V_LAG],
                        (const double*)c->f3[LES_F_W_LAG], (const double*)c->ust, (const double*)c->vst,
