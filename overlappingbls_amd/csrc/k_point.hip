@@ -446,6 +446,93 @@ __global__ void k_stress(Strain<W> S, const double* __restrict__ Cs, W* Nu, W* t
     }
 }
 
+// sgs_stag's strain (:5-158) and Nu_t / tau_ij (:206-315) in one pass, for the
+// steps whose strain nothing else reads: the w-node strain of levels k and
+// k + 1 is formed from the gradients (each value rounded to W as k_strain
+// stores it) and used at once, so the six S_ij fields are neither written
+// nor re-read.  Every output is bitwise equal to k_strain + k_stress.
+template <typename W>
+__device__ __forceinline__ void strain_at(const Grad<W>& g, long i, int k, long P, int nz, double s[6]) {
+    double ux, uy, uz, vx, vy, vz, wx, wy, wz;
+    if (k == 1) {
+        ux = g.ux[i]; uy = g.uy[i]; uz = g.uz[i]; vx = g.vx[i]; vy = g.vy[i]; vz = g.vz[i];
+        wx = 0.5 * (g.wx[i] + g.wx[i + P]);
+        wy = 0.5 * (g.wy[i] + g.wy[i + P]);
+        wz = g.wz[i];
+    } else if (k == nz) {
+        const long m = i - P;
+        ux = g.ux[m]; uy = g.uy[m]; uz = g.uz[i]; vx = g.vx[m]; vy = g.vy[m]; vz = g.vz[i];
+        wx = 0.5 * (g.wx[m] + g.wx[i]);
+        wy = 0.5 * (g.wy[m] + g.wy[i]);
+        wz = g.wz[m];
+    } else {
+        const long m = i - P;
+        ux = 0.5 * (g.ux[i] + g.ux[m]); uy = 0.5 * (g.uy[i] + g.uy[m]); uz = g.uz[i];
+        vx = 0.5 * (g.vx[i] + g.vx[m]); vy = 0.5 * (g.vy[i] + g.vy[m]); vz = g.vz[i];
+        wx = g.wx[i]; wy = g.wy[i]; wz = 0.5 * (g.wz[i] + g.wz[m]);
+    }
+    s[0] = (double)(W)ux;
+    s[1] = (double)(W)(0.5 * (uy + vx));
+    s[2] = (double)(W)(0.5 * (uz + wx));
+    s[3] = (double)(W)vy;
+    s[4] = (double)(W)(0.5 * (vz + wy));
+    s[5] = (double)(W)wz;
+}
+
+template <typename W>
+__global__ void k_strain_stress(Grad<W> g, const double* __restrict__ Cs, W* Nu, W* txx, W* txy, W* txz, W* tyy,
+                                W* tyz, W* tzz, KP p) {
+    const long n = p.plane * (p.nz + 1);
+    const double l2 = p.delta * p.delta;
+    GRID_LOOP(i, n) {
+        const int k = (int)(i / p.plane);
+        if (k == 0) {
+            txx[i] = BOGUS; txy[i] = BOGUS; txz[i] = BOGUS; tyy[i] = BOGUS; tyz[i] = BOGUS; tzz[i] = BOGUS;
+            continue;
+        }
+        double s[6];
+        strain_at<W>(g, i, k, p.plane, p.nz, s);
+        const double nu_k = smag(s[0], s[1], s[2], s[3], s[4], s[5]) * Cs[i] * l2;
+        Nu[i] = nu_k;
+        if (k == p.nz) {
+            txx[i] = BOGUS; txy[i] = BOGUS; tyy[i] = BOGUS; tzz[i] = BOGUS;
+            continue;  // txz, tyz(nz) from wallstress_top
+        }
+        if (k == 1) {
+            const double f = -2.0 * (nu_k + p.nu);
+            txx[i] = f * s[0]; txy[i] = f * s[1]; tyy[i] = f * s[3]; tzz[i] = f * s[5];
+        } else {
+            const long t = i + p.plane;
+            double u[6];
+            strain_at<W>(g, t, k + 1, p.plane, p.nz, u);
+            const double nu_t = smag(u[0], u[1], u[2], u[3], u[4], u[5]) * Cs[t] * l2;
+            if (k == p.nz - 1) {
+                const double f = -2.0 * (nu_t + p.nu);
+                txx[i] = f * u[0]; txy[i] = f * u[1]; tyy[i] = f * u[3]; tzz[i] = f * u[5];
+            } else {
+                const double kk = 0.5 * (nu_k + nu_t);
+                const double f = -2.0 * (kk + p.nu);
+                // the sums of two stored strains are taken in W, as k_stress takes them
+                txx[i] = f * 0.5 * (double)((W)s[0] + (W)u[0]);
+                txy[i] = f * 0.5 * (double)((W)s[1] + (W)u[1]);
+                tyy[i] = f * 0.5 * (double)((W)s[3] + (W)u[3]);
+                tzz[i] = f * 0.5 * (double)((W)s[5] + (W)u[5]);
+            }
+            txz[i] = -2.0 * (nu_k + p.nu) * s[2];
+            tyz[i] = -2.0 * (nu_k + p.nu) * s[4];
+        }
+    }
+}
+
+template <typename W>
+void strain_stress_dev(les_ctx* c) {
+    KP p = kp(c);
+    hipLaunchKernelGGL(k_strain_stress<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, grads<W>(c),
+                       (const double*)c->f3[LES_F_CS_OPT2], WF<W>(c, LES_F_NU_T), WF<W>(c, LES_F_TXX),
+                       WF<W>(c, LES_F_TXY), WF<W>(c, LES_F_TXZ), WF<W>(c, LES_F_TYY), WF<W>(c, LES_F_TYZ),
+                       WF<W>(c, LES_F_TZZ), p);
+}
+
 template <typename W>
 void nu_t_stress_dev(les_ctx* c) {
     KP p = kp(c);
@@ -1311,6 +1398,7 @@ void max3_dev(les_ctx* c, double* out) {
     template void wallstress_dev<W>(les_ctx*, int, const double*, const double*);                           \
     template void strain_dev<W>(les_ctx*);                                                                  \
     template void nu_t_stress_dev<W>(les_ctx*);                                                             \
+    template void strain_stress_dev<W>(les_ctx*);                                                           \
     template void lagrange_prep_dev<W>(les_ctx*, W*);                                                       \
     template void lagrange_contract_dev<W>(les_ctx*, const W*, const W*, int, int);                         \
     template void divstress_combine_dev<W>(les_ctx*, W*, const W*, const W*, const W*, int);                \

@@ -282,13 +282,23 @@ void lagrange(les_ctx* c, int jt) {
     if (init_qn) c->f_qn_nn_init = 1;
 }
 
+// keep_s: the strain fields S_ij must hold this step's values afterwards
+// (the routine entry, taped steps, statistics steps).  Otherwise, on the
+// steps without the dynamic update, the strain is formed inside the Nu_t /
+// tau kernel and not stored.
 template <typename W>
-void sgs_stag(les_ctx* c, int jt) {
+void sgs_stag(les_ctx* c, int jt, bool keep_s = true) {
     const les_params& P = c->prm;
-    strain_dev<W>(c);
+    const bool init = (jt == 1 && P.inilag);
+    const bool update = !c->sgs_frozen && (jt >= P.dyn_init || P.restart_vel) && (jt % P.cs_count) == 0;
     if (!c->sgs_frozen) lag_accumulate_dev(c);
-    if (jt == 1 && P.inilag) cs_init_dev(c);
-    else if (!c->sgs_frozen && (jt >= P.dyn_init || P.restart_vel) && (jt % P.cs_count) == 0) lagrange<W>(c, jt);
+    if (!keep_s && !init && !update) {
+        strain_stress_dev<W>(c);
+        return;
+    }
+    strain_dev<W>(c);
+    if (init) cs_init_dev(c);
+    else if (update) lagrange<W>(c, jt);
     nu_t_stress_dev<W>(c);
 }
 
@@ -508,7 +518,7 @@ void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     wallstress<W>(c, 0);
     if (tape) tape_save(c, tape, TAPE_WALL);
     wallstress<W>(c, 1);
-    sgs_stag<W>(c, jt);
+    sgs_stag<W>(c, jt, tape != nullptr || (stats && stats_step(c, jt)));
     W *txx = WF<W>(c, LES_F_TXX), *txy = WF<W>(c, LES_F_TXY), *txz = WF<W>(c, LES_F_TXZ);
     W *tyy = WF<W>(c, LES_F_TYY), *tyz = WF<W>(c, LES_F_TYZ), *tzz = WF<W>(c, LES_F_TZZ);
     divstress<W>(c, WF<W>(c, LES_F_DIVTX), txx, txy, txz, 0);
