@@ -965,6 +965,26 @@ __global__ void k_scalar_accum(double* rhs, const W* __restrict__ d, const W* __
     }
 }
 
+// ddz_w of the vertical flux (rounded to W as k_ddz_w stores it) added to the
+// RHS at once (k_scalar_accum without adv), levels 1..nz-1
+template <typename W>
+__global__ void k_ddz_w_accum(double* rhs, const W* __restrict__ f, KP p) {
+    const long n = p.plane * (p.nz - 1);
+    const double cst = 1.0 / p.dz;
+    GRID_LOOP(t, n) {
+        const long i = p.plane + t;
+        const W d = (W)(cst * ((double)f[i + p.plane] - (double)f[i]));
+        rhs[i] = rhs[i] + (double)d;
+    }
+}
+
+template <typename W>
+void ddz_w_accum_dev(les_ctx* c, const W* f) {
+    KP p = kp(c);
+    hipLaunchKernelGGL(k_ddz_w_accum<W>, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream,
+                       c->f3[LES_F_RHST], f, p);
+}
+
 template <typename W>
 void scalar_accum_dev(les_ctx* c, const W* dtemp, const W* adv) {
     KP p = kp(c);
@@ -1398,6 +1418,7 @@ void max3_dev(les_ctx* c, double* out) {
     template void wallstress_dev<W>(les_ctx*, int, const double*, const double*);                           \
     template void strain_dev<W>(les_ctx*);                                                                  \
     template void nu_t_stress_dev<W>(les_ctx*);                                                             \
+    template void ddz_w_accum_dev<W>(les_ctx*, const W*);                                                    \
     template void strain_stress_dev<W>(les_ctx*);                                                           \
     template void lagrange_prep_dev<W>(les_ctx*, W*);                                                       \
     template void lagrange_contract_dev<W>(les_ctx*, const W*, const W*, int, int);                         \

@@ -427,8 +427,7 @@ void theta_post(les_ctx* c, int jt, Tape* tape = nullptr) {
     ddy_f<W, W>(c, dtemp, temp, 0, (int)c->nlev);
     scalar_accum_dev<W>(c, dtemp, (const W*)nullptr);
     scalar_flux_dev<W>(c, temp, 2);
-    ddz_w_dev<W, W>(c, dtemp, temp);
-    scalar_accum_dev<W>(c, dtemp, (const W*)nullptr);
+    ddz_w_accum_dev<W>(c, temp);  // the vertical term added as it is differenced
     les_note(c, hipStreamWaitEvent(c->stream, c->ev_th, 0));
     calc_beta_dev<W>(c, c->th_mean);
     if (tape) tape_save(c, tape, TAPE_THETA);

@@ -254,6 +254,7 @@ template <typename W> void strain_dev(les_ctx* c);
 void lag_accumulate_dev(les_ctx* c);
 template <typename W> void nu_t_stress_dev(les_ctx* c);
 template <typename W> void strain_stress_dev(les_ctx* c);
+template <typename W> void ddz_w_accum_dev(les_ctx* c, const W* f);
 void cs_init_dev(les_ctx* c);
 void interpolag_dev(les_ctx* c);
 template <typename W> void lagrange_prep_dev(les_ctx* c, W* buf21);
