@@ -292,7 +292,7 @@ void sgs_stag(les_ctx* c, int jt, bool keep_s = true) {
     const bool init = (jt == 1 && P.inilag);
     const bool update = !c->sgs_frozen && (jt >= P.dyn_init || P.restart_vel) && (jt % P.cs_count) == 0;
     if (!c->sgs_frozen) lag_accumulate_dev(c);
-    if (!keep_s && !init && !update) {
+    if (c->fuse && !keep_s && !init && !update) {
         strain_stress_dev<W>(c);
         return;
     }
@@ -327,7 +327,8 @@ void convec(les_ctx* c) {
     // big[3..5]), then the y passes into big[0..2]
     const VortIn<W> vg{WF<W>(c, LES_F_DUDY), WF<W>(c, LES_F_DUDZ), WF<W>(c, LES_F_DVDX),
                        WF<W>(c, LES_F_DVDZ), WF<W>(c, LES_F_DWDX), WF<W>(c, LES_F_DWDY)};
-    if (vort_pad_x<W>(c->stream, vg, big[3], big[4], big[5], nz, c->ny, c->tw_x, c->tw_x2)) {
+    const bool fz = c->fuse && g_use_ct;
+    if (fz && vort_pad_x<W>(c->stream, vg, big[3], big[4], big[5], nz, c->ny, c->tw_x, c->tw_x2)) {
         for (int m = 0; m < 3; ++m)
             yline_real<W>(c->stream, big[3 + m], big[m] + np2, nz, c->nx2, c->nx2, c->nx2, (long)c->ny * c->nx2,
                           (long)np2, c->tw_y, c->tw_y2, OP_PAD, 0.0);
@@ -344,7 +345,7 @@ void convec(les_ctx* c) {
     // cross product and the x pass of its unpads in one kernel (x-unpadded
     // rows in big[3..5]), then the y passes
     const ConvIn<W> ci{ub, vb, wb, big[0], big[1], big[2], c->ust, c->vst};
-    if (cross_unpad_x<W>(c->stream, ci, 0, big[3], big[4], big[5], nz, c->ny2, c->tw_x2, c->tw_x)) {
+    if (fz && cross_unpad_x<W>(c->stream, ci, 0, big[3], big[4], big[5], nz, c->ny2, c->tw_x2, c->tw_x)) {
         W* dst[3] = {cxf, cyf, czf};
         for (int m = 0; m < 3; ++m)
             yline_real<W>(c->stream, big[3 + m], dst[m] + np, nz - 1, c->nx, c->nx, c->nx, (long)c->ny2 * c->nx,
@@ -411,7 +412,7 @@ void theta_post(les_ctx* c, int jt, Tape* tape = nullptr) {
     // big3[3]), then the y pass
     const ConvIn<W> si{scr<W>(c->u_big), scr<W>(c->v_big), scr<W>(c->w_big), scr<W>(c->tbig[0]),
                        scr<W>(c->tbig[1]), scr<W>(c->tbig[2]), c->ust, c->vst};
-    if (cross_unpad_x<W>(c->stream, si, 1, rm, rm, rm, nz, c->ny2, c->tw_x2, c->tw_x)) {
+    if (c->fuse && g_use_ct && cross_unpad_x<W>(c->stream, si, 1, rm, rm, rm, nz, c->ny2, c->tw_x2, c->tw_x)) {
         yline_real<W>(c->stream, rm, adv + np, nz - 1, c->nx, c->nx, c->nx, (long)c->ny2 * c->nx, (long)np,
                       c->tw_y2, c->tw_y, OP_UNPAD, 0.0);
     } else {
@@ -427,7 +428,12 @@ void theta_post(les_ctx* c, int jt, Tape* tape = nullptr) {
     ddy_f<W, W>(c, dtemp, temp, 0, (int)c->nlev);
     scalar_accum_dev<W>(c, dtemp, (const W*)nullptr);
     scalar_flux_dev<W>(c, temp, 2);
-    ddz_w_accum_dev<W>(c, temp);  // the vertical term added as it is differenced
+    if (c->fuse) {
+        ddz_w_accum_dev<W>(c, temp);  // the vertical term added as it is differenced
+    } else {
+        ddz_w_dev<W, W>(c, dtemp, temp);
+        scalar_accum_dev<W>(c, dtemp, (const W*)nullptr);
+    }
     les_note(c, hipStreamWaitEvent(c->stream, c->ev_th, 0));
     calc_beta_dev<W>(c, c->th_mean);
     if (tape) tape_save(c, tape, TAPE_THETA);
@@ -628,6 +634,10 @@ int les_create(const les_params* prm, les_ctx** out) {
     if (const char* lk = getenv("LES_LINE_KERNELS")) g_use_ct = strcmp(lk, "generic") != 0;
     les_ctx* c = new les_ctx();
     c->prm = *prm;
+    // LES_FUSE=0: the separate kernels instead of the fused ones (bitwise the
+    // same results; for checking that)
+    c->fuse = 1;
+    if (const char* fz = getenv("LES_FUSE")) c->fuse = strcmp(fz, "0") != 0;
     c->nx = prm->nxt; c->ny = prm->nyt; c->nz = prm->nzt;
     c->nx2 = 3 * c->nx / 2; c->ny2 = 3 * c->ny / 2;
     c->lhx = c->nx / 2 + 1; c->lhy = c->ny / 2 + 1;

@@ -122,6 +122,7 @@ struct les_ctx {
     int cfl_cap;
     int f_lm_mm_init, f_qn_nn_init;
     int sgs_frozen;            // skip the Lagrangian SGS update (Cs_opt2, F_* held)
+    int fuse;                  // fused kernels (LES_FUSE=0 at creation: the separate ones)
     les::LinCtx* lin;          // linearised-step buffers (les_lin.hip), or null
     double cfl;
     hipError_t err;            // first asynchronous HIP error since the last finish()
