@@ -9,7 +9,7 @@
 // The first stage of each transform reads global memory directly, the last
 // stage writes it directly; stages exchange through LDS.
 #include "fft_rg.hpp"
-#include "les_internal.hpp"
+#include "les_lag.hpp"
 
 namespace les {
 
@@ -783,6 +783,87 @@ __global__ __launch_bounds__((rg_threads<T, NIN, NOUT, false>())) void kr_fused_
     line_pipeline<T, NIN, NOUT, OP, NL, NTH, false, true>(ld, st, buf, twi, twop, T(0));
 }
 
+// ---- lagrange_Sdep: filtered fields contracted as they are transformed back ----
+// (106_03_lagrange_Sdep.f90:132-331).  A workgroup takes one row pair (rows
+// 2 rp, 2 rp + 1 of levels 1..nz) of all 21 test-filtered fields: the x c2r
+// of the 21 lines (pruned half spectra, as kr_x_c2r computes it), the real
+// rows to LDS, then the contraction at each of the 2 N points.  The 21
+// filtered fields are never written and re-read.
+//  SCALE 2: L_ij M_ij and M_ij M_ij of the 2 delta filter to lm, mm (FP64).
+//  SCALE 4: Q_ij N_ij and N_ij N_ij of the 4 delta filter, then with lm, mm
+//           the relaxation of F_LM .. F_NN and Cs^2 (lag_update).
+constexpr int LAG_NF = 21;
+template <typename T, int N>
+constexpr int lag_threads() {
+    return round64(LAG_NF * rg_maxbf(N));
+}
+
+template <typename T, int N, int SCALE>
+__global__ __launch_bounds__((lag_threads<T, N>())) void kr_lag_contract(const cx<T>* __restrict__ spt, int ldc,
+                                                                        int kst, long nrp, int ny, long plane,
+                                                                        double* lm, double* mm, LagUpdate u,
+                                                                        double cst,
+                                                                        const cx<T>* __restrict__ tw_g) {
+    constexpr int NL = LAG_NF;
+    constexpr int NTH = lag_threads<T, N>();
+    constexpr int LS = rg_ls(N);
+    constexpr int SI = rg_nst(N);
+    static_assert(LAG_NF * 2 * N * (int)sizeof(T) <= NL * LS * (int)sizeof(cx<T>), "rows fit the line buffer");
+    __shared__ cx<T> buf[NL * LS];
+    __shared__ cx<T> tw[N];
+    load_twr(tw, tw_g, N);
+    const long rp = blockIdx.x;
+    const long nzny = 2 * nrp;            // rows per field (levels 1..nz)
+    __syncthreads();
+    Stage<T, N, 0, +1, NL, NTH, false> i0;
+    i0.load([&](int line, int k) {        // field `line`, rows 2 rp, 2 rp + 1
+        const int kk = (2 * k <= N) ? k : N - k;
+        const long ga = (long)line * nzny + 2 * rp, gb = ga + 1;
+        cx<T> a = (kk < kst) ? spt[ga * ldc + kk] : cx<T>{T(0), T(0)};
+        cx<T> b = (kk < kst) ? spt[gb * ldc + kk] : cx<T>{T(0), T(0)};
+        if (kk == 0 || 2 * kk == N) {
+            a.y = T(0);
+            b.y = T(0);
+        }
+        if (2 * k > N) {
+            a.y = -a.y;
+            b.y = -b.y;
+        }
+        return cx<T>{a.x - b.y, a.y + b.x};
+    });
+    i0.compute(tw);
+    i0.template write<LS>(buf);
+    __syncthreads();
+    middle_stages<T, N, +1, NL, NTH, false, LS>(buf, tw);
+    Stage<T, N, SI - 1, +1, NL, NTH, false> il;
+    il.template read<LS>(buf);
+    il.compute(tw);
+    __syncthreads();  // every line read: the buffer now takes the real rows
+    T* rows = reinterpret_cast<T*>(buf);  // [field][row of the pair][x]
+    il.store([&](int line, int p, cx<T> z) {
+        rows[(line * 2) * N + p] = z.x;
+        rows[(line * 2 + 1) * N + p] = z.y;
+    });
+    __syncthreads();
+    const int ra = 2 * (int)rp;
+    const int k = 1 + ra / ny;
+    const long row0 = (long)k * plane + (long)(ra - (k - 1) * ny) * N;  // field index of the pair's first row
+    for (int q = threadIdx.x; q < 2 * N; q += NTH) {
+        const int r = q / N, x = q - r * N;
+        const long i = row0 + (long)r * N + x;
+        auto v = [&](int j) { return (double)rows[(j * 2 + r) * N + x]; };
+        double a, b;
+        if constexpr (SCALE == 2) {
+            lag_scale(v, cst, 4.0, a, b);
+            lm[i] = a;
+            mm[i] = b;
+        } else {
+            lag_scale(v, cst, 16.0, a, b);
+            lag_update(u, i, k, lm[i], mm[i], a, b);
+        }
+    }
+}
+
 template <typename T>
 const cx<T>* twp(const Twiddle& t);
 template <>
@@ -993,6 +1074,30 @@ bool vort_pad_x(hipStream_t s, const VortIn<T>& g, T* o0, T* o1, T* o2, int nz, 
 #undef VCASE
     return false;
 }
+template <typename T>
+bool lag_contract_x(hipStream_t s, int scale, const cx<T>* spt, int ldc, int kst, int nz, int ny, double* lm,
+                    double* mm, const LagUpdate& u, double cst, const Twiddle& tw) {
+    const long nrp = (long)nz * ny / 2;
+    if (ny % 2 != 0) return false;
+#define LCCASE(NN_)                                                                                         \
+    if (tw.n == NN_) {                                                                                      \
+        const dim3 g((unsigned)nrp), b(lag_threads<T, NN_>());                                              \
+        if (scale == 2)                                                                                     \
+            hipLaunchKernelGGL((kr_lag_contract<T, NN_, 2>), g, b, 0, s, spt, ldc, kst, nrp, ny,            \
+                               (long)NN_ * ny, lm, mm, u, cst, twp<T>(tw));                                 \
+        else                                                                                                \
+            hipLaunchKernelGGL((kr_lag_contract<T, NN_, 4>), g, b, 0, s, spt, ldc, kst, nrp, ny,            \
+                               (long)NN_ * ny, lm, mm, u, cst, twp<T>(tw));                                 \
+        return true;                                                                                        \
+    }
+    LCCASE(64) LCCASE(128) LCCASE(256)
+#undef LCCASE
+    return false;
+}
+template bool lag_contract_x<double>(hipStream_t, int, const cx<double>*, int, int, int, int, double*, double*,
+                                     const LagUpdate&, double, const Twiddle&);
+template bool lag_contract_x<float>(hipStream_t, int, const cx<float>*, int, int, int, int, double*, double*,
+                                    const LagUpdate&, double, const Twiddle&);
 template bool cross_unpad_x<double>(hipStream_t, const ConvIn<double>&, int, double*, double*, double*, int, int,
                                     const Twiddle&, const Twiddle&);
 template bool cross_unpad_x<float>(hipStream_t, const ConvIn<float>&, int, float*, float*, float*, int, int,

@@ -5,7 +5,7 @@
 // one write of each output per point, grid-stride over (level, plane).
 #include <utility>
 #include "fft_rg.hpp"
-#include "les_internal.hpp"
+#include "les_lag.hpp"
 
 namespace les {
 
@@ -687,68 +687,23 @@ void lagrange_prep_dev(les_ctx* c, W* buf21) {
                        strains<W>(c), buf21, p, um, vm);
 }
 
-__device__ __forceinline__ void lag_relax(double& F1, double& F2, double A, double B, double opftdelta,
-                                          double lagran_dt) {
-    const double CLIP = (double)1.e-24f;
-    double Tn = fmax(F1 * F2, CLIP);
-    Tn = opftdelta * pow(Tn, -1.0 / 8.0);
-    Tn = fmax(CLIP, Tn);
-    const double dumfac = lagran_dt / Tn;
-    const double epsi = dumfac / (1.0 + dumfac);
-    F1 = epsi * A + (1.0 - epsi) * F1;
-    F2 = epsi * B + (1.0 - epsi) * F2;
-    F1 = fmax(CLIP, F1);
-}
-
 // Two waves per SIMD: the FP64 contraction (42 filtered values and three
 // pow calls per point) spills to scratch under the default four.
 template <typename W>
-__global__ __launch_bounds__(256, 2) void k_lag_contract(const W* __restrict__ b2, const W* __restrict__ b4, double* FLM,
-                               double* FMM, double* FQN, double* FNN, double* Cs, KP p, int init_lm, int init_qn,
-                               double lagran_dt, double bexp) {
+__global__ __launch_bounds__(256, 2) void k_lag_contract(const W* __restrict__ b2, const W* __restrict__ b4,
+                                                         LagUpdate u, KP p) {
     const long n = p.plane * p.nz;
     const long fs = n;
-    const double CLIP = (double)1.e-24f;
     const double cst = 2.0 * (p.delta * p.delta);
-    const double opftdelta = OPFTIME * p.delta;
     GRID_LOOP(t, n) {
         const long i = p.plane + t;
         const int k = (int)(i / p.plane);
         const W* a = b2 + t;
         const W* h = b4 + t;
-        const double ub = a[0], vb = a[fs], wb = a[2 * fs];
-        const double uh = h[0], vh = h[fs], wh = h[2 * fs];
-        const double l11 = a[3 * fs] - ub * ub, l12 = a[4 * fs] - ub * vb, l13 = a[5 * fs] - ub * wb;
-        const double l22 = a[6 * fs] - vb * vb, l23 = a[7 * fs] - vb * wb, l33 = a[8 * fs] - wb * wb;
-        const double q11 = h[3 * fs] - uh * uh, q12 = h[4 * fs] - uh * vh, q13 = h[5 * fs] - uh * wh;
-        const double q22 = h[6 * fs] - vh * vh, q23 = h[7 * fs] - vh * wh, q33 = h[8 * fs] - wh * wh;
-        double sb[6], sh[6];
-#pragma unroll
-        for (int m = 0; m < 6; ++m) { sb[m] = a[(9 + m) * fs]; sh[m] = h[(9 + m) * fs]; }
-        const double S_bar = smag(sb[0], sb[1], sb[2], sb[3], sb[4], sb[5]);
-        const double S_hat = smag(sh[0], sh[1], sh[2], sh[3], sh[4], sh[5]);
-        double M[6], N[6];
-#pragma unroll
-        for (int m = 0; m < 6; ++m) {
-            M[m] = cst * (a[(15 + m) * fs] - 4.0 * S_bar * sb[m]);
-            N[m] = cst * (h[(15 + m) * fs] - 16.0 * S_hat * sh[m]);
-        }
-        const double LM = l11 * M[0] + l22 * M[3] + l33 * M[5] + 2.0 * (l12 * M[1] + l13 * M[2] + l23 * M[4]);
-        const double MM = M[0] * M[0] + M[3] * M[3] + M[5] * M[5] + 2.0 * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]);
-        const double QN = q11 * N[0] + q22 * N[3] + q33 * N[5] + 2.0 * (q12 * N[1] + q13 * N[2] + q23 * N[4]);
-        const double NN = N[0] * N[0] + N[3] * N[3] + N[5] * N[5] + 2.0 * (N[1] * N[1] + N[2] * N[2] + N[4] * N[4]);
-        double flm = FLM[i], fmm = FMM[i], fqn = FQN[i], fnn = FNN[i];
-        if (init_lm) { fmm = MM; flm = 0.03 * MM; }
-        lag_relax(flm, fmm, LM, MM, opftdelta, lagran_dt);
-        const double cs2d = fmax(CLIP, flm / fmm);
-        if (init_qn) { fnn = NN; fqn = 0.03 * NN; }
-        lag_relax(fqn, fnn, QN, NN, opftdelta, lagran_dt);
-        const double cs4d = fmax(CLIP, fqn / fnn);
-        double beta = pow(cs4d / cs2d, bexp);
-        if (k == p.nz) beta = 1.0;
-        const double betaclip = fmax(beta, 1.0 / (2.0 * 4.0));
-        FLM[i] = flm; FMM[i] = fmm; FQN[i] = fqn; FNN[i] = fnn;
-        Cs[i] = fmax(CLIP, cs2d / betaclip);
+        double LM, MM, QN, NN;
+        lag_scale([&](int j) { return (double)a[j * fs]; }, cst, 4.0, LM, MM);
+        lag_scale([&](int j) { return (double)h[j * fs]; }, cst, 16.0, QN, NN);
+        lag_update(u, i, k, LM, MM, QN, NN);
     }
 }
 
@@ -756,9 +711,10 @@ template <typename W>
 void lagrange_contract_dev(les_ctx* c, const W* b2, const W* b4, int init_lm, int init_qn) {
     KP p = kp(c);
     const double bexp = log(2.0) / (log(4.0) - log(2.0));
-    hipLaunchKernelGGL(k_lag_contract<W>, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, b2, b4,
-                       c->f3[LES_F_F_LM], c->f3[LES_F_F_MM], c->f3[LES_F_F_QN], c->f3[LES_F_F_NN],
-                       c->f3[LES_F_CS_OPT2], p, init_lm, init_qn, c->prm.dt * (double)c->prm.cs_count, bexp);
+    const LagUpdate u{c->f3[LES_F_F_LM], c->f3[LES_F_F_MM], c->f3[LES_F_F_QN], c->f3[LES_F_F_NN],
+                      c->f3[LES_F_CS_OPT2], init_lm, init_qn, p.nz, OPFTIME * p.delta,
+                      c->prm.dt * (double)c->prm.cs_count, bexp};
+    hipLaunchKernelGGL(k_lag_contract<W>, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, b2, b4, u, p);
 }
 
 // ---- vertical differences, 004_derivatives.f90:85-183 --------------------

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <utility>
 #include <vector>
+#include "les_lag.hpp"
 #include "les_ops.hpp"
 
 using namespace les;
@@ -272,12 +273,28 @@ void lagrange(les_ctx* c, int jt) {
     x_r2c<W>(c->stream, raw, spf, np21 * c->ny, c->nx, kc, c->tw_x, 1.0, kc);
     yline_spec<W>(c->stream, spf, spf, (int)np21, kc, psp, c->tw_y, SP_FWD, nullptr, kc, -1, mc);
     yline_spec<W>(c->stream, spf, spt, (int)np21, kc, psp, c->tw_y, SP_MULINV, c->Gtest, kc, c->lhx, mc);
-    x_c2r<W>(c->stream, spt, b2, np21 * c->ny, kc, c->nx, c->tw_x, 0, 0.0, kc);
-    yline_spec<W>(c->stream, spf, spt, (int)np21, kc4, psp, c->tw_y, SP_MULINV, c->Gtest2, kc, c->lhx, mc);
-    x_c2r<W>(c->stream, spt, b4, np21 * c->ny, kc, c->nx, c->tw_x, 0, 0.0, kc4);
     const int init_lm = P.inilag && !c->f_lm_mm_init && (jt == P.cs_count || jt == P.dyn_init);
     const int init_qn = P.inilag && !c->f_qn_nn_init && (jt == P.cs_count || jt == P.dyn_init);
-    lagrange_contract_dev<W>(c, b2, b4, init_lm, init_qn);
+    // each filter's fields contracted as they are transformed back (L M, M M
+    // of the 2 delta filter kept in the spent product buffer), else the
+    // filtered fields are stored and contracted by k_lag_contract
+    const double delta = 1.0 * pow(c->dx * c->dy * c->dz, 1.0 / 3.0);
+    const LagUpdate u{c->f3[LES_F_F_LM], c->f3[LES_F_F_MM], c->f3[LES_F_F_QN], c->f3[LES_F_F_NN],
+                      c->f3[LES_F_CS_OPT2], init_lm, init_qn, nz, OPFTIME * delta,
+                      P.dt * (double)P.cs_count, log(2.0) / (log(4.0) - log(2.0))};
+    double* lmb = c->lag_buf;
+    double* mmb = c->lag_buf + c->plane * c->nlev;
+    const double cst = 2.0 * (delta * delta);
+    if (c->fuse && g_use_ct &&
+        lag_contract_x<W>(c->stream, 2, spt, kc, kc, nz, c->ny, lmb, mmb, u, cst, c->tw_x)) {
+        yline_spec<W>(c->stream, spf, spt, (int)np21, kc4, psp, c->tw_y, SP_MULINV, c->Gtest2, kc, c->lhx, mc);
+        lag_contract_x<W>(c->stream, 4, spt, kc, kc4, nz, c->ny, lmb, mmb, u, cst, c->tw_x);
+    } else {
+        x_c2r<W>(c->stream, spt, b2, np21 * c->ny, kc, c->nx, c->tw_x, 0, 0.0, kc);
+        yline_spec<W>(c->stream, spf, spt, (int)np21, kc4, psp, c->tw_y, SP_MULINV, c->Gtest2, kc, c->lhx, mc);
+        x_c2r<W>(c->stream, spt, b4, np21 * c->ny, kc, c->nx, c->tw_x, 0, 0.0, kc4);
+        lagrange_contract_dev<W>(c, b2, b4, init_lm, init_qn);
+    }
     if (init_lm) c->f_lm_mm_init = 1;
     if (init_qn) c->f_qn_nn_init = 1;
 }
