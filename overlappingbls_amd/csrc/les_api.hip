@@ -415,14 +415,60 @@ void theta_pre(les_ctx* c, int jt) {
 
 // advection, SGS fluxes, calc_beta and step_scalar (:466-671)
 template <typename W>
+void theta_rhs(les_ctx* c);
+
+template <typename W>
 void theta_post(les_ctx* c, int jt, Tape* tape = nullptr) {
     const les_params& P = c->prm;
-    const int nz = c->nz;
-    const size_t np = c->plane, np2 = c->plane2;
+    const size_t np = c->plane;
     if (!theta_active(c, jt)) {
         fill_levels(c->stream, WF<W>(c, LES_F_BETA_SCAL), np, 0, (int)c->nlev, 0.0);
         return;
     }
+    theta_rhs<W>(c);
+    les_note(c, hipStreamWaitEvent(c->stream, c->ev_th, 0));
+    calc_beta_dev<W>(c, c->th_mean);
+    if (tape) tape_save(c, tape, TAPE_THETA);
+    if (jt == P.theta_init_time && !P.restart_theta)
+        les_note(c, hipMemcpyAsync(F(c, LES_F_RHST_F), F(c, LES_F_RHST), np * c->nlev * 8, hipMemcpyDeviceToDevice,
+                                   c->stream));
+    step_scalar_dev(c);
+}
+
+// The same with theta's RHS and update on the auxiliary stream, concurrent
+// with the momentum RHS and the pressure solve: calc_beta (which the momentum
+// RHS reads, and which must read theta before step_scalar updates it) runs on
+// the context's stream first; the rest runs on aux with its own scratch
+// (theta reads nothing the momentum path writes after convec, and writes
+// nothing it reads).  The caller joins the streams before the step ends.
+template <typename W>
+void theta_post_overlap(les_ctx* c, int jt) {
+    const les_params& P = c->prm;
+    const size_t np = c->plane;
+    if (!theta_active(c, jt)) {
+        fill_levels(c->stream, WF<W>(c, LES_F_BETA_SCAL), np, 0, (int)c->nlev, 0.0);
+        return;
+    }
+    les_note(c, hipStreamWaitEvent(c->stream, c->ev_th, 0));
+    calc_beta_dev<W>(c, c->th_mean);
+    fork(c);
+    double** scratch[5] = {&c->work3[0], &c->work3[1], &c->work3[2], &c->work3[6], &c->big3[3]};
+    std::swap(c->stream, c->aux);
+    for (int i = 0; i < 5; ++i) std::swap(*scratch[i], c->tscr[i]);
+    theta_rhs<W>(c);
+    if (jt == P.theta_init_time && !P.restart_theta)
+        les_note(c, hipMemcpyAsync(F(c, LES_F_RHST_F), F(c, LES_F_RHST), np * c->nlev * 8, hipMemcpyDeviceToDevice,
+                                   c->stream));
+    step_scalar_dev(c);
+    for (int i = 0; i < 5; ++i) std::swap(*scratch[i], c->tscr[i]);
+    std::swap(c->stream, c->aux);
+}
+
+// theta's RHS: advection and the SGS flux divergence (:466-594)
+template <typename W>
+void theta_rhs(les_ctx* c) {
+    const int nz = c->nz;
+    const size_t np = c->plane, np2 = c->plane2;
     W* rm = scr<W>(c->big3[3]);
     W* adv = scr<W>(c->work3[2]);
     // advection and the x pass of its unpad in one kernel (x-unpadded rows in
@@ -451,13 +497,6 @@ void theta_post(les_ctx* c, int jt, Tape* tape = nullptr) {
         ddz_w_dev<W, W>(c, dtemp, temp);
         scalar_accum_dev<W>(c, dtemp, (const W*)nullptr);
     }
-    les_note(c, hipStreamWaitEvent(c->stream, c->ev_th, 0));
-    calc_beta_dev<W>(c, c->th_mean);
-    if (tape) tape_save(c, tape, TAPE_THETA);
-    if (jt == P.theta_init_time && !P.restart_theta)
-        les_note(c, hipMemcpyAsync(F(c, LES_F_RHST_F), F(c, LES_F_RHST), np * c->nlev * 8, hipMemcpyDeviceToDevice,
-                                   c->stream));
-    step_scalar_dev(c);
 }
 
 template <typename W>
@@ -547,15 +586,18 @@ void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     divstress<W>(c, WF<W>(c, LES_F_DIVTY), txy, tyy, tyz, 0);
     divstress<W>(c, WF<W>(c, LES_F_DIVTZ), txz, tyz, tzz, 1);
     convec<W>(c);
-    theta_post<W>(c, jt, tape);
+    const bool st = stats && stats_step(c, jt);
+    const bool ovl = c->overlap && !tape && !st;  // taped and statistics steps stay in order
+    if (ovl) theta_post_overlap<W>(c, jt);
+    else theta_post<W>(c, jt, tape);
     rhs_ab2_dev<W>(c, jt);
     press_stag_array<W>(c);
     gradp_project_dev<W>(c, jt);
-    const bool st = stats && stats_step(c, jt);
     if (st) diagnostics_ke(c);
     filter_f<double>(c, F(c, LES_F_U), 0, nl);
     filter_f<double>(c, F(c, LES_F_V), 0, nl);
     filter_f<double>(c, F(c, LES_F_W), 0, nl);
+    if (ovl) join(c);
     if (st) avg_slices<W>(c, jt);
     if (tape) tape_save(c, tape, TAPE_END);
 }
@@ -655,6 +697,9 @@ int les_create(const les_params* prm, les_ctx** out) {
     // same results; for checking that)
     c->fuse = 1;
     if (const char* fz = getenv("LES_FUSE")) c->fuse = strcmp(fz, "0") != 0;
+    // LES_OVERLAP=0: theta's RHS and update in order on the context's stream
+    c->overlap = 1;
+    if (const char* ov = getenv("LES_OVERLAP")) c->overlap = strcmp(ov, "0") != 0;
     c->nx = prm->nxt; c->ny = prm->nyt; c->nz = prm->nzt;
     c->nx2 = 3 * c->nx / 2; c->ny2 = 3 * c->ny / 2;
     c->lhx = c->nx / 2 + 1; c->lhy = c->ny / 2 + 1;
@@ -690,6 +735,9 @@ int les_create(const les_params* prm, les_ctx** out) {
     if (e == hipSuccess) e = dalloc(c, &c->th_mean, c->nlev);
     if (e == hipSuccess) e = dalloc(c, &c->th1_copy, c->plane);
     for (int i = 0; i < 8 && e == hipSuccess; ++i) e = dalloc(c, &c->work3[i], i == 6 ? (F3 * 3) / 2 + c->plane2 : F3);
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = dalloc(c, &c->tscr[i], F3);
+    if (e == hipSuccess) e = dalloc(c, &c->tscr[3], (F3 * 3) / 2 + c->plane2);
+    if (e == hipSuccess) e = dalloc(c, &c->tscr[4], B3);
     for (int i = 0; i < 4 && e == hipSuccess; ++i) e = dalloc(c, &c->spec3[i], c->ns * c->nlev);
     const size_t np21 = (size_t)21 * c->nz;
     if (e == hipSuccess) e = dalloc(c, &c->lag_buf, 3 * np21 * c->plane);

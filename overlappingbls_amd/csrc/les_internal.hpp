@@ -123,6 +123,9 @@ struct les_ctx {
     int f_lm_mm_init, f_qn_nn_init;
     int sgs_frozen;            // skip the Lagrangian SGS update (Cs_opt2, F_* held)
     int fuse;                  // fused kernels (LES_FUSE=0 at creation: the separate ones)
+    int overlap;               // theta's RHS and update on aux (LES_OVERLAP=0: in order)
+    double* tscr[5];           // theta's private scratch while it runs on aux:
+                               // work3[0..2], work3[6], big3[3] stand-ins
     les::LinCtx* lin;          // linearised-step buffers (les_lin.hip), or null
     double cfl;
     hipError_t err;            // first asynchronous HIP error since the last finish()

@@ -1,4 +1,5 @@
-"""Fused kernels against the separate kernels they replace.
+"""Fused kernels against the separate kernels they replace, and theta's
+update on the auxiliary stream against the in-order step.
 
 The fused first passes of the 3/2-grid transforms (vorticity -> pad, cross
 product / scalar advection -> unpad), the strain formed inside the Nu_t /
@@ -6,7 +7,9 @@ tau kernel and the vertical flux divergence added as it is differenced each
 compute every value with the same operations, in the same precision, as
 the kernels they replace.  So a run with them (default) must equal a run
 with LES_FUSE=0 bit for bit, in both precisions, over steps that include
-the dynamic SGS update (cs_count steps) and the plain steps between.
+the dynamic SGS update (cs_count steps) and the plain steps between.  The
+same holds for LES_OVERLAP=0 (theta's RHS and update in order instead of
+concurrent with the momentum RHS and the pressure solve).
 """
 import os
 
@@ -20,17 +23,17 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("U", "V", "W", "THETA", "RHSX", "RHSY", "RHSZ", "RHST", "CS_OPT2", "F_LM", "F_QN", "TXX", "TXZ", "NU_T")
 
 
-def run(case, gpu_lib, fuse, mixed, nsteps):
-    old = os.environ.get("LES_FUSE")
-    os.environ["LES_FUSE"] = "1" if fuse else "0"
+def run(case, gpu_lib, fuse, mixed, nsteps, var="LES_FUSE"):
+    old = os.environ.get(var)
+    os.environ[var] = "1" if fuse else "0"
     try:
         nl = config.workload(case)
         g = gpu_lib.create(nl.params())
     finally:
         if old is None:
-            del os.environ["LES_FUSE"]
+            del os.environ[var]
         else:
-            os.environ["LES_FUSE"] = old
+            os.environ[var] = old
     if mixed:
         from overlappingbls_amd.abi import LES_MIXED
         g.set_precision(LES_MIXED)
@@ -39,11 +42,12 @@ def run(case, gpu_lib, fuse, mixed, nsteps):
     return {n: g.get(n) for n in FIELDS}
 
 
+@pytest.mark.parametrize("var", ["LES_FUSE", "LES_OVERLAP"])
 @pytest.mark.parametrize("mixed", [False, True])
 @pytest.mark.parametrize("case", ["mid", "sbl128"])
-def test_fused_equals_separate(case, mixed, gpu_lib):
+def test_fused_equals_separate(case, mixed, var, gpu_lib):
     nsteps = 6
-    a = run(case, gpu_lib, True, mixed, nsteps)
-    b = run(case, gpu_lib, False, mixed, nsteps)
+    a = run(case, gpu_lib, True, mixed, nsteps, var)
+    b = run(case, gpu_lib, False, mixed, nsteps, var)
     for n in FIELDS:
         assert np.array_equal(a[n], b[n], equal_nan=True), n
