@@ -1222,11 +1222,11 @@ void ab2_dev(les_ctx* c) {
 
 // RHS -= grad p (103_flosol.f90:179-181; the dudt diagnostics likewise,
 // :183-185, when du != null) fused with project (110_project.f90:16-40):
-// u, v on levels 1..nz-1, w from level 2
+// u, v on levels 1..nz-1, w from level 2 (uvw = 0: RHS and dudt only)
 template <typename W>
 __global__ void k_gradp_project(double* rx, double* ry, double* rz, double* u, double* v, double* w,
                                 const W* __restrict__ px, const W* __restrict__ py, const W* __restrict__ pz, KP p,
-                                double* du, double* dv, double* dw) {
+                                double* du, double* dv, double* dw, int uvw) {
     const long n = p.plane * (p.nz - 1);
     const double tadv1 = 1.5;
     GRID_LOOP(t, n) {
@@ -1241,9 +1241,11 @@ __global__ void k_gradp_project(double* rx, double* ry, double* rz, double* u, d
             dv[i] = dv[i] - gy;
             dw[i] = dw[i] - gz;
         }
-        u[i] = u[i] + p.dt * (-tadv1 * gx);
-        v[i] = v[i] + p.dt * (-tadv1 * gy);
-        if (k >= 2) w[i] = w[i] + p.dt * (-tadv1 * gz);
+        if (uvw) {
+            u[i] = u[i] + p.dt * (-tadv1 * gx);
+            v[i] = v[i] + p.dt * (-tadv1 * gy);
+            if (k >= 2) w[i] = w[i] + p.dt * (-tadv1 * gz);
+        }
     }
 }
 
@@ -1282,8 +1284,24 @@ __global__ void k_project_bc(double* u, double* v, double* w, double* th, KP p, 
     }
 }
 
+// theta's rows of k_project_bc, for a theta update that runs on the
+// auxiliary stream: enqueued there after that update, so the rows are written
+// last, as in the serial order
+__global__ void k_theta_rows(double* th, KP p) {
+    GRID_LOOP(q, p.plane) {
+        th[q] = BOGUS;
+        th[(long)p.nz * p.plane + q] = BOGUS;
+    }
+}
+
+void theta_rows_dev(les_ctx* c) {
+    KP p = kp(c);
+    hipLaunchKernelGGL(k_theta_rows, grid_for(p.plane), dim3(256), 0, c->stream, c->f3[LES_F_THETA], p);
+}
+
+// theta_rows = false: theta's rows are left to theta_rows_dev
 template <typename W>
-void gradp_project_dev(les_ctx* c, int jt) {
+void gradp_project_dev(les_ctx* c, int jt, bool theta_rows, bool uvw) {
     KP p = kp(c);
     const les_params& P = c->prm;
     const bool st = stats_step(c, jt);
@@ -1291,10 +1309,14 @@ void gradp_project_dev(les_ctx* c, int jt) {
                        c->f3[LES_F_RHSX], c->f3[LES_F_RHSY], c->f3[LES_F_RHSZ], c->f3[LES_F_U], c->f3[LES_F_V],
                        c->f3[LES_F_W], (const W*)WF<W>(c, LES_F_DPDX), (const W*)WF<W>(c, LES_F_DPDY),
                        (const W*)WF<W>(c, LES_F_DPDZ), p, st ? c->st_diag[0] : nullptr,
-                       st ? c->st_diag[1] : nullptr, st ? c->st_diag[2] : nullptr);
+                       st ? c->st_diag[1] : nullptr, st ? c->st_diag[2] : nullptr, (int)uvw);
+    if (!uvw) {
+        if (P.theta_flag && theta_rows) theta_rows_dev(c);
+        return;
+    }
     hipLaunchKernelGGL(k_project_bc, grid_for(p.plane), dim3(256), 0, c->stream, c->f3[LES_F_U], c->f3[LES_F_V],
                        c->f3[LES_F_W], c->f3[LES_F_THETA], p, (int)(P.ubc_mom == LES_BC_STRESS_FREE),
-                       P.theta_flag);
+                       (int)(P.theta_flag && theta_rows));
 }
 
 template <typename W>
@@ -1393,7 +1415,7 @@ void max3_dev(les_ctx* c, double* out) {
     template void press_hz_sums_dev<W>(les_ctx*, double*);                                                  \
     template void dpdz_dev<W>(les_ctx*);                                                                    \
     template void rhs_ab2_dev<W>(les_ctx*, int);                                                            \
-    template void gradp_project_dev<W>(les_ctx*, int);                                                      \
+    template void gradp_project_dev<W>(les_ctx*, int, bool, bool);                                                      \
     template void project_dev<W>(les_ctx*);
 LES_INST_POINT(double)
 LES_INST_POINT(float)

@@ -440,7 +440,10 @@ void theta_post(les_ctx* c, int jt, Tape* tape = nullptr) {
 // RHS reads, and which must read theta before step_scalar updates it) runs on
 // the context's stream first; the rest runs on aux with its own scratch
 // (theta reads nothing the momentum path writes after convec, and writes
-// nothing it reads).  The caller joins the streams before the step ends.
+// nothing it reads or writes: project's boundary rows of theta, levels 0 and
+// nz, are written on aux after the update, not by k_project_bc, so that they
+// come last as in the serial order).  The caller joins the streams before the
+// step ends.
 template <typename W>
 void theta_post_overlap(les_ctx* c, int jt) {
     const les_params& P = c->prm;
@@ -460,6 +463,7 @@ void theta_post_overlap(les_ctx* c, int jt) {
         les_note(c, hipMemcpyAsync(F(c, LES_F_RHST_F), F(c, LES_F_RHST), np * c->nlev * 8, hipMemcpyDeviceToDevice,
                                    c->stream));
     step_scalar_dev(c);
+    theta_rows_dev(c);  // project's theta rows (left out of k_project_bc on the context's stream)
     for (int i = 0; i < 5; ++i) std::swap(*scratch[i], c->tscr[i]);
     std::swap(c->stream, c->aux);
 }
@@ -549,6 +553,27 @@ void avg_slices(les_ctx* c, int jt) {
     }
 }
 
+// RHS -= grad p with project (110_project.f90:16-40) evaluated inside
+// filter_data's x pass of u, v, w (103_flosol.f90:179-181, :242-244): the
+// projected fields are not written and re-read.  The x pass writes
+// work3[3..5], the y pass those into u, v, w.  Every value is computed and
+// rounded as gradp_project_dev and filter_f compute it.  False when the line
+// length has no register-radix specialisation.
+template <typename W>
+bool project_filter(les_ctx* c, int jt, bool theta_rows) {
+    const les_params& P = c->prm;
+    const int nl = (int)c->nlev;
+    const ProjIn<W> a{F(c, LES_F_U), F(c, LES_F_V), F(c, LES_F_W), WF<W>(c, LES_F_DPDX), WF<W>(c, LES_F_DPDY),
+                      WF<W>(c, LES_F_DPDZ), P.dt, (int)(P.ubc_mom == LES_BC_STRESS_FREE)};
+    double* t[3] = {c->work3[3], c->work3[4], c->work3[5]};
+    if (!proj_nyq_x<W>(c->stream, a, t[0], t[1], t[2], c->nz, c->ny, c->tw_x)) return false;
+    gradp_project_dev<W>(c, jt, theta_rows, false);
+    for (int m = 0; m < 3; ++m)
+        yline_real<double>(c->stream, t[m], F(c, LES_F_U + m), nl, c->nx, c->nx, c->nx, (long)c->plane,
+                           (long)c->plane, c->tw_y, c->tw_y, OP_NYQ, 0.0);
+    return true;
+}
+
 template <typename W>
 void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     // newstep: RHS_f = RHS (103_flosol.f90:311-313) as a pointer exchange
@@ -592,11 +617,16 @@ void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     else theta_post<W>(c, jt, tape);
     rhs_ab2_dev<W>(c, jt);
     press_stag_array<W>(c);
-    gradp_project_dev<W>(c, jt);
-    if (st) diagnostics_ke(c);
-    filter_f<double>(c, F(c, LES_F_U), 0, nl);
-    filter_f<double>(c, F(c, LES_F_V), 0, nl);
-    filter_f<double>(c, F(c, LES_F_W), 0, nl);
+    const bool th_rows = !(ovl && theta_active(c, jt));
+    if (!st && c->fuse && g_use_ct && project_filter<W>(c, jt, th_rows)) {
+        // project evaluated inside filter_data's x pass (below)
+    } else {
+        gradp_project_dev<W>(c, jt, th_rows);
+        if (st) diagnostics_ke(c);
+        filter_f<double>(c, F(c, LES_F_U), 0, nl);
+        filter_f<double>(c, F(c, LES_F_V), 0, nl);
+        filter_f<double>(c, F(c, LES_F_W), 0, nl);
+    }
     if (ovl) join(c);
     if (st) avg_slices<W>(c, jt);
     if (tape) tape_save(c, tape, TAPE_END);
