@@ -715,20 +715,6 @@ __device__ __forceinline__ double fused_value(const VortIn<T>& g, int k, long i,
     }
 }
 
-// project (k_gradp_project's update of u, v, w) and the boundary rows
-// k_project_bc then writes; every load in the column, the row chosen by
-// selects
-template <typename T, int COMP, typename W>
-__device__ __forceinline__ double fused_value(const ProjIn<W>& a, int k, long i, long P, int nz) {
-    const double* f = (COMP == 0) ? a.u : ((COMP == 1) ? a.v : a.w);
-    const W* g = (COMP == 0) ? a.px : ((COMP == 1) ? a.py : a.pz);
-    const long is = (COMP < 2 && a.sf && k == nz - 1) ? i - P : i;
-    const double gv = g[is];
-    const double pv = f[is] + a.dt * (-1.5 * gv);
-    if constexpr (COMP < 2) return (k == 0 || k == nz) ? BOGUS : pv;
-    else return (k == 0) ? BOGUS : ((k == 1 || k == nz) ? 0.0 : pv);
-}
-
 // inputs of a block's NL lines into the line buffer (component COMP)
 template <typename T, int NIN, int NL, int NTH, int LS, int COMP, typename IN>
 __device__ __forceinline__ void fused_inputs(const IN& a, cx<T>* buf, long rp0, long nrp, int ny_in, long plane_in,
@@ -746,7 +732,7 @@ __device__ __forceinline__ void fused_inputs(const IN& a, cx<T>* buf, long rp0, 
         const int rpc = (int)((rp < nrp) ? rp : nrp - 1);  // rows past the end read the last pair
         const int ra = 2 * rpc;
         const int km = ra / ny_in;
-        const int k = IN::K0 + km;
+        const int k = 1 + km;
         const long ia = (long)k * plane_in + (long)(ra - km * ny_in) * NIN + p;
         const double va = fused_value<T, COMP>(a, k, ia, plane_in, nz);
         const double vb = fused_value<T, COMP>(a, k, ia + NIN, plane_in, nz);
@@ -1088,20 +1074,6 @@ bool vort_pad_x(hipStream_t s, const VortIn<T>& g, T* o0, T* o1, T* o2, int nz, 
 #undef VCASE
     return false;
 }
-template <typename W>
-bool proj_nyq_x(hipStream_t s, const ProjIn<W>& a, double* o0, double* o1, double* o2, int nz, int ny,
-                const Twiddle& tw) {
-    const long plane = (long)tw.n * ny;
-    const long nrp = (long)(nz + 1) * ny / 2;
-#define PCASE(NN_)                                                                                          \
-    if (tw.n == NN_ && ny % 2 == 0) {                                                                       \
-        launch_fused<double, NN_, NN_, OP_NYQ, 3>(s, a, 0, o0, o1, o2, nrp, ny, plane, nz, tw, tw);          \
-        return true;                                                                                        \
-    }
-    LES_SIZES(PCASE)
-#undef PCASE
-    return false;
-}
 template <typename T>
 bool lag_contract_x(hipStream_t s, int scale, const cx<T>* spt, int ldc, int kst, int nz, int ny, double* lm,
                     double* mm, const LagUpdate& u, double cst, const Twiddle& tw) {
@@ -1130,10 +1102,6 @@ template bool cross_unpad_x<double>(hipStream_t, const ConvIn<double>&, int, dou
                                     const Twiddle&, const Twiddle&);
 template bool cross_unpad_x<float>(hipStream_t, const ConvIn<float>&, int, float*, float*, float*, int, int,
                                    const Twiddle&, const Twiddle&);
-template bool proj_nyq_x<double>(hipStream_t, const ProjIn<double>&, double*, double*, double*, int, int,
-                                 const Twiddle&);
-template bool proj_nyq_x<float>(hipStream_t, const ProjIn<float>&, double*, double*, double*, int, int,
-                                const Twiddle&);
 template bool vort_pad_x<double>(hipStream_t, const VortIn<double>&, double*, double*, double*, int, int,
                                  const Twiddle&, const Twiddle&);
 template bool vort_pad_x<float>(hipStream_t, const VortIn<float>&, float*, float*, float*, int, int, const Twiddle&,

@@ -1222,11 +1222,11 @@ void ab2_dev(les_ctx* c) {
 
 // RHS -= grad p (103_flosol.f90:179-181; the dudt diagnostics likewise,
 // :183-185, when du != null) fused with project (110_project.f90:16-40):
-// u, v on levels 1..nz-1, w from level 2 (uvw = 0: RHS and dudt only)
+// u, v on levels 1..nz-1, w from level 2
 template <typename W>
 __global__ void k_gradp_project(double* rx, double* ry, double* rz, double* u, double* v, double* w,
                                 const W* __restrict__ px, const W* __restrict__ py, const W* __restrict__ pz, KP p,
-                                double* du, double* dv, double* dw, int uvw) {
+                                double* du, double* dv, double* dw) {
     const long n = p.plane * (p.nz - 1);
     const double tadv1 = 1.5;
     GRID_LOOP(t, n) {
@@ -1241,11 +1241,9 @@ __global__ void k_gradp_project(double* rx, double* ry, double* rz, double* u, d
             dv[i] = dv[i] - gy;
             dw[i] = dw[i] - gz;
         }
-        if (uvw) {
-            u[i] = u[i] + p.dt * (-tadv1 * gx);
-            v[i] = v[i] + p.dt * (-tadv1 * gy);
-            if (k >= 2) w[i] = w[i] + p.dt * (-tadv1 * gz);
-        }
+        u[i] = u[i] + p.dt * (-tadv1 * gx);
+        v[i] = v[i] + p.dt * (-tadv1 * gy);
+        if (k >= 2) w[i] = w[i] + p.dt * (-tadv1 * gz);
     }
 }
 
@@ -1301,7 +1299,7 @@ void theta_rows_dev(les_ctx* c) {
 
 // theta_rows = false: theta's rows are left to theta_rows_dev
 template <typename W>
-void gradp_project_dev(les_ctx* c, int jt, bool theta_rows, bool uvw) {
+void gradp_project_dev(les_ctx* c, int jt, bool theta_rows) {
     KP p = kp(c);
     const les_params& P = c->prm;
     const bool st = stats_step(c, jt);
@@ -1309,11 +1307,7 @@ void gradp_project_dev(les_ctx* c, int jt, bool theta_rows, bool uvw) {
                        c->f3[LES_F_RHSX], c->f3[LES_F_RHSY], c->f3[LES_F_RHSZ], c->f3[LES_F_U], c->f3[LES_F_V],
                        c->f3[LES_F_W], (const W*)WF<W>(c, LES_F_DPDX), (const W*)WF<W>(c, LES_F_DPDY),
                        (const W*)WF<W>(c, LES_F_DPDZ), p, st ? c->st_diag[0] : nullptr,
-                       st ? c->st_diag[1] : nullptr, st ? c->st_diag[2] : nullptr, (int)uvw);
-    if (!uvw) {
-        if (P.theta_flag && theta_rows) theta_rows_dev(c);
-        return;
-    }
+                       st ? c->st_diag[1] : nullptr, st ? c->st_diag[2] : nullptr);
     hipLaunchKernelGGL(k_project_bc, grid_for(p.plane), dim3(256), 0, c->stream, c->f3[LES_F_U], c->f3[LES_F_V],
                        c->f3[LES_F_W], c->f3[LES_F_THETA], p, (int)(P.ubc_mom == LES_BC_STRESS_FREE),
                        (int)(P.theta_flag && theta_rows));
@@ -1415,7 +1409,7 @@ void max3_dev(les_ctx* c, double* out) {
     template void press_hz_sums_dev<W>(les_ctx*, double*);                                                  \
     template void dpdz_dev<W>(les_ctx*);                                                                    \
     template void rhs_ab2_dev<W>(les_ctx*, int);                                                            \
-    template void gradp_project_dev<W>(les_ctx*, int, bool, bool);                                                      \
+    template void gradp_project_dev<W>(les_ctx*, int, bool);                                                      \
     template void project_dev<W>(les_ctx*);
 LES_INST_POINT(double)
 LES_INST_POINT(float)

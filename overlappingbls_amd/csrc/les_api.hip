@@ -553,27 +553,6 @@ void avg_slices(les_ctx* c, int jt) {
     }
 }
 
-// RHS -= grad p with project (110_project.f90:16-40) evaluated inside
-// filter_data's x pass of u, v, w (103_flosol.f90:179-181, :242-244): the
-// projected fields are not written and re-read.  The x pass writes
-// work3[3..5], the y pass those into u, v, w.  Every value is computed and
-// rounded as gradp_project_dev and filter_f compute it.  False when the line
-// length has no register-radix specialisation.
-template <typename W>
-bool project_filter(les_ctx* c, int jt, bool theta_rows) {
-    const les_params& P = c->prm;
-    const int nl = (int)c->nlev;
-    const ProjIn<W> a{F(c, LES_F_U), F(c, LES_F_V), F(c, LES_F_W), WF<W>(c, LES_F_DPDX), WF<W>(c, LES_F_DPDY),
-                      WF<W>(c, LES_F_DPDZ), P.dt, (int)(P.ubc_mom == LES_BC_STRESS_FREE)};
-    double* t[3] = {c->work3[3], c->work3[4], c->work3[5]};
-    if (!proj_nyq_x<W>(c->stream, a, t[0], t[1], t[2], c->nz, c->ny, c->tw_x)) return false;
-    gradp_project_dev<W>(c, jt, theta_rows, false);
-    for (int m = 0; m < 3; ++m)
-        yline_real<double>(c->stream, t[m], F(c, LES_F_U + m), nl, c->nx, c->nx, c->nx, (long)c->plane,
-                           (long)c->plane, c->tw_y, c->tw_y, OP_NYQ, 0.0);
-    return true;
-}
-
 template <typename W>
 void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     // newstep: RHS_f = RHS (103_flosol.f90:311-313) as a pointer exchange
@@ -617,16 +596,11 @@ void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     else theta_post<W>(c, jt, tape);
     rhs_ab2_dev<W>(c, jt);
     press_stag_array<W>(c);
-    const bool th_rows = !(ovl && theta_active(c, jt));
-    if (!st && c->fuse && g_use_ct && project_filter<W>(c, jt, th_rows)) {
-        // project evaluated inside filter_data's x pass (below)
-    } else {
-        gradp_project_dev<W>(c, jt, th_rows);
-        if (st) diagnostics_ke(c);
-        filter_f<double>(c, F(c, LES_F_U), 0, nl);
-        filter_f<double>(c, F(c, LES_F_V), 0, nl);
-        filter_f<double>(c, F(c, LES_F_W), 0, nl);
-    }
+    gradp_project_dev<W>(c, jt, !(ovl && theta_active(c, jt)));
+    if (st) diagnostics_ke(c);
+    filter_f<double>(c, F(c, LES_F_U), 0, nl);
+    filter_f<double>(c, F(c, LES_F_V), 0, nl);
+    filter_f<double>(c, F(c, LES_F_W), 0, nl);
     if (ovl) join(c);
     if (st) avg_slices<W>(c, jt);
     if (tape) tape_save(c, tape, TAPE_END);

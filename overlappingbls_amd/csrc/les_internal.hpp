@@ -198,29 +198,15 @@ bool yline_spec_ct(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int 
 // register-radix specialisation (the caller then runs the separate kernels).
 // ConvIn: padded u, v, w and b1..b3 = padded vorticity (cross product) or
 // padded theta gradients (scalar advection); ust, vst the Galilean shift.
-// K0: the level of the first row an input evaluates.
 template <typename T>
 struct ConvIn {
-    static constexpr int K0 = 1;
     const T *ub, *vb, *wb, *b1, *b2, *b3;
     const double *ust, *vst;
 };
 // VortIn: the velocity gradients of the vorticity
 template <typename T>
 struct VortIn {
-    static constexpr int K0 = 1;
     const T *uy, *uz, *vx, *vz, *wx, *wy;
-};
-// ProjIn: u, v, w before project and the pressure gradient: project
-// (110_project.f90:16-40) and its boundary rows (k_project_bc), as the input
-// of filter_data's x pass
-template <typename W>
-struct ProjIn {
-    static constexpr int K0 = 0;
-    const double *u, *v, *w;
-    const W *px, *py, *pz;
-    double dt;
-    int sf;  // stress-free top: level nz-1 of u, v takes level nz-2
 };
 // cross product (scalar = 0: components to o0, o1, o2) or scalar advection
 // (scalar = 1: to o0) on levels 1..nz-1 of the 3/2 grid, then the x unpad:
@@ -233,11 +219,6 @@ bool cross_unpad_x(hipStream_t s, const ConvIn<T>& a, int scalar, T* o0, T* o1, 
 template <typename T>
 bool vort_pad_x(hipStream_t s, const VortIn<T>& g, T* o0, T* o1, T* o2, int nz, int ny, const Twiddle& tin,
                 const Twiddle& tout);
-// project of u, v, w on levels 0..nz, then filter_data's x pass: o0..o2 hold
-// (nz + 1) * ny rows of length tw.n (FP64, as the state)
-template <typename W>
-bool proj_nyq_x(hipStream_t s, const ProjIn<W>& a, double* o0, double* o1, double* o2, int nz, int ny,
-                const Twiddle& tw);
 // process-wide switch: 0 forces the generic runtime-length kernels
 extern int g_use_ct;
 
@@ -304,9 +285,7 @@ template <typename W> void dpdz_dev(les_ctx* c);
 // RHS assembly + Admas_Bashforth, and RHS -= grad p + project, each one pass
 template <typename W> void rhs_ab2_dev(les_ctx* c, int jt);
 void ab2_dev(les_ctx* c);
-// uvw = false: the RHS (and dudt) update only; u, v, w and their boundary
-// rows are left to proj_nyq_x
-template <typename W> void gradp_project_dev(les_ctx* c, int jt, bool theta_rows = true, bool uvw = true);
+template <typename W> void gradp_project_dev(les_ctx* c, int jt, bool theta_rows = true);
 void theta_rows_dev(les_ctx* c);
 template <typename W> void project_dev(les_ctx* c);
 void max3_dev(les_ctx* c, double* out);
