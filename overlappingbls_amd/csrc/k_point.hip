@@ -1328,7 +1328,7 @@ void project_dev(les_ctx* c) {
 // ---- check_cfl, 901_debug.f90:85-99 ----------------------------------------------
 // Two-stage max over levels 1..nz-1 of u, v, w; the final block forms
 // u_max dt/dx + v_max dt/dy + w_max dt/dz.
-constexpr int MAXB = 512;
+constexpr int MAXB = MAX3_BLOCKS;
 
 __global__ __launch_bounds__(256) void k_max3_part(const double* __restrict__ u, const double* __restrict__ v,
                                                    const double* __restrict__ w, long lo, long hi,
@@ -1339,6 +1339,38 @@ __global__ __launch_bounds__(256) void k_max3_part(const double* __restrict__ u,
         m[0] = fmax(m[0], u[i]);
         m[1] = fmax(m[1], v[i]);
         m[2] = fmax(m[2], w[i]);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) sh[a][threadIdx.x] = m[a];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) sh[a][threadIdx.x] = fmax(sh[a][threadIdx.x], sh[a][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) part[a * MAXB + blockIdx.x] = sh[a][0];
+}
+
+// k_max3_part over levels lo..hi fused with k_lag_acc over all n values:
+// u, v, w are read once
+__global__ __launch_bounds__(256) void k_max3_lag(const double* __restrict__ u, const double* __restrict__ v,
+                                                  const double* __restrict__ w, double* ul, double* vl, double* wl,
+                                                  long n, long lo, long hi, double* __restrict__ part) {
+    __shared__ double sh[3][256];
+    double m[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const double a = u[i], b = v[i], d = w[i];
+        ul[i] = ul[i] + a;
+        vl[i] = vl[i] + b;
+        wl[i] = wl[i] + d;
+        if (i >= lo && i < hi) {
+            m[0] = fmax(m[0], a);
+            m[1] = fmax(m[1], b);
+            m[2] = fmax(m[2], d);
+        }
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a) sh[a][threadIdx.x] = m[a];
@@ -1373,14 +1405,20 @@ __global__ __launch_bounds__(256) void k_max3_final(const double* __restrict__ p
     if (threadIdx.x == 0) *out = sh[0][0] * cu + sh[1][0] * cv + sh[2][0] * cw;
 }
 
-void max3_dev(les_ctx* c, double* out) {
+void max3_dev(les_ctx* c, double* out, bool lag) {
     const long lo = (long)c->plane, hi = (long)c->plane * c->nz;
-    long nb = (hi - lo + 255) / 256;
+    const long n = lag ? (long)c->plane * (long)c->nlev : hi - lo;
+    long nb = (n + 255) / 256;
     if (nb > MAXB) nb = MAXB;
     const double dt = c->prm.dt;
     double* part = c->red + 64;  // red holds >= 64 + 3*MAXB doubles
-    hipLaunchKernelGGL(k_max3_part, dim3((unsigned)nb), dim3(256), 0, c->stream, (const double*)c->f3[LES_F_U],
-                       (const double*)c->f3[LES_F_V], (const double*)c->f3[LES_F_W], lo, hi, part);
+    if (lag)
+        hipLaunchKernelGGL(k_max3_lag, dim3((unsigned)nb), dim3(256), 0, c->stream, (const double*)c->f3[LES_F_U],
+                           (const double*)c->f3[LES_F_V], (const double*)c->f3[LES_F_W], c->f3[LES_F_U_LAG],
+                           c->f3[LES_F_V_LAG], c->f3[LES_F_W_LAG], n, lo, hi, part);
+    else
+        hipLaunchKernelGGL(k_max3_part, dim3((unsigned)nb), dim3(256), 0, c->stream, (const double*)c->f3[LES_F_U],
+                           (const double*)c->f3[LES_F_V], (const double*)c->f3[LES_F_W], lo, hi, part);
     hipLaunchKernelGGL(k_max3_final, dim3(1), dim3(256), 0, c->stream, (const double*)part, (int)nb, out,
                        dt / c->dx, dt / c->dy, dt / c->dz);
 }
@@ -1409,7 +1447,7 @@ void max3_dev(les_ctx* c, double* out) {
     template void press_hz_sums_dev<W>(les_ctx*, double*);                                                  \
     template void dpdz_dev<W>(les_ctx*);                                                                    \
     template void rhs_ab2_dev<W>(les_ctx*, int);                                                            \
-    template void gradp_project_dev<W>(les_ctx*, int, bool);                                                      \
+    template void gradp_project_dev<W>(les_ctx*, int, bool);                                                \
     template void project_dev<W>(les_ctx*);
 LES_INST_POINT(double)
 LES_INST_POINT(float)

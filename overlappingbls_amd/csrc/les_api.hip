@@ -308,7 +308,7 @@ void sgs_stag(les_ctx* c, int jt, bool keep_s = true) {
     const les_params& P = c->prm;
     const bool init = (jt == 1 && P.inilag);
     const bool update = !c->sgs_frozen && (jt >= P.dyn_init || P.restart_vel) && (jt % P.cs_count) == 0;
-    if (!c->sgs_frozen) lag_accumulate_dev(c);
+    if (!c->sgs_frozen && !c->lag_done) lag_accumulate_dev(c);
     if (c->fuse && !keep_s && !init && !update) {
         strain_stress_dev<W>(c);
         return;
@@ -748,7 +748,7 @@ int les_create(const les_params* prm, les_ctx** out) {
     if (e == hipSuccess) e = dalloc(c, &c->lag_spec, 2 * np21 * c->ns);
     for (int i = 0; i < 4 && e == hipSuccess; ++i) e = dalloc(c, &c->fold[i], F3);
     if (e == hipSuccess) e = dalloc(c, &c->scal, 64);
-    if (e == hipSuccess) e = dalloc(c, &c->red, 64 + 3 * 512 + c->nlev);
+    if (e == hipSuccess) e = dalloc(c, &c->red, 64 + 3 * MAX3_BLOCKS + c->nlev);
     c->cfl_cap = 4096;
     if (e == hipSuccess) e = dalloc(c, &c->cfl_dev, (size_t)c->cfl_cap);
     if (e == hipSuccess) e = dalloc(c, &c->ust, c->nlev);
@@ -1090,8 +1090,13 @@ int les_run(les_ctx* c, int jt0, int nsteps, int* failed_jt) {
     while (done < nsteps) {
         const int chunk = (nsteps - done) < c->cfl_cap ? (nsteps - done) : c->cfl_cap;
         for (int s = 0; s < chunk; ++s) {
-            max3_dev(c, c->cfl_dev + s);  // check_cfl of newstep (901_debug.f90:85-99)
+            // check_cfl of newstep (901_debug.f90:85-99), with sgs_stag's u_lag
+            // accumulation in the same pass (u, v, w do not change in between)
+            const bool lag = c->fuse && !c->sgs_frozen;
+            max3_dev(c, c->cfl_dev + s, lag);
+            c->lag_done = lag;
             LES_DISPATCH(step_body, c, jt0 + done + s);
+            c->lag_done = 0;
         }
         les_note(c, hipMemcpyAsync(c->cfl_host, c->cfl_dev, sizeof(double) * chunk, hipMemcpyDeviceToHost, c->stream));
         const int st = finish(c);

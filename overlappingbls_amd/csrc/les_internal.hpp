@@ -124,6 +124,7 @@ struct les_ctx {
     int sgs_frozen;            // skip the Lagrangian SGS update (Cs_opt2, F_* held)
     int fuse;                  // fused kernels (LES_FUSE=0 at creation: the separate ones)
     int overlap;               // theta's RHS and update on aux (LES_OVERLAP=0: in order)
+    int lag_done;              // this step's u_lag accumulation already done (k_max3_lag)
     double* tscr[5];           // theta's private scratch while it runs on aux:
                                // work3[0..2], work3[6], big3[3] stand-ins
     les::LinCtx* lin;          // linearised-step buffers (les_lin.hip), or null
@@ -288,7 +289,10 @@ void ab2_dev(les_ctx* c);
 template <typename W> void gradp_project_dev(les_ctx* c, int jt, bool theta_rows = true);
 void theta_rows_dev(les_ctx* c);
 template <typename W> void project_dev(les_ctx* c);
-void max3_dev(les_ctx* c, double* out);
+// check_cfl's maxima to out; lag: with this step's u_lag accumulation
+// (lag_accumulate_dev) in the same pass over u, v, w
+constexpr int MAX3_BLOCKS = 2048;  // partial maxima per component (c->red + 64)
+void max3_dev(les_ctx* c, double* out, bool lag = false);
 
 // ---- forward step and the linearised step (les_api.hip, les_lin.hip) ---------
 template <typename W>
