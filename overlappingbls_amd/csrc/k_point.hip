@@ -55,6 +55,38 @@ void fill_levels(hipStream_t s, T* f, size_t plane, int lev0, int nlev, double v
 template void fill_levels<double>(hipStream_t, double*, size_t, int, int, double);
 template void fill_levels<float>(hipStream_t, float*, size_t, int, int, double);
 
+// up to FILL_MAX runs of n values in one launch (run blockIdx.y)
+template <typename T>
+struct FillRuns {
+    T* p[FILL_MAX];
+};
+template <typename T>
+__global__ void k_fill_runs(FillRuns<T> r, long n, double v) {
+    T* f = r.p[blockIdx.y];
+    GRID_LOOP(i, n) f[i] = (T)v;
+}
+
+template <typename T>
+void fill_runs(hipStream_t s, long n, double v, std::initializer_list<T*> ps) {
+    FillRuns<T> r{};
+    int m = 0;
+    dim3 g = grid_for(n);
+    for (T* q : ps) {
+        r.p[m++] = q;
+        if (m == FILL_MAX) {  // launch each full set of FILL_MAX runs
+            g.y = (unsigned)m;
+            hipLaunchKernelGGL(k_fill_runs<T>, g, dim3(256), 0, s, r, n, v);
+            m = 0;
+        }
+    }
+    if (m > 0) {
+        g.y = (unsigned)m;
+        hipLaunchKernelGGL(k_fill_runs<T>, g, dim3(256), 0, s, r, n, v);
+    }
+}
+template void fill_runs<double>(hipStream_t, long, double, std::initializer_list<double*>);
+template void fill_runs<float>(hipStream_t, long, double, std::initializer_list<float*>);
+
 // element-wise conversion between storage precisions
 template <typename TO, typename TF>
 __global__ void k_convert(TO* __restrict__ out, const TF* __restrict__ in, long n) {
@@ -313,10 +345,7 @@ void wallstress_dev(les_ctx* c, int top, const double* u1, const double* v1) {
     W* dvdz = WF<W>(c, LES_F_DVDZ) + lev * np;
     const int sf = top ? (P.ubc_mom == LES_BC_STRESS_FREE) : (P.lbc_mom == LES_BC_STRESS_FREE);
     if (sf) {
-        fill_levels(c->stream, txz, c->plane, 0, 1, 0.0);
-        fill_levels(c->stream, tyz, c->plane, 0, 1, 0.0);
-        fill_levels(c->stream, dudz, c->plane, 0, 1, 0.0);
-        fill_levels(c->stream, dvdz, c->plane, 0, 1, 0.0);
+        fill_runs<W>(c->stream, np, 0.0, {txz, tyz, dudz, dvdz});
         return;
     }
     hipLaunchKernelGGL(k_wall<W>, grid_for(np), dim3(256), 0, c->stream, txz, tyz, dudz, dvdz, c->ustar,

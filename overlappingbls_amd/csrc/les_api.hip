@@ -373,11 +373,7 @@ void convec(les_ctx* c) {
         unpad_f<W>(c, big[4] + np2, cyf + np, nz - 1);
         unpad_f<W>(c, big[5] + np2, czf + np, nz - 1);
     }
-    fill_levels(c->stream, cxf, np, 0, 1, BOGUS);
-    fill_levels(c->stream, cyf, np, 0, 1, BOGUS);
-    fill_levels(c->stream, czf, np, 0, 1, BOGUS);
-    fill_levels(c->stream, cxf, np, nz, 1, BOGUS);
-    fill_levels(c->stream, cyf, np, nz, 1, BOGUS);
+    fill_runs<W>(c->stream, (long)np, BOGUS, {cxf, cyf, czf, cxf + (size_t)nz * np, cyf + (size_t)nz * np});
 }
 
 // theta_all_in_one (010_scalars.f90:111-671) in two parts.  The first needs

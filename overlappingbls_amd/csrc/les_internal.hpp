@@ -1,5 +1,6 @@
 // Internal declarations of the MI355X LES timestep library.
 #pragma once
+#include <initializer_list>
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
@@ -236,6 +237,10 @@ void tdma_press(les_ctx* c, const cx<T>* Hx, const cx<T>* Hy, const cx<T>* Hz, c
 // Lagrangian state, the surface planes and the reductions are FP64 always.
 template <typename T>
 void fill_levels(hipStream_t s, T* f, size_t plane, int lev0, int nlev, double v);
+// v into n values at each of up to FILL_MAX addresses, in one launch
+constexpr int FILL_MAX = 8;
+template <typename T>
+void fill_runs(hipStream_t s, long n, double v, std::initializer_list<T*> ps);
 template <typename TO, typename TF>
 void convert(hipStream_t s, TO* out, const TF* in, long n);
 void level_means(hipStream_t s, const double* f, size_t plane, int lev0, int nlev, double inv_n, double* out);
