@@ -51,3 +51,22 @@ def test_fused_equals_separate(case, mixed, var, gpu_lib):
     b = run(case, gpu_lib, False, mixed, nsteps, var)
     for n in FIELDS:
         assert np.array_equal(a[n], b[n], equal_nan=True), n
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_theta_boundary_rows_after_overlapped_steps(mixed, gpu_lib):
+    """project's boundary rows of theta (levels 0 and nz) are the last write
+    of a step, also when theta's update runs on the auxiliary stream: both
+    rows hold BOGUS after every step, as in the serial order."""
+    nl = config.workload("sbl128")
+    g = gpu_lib.create(nl.params())
+    if mixed:
+        from overlappingbls_amd.abi import LES_MIXED
+        g.set_precision(LES_MIXED)
+    config.load_state(g, config.initial_fields(nl, seed=78))
+    nz = nl.params().nzt
+    for jt in range(1, 5):
+        g.step(jt)
+        th = g.get("THETA")
+        assert th.shape[0] == nz + 1
+        assert np.all(th[0] == config.BOGUS) and np.all(th[nz] == config.BOGUS), jt
