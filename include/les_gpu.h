@@ -39,6 +39,7 @@ extern "C" {
 #define LES_E_CFL      -3   /* CFL > 0.6 (901_debug.f90:123-128)            */
 #define LES_E_NOMEM    -4   /* device allocation failed                    */
 #define LES_E_NODEV    -5   /* no GPU visible                              */
+#define LES_E_IO       -6   /* restart file missing, short or not writable */
 
 /* ---- precision --------------------------------------------------------- */
 #define LES_FP64   0
@@ -147,6 +148,25 @@ int les_download(les_ctx* ctx);
  * host block is nlev planes of nyt rows with row stride ld (doubles). */
 int les_set_field(les_ctx* ctx, int field, const double* host, int ld, int lev0, int nlev);
 int les_get_field(les_ctx* ctx, int field, double* host, int ld, int lev0, int nlev);
+/* ---- restart files ------------------------------------------------------------
+ * The reference's restart files, read and written on the host:
+ *   <path>vel_tt<nums>.out  u, v, w, RHSx, RHSy, RHSz, Cs_opt2, F_LM, F_MM, F_QN, F_NN
+ *   <path>temp_tt<nums>.out theta, RHS_T, sgs_t3 (wall level), psi_m (theta_flag only)
+ * Each 3-D field is levels 1..nzt-1 as nzt-1 planes of nyt rows of nxt
+ * native-endian doubles; each plane field one such plane (the MPI-IO layout of
+ * 202_parallel.f90:1084-1117, 1252-1285).  nums is the %08d step counter of
+ * the names; path is path_restart, ending in '/'.  Returns LES_E_IO when a
+ * file cannot be opened or has the wrong size.
+ * les_write_restart: checkpoint_final, 012_io.f90:496-559.
+ * les_read_restart: read_flow_field, 102_01_ic.f90:682-734, and with
+ * read_theta init_temperature_field, 102_initial.f90:740-797; then the
+ * boundary ghost rows of data_exchange (202_parallel.f90:722-733) and
+ * w(nzt) = 0 (110_project.f90:54).  Other fields and levels are left as
+ * they are; the caller sets the restart flags (les_params.restart_vel,
+ * restart_theta) and the SGS init flags as the reference's run would. */
+int les_write_restart(les_ctx* ctx, const char* path, int nums);
+int les_read_restart(les_ctx* ctx, const char* path, int nums, int read_theta);
+
 /* Scalar state carried across steps (lagrange_Sdep's saved init flags). */
 int les_set_sgs_init_flags(les_ctx* ctx, int f_lm_mm_init, int f_qn_nn_init);
 int les_get_sgs_init_flags(les_ctx* ctx, int* f_lm_mm_init, int* f_qn_nn_init);

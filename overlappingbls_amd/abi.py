@@ -8,6 +8,7 @@ reads like the reference's call sequence in flosol (103_flosol.f90:4-290).
 from __future__ import annotations
 
 import ctypes as C
+import os
 import numpy as np
 
 LES_OK = 0
@@ -16,6 +17,7 @@ LES_E_HIP = -2
 LES_E_CFL = -3
 LES_E_NOMEM = -4
 LES_E_NODEV = -5
+LES_E_IO = -6
 
 K_TDMA, K_XDERIV, K_YDERIV, K_PAD, K_COPY = 0, 1, 2, 3, 4
 
@@ -112,6 +114,8 @@ def bind(lib: C.CDLL, prefix: str):
         "get_field": [P, i, _PD, i, i, i],
         "set_sgs_init_flags": [P, i, i],
         "get_sgs_init_flags": [P, C.POINTER(i), C.POINTER(i)],
+        "write_restart": [P, C.c_char_p, i],
+        "read_restart": [P, C.c_char_p, i, i],
         "ddx": [P, i, i], "ddy": [P, i, i], "ddz_uv": [P, i, i], "ddz_w": [P, i, i],
         "filter_data": [P, i],
         "test_filter": [P, i, i, i],
@@ -240,6 +244,16 @@ class LesContext:
 
     def set_sgs_init_flags(self, a, b):
         self._call("set_sgs_init_flags", self._h, int(a), int(b))
+
+    def write_restart(self, path, nums):
+        """checkpoint_final (012_io.f90:496-559): <path>vel_tt<nums>.out and,
+        with theta_flag, <path>temp_tt<nums>.out; path ends in '/'."""
+        self._call("write_restart", self._h, os.fsencode(path), int(nums))
+
+    def read_restart(self, path, nums, theta=True):
+        """read_flow_field (102_01_ic.f90:682-734) and, with theta,
+        init_temperature_field (102_initial.f90:740-797)."""
+        self._call("read_restart", self._h, os.fsencode(path), int(nums), int(bool(theta)))
 
     def cfl(self):
         x = C.c_double()

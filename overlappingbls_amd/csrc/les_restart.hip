@@ -1,0 +1,146 @@
+// Restart files of the reference: checkpoint_final (012_io.f90:496-546,
+// :548-559) writes them; read_flow_field (102_01_ic.f90:682-734) and
+// init_temperature_field (102_initial.f90:740-797) read them.
+//
+// Format: MPI-IO with a 'native' view of the global array (nxt, nyt, nzt-1)
+// in Fortran order (mpi_type_create_subarray, 202_parallel.f90:1084-1117;
+// sizes and starts 102_initial.f90:154-162).  The file is therefore plain
+// native-endian doubles: x fastest, then y, then z over the global levels
+// 1..nzt-1, one whole field after another.  A plane field
+// (write_file_mpi_xy, 202_parallel.f90:1252-1285) is the (nxt, nyt) array.
+//   vel_tt<nums>.out:  u, v, w, RHSx, RHSy, RHSz, Cs_opt2, F_LM, F_MM, F_QN, F_NN
+//   temp_tt<nums>.out: theta, RHS_T, sgs_t3(:,:,1), psi_m
+// The host side moves fields through les_get_field / les_set_field, so both
+// precisions read and write the FP64 values the caller would see.
+#include <cstdio>
+#include <string>
+#include <vector>
+#include "les_internal.hpp"
+
+namespace {
+
+constexpr int VEL_FIELDS[] = {LES_F_U,       LES_F_V,    LES_F_W,    LES_F_RHSX, LES_F_RHSY, LES_F_RHSZ,
+                              LES_F_CS_OPT2, LES_F_F_LM, LES_F_F_MM, LES_F_F_QN, LES_F_F_NN};
+constexpr int N_VEL = sizeof(VEL_FIELDS) / sizeof(VEL_FIELDS[0]);
+
+// path_restart//'vel_tt'//i8.8//'.out' (012_io.f90:524)
+bool file_name(std::string& out, const char* path, const char* stem, int nums) {
+    if (!path || nums < 0 || nums > 99999999) return false;
+    char b[32];
+    std::snprintf(b, sizeof b, "%s%08d.out", stem, nums);
+    out = std::string(path) + b;
+    return true;
+}
+
+struct File {
+    std::FILE* f = nullptr;
+    bool close() {  // false when buffered data could not be written
+        const int rc = std::fclose(f);
+        f = nullptr;
+        return rc == 0;
+    }
+    ~File() {
+        if (f) std::fclose(f);
+    }
+};
+
+// levels 1..nzt-1 of a field <-> the next block of the file
+int put_field(les_ctx* c, std::FILE* f, int id, std::vector<double>& buf, int lev0, int nlev) {
+    const size_t n = (size_t)nlev * c->plane;
+    buf.resize(n);
+    const int st = les_get_field(c, id, buf.data(), c->nx, lev0, nlev);
+    if (st != LES_OK) return st;
+    return std::fwrite(buf.data(), sizeof(double), n, f) == n ? LES_OK : LES_E_IO;
+}
+int take_field(les_ctx* c, std::FILE* f, int id, std::vector<double>& buf, int lev0, int nlev) {
+    const size_t n = (size_t)nlev * c->plane;
+    buf.resize(n);
+    if (std::fread(buf.data(), sizeof(double), n, f) != n) return LES_E_IO;
+    return les_set_field(c, id, buf.data(), c->nx, lev0, nlev);
+}
+
+// the file holds exactly `bytes` bytes
+bool has_size(std::FILE* f, long long bytes) {
+    if (std::fseek(f, 0, SEEK_END) != 0) return false;
+    const long long n = std::ftell(f);
+    std::rewind(f);
+    return n == bytes;
+}
+
+// one plane of v at level lev of field id
+int fill_plane(les_ctx* c, int id, int lev, double v) {
+    std::vector<double> p((size_t)c->plane, v);
+    return les_set_field(c, id, p.data(), c->nx, lev, 1);
+}
+
+}  // namespace
+
+int les_write_restart(les_ctx* c, const char* path, int nums) {
+    if (!c) return LES_E_BADARG;
+    std::string fn;
+    if (!file_name(fn, path, "vel_tt", nums)) return LES_E_BADARG;
+    const int nl = c->nz - 1;
+    std::vector<double> buf;
+    {
+        File f;
+        f.f = std::fopen(fn.c_str(), "wb");
+        if (!f.f) return LES_E_IO;
+        for (int m = 0; m < N_VEL; ++m) {
+            const int st = put_field(c, f.f, VEL_FIELDS[m], buf, 1, nl);
+            if (st != LES_OK) return st;
+        }
+        if (!f.close()) return LES_E_IO;
+    }
+    if (!c->prm.theta_flag) return LES_OK;
+    if (!file_name(fn, path, "temp_tt", nums)) return LES_E_BADARG;
+    File f;
+    f.f = std::fopen(fn.c_str(), "wb");
+    if (!f.f) return LES_E_IO;
+    int st = put_field(c, f.f, LES_F_THETA, buf, 1, nl);
+    if (st == LES_OK) st = put_field(c, f.f, LES_F_RHST, buf, 1, nl);
+    if (st == LES_OK) st = put_field(c, f.f, LES_F_SGS_T3, buf, 1, 1);
+    if (st == LES_OK) st = put_field(c, f.f, LES_F_PSI_M_SFC, buf, 0, 1);
+    if (st == LES_OK && !f.close()) st = LES_E_IO;
+    return st;
+}
+
+int les_read_restart(les_ctx* c, const char* path, int nums, int read_theta) {
+    if (!c) return LES_E_BADARG;
+    if (read_theta && !c->prm.theta_flag) return LES_E_BADARG;  // 102_initial.f90:798-803
+    std::string fn;
+    if (!file_name(fn, path, "vel_tt", nums)) return LES_E_BADARG;
+    const int nl = c->nz - 1;
+    const long long F = (long long)c->plane * nl * (long long)sizeof(double);
+    const long long P = (long long)c->plane * (long long)sizeof(double);
+    std::vector<double> buf;
+    {
+        File f;
+        f.f = std::fopen(fn.c_str(), "rb");
+        if (!f.f || !has_size(f.f, N_VEL * F)) return LES_E_IO;
+        for (int m = 0; m < N_VEL; ++m) {
+            const int st = take_field(c, f.f, VEL_FIELDS[m], buf, 1, nl);
+            if (st != LES_OK) return st;
+        }
+    }
+    // the ghost rows data_exchange leaves at the physical boundaries
+    // (202_parallel.f90:722-733): levels 0 of u, v, w and nzt of u, v are
+    // BOGUS; w at nzt is the top boundary's 0 (110_project.f90:48-55)
+    int st = LES_OK;
+    for (int id : {LES_F_U, LES_F_V, LES_F_W})
+        if (st == LES_OK) st = fill_plane(c, id, 0, les::BOGUS);
+    for (int id : {LES_F_U, LES_F_V})
+        if (st == LES_OK) st = fill_plane(c, id, c->nz, les::BOGUS);
+    if (st == LES_OK) st = fill_plane(c, LES_F_W, c->nz, 0.0);
+    if (st != LES_OK || !read_theta) return st;
+    if (!file_name(fn, path, "temp_tt", nums)) return LES_E_BADARG;
+    File f;
+    f.f = std::fopen(fn.c_str(), "rb");
+    if (!f.f || !has_size(f.f, 2 * F + 2 * P)) return LES_E_IO;
+    st = take_field(c, f.f, LES_F_THETA, buf, 1, nl);
+    if (st == LES_OK) st = take_field(c, f.f, LES_F_RHST, buf, 1, nl);
+    if (st == LES_OK) st = take_field(c, f.f, LES_F_SGS_T3, buf, 1, 1);
+    if (st == LES_OK) st = take_field(c, f.f, LES_F_PSI_M_SFC, buf, 0, 1);
+    if (st == LES_OK) st = fill_plane(c, LES_F_THETA, 0, les::BOGUS);
+    if (st == LES_OK) st = fill_plane(c, LES_F_THETA, c->nz, les::BOGUS);
+    return st;
+}
