@@ -278,6 +278,14 @@ enum les_slice {
 int les_set_avgslice(les_ctx* ctx, int use_avgslice, int c_count, int p_count);
 /* window 0: running sums; 1: the last closed window. */
 int les_get_avgslice(les_ctx* ctx, int slice, int window, double* host);
+/* write_avgslice (011_scalars2.f90:393-442) for every slice of the last
+ * closed window (the theta slices when theta_flag): appends to
+ * <path>result/aver_<name>.out, names as flow_slice / theta_slice pass them
+ * (:203-244, 305-311).  average_dim_num 1: one line per level 1..nzt-1 of
+ * nums*dt and the nxt values; 2: one line of nums*dt and the x-mean of each
+ * level.  Values in Fortran E14.5 (fmt_5168, :21).  Call it where the
+ * reference does, on the steps that close a window (mod(jt, p_count) == 0). */
+int les_write_avgslice(les_ctx* ctx, const char* path, int nums, int average_dim_num);
 
 /* ---- tangent-linear model and adjoint -------------------------------------- */
 /* No reference counterpart: the reference has no derivative code.  These

@@ -10,6 +10,8 @@
  * here element by element from the oracle's level-major arrays. */
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
+#include <math.h>
 #include "orc_internal.h"
 
 static const int vel_ids[11] = {LES_F_U, LES_F_V, LES_F_W, LES_F_RHSX, LES_F_RHSY, LES_F_RHSZ,
@@ -98,4 +100,73 @@ int orc_read_restart(les_ctx* c, const char* path, int nums, int read_theta) {
         c->f3[LES_F_THETA][top + q] = ORC_BOGUS;
     }
     return LES_OK;
+}
+
+/* write_avgslice (011_scalars2.f90:393-442): one file per slice,
+ * path//'result/aver_'//name//'.out' opened for appending, records written
+ * with fmt_5168 = "(1400(E14.5))" (:21).  Names as flow_slice and
+ * theta_slice pass them (:203-244, 305-311), indexed by les_slice. */
+int orc_get_avgslice(les_ctx* c, int slice, int window, double* host); /* orc_step.c */
+
+static const char* const orc_slice_names[LES_N_SLICES] = {
+    "u", "v", "w", "p", "dudt", "dvdt", "dwdt", "dudz", "dvdz", "u2", "v2", "w2", "p2", "uv", "uw", "vw",
+    "wp", "txx", "txy", "txz", "tyy", "tyz", "tzz", "u3", "v3", "w3", "u2fluc", "v2fluc", "uvfluc",
+    "u2wfluc", "v2wfluc", "dissip", "utxz_fluc", "vtyz_fluc", "wtzz", "Cs", "theta", "theta2", "Tfluc",
+    "dTdz", "sgs_t3", "wt", "Nu_t"};
+
+/* E14.5 edit descriptor: the value as 0.d1d2d3d4d5 x 10^e, d1 != 0 unless
+ * the value is zero, in a 14-character field padded on the left; the
+ * exponent as E+ee, or +eee when |e| > 99 */
+static void orc_e14_5(FILE* f, double x) {
+    char field[32];
+    if (x != x) {
+        snprintf(field, sizeof field, "NaN");
+    } else if (x == 0.0) {
+        snprintf(field, sizeof field, "%s0.00000E+00", signbit(x) ? "-" : "");
+    } else {
+        char sci[32];
+        snprintf(sci, sizeof sci, "%.4E", fabs(x)); /* d.ddddE+xx, rounded to 5 digits */
+        const int e = atoi(strchr(sci, 'E') + 1) + 1;
+        char mant[6] = {sci[0], sci[2], sci[3], sci[4], sci[5], 0};
+        if (e >= -99 && e <= 99)
+            snprintf(field, sizeof field, "%s0.%sE%c%02d", x < 0 ? "-" : "", mant, e < 0 ? '-' : '+', e < 0 ? -e : e);
+        else
+            snprintf(field, sizeof field, "%s0.%s%c%03d", x < 0 ? "-" : "", mant, e < 0 ? '-' : '+', e < 0 ? -e : e);
+    }
+    fprintf(f, "%14s", field);
+}
+
+int orc_write_avgslice(les_ctx* c, const char* path, int nums, int average_dim_num) {
+    if (!path || !c->sl_run || (average_dim_num != 1 && average_dim_num != 2)) return LES_E_BADARG;
+    const int nx = c->nx, nl = c->nz - 1, ns = c->prm.theta_flag ? LES_N_SLICES : LES_N_FLOW_SLICES;
+    const double t = nums * c->prm.dt;
+    double* a = malloc(sizeof(double) * (size_t)nx * nl);
+    if (!a) return LES_E_NOMEM;
+    int st = LES_OK;
+    for (int s = 0; s < ns && st == LES_OK; ++s) {
+        char fn[4096];
+        st = orc_get_avgslice(c, s, 1, a);
+        if (st != LES_OK) break;
+        snprintf(fn, sizeof fn, "%sresult/aver_%s.out", path, orc_slice_names[s]);
+        FILE* f = fopen(fn, "a");
+        if (!f) { st = LES_E_IO; break; }
+        if (average_dim_num == 1) {
+            for (int k = 0; k < nl; ++k) {
+                orc_e14_5(f, t);
+                for (int i = 0; i < nx; ++i) orc_e14_5(f, a[(size_t)k * nx + i]);
+                fputc('\n', f);
+            }
+        } else {
+            orc_e14_5(f, t);
+            for (int k = 0; k < nl; ++k) {
+                double sum = 0.0; /* sum(arr2d(:, k), dim=1) in x order */
+                for (int i = 0; i < nx; ++i) sum += a[(size_t)k * nx + i];
+                orc_e14_5(f, sum / nx);
+            }
+            fputc('\n', f);
+        }
+        if (fclose(f) != 0) st = LES_E_IO;
+    }
+    free(a);
+    return st;
 }

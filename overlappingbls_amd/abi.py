@@ -116,6 +116,7 @@ def bind(lib: C.CDLL, prefix: str):
         "get_sgs_init_flags": [P, C.POINTER(i), C.POINTER(i)],
         "write_restart": [P, C.c_char_p, i],
         "read_restart": [P, C.c_char_p, i, i],
+        "write_avgslice": [P, C.c_char_p, i, i],
         "ddx": [P, i, i], "ddy": [P, i, i], "ddz_uv": [P, i, i], "ddz_w": [P, i, i],
         "filter_data": [P, i],
         "test_filter": [P, i, i, i],
@@ -249,6 +250,11 @@ class LesContext:
         """checkpoint_final (012_io.f90:496-559): <path>vel_tt<nums>.out and,
         with theta_flag, <path>temp_tt<nums>.out; path ends in '/'."""
         self._call("write_restart", self._h, os.fsencode(path), int(nums))
+
+    def write_avgslice(self, path, nums, average_dim_num=2):
+        """write_avgslice (011_scalars2.f90:393-442) of the last closed
+        window: appends to <path>result/aver_<name>.out."""
+        self._call("write_avgslice", self._h, os.fsencode(path), int(nums), int(average_dim_num))
 
     def read_restart(self, path, nums, theta=True):
         """read_flow_field (102_01_ic.f90:682-734) and, with theta,

@@ -1,4 +1,7 @@
-// Restart files of the reference: checkpoint_final (012_io.f90:496-546,
+// Output files of the reference, written and read on the host: the restart
+// files and the time-averaged slices.
+//
+// Restart files: checkpoint_final (012_io.f90:496-546,
 // :548-559) writes them; read_flow_field (102_01_ic.f90:682-734) and
 // init_temperature_field (102_initial.f90:740-797) read them.
 //
@@ -12,7 +15,10 @@
 //   temp_tt<nums>.out: theta, RHS_T, sgs_t3(:,:,1), psi_m
 // The host side moves fields through les_get_field / les_set_field, so both
 // precisions read and write the FP64 values the caller would see.
+#include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <string>
 #include <vector>
 #include "les_internal.hpp"
@@ -143,4 +149,99 @@ int les_read_restart(les_ctx* c, const char* path, int nums, int read_theta) {
     if (st == LES_OK) st = fill_plane(c, LES_F_THETA, 0, les::BOGUS);
     if (st == LES_OK) st = fill_plane(c, LES_F_THETA, c->nz, les::BOGUS);
     return st;
+}
+
+// ---- time-averaged slices: write_avgslice (011_scalars2.f90:393-442) -----------
+// Appends to <path>result/aver_<name>.out, names as flow_slice / theta_slice
+// pass them (011_scalars2.f90:203-244, 305-311).  average_dim_num 1: one line
+// per level 1..nzt-1 holding nums*dt and the slice's nxt values; 2: one line
+// holding nums*dt and the x-mean of each level (sum(arr2d, dim=1)/nxt, summed
+// in x order).  Every value is written with Fortran's E14.5 edit descriptor
+// (fmt_5168 = "(1400(E14.5))", :21): right-justified in 14 characters as
+// [-]0.ddddd followed by E+ee, or +eee without the E when the exponent has
+// three digits.  The 5 significant digits come from the C library's
+// correctly rounded conversion.
+namespace {
+
+const char* const SLICE_NAMES[LES_N_SLICES] = {
+    "u", "v", "w", "p", "dudt", "dvdt", "dwdt", "dudz", "dvdz",
+    "u2", "v2", "w2", "p2", "uv", "uw", "vw", "wp",
+    "txx", "txy", "txz", "tyy", "tyz", "tzz",
+    "u3", "v3", "w3",
+    "u2fluc", "v2fluc", "uvfluc", "u2wfluc", "v2wfluc", "dissip",
+    "utxz_fluc", "vtyz_fluc", "wtzz", "Cs",
+    "theta", "theta2", "Tfluc", "dTdz", "sgs_t3", "wt", "Nu_t"};
+
+// Fortran Ew.d output of x into out (w characters, no terminator)
+void fortran_e(char* out, double x, int w, int d) {
+    char t[64];
+    int n;
+    if (std::isnan(x)) {
+        n = std::snprintf(t, sizeof t, "NaN");
+    } else if (std::isinf(x)) {
+        n = std::snprintf(t, sizeof t, "%s", x < 0 ? "-Infinity" : "Infinity");
+    } else {
+        char m[64];
+        int e10 = 0;
+        std::string digits(d, '0');
+        if (x != 0.0) {
+            std::snprintf(m, sizeof m, "%.*e", d - 1, std::fabs(x));  // D.DDDDe+XX
+            digits = std::string(1, m[0]) + std::string(m + 2, m + 1 + d);
+            e10 = std::atoi(std::strchr(m, 'e') + 1) + 1;  // 0.DDDDD x 10^e10
+        }
+        const char* sg = std::signbit(x) ? "-" : "";
+        if (std::abs(e10) <= 99)
+            n = std::snprintf(t, sizeof t, "%s0.%sE%c%02d", sg, digits.c_str(), e10 < 0 ? '-' : '+', std::abs(e10));
+        else
+            n = std::snprintf(t, sizeof t, "%s0.%s%c%03d", sg, digits.c_str(), e10 < 0 ? '-' : '+', std::abs(e10));
+    }
+    if (n > w) {
+        for (int i = 0; i < w; ++i) out[i] = '*';
+        return;
+    }
+    for (int i = 0; i < w - n; ++i) out[i] = ' ';
+    for (int i = 0; i < n; ++i) out[w - n + i] = t[i];
+}
+
+void put_line(std::string& line, double v) {
+    char b[14];
+    fortran_e(b, v, 14, 5);
+    line.append(b, 14);
+}
+
+}  // namespace
+
+int les_write_avgslice(les_ctx* c, const char* path, int nums, int average_dim_num) {
+    if (!c || !path || !c->st_run || (average_dim_num != 1 && average_dim_num != 2)) return LES_E_BADARG;
+    const int nx = c->nx, nl = c->nz - 1;
+    const int ns = c->prm.theta_flag ? (int)LES_N_SLICES : (int)LES_N_FLOW_SLICES;
+    const double time = nums * c->prm.dt;
+    std::vector<double> a((size_t)nx * nl);
+    for (int s = 0; s < ns; ++s) {
+        const int st = les_get_avgslice(c, s, 1, a.data());
+        if (st != LES_OK) return st;
+        std::string text, line;
+        if (average_dim_num == 1) {
+            for (int k = 0; k < nl; ++k) {
+                line.clear();
+                put_line(line, time);
+                for (int i = 0; i < nx; ++i) put_line(line, a[(size_t)k * nx + i]);
+                text += line + "\n";
+            }
+        } else {
+            put_line(text, time);
+            for (int k = 0; k < nl; ++k) {
+                double sum = 0.0;
+                for (int i = 0; i < nx; ++i) sum += a[(size_t)k * nx + i];
+                put_line(text, sum / nx);
+            }
+            text += "\n";
+        }
+        const std::string fn = std::string(path) + "result/aver_" + SLICE_NAMES[s] + ".out";
+        File f;
+        f.f = std::fopen(fn.c_str(), "ab");
+        if (!f.f) return LES_E_IO;
+        if (std::fwrite(text.data(), 1, text.size(), f.f) != text.size() || !f.close()) return LES_E_IO;
+    }
+    return LES_OK;
 }

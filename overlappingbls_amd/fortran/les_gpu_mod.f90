@@ -58,7 +58,7 @@ public :: les_create, les_destroy, les_set_field, les_get_field, les_step, les_r
           les_ab2, les_tdma_solve, les_bind_field, les_upload, les_download, &
           les_lin_set, les_lin_get, les_lin_zero, les_set_sgs_frozen, les_tlm_window, &
           les_adjoint_window, les_step_tlm, les_step_adj, &
-          les_write_restart, les_read_restart, &
+          les_write_restart, les_read_restart, les_write_avgslice, &
           les_check
 
 interface
@@ -248,6 +248,13 @@ interface
         type(c_ptr), value :: ctx
         character(kind=c_char), intent(in) :: path(*)
         integer(c_int), value :: nums, read_theta
+    end function
+    ! path: path//c_null_char (write_avgslice appends to path//'result/aver_*.out')
+    integer(c_int) function les_write_avgslice(ctx, path, nums, average_dim_num) bind(C, name="les_write_avgslice")
+        import :: c_int, c_ptr, c_char
+        type(c_ptr), value :: ctx
+        character(kind=c_char), intent(in) :: path(*)
+        integer(c_int), value :: nums, average_dim_num
     end function
     integer(c_int) function les_bind_field(ctx, f, h, ld, lev0, nlev) bind(C, name="les_bind_field")
         import :: c_int, c_ptr, c_double

@@ -80,3 +80,23 @@ def test_resumed_run_matches_the_oracle_resumed_run(oracle_mod, gpu_lib, tmp_pat
     # Cs^2 is a clipped ratio of small plane sums, the least well-conditioned
     # field (free-running parity at 128^3 leaves it within 1e-8)
     assert rel(g.get("CS_OPT2")[1:nz], o.get("CS_OPT2")[1:nz]) < 1e-6
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_gpu_writes_the_avgslice_records(gpu_lib, tmp_path, dim):
+    """write_avgslice's files from the device window sums: every record is
+    E14.5 of the window slice the library returns (the formatter is pinned
+    in tests/test_oracle_restart.py)."""
+    from avgslice_text import NAMES, records
+    nl = config.workload("small")
+    p = nl.params()
+    g = gpu_lib.create(p)
+    config.load_state(g, config.initial_fields(nl, seed=14))
+    g.set_avgslice(True, 1, 2)
+    g.run(1, 2)
+    (tmp_path / "result").mkdir()
+    path = str(tmp_path) + "/"
+    g.write_avgslice(path, 2, dim)
+    for name, stem in NAMES.items():
+        lines = (tmp_path / "result" / f"aver_{stem}.out").read_text().splitlines()
+        assert lines == records(g.avgslice(name, 1), 2 * p.dt, dim), name

@@ -80,3 +80,48 @@ def test_restart_reader_rejects_missing_short_and_long_files(oracle_mod, tmp_pat
         with pytest.raises(LesError) as e:
             r.read_restart(path, 1)
         assert e.value.status == LES_E_IO
+
+
+# ---- write_avgslice (011_scalars2.f90:393-442) ------------------------------------
+
+def test_e14_5_fields():
+    """Fortran E editing (Ew.d, no Ee): [-]0. then d digits, then E+ee, or
+    +eee without the E when the exponent needs three digits; right-justified."""
+    from avgslice_text import e14_5
+    cases = {
+        12.3456: "   0.12346E+02",
+        -0.000123456: "  -0.12346E-03",
+        0.0: "   0.00000E+00",
+        1.0: "   0.10000E+01",
+        9.999996: "   0.10000E+02",
+        -2.5e-7: "  -0.25000E-06",
+        1e-120: "   0.10000-119",
+        6.02e123: "   0.60200+124",
+    }
+    for x, s in cases.items():
+        assert e14_5(x) == s, (x, e14_5(x), s)
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_oracle_writes_the_avgslice_records(oracle_mod, tmp_path, dim):
+    from avgslice_text import NAMES, THETA_SLICES, records
+    nl = config.workload("tiny")
+    p = nl.params()
+    o = oracle_mod.create(p)
+    config.load_state(o, config.initial_fields(nl, seed=6))
+    o.set_avgslice(True, 1, 2)
+    o.run(1, 2)                       # jt = 2 closes the first window
+    (tmp_path / "result").mkdir()
+    path = str(tmp_path) + "/"
+    o.write_avgslice(path, 2, dim)
+    o.run(3, 2)
+    o.write_avgslice(path, 4, dim)    # appended, as the reference's position="append"
+    files = sorted(f.name for f in (tmp_path / "result").iterdir())
+    assert files == sorted(f"aver_{v}.out" for v in NAMES.values())
+    for name, stem in NAMES.items():
+        lines = (tmp_path / "result" / f"aver_{stem}.out").read_text().splitlines()
+        per = (p.nzt - 1) if dim == 1 else 1
+        assert len(lines) == 2 * per, name
+        assert all(len(l) == 14 * (1 + (p.nxt if dim == 1 else p.nzt - 1)) for l in lines), name
+        assert lines[per:] == records(o.avgslice(name, 1), 4 * p.dt, dim), name
+    assert THETA_SLICES[0] in NAMES
