@@ -14,6 +14,8 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import os
+import re
 
 import numpy as np
 
@@ -63,6 +65,8 @@ class Namelist:
     wt_s: float = 0.0                   # K m/s
     dTdz_top: float = 0.0               # K/m
     npz: int = 2                        # only lz*npz of step_scalar depends on it
+    ocean_flag: bool = False            # the ocean branch: les_create rejects it
+    lbc: int = 1                        # 1: prescribed surface flux (the only one supported)
 
     def params(self) -> LesParams:
         p = LesParams()
@@ -77,9 +81,9 @@ class Namelist:
         p.ubc, p.damping_method = self.ubc, self.damping_method
         p.coriolis_forcing = int(self.coriolis_forcing)
         p.use_mean_p_force = int(self.use_mean_p_force)
-        p.ocean_flag = 0
+        p.ocean_flag = int(self.ocean_flag)
         p.theta_flag, p.restart_theta = int(self.theta_flag), int(self.restart_theta)
-        p.theta_init_time, p.passive_scalar, p.lbc = self.theta_init_time, int(self.passive_scalar), 1
+        p.theta_init_time, p.passive_scalar, p.lbc = self.theta_init_time, int(self.passive_scalar), self.lbc
         # init_nondimensional, 102_initial.f90:186-225
         z_i, us = self.z_i, self.u_scale
         p.lx_tot, p.ly_tot, p.lz_tot = self.lx_tot / z_i, self.ly_tot / z_i, self.lz_tot / z_i
@@ -96,6 +100,93 @@ class Namelist:
         lz = self.lz_tot / self.npz / z_i          # 102_initial.f90:142, 186
         p.lz_npz = lz * self.npz
         return p
+
+
+# ---- the reference's namelist file (001_param.f90 namelists, read in
+# 102_initial.f90:65-110) -------------------------------------------------------
+_NML_TOKEN = re.compile(r"!.*|&\w+|/|'[^']*'|\"[^\"]*\"|=|[^\s,=/!'\"]+")
+
+
+def _nml_value(tok: str):
+    """One namelist value: logical, quoted string, integer or real (d/D
+    exponents as Fortran writes double precision)."""
+    if tok[0] in "'\"":
+        return tok[1:-1]
+    low = tok.lower()
+    if low in (".true.", ".t.", "t"):
+        return True
+    if low in (".false.", ".f.", "f"):
+        return False
+    num = low.replace("d", "e")
+    try:
+        return int(num)
+    except ValueError:
+        return float(num)
+
+
+def parse_namelist(text: str) -> dict:
+    """{group: {variable: value or list of values}} of a Fortran namelist
+    text; names in lower case; '!' starts a comment; separators are commas
+    or white space."""
+    groups, cur, name, vals = {}, None, None, []
+
+    def flush():
+        if cur is not None and name is not None:
+            groups[cur][name] = vals[0] if len(vals) == 1 else list(vals)
+
+    toks = [t for t in _NML_TOKEN.findall(text) if not t.startswith("!")]
+    i = 0
+    while i < len(toks):
+        t = toks[i]
+        if t.startswith("&"):
+            cur, name, vals = t[1:].lower(), None, []
+            groups.setdefault(cur, {})
+        elif t == "/":
+            flush()
+            cur, name, vals = None, None, []
+        elif cur is not None and i + 1 < len(toks) and toks[i + 1] == "=":
+            flush()
+            name, vals = t.lower(), []
+            i += 1
+        elif cur is not None and name is not None:
+            vals.append(_nml_value(t))
+        i += 1
+    return groups
+
+
+# namelist variable -> Namelist field (the variables the step reads)
+_NML_FIELDS = {f.lower(): f for f in (
+    "nxt", "nyt", "nzt", "lx_tot", "ly_tot", "lz_tot", "z_i", "dt", "u_scale", "cs_count", "dyn_init",
+    "inilag", "restart_vel", "walltype", "lbc_mom", "ubc_mom", "ubc", "damping_method", "coriolis_forcing",
+    "freq_coriolis", "ug_dim", "vg_dim", "use_mean_p_force", "mean_p_force", "nu_molec", "molec", "Pr",
+    "theta_flag", "restart_theta", "theta_init_time", "passive_scalar", "T_scale", "wt_s", "dTdz_top", "npz",
+    "ocean_flag", "lbc")}
+
+
+def read_namelist(path_or_text: str) -> Namelist:
+    """Namelist from the reference's param.nml (a path, or its text): the
+    variables of every group that the step reads, the rest at their
+    defaults.  Options the step does not support (ocean_flag, lbc = 0,
+    damping methods 2 and 3) pass through, and les_create rejects them."""
+    text = path_or_text
+    if "&" not in path_or_text and os.path.exists(path_or_text):
+        with open(path_or_text) as f:
+            text = f.read()
+    nl = Namelist()
+    for group in parse_namelist(text).values():
+        for var, val in group.items():
+            field = _NML_FIELDS.get(var)
+            if field is None:
+                continue
+            cur = getattr(nl, field)
+            if isinstance(cur, bool):
+                val = bool(val)
+            elif isinstance(cur, int) and not isinstance(val, bool):
+                val = int(val)
+            elif isinstance(cur, float):
+                val = float(val)
+            setattr(nl, field, val)
+    return nl
 
 
 def workload(name: str) -> Namelist:

@@ -55,3 +55,17 @@ def test_bind_upload_download(gpu_lib):
     g.set("U", 3.0 * u)
     g.download()
     assert np.array_equal(u, 3.0 * config.uniform(3, 0, sh))
+
+
+def test_namelist_options_the_step_does_not_support_are_rejected(gpu_lib):
+    """param.nml options outside the step's scope reach les_create, which
+    refuses them (the ocean branch, prescribed surface temperature, damping
+    methods 2 and 3 with ubc = 1)."""
+    from overlappingbls_amd import config
+    from overlappingbls_amd.abi import LES_E_BADARG, LesError
+    base = "&domain_param nxt = 16, nyt = 16, nzt = 9 /\n"
+    for extra in ("&flow_param ocean_flag = .true. /", "&temperature_param lbc = 0 /",
+                  "&flow_param ubc = 1, damping_method = 2 /"):
+        with pytest.raises(LesError) as e:
+            gpu_lib.create(config.read_namelist(base + extra).params())
+        assert e.value.status == LES_E_BADARG, extra
