@@ -182,3 +182,25 @@ def test_mixed_then_fp64(oracle_mod, gpu_lib):
     g.step(2)
     bad = compare_state(o, g, ("U", "V", "W", "THETA"))
     assert not bad, bad
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_state_has_no_nyquist_modes_after_a_step(gpu_lib, mixed):
+    """filter_data of u, v, w ends the step (103_flosol.f90:242-244): their
+    kx = nx/2 and ky = ny/2 modes are zero, also with the mixed step's
+    alternating-sum form of the filter."""
+    from overlappingbls_amd.abi import LES_MIXED
+    nl = config.workload("small")
+    g = gpu_lib.create(nl.params())
+    if mixed:
+        g.set_precision(LES_MIXED)
+    config.load_state(g, config.initial_fields(nl, seed=21))
+    g.run(1, 2)
+    nz = nl.params().nzt
+    for name in ("U", "V", "W"):
+        a = g.get(name)[1:nz]
+        F = np.fft.fft2(a)
+        ny, nx = a.shape[1:]
+        scale = np.abs(F).max()
+        assert np.abs(F[:, ny // 2, :]).max() < 1e-13 * scale, name
+        assert np.abs(F[:, :, nx // 2]).max() < 1e-13 * scale, name
