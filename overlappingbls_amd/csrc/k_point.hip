@@ -654,9 +654,7 @@ void interpolag_dev(les_ctx* c) {
                        (const double*)c->fold[3], c->f3[ids[0]], c->f3[ids[1]], c->f3[ids[2]], c->f3[ids[3]], p,
                        (double)c->prm.cs_count);
     const long n = (long)(c->plane * c->nlev);                              // :330-332
-    hipLaunchKernelGGL(k_fill<double>, grid_for(n), dim3(256), 0, c->stream, c->f3[LES_F_U_LAG], n, 0.0);
-    hipLaunchKernelGGL(k_fill<double>, grid_for(n), dim3(256), 0, c->stream, c->f3[LES_F_V_LAG], n, 0.0);
-    hipLaunchKernelGGL(k_fill<double>, grid_for(n), dim3(256), 0, c->stream, c->f3[LES_F_W_LAG], n, 0.0);
+    fill_runs<double>(c->stream, n, 0.0, {c->f3[LES_F_U_LAG], c->f3[LES_F_V_LAG], c->f3[LES_F_W_LAG]});
 }
 
 // ---- lagrange_Sdep, 106_03_lagrange_Sdep.f90:70-331 -------------------------
