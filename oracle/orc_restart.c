@@ -121,6 +121,8 @@ static void orc_e14_5(FILE* f, double x) {
     char field[32];
     if (x != x) {
         snprintf(field, sizeof field, "NaN");
+    } else if (isinf(x)) {
+        snprintf(field, sizeof field, "%s", x < 0 ? "-Infinity" : "Infinity");
     } else if (x == 0.0) {
         snprintf(field, sizeof field, "%s0.00000E+00", signbit(x) ? "-" : "");
     } else {
