@@ -22,6 +22,8 @@ def e14_5(x):
     """Fortran E14.5: [-]0.ddddd then E+ee (|e| <= 99) or +eee, right-justified."""
     if math.isnan(x):
         s = "NaN"
+    elif math.isinf(x):
+        s = "-Infinity" if x < 0 else "Infinity"
     elif x == 0.0:
         s = ("-" if math.copysign(1.0, x) < 0 else "") + "0.00000E+00"
     else:
