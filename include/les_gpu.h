@@ -155,8 +155,9 @@ int les_get_field(les_ctx* ctx, int field, double* host, int ld, int lev0, int n
  * Each 3-D field is levels 1..nzt-1 as nzt-1 planes of nyt rows of nxt
  * native-endian doubles; each plane field one such plane (the MPI-IO layout of
  * 202_parallel.f90:1084-1117, 1252-1285).  nums is the %08d step counter of
- * the names; path is path_restart, ending in '/'.  Returns LES_E_IO when a
- * file cannot be opened or has the wrong size.
+ * the names; path is path_restart, ending in '/' (the directory must
+ * exist).  Returns LES_E_IO when a file cannot be opened or has the wrong
+ * size.
  * les_write_restart: checkpoint_final, 012_io.f90:496-559.
  * les_read_restart: read_flow_field, 102_01_ic.f90:682-734, and with
  * read_theta init_temperature_field, 102_initial.f90:740-797; then the
@@ -284,7 +285,9 @@ int les_get_avgslice(les_ctx* ctx, int slice, int window, double* host);
  * (:203-244, 305-311).  average_dim_num 1: one line per level 1..nzt-1 of
  * nums*dt and the nxt values; 2: one line of nums*dt and the x-mean of each
  * level.  Values in Fortran E14.5 (fmt_5168, :21).  Call it where the
- * reference does, on the steps that close a window (mod(jt, p_count) == 0). */
+ * reference does, on the steps that close a window (mod(jt, p_count) == 0).
+ * The directory <path>result/ must exist (LES_E_IO otherwise); LES_E_BADARG
+ * before les_set_avgslice. */
 int les_write_avgslice(les_ctx* ctx, const char* path, int nums, int average_dim_num);
 
 /* ---- tangent-linear model and adjoint -------------------------------------- */
