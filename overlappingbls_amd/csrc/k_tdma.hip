@@ -28,6 +28,8 @@ namespace les {
 
 constexpr int TDF = 24;  // forward-sweep levels per load batch
 constexpr int TDB = 32;  // backward-sweep levels per load batch
+constexpr int TDL = 48;  // top levels whose d' stays in LDS (48 KB per 128-lane block:
+                         // three blocks fit a CU, so 2 ns lanes still run in one round)
 
 // Coefficients of one system (109_press_stag_array.f90:4-103 as
 // init_spectral forms them) and its right-hand-side factors.
@@ -85,6 +87,9 @@ __global__ __launch_bounds__(128) void k_tdma(const TH* __restrict__ Hx, const T
         return;
     }
     const int kt = nz - 1;             // top row of the system
+    // d' of levels kl..kt stays in LDS (the backward sweep reads it first)
+    __shared__ double dl[TDL][128];
+    const int kl = max(kt - TDL + 1, 1);
     const double k2 = k2a[q], kx = kx2d[q], ky = ky2d[q];
     const bool wave = k2 > 0.0;
     const double idz = 1.0 / dz;
@@ -117,7 +122,12 @@ __global__ __launch_bounds__(128) void k_tdma(const TH* __restrict__ Hx, const T
             for (int d = 0; d < TDF; ++d) hz[d] = hzm[k0 + d + 1];
         }
 #pragma unroll
-        for (int d = 0; d < TDF; ++d) dsc[(k0 + d) * s2 + t] = tdma_level(C, k0 + d, (double)hx[d], (double)hy[d], hz[d], hzk, dprev, cprev);
+        for (int d = 0; d < TDF; ++d) {
+            const int k = k0 + d;
+            const double v = tdma_level(C, k, (double)hx[d], (double)hy[d], hz[d], hzk, dprev, cprev);
+            if (k >= kl) dl[k - kl][threadIdx.x] = v;
+            else dsc[k * s2 + t] = v;
+        }
     }
     if (k0 <= kt) {
         TH hx[TDF], hy[TDF];
@@ -134,8 +144,14 @@ __global__ __launch_bounds__(128) void k_tdma(const TH* __restrict__ Hx, const T
             for (int d = 0; d < TDF; ++d) hz[d] = hzm[min(k0 + d, kt) + 1];
         }
 #pragma unroll
-        for (int d = 0; d < TDF; ++d)
-            if (k0 + d <= kt) dsc[(k0 + d) * s2 + t] = tdma_level(C, k0 + d, (double)hx[d], (double)hy[d], hz[d], hzk, dprev, cprev);
+        for (int d = 0; d < TDF; ++d) {
+            const int k = k0 + d;
+            if (k <= kt) {
+                const double v = tdma_level(C, k, (double)hx[d], (double)hy[d], hz[d], hzk, dprev, cprev);
+                if (k >= kl) dl[k - kl][threadIdx.x] = v;
+                else dsc[k * s2 + t] = v;
+            }
+        }
     }
     // ---- backward sweep ----
     double pn = dprev;  // level nz-1 (c'(nz-1) = 0)
@@ -145,8 +161,10 @@ __global__ __launch_bounds__(128) void k_tdma(const TH* __restrict__ Hx, const T
         double dv[TDB], c[TDB];
 #pragma unroll
         for (int d = 0; d < TDB; ++d) {
-            dv[d] = dsc[(k0 - d) * s2 + t];
-            c[d] = cc[(k0 - d) * ns + q];
+            const int k = k0 - d;
+            if (k >= kl) dv[d] = dl[k - kl][threadIdx.x];
+            else dv[d] = dsc[k * s2 + t];
+            c[d] = cc[k * ns + q];
         }
 #pragma unroll
         for (int d = 0; d < TDB; ++d) {
@@ -159,7 +177,8 @@ __global__ __launch_bounds__(128) void k_tdma(const TH* __restrict__ Hx, const T
 #pragma unroll
         for (int d = 0; d < TDB; ++d) {
             const int k = max(k0 - d, 1);
-            dv[d] = dsc[k * s2 + t];
+            if (k >= kl) dv[d] = dl[k - kl][threadIdx.x];
+            else dv[d] = dsc[k * s2 + t];
             c[d] = cc[k * ns + q];
         }
 #pragma unroll
