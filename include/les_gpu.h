@@ -154,7 +154,7 @@ int les_get_field(les_ctx* ctx, int field, double* host, int ld, int lev0, int n
  *   <path>temp_tt<nums>.out theta, RHS_T, sgs_t3 (wall level), psi_m (theta_flag only)
  * Each 3-D field is levels 1..nzt-1 as nzt-1 planes of nyt rows of nxt
  * native-endian doubles; each plane field one such plane (the MPI-IO layout of
- * 202_parallel.f90:1084-1117, 1252-1285).  nums is the %08d step counter of
+ * 202_parallel.f90:1084-1117, 1252-1284).  nums is the %08d step counter of
  * the names; path is path_restart, ending in '/' (the directory must
  * exist).  Returns LES_E_IO when a file cannot be opened or has the wrong
  * size.

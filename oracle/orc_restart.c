@@ -3,7 +3,7 @@
  * read_flow_field (102_01_ic.f90:682-734) and init_temperature_field
  * (102_initial.f90:740-797).  Each field goes through
  * write_file_mpi_3d / write_file_mpi_xy (202_parallel.f90:1084-1117,
- * 1252-1285): an MPI-IO subarray of the global (nxt, nyt, nzt-1) array in
+ * 1252-1284): an MPI-IO subarray of the global (nxt, nyt, nzt-1) array in
  * Fortran order with a 'native' view, so the file is the global array's
  * doubles, x fastest, levels 1..nzt-1, fields one after another at
  * displacements disp3d = nxt*nyt*(nzt-1)*8 (012_io.f90:520-521).  Written

@@ -10,7 +10,7 @@
 // sizes and starts 102_initial.f90:154-162).  The file is therefore plain
 // native-endian doubles: x fastest, then y, then z over the global levels
 // 1..nzt-1, one whole field after another.  A plane field
-// (write_file_mpi_xy, 202_parallel.f90:1252-1285) is the (nxt, nyt) array.
+// (write_file_mpi_xy, 202_parallel.f90:1252-1284) is the (nxt, nyt) array.
 //   vel_tt<nums>.out:  u, v, w, RHSx, RHSy, RHSz, Cs_opt2, F_LM, F_MM, F_QN, F_NN
 //   temp_tt<nums>.out: theta, RHS_T, sgs_t3(:,:,1), psi_m
 // The host side moves fields through les_get_field / les_set_field, so both
