@@ -50,6 +50,10 @@ extern "C" {
 #define LES_WALL_SMOOTH  1   /* walltype = 'smooth' */
 #define LES_BC_WALL        0 /* lbc_mom / ubc_mom = 'wall'        */
 #define LES_BC_STRESS_FREE 1 /* lbc_mom / ubc_mom = 'stress free' */
+/* ocean_param stokes_action (014_stokes_drift.f90:45-56) */
+#define LES_STOKES_DP       0 /* 'donelan.pierson1987' (not supported)        */
+#define LES_STOKES_MONO     1 /* 'mono': amp_w, lambda_w, agl_Ust             */
+#define LES_STOKES_DYNAMIC  2 /* 'dynamic': the same with gwave.in's record 1 */
 
 /*
  * Run parameters, already nondimensionalised as init_nondimensional leaves
@@ -60,7 +64,8 @@ extern "C" {
  */
 typedef struct les_params {
     int32_t nxt, nyt, nzt;     /* domain_param; nzt counts the top ghost    */
-    int32_t model;             /* flow_param model; only 5 (Lagrangian Sdep)*/
+    int32_t model;             /* flow_param model (1 Smagorinsky, 5 Lagrangian
+                                  scale-dependent)                          */
     int32_t cs_count;          /* time_param cs_count                       */
     int32_t dyn_init;          /* flow_param DYN_init                       */
     int32_t inilag;            /* flow_param inilag                         */
@@ -70,16 +75,16 @@ typedef struct les_params {
     int32_t walltype;          /* LES_WALL_*                                */
     int32_t lbc_mom, ubc_mom;  /* LES_BC_*                                  */
     int32_t ubc;               /* flow_param ubc (1: sponge on)             */
-    int32_t damping_method;    /* flow_param damping_method (only 1)        */
+    int32_t damping_method;    /* flow_param damping_method (1, 2, 3)       */
     int32_t coriolis_forcing;  /* flow_param coriolis_forcing               */
     int32_t use_mean_p_force;  /* flow_param use_mean_p_force               */
-    int32_t ocean_flag;        /* flow_param ocean_flag (only 0)            */
+    int32_t ocean_flag;        /* flow_param ocean_flag                     */
     int32_t theta_flag;        /* temperature_param theta_flag              */
     int32_t restart_theta;     /* temperature_param restart_theta           */
     int32_t theta_init_time;   /* temperature_param theta_init_time         */
     int32_t passive_scalar;    /* 001_param.f90:512 compile-time switch     */
-    int32_t lbc;               /* temperature_param lbc (only 1: flux)      */
-    int32_t pad0;
+    int32_t lbc;               /* temperature_param lbc (1 flux, 0 temperature) */
+    int32_t stokes_flag;       /* ocean_param stokes_flag                   */
     double lx_tot, ly_tot, lz_tot;   /* / z_i                               */
     double dt;                       /* nondimensional                      */
     double z_i, u_scale, T_scale;    /* scales                              */
@@ -91,6 +96,19 @@ typedef struct les_params {
     double coriol, ug, vg;           /* nondimensional Coriolis forcing     */
     double mean_p_force;             /* nondimensional                      */
     double lz_npz;                   /* lz*npz of step_scalar (= lz_tot)     */
+    /* ocean branch, Stokes drift, damping and lbc = 0 (001_param.f90:304-333,
+     * 102_initial.f90:226-250, 008_bottombc.f90:116-127) */
+    int32_t stokes_action;           /* LES_STOKES_*                        */
+    int32_t flag_crossvel;           /* ocean_param flag_crossVel           */
+    double u_star;                   /* wind-stress friction velocity / u_scale */
+    double rad_stress;               /* wind-stress angle (radians)         */
+    double alpha_w;                  /* thermal expansion alpha_w * T_scale */
+    double u_cross, v_cross;         /* cross-flow velocity / u_scale       */
+    double amp_w, lambda_w;          /* wave amplitude and length, metres   */
+    double agl_ust;                  /* Stokes drift direction, degrees     */
+    double inv_strength;             /* K/m, damping method 3               */
+    double T_s;                      /* surface temperature / T_scale       */
+    double Co;                       /* flow_param Co (model 1)             */
 } les_params;
 
 /* ---- field identifiers ----------------------------------------------------
@@ -116,6 +134,7 @@ enum les_field {
     /* one-plane surface fields (bottombc module) */
     LES_F_USTAR_SFC = 100, LES_F_PSI_M_SFC, LES_F_PHI_M_SFC,
     LES_F_PSI_H_SFC, LES_F_PHI_H_SFC,
+    LES_F_USTAR_TOP_SFC,   /* ocean branch: ustar_avg of the top wall (obukhov) */
     LES_F_END
 };
 
@@ -131,8 +150,17 @@ int les_set_precision(les_ctx* ctx, int precision);
 /* Stokes-drift profiles ust(0:nzt), vst(0:nzt) (014_stokes_drift.f90:29-115). */
 int les_set_stokes(les_ctx* ctx, const double* ust, const double* vst);
 /* Sponge profile sponge(0:nzt) (007_topbc.f90:15-85); computed by les_create
- * for damping_method 1, overridable here. */
+ * for damping_method 1, 2 and 3, overridable here. */
 int les_set_sponge(les_ctx* ctx, const double* sponge);
+/* The Stokes drift and sponge profiles the context holds (nzt+1 values
+ * each, levels 0..nzt; any pointer may be null). */
+int les_get_profiles(les_ctx* ctx, double* ust, double* vst, double* sponge);
+/* Ocean wind stress used by wallstress_bottom from now on
+ * (104_wallstress.f90:118-136): ustar = friction velocity / u_scale, agl in
+ * degrees.  les_create takes u_star and rad_stress from les_params; with
+ * flag_dynStress the caller reads ustar.in's record nums+ttshift_dynStress+1
+ * (104_wallstress.f90:120-126) and passes it here before each step. */
+int les_set_wind_stress(les_ctx* ctx, double ustar, double agl_deg);
 
 /* ---- registration (the reference's module arrays stay the caller's) ------- */
 /* Register a caller array for field `field` (levels lev0..lev0+nlev-1, row

@@ -57,6 +57,8 @@ struct les_ctx {
     double *sponge;          /* (0:nz)                                        */
     double* f3[LES_N_FIELDS3D];
     double *ustar_avg, *psi_m, *phi_m, *psi_h, *phi_h;  /* surface planes */
+    double* ustar_top;       /* ocean obukhov: ustar_avg of the top wall     */
+    double wind_ustar, wind_rad;  /* ocean wind stress (u_star, rad_stress) */
     double *u_big, *v_big, *w_big;     /* padded velocity kept for theta (convec) */
     int f_lm_mm_init, f_qn_nn_init;
     int sgs_frozen;

@@ -28,6 +28,7 @@
 #define CLIP24 ((double)1.e-24f)   /* real(1.e-24, kind=rprec) */
 
 static double* F(les_ctx* c, int id) { return c->f3[id]; }
+int orc_destroy(les_ctx* c);
 
 /* --------------------------------------------------------------------------
  * lifecycle
@@ -120,22 +121,90 @@ static void init_coef_press(les_ctx* c) {
 }
 
 static void init_sponge(les_ctx* c) {
-    /* setsponge damping_method 1, 007_topbc.f90:26-36, global k */
+    /* setsponge, 007_topbc.f90:15-85, global level k; zuvp(k) = (k - 1/2) dz
+     * (init_meshgrid, 102_initial.f90:275-278) */
+    const les_params* P = &c->prm;
     const int nzt = c->nz;
     memset(c->sponge, 0, sizeof(double) * c->nlev);
-    if (c->prm.ubc != 1) return;
-    double factor = 9.0 / (double)(nzt - 3 * nzt / 4 + 1);
-    double sponge_top = c->prm.z_i / (50.0 * c->prm.u_scale);
-    for (int k = 1; k <= nzt - 1; ++k)
-        if (k > 3 * nzt / 4 + 1) c->sponge[k] = sponge_top * pow(5.0, (double)(k - nzt) * factor);
+    if (P->ubc != 1) return;
+    if (P->damping_method == 1) {                                /* :26-36 */
+        double factor = 9.0 / (double)(nzt - 3 * nzt / 4 + 1);
+        double sponge_top = P->z_i / (50.0 * P->u_scale);
+        for (int k = 1; k <= nzt - 1; ++k)
+            if (k > 3 * nzt / 4 + 1) c->sponge[k] = sponge_top * pow(5.0, (double)(k - nzt) * factor);
+    } else if (P->damping_method == 2) {                         /* :37-50 */
+        const double z_d = 0.75 * P->lz_tot, cfrdmp = 3.9;
+        for (int k = 1; k <= nzt - 1; ++k) {
+            const double z = ((double)k - 0.5) * c->dz;
+            if (z >= z_d && z <= P->lz_tot)
+                c->sponge[k] = cfrdmp * 0.5 * (1.0 - cos(ORC_PI * (z - z_d) / (P->lz_tot - z_d)));
+        }
+    } else {                                                     /* :55-82 */
+        const double g_hat = ORC_G * (P->z_i / (P->u_scale * P->u_scale));
+        double fac, bv;
+        if (P->ocean_flag) {
+            fac = (double)0.22f;                                 /* default-real literal */
+            bv = sqrt(g_hat * P->alpha_w * P->inv_strength / P->T_scale * P->z_i);
+        } else {
+            fac = 1.5;
+            bv = sqrt(g_hat * P->inv_strength / P->T_scale * P->z_i);
+        }
+        const double sigma_z = 0.8 * P->lz_tot;
+        const double sigma_max = fac * bv / (double)(8.0f * atanf(1.0f));   /* 8.0*atan(1.0), default real */
+        for (int k = 1; k <= nzt; ++k) {
+            const double z = ((double)k - 0.5) * c->dz;
+            if (z >= sigma_z && z <= P->lz_tot) {
+                const double ell = (z - sigma_z) / (P->lz_tot - sigma_z);
+                c->sponge[k] = ell * ell * sigma_max;
+            }
+        }
+    }
+}
+
+/* init_stokes_drift, 014_stokes_drift.f90:29-115 (monochromatic, its
+ * shallow-water form under a wall top, and 'dynamic' with the values of
+ * gwave.in's first record); zuvp(k) = (k - 1/2) dz */
+static int init_stokes(les_ctx* c) {
+    const les_params* P = &c->prm;
+    memset(c->ust, 0, sizeof(double) * c->nlev);
+    memset(c->vst, 0, sizeof(double) * c->nlev);
+    if (!P->stokes_flag) return LES_OK;
+    if (P->stokes_action != LES_STOKES_MONO && P->stokes_action != LES_STOKES_DYNAMIC) return LES_E_BADARG;
+    const double rad = P->agl_ust / 180.0 * ORC_PI;              /* deg2rad, :300-307 */
+    const double amp = P->amp_w / P->z_i, lam = P->lambda_w / P->z_i;
+    const double wn = 2.0 * ORC_PI / lam;
+    const double gnd = ORC_G * P->z_i / (P->u_scale * P->u_scale);
+    if (P->ubc_mom == LES_BC_STRESS_FREE) {                      /* monochromatic, :146-160 */
+        const double omega = sqrt(gnd * wn);
+        const double us = omega * wn * amp * amp;
+        for (int k = 0; k <= c->nz; ++k) {                       /* :59-63 */
+            const double z = ((double)k - 0.5) * c->dz;
+            c->ust[k] = us * exp(-2.0 * wn * z) * cos(rad);
+            c->vst[k] = us * exp(-2.0 * wn * z) * sin(rad);
+        }
+    } else {                                                     /* mono_shallow_water, :164-178 */
+        const double H = P->lz_tot;
+        const double omega = sqrt(gnd * wn * tanh(wn * H));
+        const double us = omega * wn * amp * amp / tanh(wn * H);
+        for (int k = 0; k <= c->nz; ++k) {                       /* :76-81 */
+            const double z = ((double)k - 0.5) * c->dz;
+            const double sh = sinh(wn * H);
+            c->ust[k] = us * cosh(2.0 * wn * (-z + H)) / 2.0 / (sh * sh) * cos(rad);
+            c->vst[k] = us * cosh(2.0 * wn * (-z + H)) / 2.0 / (sh * sh) * sin(rad);
+        }
+    }
+    return LES_OK;
 }
 
 int orc_create(const les_params* prm, les_ctx** out) {
     *out = NULL;
     if (!good_size(prm->nxt) || !good_size(prm->nyt) || prm->nzt < 4) return LES_E_BADARG;
-    if (prm->model != 5 || prm->ocean_flag || !prm->sgs || prm->lbc != 1) return LES_E_BADARG;
-    if (prm->ubc == 1 && prm->damping_method != 1) return LES_E_BADARG;
-    if (prm->cs_count < 1) return LES_E_BADARG;
+    if (prm->model != 5 || !prm->sgs || prm->cs_count < 1) return LES_E_BADARG;
+    if (prm->lbc != 0 && prm->lbc != 1) return LES_E_BADARG;
+    /* lbc = 0 reads ustar_avg and psi_h, which the ocean branch never sets on
+     * the surface ranks (010_scalars.f90:380-390, :696-716) */
+    if (prm->lbc == 0 && prm->ocean_flag) return LES_E_BADARG;
+    if (prm->damping_method < 1 || prm->damping_method > 3) return LES_E_BADARG;
     les_ctx* c = (les_ctx*)calloc(1, sizeof(les_ctx));
     c->prm = *prm;
     c->nx = prm->nxt; c->ny = prm->nyt; c->nz = prm->nzt;
@@ -181,6 +250,13 @@ int orc_create(const les_params* prm, les_ctx** out) {
     init_test_filter(c, 4.0, c->Gtest2);     /* 102_initial.f90:42 */
     init_coef_press(c);
     init_sponge(c);
+    c->ustar_top = (double*)calloc(c->plane, sizeof(double));
+    c->wind_ustar = prm->u_star;                  /* 102_initial.f90:201, 246 */
+    c->wind_rad = prm->rad_stress;
+    if (init_stokes(c) != LES_OK) {
+        orc_destroy(c);
+        return LES_E_BADARG;
+    }
     *out = c;
     return LES_OK;
 }
@@ -194,7 +270,7 @@ int orc_destroy(les_ctx* c) {
     free(c->coef_a); free(c->coef_b); free(c->coef_c);
     free(c->ust); free(c->vst); free(c->sponge);
     for (int f = 0; f < LES_N_FIELDS3D; ++f) free(c->f3[f]);
-    free(c->ustar_avg); free(c->psi_m); free(c->phi_m); free(c->psi_h); free(c->phi_h);
+    free(c->ustar_avg); free(c->psi_m); free(c->phi_m); free(c->psi_h); free(c->phi_h); free(c->ustar_top);
     free(c->u_big); free(c->v_big); free(c->w_big);
     free(c->dudt); free(c->dvdt); free(c->dwdt); free(c->ke); free(c->ke_temp); free(c->dke);
     free(c->sl_run); free(c->sl_win);
@@ -213,6 +289,19 @@ int orc_set_sponge(les_ctx* c, const double* sp) {
     return LES_OK;
 }
 
+int orc_get_profiles(les_ctx* c, double* ust, double* vst, double* sponge) {
+    if (ust) memcpy(ust, c->ust, sizeof(double) * c->nlev);
+    if (vst) memcpy(vst, c->vst, sizeof(double) * c->nlev);
+    if (sponge) memcpy(sponge, c->sponge, sizeof(double) * c->nlev);
+    return LES_OK;
+}
+
+int orc_set_wind_stress(les_ctx* c, double ustar, double agl_deg) {
+    c->wind_ustar = ustar;
+    c->wind_rad = agl_deg / 180.0 * ORC_PI;       /* deg2rad, 014_stokes_drift.f90:300-307 */
+    return LES_OK;
+}
+
 static double* surf_field(les_ctx* c, int id) {
     switch (id) {
         case LES_F_USTAR_SFC: return c->ustar_avg;
@@ -220,6 +309,7 @@ static double* surf_field(les_ctx* c, int id) {
         case LES_F_PHI_M_SFC: return c->phi_m;
         case LES_F_PSI_H_SFC: return c->psi_h;
         case LES_F_PHI_H_SFC: return c->phi_h;
+        case LES_F_USTAR_TOP_SFC: return c->ustar_top;
         default: return NULL;
     }
 }
@@ -395,6 +485,23 @@ int orc_obukhov(les_ctx* c, int jt) {
     const double* v1 = LEV(c, F(c, LES_F_V), 1);
     const double nn = (double)(c->nx * c->ny);
     const double dz = c->dz;
+    if (P->ocean_flag) {                                       /* :696-716 */
+        /* the top wall's ranks set their own ustar_avg (to ustar_top here)
+         * and psi/phi; the surface ranks' planes are left as they are */
+        if (P->ubc_mom != LES_BC_WALL) return LES_OK;
+        const double* ut = LEV(c, F(c, LES_F_U), c->nz - 1);
+        const double* vt = LEV(c, F(c, LES_F_V), c->nz - 1);
+        double su = 0.0;
+        for (size_t q = 0; q < np; ++q) su += sqrt(ut[q] * ut[q] + vt[q] * vt[q]);
+        const double nf = (double)(float)(c->nx * c->ny);      /* float(nxt*nynpy) */
+        double us;
+        if (P->walltype == LES_WALL_SMOOTH)
+            us = su / nf / (log(0.5 * dz / (P->nu_molec / P->u_scale / P->z_i)) / ORC_VONK + ORC_B_SMOOTH);
+        else
+            us = su / nf * ORC_VONK / log(0.5 * dz / P->zo);
+        for (size_t q = 0; q < np; ++q) c->ustar_top[q] = us;
+        return LES_OK;
+    }
     if (!P->theta_flag || jt < P->theta_init_time) {          /* :718-738 */
         if (!P->restart_theta) for (size_t q = 0; q < np; ++q) c->psi_m[q] = 0.0;
         double su = 0.0, sp = 0.0;
@@ -456,6 +563,19 @@ int orc_wallstress_bottom(les_ctx* c) {
     double* tyz = LEV(c, F(c, LES_F_TYZ), 1);
     double* dudz = LEV(c, F(c, LES_F_DUDZ), 1);
     double* dvdz = LEV(c, F(c, LES_F_DVDZ), 1);
+    if (P->ocean_flag) {                                        /* :118-136 */
+        const double us = c->wind_ustar, rad = c->wind_rad;
+        const double* dudz2 = LEV(c, F(c, LES_F_DUDZ), 2);
+        const double* dvdz2 = LEV(c, F(c, LES_F_DVDZ), 2);
+        for (size_t q = 0; q < np; ++q) {
+            txz[q] = us * us * cos(rad);
+            tyz[q] = us * us * sin(rad);
+            const double a = dudz2[q] * cos(rad) + dvdz2[q] * sin(rad);
+            dudz[q] = a * cos(rad);
+            dvdz[q] = a * sin(rad);
+        }
+        return LES_OK;
+    }
     if (P->lbc_mom == LES_BC_STRESS_FREE) {                     /* :178-182 */
         for (size_t q = 0; q < np; ++q) txz[q] = tyz[q] = dudz[q] = dvdz[q] = 0.0;
         return LES_OK;
@@ -1012,7 +1132,19 @@ int orc_theta_all_in_one(les_ctx* c, int jt) {
     for (size_t q = 0; q < np; ++q) LEV(c, dTdz, nz)[q] = P->dTdz_top / P->T_scale * P->z_i;   /* :140-142 */
     memcpy(RHSf, RHS, sizeof(double) * np * nl);                 /* :163 */
     /* ---- scalar_RHS_calc ---- */
-    const double surf_flux = P->wt_s / P->T_scale / P->u_scale;  /* :372-379 */
+    /* surface flux (:359-394): lbc = 1 the prescribed wt_s; lbc = 0 the bulk
+     * Monin-Obukhov flux from the surface temperature T_s, the scalar
+     * roughness z_os = zo/10 (008_bottombc.f90:126) and the filtered theta
+     * of level 1, with ustar_avg as wallstress_bottom (or obukhov) left it */
+    double* surf_flux = (double*)malloc(sizeof(double) * np);
+    {
+        const double* th1 = LEV(c, th, 1);
+        const double z_os = (1.0 / 10.0) * P->zo;
+        for (size_t q = 0; q < np; ++q) {
+            if (P->lbc == 1) surf_flux[q] = P->wt_s / P->T_scale / P->u_scale;
+            else surf_flux[q] = (P->T_s - th1[q]) * ORC_VONK * c->ustar_avg[q] / (log(c->dz / (2.0 * z_os)) - c->psi_h[q]);
+        }
+    }
     double* dsx = (double*)calloc(np2 * nl, sizeof(double));
     double* dsy = (double*)calloc(np2 * nl, sizeof(double));
     double* dsz = (double*)calloc(np2 * nl, sizeof(double));
@@ -1078,9 +1210,10 @@ int orc_theta_all_in_one(les_ctx* c, int jt) {
     /* z flux (:576-618) */
     memset(temp, 0, sizeof(double) * np * nl);
     for (size_t q = 0; q < np; ++q) {
-        temp[np + q] = -1.0 * surf_flux;
-        sgs_t3[np + q] = surf_flux;
+        temp[np + q] = -1.0 * surf_flux[q];
+        sgs_t3[np + q] = surf_flux[q];
     }
+    free(surf_flux);
     for (int k = 2; k <= nz; ++k)
         for (size_t q = 0; q < np; ++q) {
             size_t o = (size_t)k * np + q;
@@ -1094,7 +1227,7 @@ int orc_theta_all_in_one(les_ctx* c, int jt) {
             RHS[o] = RHS[o] + dtemp[o];
         }
     free(temp); free(dtemp);
-    /* ---- calc_beta (:204-261, land branch) ---- */
+    /* ---- calc_beta (:204-261) ---- */
     const double g_hat = ORC_G * (P->z_i / (P->u_scale * P->u_scale));
     double* bar = (double*)calloc(nl, sizeof(double));
     for (int k = 0; k <= nz - 1; ++k) bar[k] = orc_hor_avg(c, LEV(c, th, k));
@@ -1102,8 +1235,14 @@ int orc_theta_all_in_one(les_ctx* c, int jt) {
     for (int k = 2; k <= nz - 1; ++k)
         for (size_t q = 0; q < np; ++q) {
             size_t o = (size_t)k * np + q;
-            double above = (th[o] - bar[k]) / bar[k];
-            double below = (th[o - np] - bar[k - 1]) / bar[k - 1];
+            double above, below;
+            if (P->ocean_flag) {                                 /* :239-248 */
+                above = P->alpha_w * (th[o] - bar[k]);
+                below = P->alpha_w * (th[o - np] - bar[k - 1]);
+            } else {
+                above = (th[o] - bar[k]) / bar[k];
+                below = (th[o - np] - bar[k - 1]) / bar[k - 1];
+            }
             beta[o] = g_hat * (above + below) / 2.0;
         }
     free(bar);
@@ -1174,7 +1313,6 @@ int orc_press_stag_array(les_ctx* c, int jt) {
     const size_t np = c->plane, ns = (size_t)lhx * ny;
     const double cst = 1.0 / (double)(c->nx * c->ny);
     const double tadv1 = 1.5, dz = c->dz;
-    (void)jt;
     cplx* Hx = (cplx*)calloc(ns * (nz + 1), sizeof(cplx));
     cplx* Hy = (cplx*)calloc(ns * (nz + 1), sizeof(cplx));
     cplx* Hz = (cplx*)calloc(ns * (nz + 1), sizeof(cplx));
@@ -1193,7 +1331,15 @@ int orc_press_stag_array(les_ctx* c, int jt) {
     {
         double* r = (double*)malloc(sizeof(double) * np);
         const double* divtz = F(c, LES_F_DIVTZ);
-        for (size_t q = 0; q < np; ++q) r[q] = cst * LEV(c, divtz, 1)[q];    /* :195-196 */
+        /* surface-temperature term of lbc = 0 (:186-191): g z_i/u_scale^2
+         * (T_s - <T_s>) of the uniform T_s plane, <.> summed in order */
+        double modi = 0.0;
+        if (P->theta_flag && jt > P->theta_init_time && P->lbc == 0) {
+            double st = 0.0;
+            for (size_t q = 0; q < np; ++q) st += P->T_s;
+            modi = ORC_G * (P->z_i / (P->u_scale * P->u_scale)) * (P->T_s - st / (double)(c->nx * c->ny));
+        }
+        for (size_t q = 0; q < np; ++q) r[q] = cst * (LEV(c, divtz, 1)[q] - modi);   /* :192-196 */
         orc_fftn_plane(c, r, Hz);
         for (size_t q = 0; q < np; ++q) r[q] = cst * LEV(c, divtz, nz)[q];   /* :198-199 */
         orc_fftn_plane(c, r, Hz + (size_t)nz * ns);
@@ -1524,7 +1670,20 @@ int orc_step(les_ctx* c, int jt) {
     const double *dx = F(c, LES_F_DIVTX), *dy = F(c, LES_F_DIVTY), *dzt = F(c, LES_F_DIVTZ);
     const double* beta = F(c, LES_F_BETA_SCAL);
     double *u = F(c, LES_F_U), *v = F(c, LES_F_V), *w = F(c, LES_F_W);
-    const int damp = (P->ubc_mom == LES_BC_STRESS_FREE && P->ubc == 1 && P->damping_method == 1);
+    /* add_damping_layer (007_topbc.f90:92-124): method 1 damps w; methods 2
+     * and 3 damp u, v, w toward their plane means */
+    const int damp_sf = (P->ubc_mom == LES_BC_STRESS_FREE && P->ubc == 1);
+    const int damp1 = damp_sf && P->damping_method == 1;
+    const int damp23 = damp_sf && P->damping_method != 1;
+    double* mn = NULL;
+    if (damp23) {
+        mn = (double*)calloc(3 * c->nlev, sizeof(double));
+        for (int k = 1; k <= nz - 1; ++k) {
+            mn[3 * k] = orc_hor_avg(c, LEV(c, u, k));
+            mn[3 * k + 1] = orc_hor_avg(c, LEV(c, v, k));
+            mn[3 * k + 2] = orc_hor_avg(c, LEV(c, w, k));
+        }
+    }
     for (int k = 1; k <= nz - 1; ++k)
         for (size_t q = 0; q < np; ++q) {
             size_t o = (size_t)k * np + q;
@@ -1532,12 +1691,26 @@ int orc_step(les_ctx* c, int jt) {
             RY[o] = -cy[o] - dy[o];
             RZ[o] = -cz[o] - dzt[o];
             if (P->theta_flag && !P->passive_scalar) RZ[o] = RZ[o] + beta[o];
-            if (P->coriolis_forcing) {
+            if (P->ocean_flag) {                                 /* :134-149 */
+                RX[o] = RX[o] + P->coriol * (v[o] + c->vst[k]) - P->coriol * P->vg;
+                RY[o] = RY[o] - P->coriol * (u[o] + c->ust[k]) + P->coriol * P->ug;
+                if (P->flag_crossvel) {
+                    RX[o] = RX[o] - P->coriol * P->v_cross;
+                    RY[o] = RY[o] + P->coriol * P->u_cross;
+                }
+            } else if (P->coriolis_forcing) {                    /* :150-156 */
                 RX[o] = RX[o] + P->coriol * (v[o] - P->vg);
                 RY[o] = RY[o] + P->coriol * (P->ug - u[o]);
             }
-            if (damp) RZ[o] = RZ[o] - c->sponge[k] * w[o];
+            if (damp1) RZ[o] = RZ[o] - c->sponge[k] * w[o];
+            if (damp23) {
+                const double sk = 0.5 * (c->sponge[k] + c->sponge[k + 1]);
+                RX[o] = RX[o] - sk * (u[o] - mn[3 * k]);
+                RY[o] = RY[o] - sk * (v[o] - mn[3 * k + 1]);
+                RZ[o] = RZ[o] - c->sponge[k] * (w[o] - mn[3 * k + 2]);
+            }
         }
+    free(mn);
     const int st = stats_step(c, jt);
     if (st)                                                      /* dudt, :119-156 */
         for (int k = 1; k <= nz - 1; ++k)
@@ -1546,7 +1719,14 @@ int orc_step(les_ctx* c, int jt) {
                 c->dudt[o] = -dx[o];
                 c->dvdt[o] = -dy[o];
                 c->dwdt[o] = -dzt[o];
-                if (P->coriolis_forcing) {
+                if (P->ocean_flag) {
+                    c->dudt[o] = c->dudt[o] + P->coriol * (v[o] + c->vst[k]) - P->coriol * P->vg;
+                    c->dvdt[o] = c->dvdt[o] - P->coriol * (u[o] + c->ust[k]) + P->coriol * P->ug;
+                    if (P->flag_crossvel) {
+                        c->dudt[o] = c->dudt[o] - P->coriol * P->v_cross;
+                        c->dvdt[o] = c->dvdt[o] + P->coriol * P->u_cross;
+                    }
+                } else if (P->coriolis_forcing) {
                     c->dudt[o] = c->dudt[o] + P->coriol * (v[o] - P->vg);
                     c->dvdt[o] = c->dvdt[o] + P->coriol * (P->ug - u[o]);
                 }

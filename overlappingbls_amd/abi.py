@@ -26,6 +26,7 @@ LES_MIXED = 1
 
 WALL_ROUGH, WALL_SMOOTH = 0, 1
 BC_WALL, BC_STRESS_FREE = 0, 1
+STOKES_DP, STOKES_MONO, STOKES_DYNAMIC = 0, 1, 2
 
 _FIELDS3D = [
     "U", "V", "W", "THETA",
@@ -44,7 +45,8 @@ _FIELDS3D = [
 ]
 FIELD = {name: i for i, name in enumerate(_FIELDS3D)}
 N_FIELDS3D = len(_FIELDS3D)
-SURFACE = {"USTAR_SFC": 100, "PSI_M_SFC": 101, "PHI_M_SFC": 102, "PSI_H_SFC": 103, "PHI_H_SFC": 104}
+SURFACE = {"USTAR_SFC": 100, "PSI_M_SFC": 101, "PHI_M_SFC": 102, "PSI_H_SFC": 103, "PHI_H_SFC": 104,
+           "USTAR_TOP_SFC": 105}
 FIELD.update(SURFACE)
 
 # time-averaged slices, enum les_slice (flow_slice then theta_slice)
@@ -73,7 +75,7 @@ class LesParams(C.Structure):
         ("coriolis_forcing", C.c_int32), ("use_mean_p_force", C.c_int32),
         ("ocean_flag", C.c_int32), ("theta_flag", C.c_int32),
         ("restart_theta", C.c_int32), ("theta_init_time", C.c_int32),
-        ("passive_scalar", C.c_int32), ("lbc", C.c_int32), ("pad0", C.c_int32),
+        ("passive_scalar", C.c_int32), ("lbc", C.c_int32), ("stokes_flag", C.c_int32),
         ("lx_tot", C.c_double), ("ly_tot", C.c_double), ("lz_tot", C.c_double),
         ("dt", C.c_double),
         ("z_i", C.c_double), ("u_scale", C.c_double), ("T_scale", C.c_double),
@@ -81,6 +83,11 @@ class LesParams(C.Structure):
         ("Pr", C.c_double), ("wt_s", C.c_double), ("dTdz_top", C.c_double),
         ("coriol", C.c_double), ("ug", C.c_double), ("vg", C.c_double),
         ("mean_p_force", C.c_double), ("lz_npz", C.c_double),
+        ("stokes_action", C.c_int32), ("flag_crossvel", C.c_int32),
+        ("u_star", C.c_double), ("rad_stress", C.c_double), ("alpha_w", C.c_double),
+        ("u_cross", C.c_double), ("v_cross", C.c_double),
+        ("amp_w", C.c_double), ("lambda_w", C.c_double), ("agl_ust", C.c_double),
+        ("inv_strength", C.c_double), ("T_s", C.c_double), ("Co", C.c_double),
     ]
 
     def as_dict(self):
@@ -110,6 +117,8 @@ def bind(lib: C.CDLL, prefix: str):
         "set_precision": [P, i],
         "set_stokes": [P, _PD, _PD],
         "set_sponge": [P, _PD],
+        "set_wind_stress": [P, C.c_double, C.c_double],
+        "get_profiles": [P, _PD, _PD, _PD],
         "set_field": [P, i, _PD, i, i, i],
         "get_field": [P, i, _PD, i, i, i],
         "set_sgs_init_flags": [P, i, i],
@@ -234,6 +243,21 @@ class LesContext:
         u = np.ascontiguousarray(ust, dtype=np.float64)
         v = np.ascontiguousarray(vst, dtype=np.float64)
         self._call("set_stokes", self._h, _pd(u), _pd(v))
+
+    def set_sponge(self, sponge):
+        a = np.ascontiguousarray(sponge, dtype=np.float64)
+        self._call("set_sponge", self._h, _pd(a))
+
+    def profiles(self):
+        """(ust, vst, sponge) over levels 0..nzt."""
+        a = [np.zeros(self.nlev) for _ in range(3)]
+        self._call("get_profiles", self._h, *[_pd(x) for x in a])
+        return tuple(a)
+
+    def set_wind_stress(self, ustar, agl_deg):
+        """Ocean wind stress (friction velocity / u_scale, angle in degrees)
+        for the following steps (104_wallstress.f90:118-136)."""
+        self._call("set_wind_stress", self._h, float(ustar), float(agl_deg))
 
     def set_precision(self, p):
         self._call("set_precision", self._h, p)

@@ -19,11 +19,19 @@ import re
 
 import numpy as np
 
-from .abi import LesParams, WALL_ROUGH, WALL_SMOOTH, BC_WALL, BC_STRESS_FREE
+from .abi import (LesParams, WALL_ROUGH, WALL_SMOOTH, BC_WALL, BC_STRESS_FREE,
+                  STOKES_DP, STOKES_MONO, STOKES_DYNAMIC)
 
 BOGUS = -1234567890.0  # 001_param.f90:40
 ZO1 = 0.01             # 008_bottombc.f90:18, roughness in metres
 DO1 = 0.0              # 008_bottombc.f90:19
+THETA_S1 = 288.0       # 008_bottombc.f90:23, surface temperature of the patches (K)
+STOKES_ACTIONS = {"donelan.pierson1987": STOKES_DP, "mono": STOKES_MONO, "dynamic": STOKES_DYNAMIC}
+
+
+class UnsupportedOption(ValueError):
+    """A namelist option that selects physics outside the timestep this
+    library provides (the reference would run it)."""
 
 
 @dataclasses.dataclass
@@ -43,6 +51,9 @@ class Namelist:
     dyn_init: int = 5
     inilag: bool = True
     restart_vel: bool = False
+    model: int = 5                      # flow_param model (5: Lagrangian scale-dependent)
+    sgs: bool = True
+    Co: float = 0.16                    # flow_param Co (Smagorinsky, model 1)
     walltype: str = "rough"
     lbc_mom: str = "wall"
     ubc_mom: str = "stress free"
@@ -53,7 +64,7 @@ class Namelist:
     ug_dim: float = 0.0
     vg_dim: float = 0.0
     use_mean_p_force: bool = False
-    mean_p_force: float = 0.0
+    mean_p_force: float | None = None   # None: float_missing, i.e. 1/lz_tot (102_initial.f90:214-219)
     nu_molec: float = 1.14e-5
     molec: bool = False
     Pr: float = 0.4
@@ -64,17 +75,48 @@ class Namelist:
     T_scale: float = 300.0
     wt_s: float = 0.0                   # K m/s
     dTdz_top: float = 0.0               # K/m
+    inv_strength: float = 0.0           # K/m (damping method 3)
     npz: int = 2                        # only lz*npz of step_scalar depends on it
-    ocean_flag: bool = False            # the ocean branch: les_create rejects it
-    lbc: int = 1                        # 1: prescribed surface flux (the only one supported)
+    lbc: int = 1                        # 1: prescribed surface flux; 0: surface temperature
+    theta_s1: float = THETA_S1          # K, surface temperature for lbc = 0
+    # ocean branch (ocean_param, 001_param.f90:304-333)
+    ocean_flag: bool = False
+    u_star: float = 0.0                 # m/s, wind-stress friction velocity
+    agl_stress: float = 0.0             # degrees
+    flag_dynStress: bool = False        # per-step wind stress: the driver calls set_wind_stress
+    alpha_w: float = 2.0e-4             # 1/K
+    flag_crossVel: bool = False
+    udim_cross: float = 0.0
+    vdim_cross: float = 0.0
+    stokes_flag: bool = False
+    stokes_action: str = "mono"
+    amp_w: float = 0.0                  # m
+    lambda_w: float = 1.0               # m
+    agl_Ust: float = 0.0                # degrees
+    # options outside the library's step: params() refuses them when set
+    dns_bc: bool = False
+    inflow: bool = False
+    flag_canopy: bool = False
+    pcon_flag: bool = False
+    flag_relaxForce: bool = False
+    spatial_flux_flag: bool = False
+    flag_dynWT: bool = False
 
     def params(self) -> LesParams:
+        """les_params as init_nondimensional leaves them (102_initial.f90:176-256);
+        raises UnsupportedOption for physics the library does not provide."""
+        refused = [name for name in ("dns_bc", "inflow", "flag_canopy", "pcon_flag", "flag_relaxForce",
+                                     "spatial_flux_flag", "flag_dynWT") if getattr(self, name)]
+        if refused:
+            raise UnsupportedOption("namelist selects options outside the timestep: " + ", ".join(refused))
+        if self.stokes_action not in STOKES_ACTIONS:
+            raise UnsupportedOption(f"stokes_action {self.stokes_action!r} (014_stokes_drift.f90:45-56)")
         p = LesParams()
         p.nxt, p.nyt, p.nzt = self.nxt, self.nyt, self.nzt
-        p.model = 5
+        p.model, p.sgs = int(self.model), int(bool(self.sgs))
         p.cs_count, p.dyn_init = self.cs_count, self.dyn_init
         p.inilag, p.restart_vel = int(self.inilag), int(self.restart_vel)
-        p.sgs, p.molec = 1, int(self.molec)
+        p.molec = int(self.molec)
         p.walltype = {"rough": WALL_ROUGH, "smooth": WALL_SMOOTH}[self.walltype]
         bc = {"wall": BC_WALL, "stress free": BC_STRESS_FREE}
         p.lbc_mom, p.ubc_mom = bc[self.lbc_mom], bc[self.ubc_mom]
@@ -84,7 +126,7 @@ class Namelist:
         p.ocean_flag = int(self.ocean_flag)
         p.theta_flag, p.restart_theta = int(self.theta_flag), int(self.restart_theta)
         p.theta_init_time, p.passive_scalar, p.lbc = self.theta_init_time, int(self.passive_scalar), self.lbc
-        # init_nondimensional, 102_initial.f90:186-225
+        # init_nondimensional, 102_initial.f90:186-250
         z_i, us = self.z_i, self.u_scale
         p.lx_tot, p.ly_tot, p.lz_tot = self.lx_tot / z_i, self.ly_tot / z_i, self.lz_tot / z_i
         p.dt = self.dt * us / z_i
@@ -92,13 +134,25 @@ class Namelist:
         p.nu_molec = self.nu_molec
         p.zo, p.d0 = ZO1 / z_i, DO1 / z_i          # patches, 008_bottombc.f90:119-120
         p.Pr, p.wt_s, p.dTdz_top = self.Pr, self.wt_s, self.dTdz_top
-        if self.coriolis_forcing:
+        p.Co = self.Co
+        if self.coriolis_forcing or self.ocean_flag:   # :208-212, :235-238
             p.coriol = self.freq_coriolis * z_i / us
             p.ug, p.vg = self.ug_dim / us, self.vg_dim / us
-        if self.use_mean_p_force:
-            p.mean_p_force = (1.0 / p.lz_tot) if self.mean_p_force == 0.0 else self.mean_p_force / us**2 * z_i
+        if self.use_mean_p_force:                   # :214-219
+            p.mean_p_force = (1.0 / p.lz_tot) if self.mean_p_force is None else self.mean_p_force / us**2 * z_i
         lz = self.lz_tot / self.npz / z_i          # 102_initial.f90:142, 186
         p.lz_npz = lz * self.npz
+        p.u_star = self.u_star / us                # :201
+        p.alpha_w = self.alpha_w * self.T_scale if (self.ocean_flag and self.theta_flag) else self.alpha_w   # :228-230
+        p.rad_stress = self.agl_stress / 180.0 * math.pi if self.ocean_flag else 0.0   # :246
+        p.flag_crossvel = int(self.flag_crossVel and self.ocean_flag)
+        if p.flag_crossvel:                        # :249-252
+            p.u_cross, p.v_cross = self.udim_cross / us, self.vdim_cross / us
+        p.stokes_flag = int(self.stokes_flag)
+        p.stokes_action = STOKES_ACTIONS[self.stokes_action]
+        p.amp_w, p.lambda_w, p.agl_ust = self.amp_w, self.lambda_w, self.agl_Ust
+        p.inv_strength = self.inv_strength
+        p.T_s = self.theta_s1 / self.T_scale       # patches, 008_bottombc.f90:113-114
         return p
 
 
@@ -154,25 +208,39 @@ def parse_namelist(text: str) -> dict:
     return groups
 
 
-# namelist variable -> Namelist field (the variables the step reads)
+# namelist variable -> Namelist field (the variables the step reads, and the
+# switches of the options it refuses)
 _NML_FIELDS = {f.lower(): f for f in (
     "nxt", "nyt", "nzt", "lx_tot", "ly_tot", "lz_tot", "z_i", "dt", "u_scale", "cs_count", "dyn_init",
-    "inilag", "restart_vel", "walltype", "lbc_mom", "ubc_mom", "ubc", "damping_method", "coriolis_forcing",
-    "freq_coriolis", "ug_dim", "vg_dim", "use_mean_p_force", "mean_p_force", "nu_molec", "molec", "Pr",
-    "theta_flag", "restart_theta", "theta_init_time", "passive_scalar", "T_scale", "wt_s", "dTdz_top", "npz",
-    "ocean_flag", "lbc")}
+    "inilag", "restart_vel", "model", "sgs", "Co", "walltype", "lbc_mom", "ubc_mom", "ubc", "damping_method",
+    "coriolis_forcing", "freq_coriolis", "ug_dim", "vg_dim", "use_mean_p_force", "mean_p_force", "nu_molec",
+    "molec", "Pr", "theta_flag", "restart_theta", "theta_init_time", "passive_scalar", "T_scale", "wt_s",
+    "dTdz_top", "inv_strength", "npz", "lbc", "ocean_flag", "u_star", "agl_stress", "flag_dynStress",
+    "alpha_w", "flag_crossVel", "udim_cross", "vdim_cross", "stokes_flag", "stokes_action", "amp_w",
+    "lambda_w", "agl_Ust", "dns_bc", "inflow", "flag_canopy", "pcon_flag", "flag_relaxForce",
+    "spatial_flux_flag", "flag_dynWT")}
+
+# The reference's defaults (001_param.f90) where they differ from the
+# Namelist dataclass's: a param.nml that leaves a variable out gets these.
+REFERENCE_DEFAULTS = dict(
+    ubc=1, ubc_mom="wall", stokes_flag=True, stokes_action="donelan.pierson1987",
+    flag_canopy=True, flag_dynStress=True, inv_strength=BOGUS, dTdz_top=BOGUS, wt_s=BOGUS,
+    theta_flag=False, theta_init_time=-9999, dyn_init=-9999, amp_w=BOGUS, lambda_w=BOGUS, agl_Ust=BOGUS,
+    udim_cross=BOGUS, vdim_cross=BOGUS, u_star=BOGUS, nu_molec=1.14e-5)
 
 
 def read_namelist(path_or_text: str) -> Namelist:
     """Namelist from the reference's param.nml (a path, or its text): the
-    variables of every group that the step reads, the rest at their
-    defaults.  Options the step does not support (ocean_flag, lbc = 0,
-    damping methods 2 and 3) pass through, and les_create rejects them."""
+    variables of every group that the step reads; the others keep the
+    reference's defaults (001_param.f90).  Options outside the step (canopy,
+    DNS, inflow, pollen, relaxation forcing, file-driven surface flux)
+    make Namelist.params() raise UnsupportedOption; physics options the
+    library does not implement reach les_create, which rejects them."""
     text = path_or_text
     if "&" not in path_or_text and os.path.exists(path_or_text):
         with open(path_or_text) as f:
             text = f.read()
-    nl = Namelist()
+    nl = Namelist(**REFERENCE_DEFAULTS)
     for group in parse_namelist(text).values():
         for var, val in group.items():
             field = _NML_FIELDS.get(var)
@@ -183,7 +251,7 @@ def read_namelist(path_or_text: str) -> Namelist:
                 val = bool(val)
             elif isinstance(cur, int) and not isinstance(val, bool):
                 val = int(val)
-            elif isinstance(cur, float):
+            elif isinstance(cur, float) or field == "mean_p_force":
                 val = float(val)
             setattr(nl, field, val)
     return nl

@@ -245,12 +245,31 @@ __global__ void k_obukhov_active(double* ustar, double* psi_m, double* phi_m, do
     }
 }
 
+// ocean branch under a wall top (010_scalars.f90:696-716): the top wall's
+// plane-mean ustar_avg from level nz-1 (sum in sums[0]); nf = float(nxt*nynpy)
+__global__ void k_obukhov_ocean(double* ustar_top, const double* __restrict__ sums, long plane, double nf,
+                                int smooth, double zo, double dz, double nu_nd) {
+    const double su = sums[0];
+    const double us = smooth ? su / nf / (log(0.5 * dz / nu_nd) / VONK + B_SMOOTH)
+                             : su / nf * VONK / log(0.5 * dz / zo);
+    GRID_LOOP(i, plane) ustar_top[i] = us;
+}
+
 // obukhov's plane sums (and the fills that precede them) on stream s
 void obukhov_sums_dev(les_ctx* c, int jt, hipStream_t s) {
     const les_params& P = c->prm;
     const long np = (long)c->plane;
     const double* u1 = c->f3[LES_F_U] + np;
     const double* v1 = c->f3[LES_F_V] + np;
+    if (P.ocean_flag) {
+        // the surface ranks' planes are left as they are; the top wall's
+        // ranks take their ustar_avg from level nz-1
+        if (P.ubc_mom == LES_BC_WALL)
+            hipLaunchKernelGGL(k_obukhov_sums, dim3(1), dim3(64), 0, s, u1 + (long)(c->nz - 2) * np,
+                               v1 + (long)(c->nz - 2) * np, (const double*)nullptr, (const double*)nullptr,
+                               (const double*)nullptr, np, c->red);
+        return;
+    }
     if (!P.theta_flag || jt < P.theta_init_time) {
         if (!P.restart_theta) fill_levels(s, c->psi_m, c->plane, 0, 1, 0.0);
         hipLaunchKernelGGL(k_obukhov_sums, dim3(2), dim3(64), 0, s, u1, v1, c->psi_m, (const double*)nullptr,
@@ -272,6 +291,13 @@ void obukhov_finish_dev(les_ctx* c, int jt) {
     const long np = (long)c->plane;
     const double nn = (double)(c->nx * c->ny);
     hipStream_t s = c->stream;
+    if (P.ocean_flag) {
+        if (P.ubc_mom == LES_BC_WALL)
+            hipLaunchKernelGGL(k_obukhov_ocean, grid_for(np), dim3(256), 0, s, c->ustar_top, (const double*)c->red,
+                               np, (double)(float)(c->nx * c->ny), (int)(P.walltype == LES_WALL_SMOOTH), P.zo, c->dz,
+                               P.nu_molec / P.u_scale / P.z_i);
+        return;
+    }
     if (!P.theta_flag || jt < P.theta_init_time) {
         hipLaunchKernelGGL(k_obukhov_neutral, grid_for(np), dim3(256), 0, s, c->ustar, c->phi_m, c->psi_h,
                            c->phi_h, (const double*)c->red, np, nn, (int)(P.walltype == LES_WALL_SMOOTH), P.zo,
@@ -333,6 +359,19 @@ __global__ void k_wall(W* txz, W* tyz, W* dudz, W* dvdz, double* ustar,
     }
 }
 
+// ocean surface (104_wallstress.f90:118-136): the wind stress, and dudz,
+// dvdz of level 1 as the projection of level 2's on the stress direction
+template <typename W>
+__global__ void k_wall_ocean(W* txz, W* tyz, W* dudz, W* dvdz, long plane, double us, double rad) {
+    GRID_LOOP(i, plane) {
+        txz[i] = us * us * cos(rad);
+        tyz[i] = us * us * sin(rad);
+        const double a = (double)dudz[i + plane] * cos(rad) + (double)dvdz[i + plane] * sin(rad);
+        dudz[i] = a * cos(rad);
+        dvdz[i] = a * sin(rad);
+    }
+}
+
 template <typename W>
 void wallstress_dev(les_ctx* c, int top, const double* u1, const double* v1) {
     const les_params& P = c->prm;
@@ -343,6 +382,11 @@ void wallstress_dev(les_ctx* c, int top, const double* u1, const double* v1) {
     W* tyz = WF<W>(c, LES_F_TYZ) + lev * np;
     W* dudz = WF<W>(c, LES_F_DUDZ) + lev * np;
     W* dvdz = WF<W>(c, LES_F_DVDZ) + lev * np;
+    if (!top && P.ocean_flag) {
+        hipLaunchKernelGGL(k_wall_ocean<W>, grid_for(np), dim3(256), 0, c->stream, txz, tyz, dudz, dvdz, np,
+                           c->wind_ustar, c->wind_rad);
+        return;
+    }
     const int sf = top ? (P.ubc_mom == LES_BC_STRESS_FREE) : (P.lbc_mom == LES_BC_STRESS_FREE);
     if (sf) {
         fill_runs<W>(c->stream, np, 0.0, {txz, tyz, dudz, dvdz});
@@ -908,9 +952,25 @@ void scalar_adv_dev(les_ctx* c, const W* dsx, const W* dsy, const W* dsz, W* rm)
 }
 
 // SGS flux components: comp 0 x (:513-541), 1 y (:552-569), 2 z (:578-594)
+// Surface flux of scalar_RHS_calc (010_scalars.f90:359-394): lbc = 1 the
+// prescribed wt_s (surf_flux); lbc = 0 the bulk Monin-Obukhov flux from the
+// surface temperature T_s and the filtered theta of level 1, with ustar_avg
+// as the wall model left it and obukhov's psi_h (:380-385)
+struct SurfFlux {
+    int bulk;
+    double surf_flux;                  // lbc = 1
+    double T_s, log_zos;               // lbc = 0: log(dz / (2 z_os))
+    const double *th1, *ustar, *psi_h;
+};
+__device__ __forceinline__ double surf_flux_at(const SurfFlux& f, long q) {
+    if (!f.bulk) return f.surf_flux;
+    return (f.T_s - f.th1[q]) * VONK * f.ustar[q] / (f.log_zos - f.psi_h[q]);
+}
+
+// SGS flux components: comp 0 x (:513-541), 1 y (:552-569), 2 z (:578-594)
 template <typename W>
 __global__ void k_scalar_flux(W* temp, double* sgs_t3, const W* __restrict__ Nu, const W* __restrict__ ds, KP p,
-                              int comp, double iPr, double surf_flux) {
+                              int comp, double iPr, SurfFlux sf) {
     const long n = p.plane * (p.nz + 1);
     GRID_LOOP(i, n) {
         const int k = (int)(i / p.plane);
@@ -920,8 +980,14 @@ __global__ void k_scalar_flux(W* temp, double* sgs_t3, const W* __restrict__ Nu,
             else if (k == p.nz - 1) t = iPr * Nu[i + p.plane] * ds[i];
             else if (k >= 2 && k <= p.nz - 2) t = iPr * 0.5 * ((double)Nu[i] + (double)Nu[i + p.plane]) * ds[i];
         } else {
-            if (k == 1) { t = -1.0 * surf_flux; sgs_t3[i] = surf_flux; }
-            else if (k >= 2) { t = iPr * Nu[i] * ds[i]; sgs_t3[i] = -1.0 * t; }
+            if (k == 1) {
+                const double fl = surf_flux_at(sf, i - p.plane);
+                t = -1.0 * fl;
+                sgs_t3[i] = fl;
+            } else if (k >= 2) {
+                t = iPr * Nu[i] * ds[i];
+                sgs_t3[i] = -1.0 * t;
+            }
         }
         temp[i] = t;
     }
@@ -932,9 +998,16 @@ void scalar_flux_dev(les_ctx* c, W* temp, int comp) {
     KP p = kp(c);
     const les_params& P = c->prm;
     const W* ds = WF<W>(c, comp == 0 ? LES_F_DTDX : comp == 1 ? LES_F_DTDY : LES_F_DTDZ);
+    SurfFlux sf;
+    sf.bulk = (P.lbc == 0);
+    sf.surf_flux = P.wt_s / P.T_scale / P.u_scale;
+    sf.T_s = P.T_s;
+    sf.log_zos = log(c->dz / (2.0 * ((1.0 / 10.0) * P.zo)));   // z_os = zo/10, 008_bottombc.f90:126
+    sf.th1 = c->f3[LES_F_THETA] + c->plane;
+    sf.ustar = c->ustar;
+    sf.psi_h = c->psi_h;
     hipLaunchKernelGGL(k_scalar_flux<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, temp,
-                       c->f3[LES_F_SGS_T3], (const W*)WF<W>(c, LES_F_NU_T), ds, p, comp, 1.0 / P.Pr,
-                       P.wt_s / P.T_scale / P.u_scale);
+                       c->f3[LES_F_SGS_T3], (const W*)WF<W>(c, LES_F_NU_T), ds, p, comp, 1.0 / P.Pr, sf);
 }
 
 // adv != null: RHS = -adv + d (first flux, adv = the advection term), else
@@ -975,17 +1048,24 @@ void scalar_accum_dev(les_ctx* c, const W* dtemp, const W* adv) {
                        c->f3[LES_F_RHST], dtemp, adv, p);
 }
 
-// calc_beta, land branch (:214-258); means[k] = plane mean of theta(k)
+// calc_beta (:214-258); means[k] = plane mean of theta(k); alpha_w > 0
+// selects the ocean's thermal-expansion form (:239-248)
 template <typename W>
 __global__ void k_calc_beta(W* beta, const double* __restrict__ th, const double* __restrict__ bar, KP p,
-                            double g_hat) {
+                            double g_hat, int ocean, double alpha_w) {
     const long n = p.plane * (p.nz + 1);
     GRID_LOOP(i, n) {
         const int k = (int)(i / p.plane);
         double b = 0.0;
         if (k >= 2 && k <= p.nz - 1) {
-            const double above = (th[i] - bar[k]) / bar[k];
-            const double below = (th[i - p.plane] - bar[k - 1]) / bar[k - 1];
+            double above, below;
+            if (ocean) {
+                above = alpha_w * (th[i] - bar[k]);
+                below = alpha_w * (th[i - p.plane] - bar[k - 1]);
+            } else {
+                above = (th[i] - bar[k]) / bar[k];
+                below = (th[i - p.plane] - bar[k - 1]) / bar[k - 1];
+            }
             b = g_hat * (above + below) / 2.0;
         }
         beta[i] = b;
@@ -998,7 +1078,7 @@ void calc_beta_dev(les_ctx* c, const double* means) {
     const les_params& P = c->prm;
     hipLaunchKernelGGL(k_calc_beta<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream,
                        WF<W>(c, LES_F_BETA_SCAL), (const double*)c->f3[LES_F_THETA], means, p,
-                       GRAV * (P.z_i / (P.u_scale * P.u_scale)));
+                       GRAV * (P.z_i / (P.u_scale * P.u_scale)), P.ocean_flag, P.alpha_w);
 }
 
 // step_scalar (:624-671): AB2 on 1..nz-1, top boundary row
@@ -1035,15 +1115,18 @@ void step_scalar_dev(les_ctx* c) {
 // their value at the level's first point before rounding.  This changes only
 // the kx = ky = 0 coefficients, which the solve does not read from these
 // transforms (Hx, Hy: unused; Hz: replaced by the FP64 plane sums).
+// modi: the surface-temperature term of lbc = 0 taken from divtz(1) at the
+// bottom (109_press_stag_array.f90:186-196), 0 otherwise
 template <typename W>
 __global__ void k_press_prep(W* rx, W* ry, W* rz, const double* __restrict__ u, const double* __restrict__ v,
-                             const double* __restrict__ w, const W* __restrict__ divtz, KP p, double cst, int shift) {
+                             const double* __restrict__ w, const W* __restrict__ divtz, KP p, double cst, int shift,
+                             double modi) {
     const long n = p.plane * (p.nz + 1);
     const double tadv1 = 1.5;
     GRID_LOOP(i, n) {
         const int k = (int)(i / p.plane);
         if (k == 0) {
-            rz[i] = cst * divtz[i + p.plane];
+            rz[i] = cst * ((double)divtz[i + p.plane] - modi);
             rx[i] = 0.0; ry[i] = 0.0;
         } else if (k == p.nz) {
             rz[i] = cst * divtz[i];
@@ -1060,16 +1143,16 @@ __global__ void k_press_prep(W* rx, W* ry, W* rz, const double* __restrict__ u, 
 
 template <typename W>
 void press_rhs_prep_on(les_ctx* c, W* rx, W* ry, W* rz, const double* u, const double* v, const double* w,
-                       const W* divtz) {
+                       const W* divtz, double modi) {
     KP p = kp(c);
     hipLaunchKernelGGL(k_press_prep<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, rx, ry, rz, u, v, w,
-                       divtz, p, 1.0 / (double)(c->nx * c->ny), (int)(sizeof(W) < sizeof(double)));
+                       divtz, p, 1.0 / (double)(c->nx * c->ny), (int)(sizeof(W) < sizeof(double)), modi);
 }
 
 template <typename W>
 void press_rhs_prep_dev(les_ctx* c, W* rx, W* ry, W* rz) {
     press_rhs_prep_on<W>(c, rx, ry, rz, c->f3[LES_F_U], c->f3[LES_F_V], c->f3[LES_F_W],
-                         (const W*)WF<W>(c, LES_F_DIVTZ));
+                         (const W*)WF<W>(c, LES_F_DIVTZ), 0.0);
 }
 
 // Plane sums of the z right-hand side of the pressure equation in FP64 (the
@@ -1081,7 +1164,7 @@ constexpr int HZ_SPLIT = 8;
 
 template <typename W>
 __global__ __launch_bounds__(256) void k_hz_part(const double* __restrict__ w, const W* __restrict__ divtz, KP p,
-                                                  double cst, double* __restrict__ part) {
+                                                  double cst, double modi, double* __restrict__ part) {
     __shared__ double sh[256];
     const int k = blockIdx.x / HZ_SPLIT, part_id = blockIdx.x % HZ_SPLIT;
     const double tadv1 = 1.5;
@@ -1091,7 +1174,7 @@ __global__ __launch_bounds__(256) void k_hz_part(const double* __restrict__ w, c
     double s = 0.0;
     for (long q = q0 + threadIdx.x; q < q1; q += 256) {
         const long i = (long)k * p.plane + q;
-        if (k == 0) s += cst * divtz[i + p.plane];
+        if (k == 0) s += cst * ((double)divtz[i + p.plane] - modi);
         else if (k == p.nz) s += cst * divtz[i];
         else s += cst / tadv1 * (w[i] / p.dt);
     }
@@ -1113,12 +1196,12 @@ __global__ void k_hz_fold(const double* __restrict__ part, int nlev, double* __r
 }
 
 template <typename W>
-void press_hz_sums_dev(les_ctx* c, double* out) {
+void press_hz_sums_dev(les_ctx* c, double* out, double modi) {
     KP p = kp(c);
     double* part = c->work3[5];  // (nz+1) * HZ_SPLIT partial sums
     hipLaunchKernelGGL(k_hz_part<W>, dim3((p.nz + 1) * HZ_SPLIT), dim3(256), 0, c->stream,
                        (const double*)c->f3[LES_F_W], (const W*)WF<W>(c, LES_F_DIVTZ), p,
-                       1.0 / (double)(c->nx * c->ny), part);
+                       1.0 / (double)(c->nx * c->ny), modi, part);
     hipLaunchKernelGGL(k_hz_fold, dim3((p.nz + 1 + 63) / 64), dim3(64), 0, c->stream, (const double*)part, p.nz + 1,
                        out);
 }
@@ -1152,14 +1235,28 @@ struct RhsArgs {
     double *du, *dv, *dw;  // dudt diagnostics (103_flosol.f90:119-156), or null
 };
 
+// Forcing terms of the RHS assembly (103_flosol.f90:121-156) and the
+// damping layer (007_topbc.f90:92-124)
+struct RhsOpts {
+    int buoy;                 // buoyancy (theta_flag, not passive)
+    int ocean;                // ocean Coriolis with Stokes drift (:134-149)
+    int cori;                 // land Coriolis / geostrophic forcing (:150-156)
+    int crossvel;             // ocean cross flow (:143-148)
+    int damp1, damp23;        // damping method 1 (w only), 2 or 3 (u, v, w toward plane means)
+    int euler;                // Euler start (RHS_f <- RHS)
+    double coriol, ug, vg, u_cross, v_cross, force;
+    const double *ust, *vst;  // Stokes drift (0:nz)
+    const double* mn;         // plane means of u, v, w at levels 1..nz-1 (mn[k-1], mn[nlev+k-1], ...)
+    long nlev;
+};
+
 // RHS assembly (103_flosol.f90:115-157) fused with Admas_Bashforth
 // (:319-389): each point forms RHS from the same-point fields, stores it (and
 // RHS_f at an Euler start, the dudt diagnostics on sampling steps), then
 // advances u, v, w with it; level 0 of u, v, w <- BOGUS.  The arithmetic and
-// its order are those of the two loops.
+// its order are those of the loops.
 template <typename W>
-__global__ void k_rhs_ab2(RhsArgs<W> a, double* u, double* v, double* w, KP p, int buoy, int cori, double coriol,
-                          double ug, double vg, int damp, int euler, double force) {
+__global__ void k_rhs_ab2(RhsArgs<W> a, double* u, double* v, double* w, KP p, RhsOpts o) {
     const long n = p.plane * p.nz;  // levels 0..nz-1
     const double tadv1 = 1.5, tadv2 = 1.0 - tadv1;
     GRID_LOOP(i, n) {
@@ -1171,29 +1268,49 @@ __global__ void k_rhs_ab2(RhsArgs<W> a, double* u, double* v, double* w, KP p, i
         double rx = -(double)a.cx[i] - (double)a.dx[i];
         double ry = -(double)a.cy[i] - (double)a.dy[i];
         double rz = -(double)a.cz[i] - (double)a.dz[i];
-        if (buoy) rz = rz + a.beta[i];
-        if (cori) {
-            rx = rx + coriol * (v[i] - vg);
-            ry = ry + coriol * (ug - u[i]);
+        if (o.buoy) rz = rz + a.beta[i];
+        if (o.ocean) {
+            rx = rx + o.coriol * (v[i] + o.vst[k]) - o.coriol * o.vg;
+            ry = ry - o.coriol * (u[i] + o.ust[k]) + o.coriol * o.ug;
+            if (o.crossvel) {
+                rx = rx - o.coriol * o.v_cross;
+                ry = ry + o.coriol * o.u_cross;
+            }
+        } else if (o.cori) {
+            rx = rx + o.coriol * (v[i] - o.vg);
+            ry = ry + o.coriol * (o.ug - u[i]);
         }
-        if (damp) rz = rz - a.sponge[k] * w[i];
+        if (o.damp1) rz = rz - a.sponge[k] * w[i];
+        if (o.damp23) {
+            const double sk = 0.5 * (a.sponge[k] + a.sponge[k + 1]);
+            rx = rx - sk * (u[i] - o.mn[k - 1]);
+            ry = ry - sk * (v[i] - o.mn[o.nlev + k - 1]);
+            rz = rz - a.sponge[k] * (w[i] - o.mn[2 * o.nlev + k - 1]);
+        }
         a.rx[i] = rx; a.ry[i] = ry; a.rz[i] = rz;
         if (a.du) {
             double du = -(double)a.dx[i], dv = -(double)a.dy[i];
-            if (cori) {
-                du = du + coriol * (v[i] - vg);
-                dv = dv + coriol * (ug - u[i]);
+            if (o.ocean) {
+                du = du + o.coriol * (v[i] + o.vst[k]) - o.coriol * o.vg;
+                dv = dv - o.coriol * (u[i] + o.ust[k]) + o.coriol * o.ug;
+                if (o.crossvel) {
+                    du = du - o.coriol * o.v_cross;
+                    dv = dv + o.coriol * o.u_cross;
+                }
+            } else if (o.cori) {
+                du = du + o.coriol * (v[i] - o.vg);
+                dv = dv + o.coriol * (o.ug - u[i]);
             }
             a.du[i] = du; a.dv[i] = dv; a.dw[i] = -(double)a.dz[i];
         }
         double rxf, ryf, rzf;
-        if (euler) {
+        if (o.euler) {
             a.rxf[i] = rx; a.ryf[i] = ry; a.rzf[i] = rz;
             rxf = rx; ryf = ry; rzf = rz;
         } else {
             rxf = a.rxf[i]; ryf = a.ryf[i]; rzf = a.rzf[i];
         }
-        u[i] = u[i] + p.dt * (tadv1 * rx + tadv2 * rxf + 0.0 + force);
+        u[i] = u[i] + p.dt * (tadv1 * rx + tadv2 * rxf + 0.0 + o.force);
         v[i] = v[i] + p.dt * (tadv1 * ry + tadv2 * ryf + 0.0);
         w[i] = w[i] + p.dt * (tadv1 * rz + tadv2 * rzf + 0.0);
     }
@@ -1214,11 +1331,30 @@ void rhs_ab2_dev(les_ctx* c, int jt) {
     a.dv = st ? c->st_diag[1] : nullptr;
     a.dw = st ? c->st_diag[2] : nullptr;
     a.sponge = c->sponge;
-    const int damp = (P.ubc_mom == LES_BC_STRESS_FREE && P.ubc == 1 && P.damping_method == 1);
+    RhsOpts o;
+    o.buoy = (int)(P.theta_flag && !P.passive_scalar);
+    o.ocean = P.ocean_flag;
+    o.cori = P.coriolis_forcing;
+    o.crossvel = P.ocean_flag && P.flag_crossvel;
+    const int damp = (P.ubc_mom == LES_BC_STRESS_FREE && P.ubc == 1);
+    o.damp1 = damp && P.damping_method == 1;
+    o.damp23 = damp && P.damping_method != 1;
+    o.euler = (int)(jt == 1 && !P.restart_vel);
+    o.coriol = P.coriol; o.ug = P.ug; o.vg = P.vg;
+    o.u_cross = P.u_cross; o.v_cross = P.v_cross;
+    o.force = P.use_mean_p_force ? P.mean_p_force : 0.0;
+    o.ust = c->ust; o.vst = c->vst;
+    o.mn = c->dmean;
+    o.nlev = (long)c->nlev;
+    if (o.damp23) {
+        // calc_hor_avg3 of u, v, w at levels 1..nz-1 (007_topbc.f90:110-112), in order
+        const double nn = (double)(c->nx * c->ny);
+        level_means_seq(c->stream, c->f3[LES_F_U], c->plane, 1, c->nz - 1, nn, c->dmean);
+        level_means_seq(c->stream, c->f3[LES_F_V], c->plane, 1, c->nz - 1, nn, c->dmean + c->nlev);
+        level_means_seq(c->stream, c->f3[LES_F_W], c->plane, 1, c->nz - 1, nn, c->dmean + 2 * c->nlev);
+    }
     hipLaunchKernelGGL(k_rhs_ab2<W>, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, a, c->f3[LES_F_U],
-                       c->f3[LES_F_V], c->f3[LES_F_W], p, (int)(P.theta_flag && !P.passive_scalar),
-                       P.coriolis_forcing, P.coriol, P.ug, P.vg, damp, (int)(jt == 1 && !P.restart_vel),
-                       P.use_mean_p_force ? P.mean_p_force : 0.0);
+                       c->f3[LES_F_V], c->f3[LES_F_W], p, o);
 }
 
 __global__ void k_ab2(double* u, double* v, double* w, const double* __restrict__ rx,
@@ -1469,9 +1605,9 @@ void max3_dev(les_ctx* c, double* out, bool lag) {
     template void calc_beta_dev<W>(les_ctx*, const double*);                                                \
     template void press_rhs_prep_dev<W>(les_ctx*, W*, W*, W*);                                              \
     template void press_rhs_prep_on<W>(les_ctx*, W*, W*, W*, const double*, const double*, const double*,   \
-                                       const W*);                                                           \
+                                       const W*, double);                                                   \
     template void dpdz_on<W>(les_ctx*, W*, const W*);                                                       \
-    template void press_hz_sums_dev<W>(les_ctx*, double*);                                                  \
+    template void press_hz_sums_dev<W>(les_ctx*, double*, double);                                          \
     template void dpdz_dev<W>(les_ctx*);                                                                    \
     template void rhs_ab2_dev<W>(les_ctx*, int);                                                            \
     template void gradp_project_dev<W>(les_ctx*, int, bool);                                                \

@@ -22,11 +22,11 @@ namespace les {
 // tangent fields.
 template <typename W>
 void press_solve(les_ctx* c, const double* u, const double* v, const double* w, const W* divtz, W* pf, W* dpdx,
-                 W* dpdy, W* dpdz) {
+                 W* dpdy, W* dpdz, double modi) {
     const int nz = c->nz;
     const size_t np = c->plane, ns = c->ns;
     W *rx = scr<W>(c->work3[0]), *ry = scr<W>(c->work3[1]), *rz = scr<W>(c->work3[2]);
-    press_rhs_prep_on<W>(c, rx, ry, rz, u, v, w, divtz);
+    press_rhs_prep_on<W>(c, rx, ry, rz, u, v, w, divtz, modi);
     cx<W>*Hx = scr<W>(c->spec3[0]), *Hy = scr<W>(c->spec3[1]), *Hz = scr<W>(c->spec3[2]);
     const long rows = (long)(nz + 1) * c->ny;
     x_r2c<W>(c->stream, rx, Hx, rows, c->nx, c->lhx, c->tw_x, 1.0);
@@ -41,7 +41,7 @@ void press_solve(les_ctx* c, const double* u, const double* v, const double* w, 
         ph = scr<W>(c->spec3[3]);
     } else {
         ph = reinterpret_cast<cx<W>*>(c->phf);
-        press_hz_sums_dev<W>(c, c->hzm);
+        press_hz_sums_dev<W>(c, c->hzm, modi);
         hzm = c->hzm;
     }
     tdma_press<W>(c, Hx, Hy, Hz, ph, reinterpret_cast<double*>(c->spec3[3]), hzm);
@@ -52,9 +52,20 @@ void press_solve(les_ctx* c, const double* u, const double* v, const double* w, 
     dpdz_on<W>(c, dpdz, pf);
 }
 template void press_solve<double>(les_ctx*, const double*, const double*, const double*, const double*, double*,
-                                  double*, double*, double*);
+                                  double*, double*, double*, double);
 template void press_solve<float>(les_ctx*, const double*, const double*, const double*, const float*, float*, float*,
-                                 float*, float*);
+                                 float*, float*, double);
+
+// lbc = 0's surface-temperature term of the pressure's bottom row on step jt
+// (109_press_stag_array.f90:186-191): g z_i/u_scale^2 (T_s - <T_s>) for the
+// uniform surface temperature of the patches, <.> a sum in the reference's order
+double press_modi(const les_ctx* c, int jt) {
+    const les_params& P = c->prm;
+    if (!(P.theta_flag && jt > P.theta_init_time && P.lbc == 0)) return 0.0;
+    double st = 0.0;
+    for (size_t q = 0; q < c->plane; ++q) st += P.T_s;
+    return GRAV * (P.z_i / (P.u_scale * P.u_scale)) * (P.T_s - st / (double)(c->nx * c->ny));
+}
 
 }  // namespace les
 
@@ -190,16 +201,80 @@ hipError_t init_spectral(les_ctx* c) {
     return hipMemcpy(c->coef_c, cc.data(), ns * nz * 8, hipMemcpyHostToDevice);
 }
 
+// setsponge (007_topbc.f90:15-85) at the global level index k, with the uvp
+// height zuvp(k) = (k - 1/2) dz (init_meshgrid, 102_initial.f90:275-278).
+// Method 2 zeroes sponge(nz) of every z-slab, a per-slab value; in one domain
+// only the top slab's remains (level nzt).
 std::vector<double> sponge_profile(const les_ctx* c) {
-    // setsponge, damping_method 1 (007_topbc.f90:26-36), global level index
+    const les_params& P = c->prm;
     std::vector<double> sp(c->nlev, 0.0);
-    if (c->prm.ubc != 1) return sp;
+    if (P.ubc != 1) return sp;
     const int nzt = c->nz;
-    const double factor = 9.0 / (double)(nzt - 3 * nzt / 4 + 1);
-    const double top = c->prm.z_i / (50.0 * c->prm.u_scale);
-    for (int k = 1; k <= nzt - 1; ++k)
-        if (k > 3 * nzt / 4 + 1) sp[k] = top * pow(5.0, (double)(k - nzt) * factor);
+    if (P.damping_method == 1) {                                     // :26-36
+        const double factor = 9.0 / (double)(nzt - 3 * nzt / 4 + 1);
+        const double top = P.z_i / (50.0 * P.u_scale);
+        for (int k = 1; k <= nzt - 1; ++k)
+            if (k > 3 * nzt / 4 + 1) sp[k] = top * pow(5.0, (double)(k - nzt) * factor);
+    } else if (P.damping_method == 2) {                              // :37-50
+        const double z_d = 0.75 * P.lz_tot, cfrdmp = 3.9;
+        for (int k = 1; k <= nzt - 1; ++k) {
+            const double z = ((double)k - 0.5) * c->dz;
+            if (z >= z_d && z <= P.lz_tot) sp[k] = cfrdmp * 0.5 * (1.0 - cos(PI * (z - z_d) / (P.lz_tot - z_d)));
+        }
+    } else {                                                         // :55-82
+        const double g_hat = GRAV * (P.z_i / (P.u_scale * P.u_scale));
+        // fac and 8.0*atan(1.0) are default-real (single precision) in the reference
+        const double fac = P.ocean_flag ? (double)0.22f : 1.5;
+        const double bv = P.ocean_flag ? sqrt(g_hat * P.alpha_w * P.inv_strength / P.T_scale * P.z_i)
+                                       : sqrt(g_hat * P.inv_strength / P.T_scale * P.z_i);
+        const double sigma_z = 0.8 * P.lz_tot;
+        const double sigma_max = fac * bv / (double)(8.0f * atanf(1.0f));
+        for (int k = 1; k <= nzt; ++k) {
+            const double z = ((double)k - 0.5) * c->dz;
+            if (z >= sigma_z && z <= P.lz_tot) {
+                const double ell = (z - sigma_z) / (P.lz_tot - sigma_z);
+                sp[k] = ell * ell * sigma_max;
+            }
+        }
+    }
     return sp;
+}
+
+// init_stokes_drift (014_stokes_drift.f90:29-115): the monochromatic wave's
+// drift, exp(-2 k z) under a stress-free top and the finite-depth form under
+// a wall top (mono_shallow_water); 'dynamic' is the monochromatic wave of
+// gwave.in's first record, whose values the caller passes as amp_w, lambda_w
+// and agl_ust.  False for the Donelan-Pierson spectrum (not provided).
+bool stokes_profile(const les_ctx* c, std::vector<double>& ust, std::vector<double>& vst) {
+    const les_params& P = c->prm;
+    ust.assign(c->nlev, 0.0);
+    vst.assign(c->nlev, 0.0);
+    if (!P.stokes_flag) return true;
+    if (P.stokes_action != LES_STOKES_MONO && P.stokes_action != LES_STOKES_DYNAMIC) return false;
+    const double rad = P.agl_ust / 180.0 * PI;
+    const double amp = P.amp_w / P.z_i, lam = P.lambda_w / P.z_i;
+    const double wn = 2.0 * PI / lam;
+    const double gnd = GRAV * P.z_i / (P.u_scale * P.u_scale);
+    if (P.ubc_mom == LES_BC_STRESS_FREE) {                           // monochromatic, :146-160
+        const double omega = sqrt(gnd * wn);
+        const double us = omega * wn * amp * amp;
+        for (int k = 0; k <= c->nz; ++k) {                           // :59-63
+            const double z = ((double)k - 0.5) * c->dz;
+            ust[k] = us * exp(-2.0 * wn * z) * cos(rad);
+            vst[k] = us * exp(-2.0 * wn * z) * sin(rad);
+        }
+    } else {                                                         // mono_shallow_water, :164-178
+        const double H = P.lz_tot;
+        const double omega = sqrt(gnd * wn * tanh(wn * H));
+        const double us = omega * wn * amp * amp / tanh(wn * H);
+        for (int k = 0; k <= c->nz; ++k) {                           // :76-81
+            const double z = ((double)k - 0.5) * c->dz;
+            const double sh = sinh(wn * H);
+            ust[k] = us * cosh(2.0 * wn * (-z + H)) / 2.0 / (sh * sh) * cos(rad);
+            vst[k] = us * cosh(2.0 * wn * (-z + H)) / 2.0 / (sh * sh) * sin(rad);
+        }
+    }
+    return true;
 }
 
 // Work fields: recomputed inside every step, stored in the precision of the
@@ -226,6 +301,7 @@ double* surf(les_ctx* c, int id) {
         case LES_F_PHI_M_SFC: return c->phi_m;
         case LES_F_PSI_H_SFC: return c->psi_h;
         case LES_F_PHI_H_SFC: return c->phi_h;
+        case LES_F_USTAR_TOP_SFC: return c->ustar_top;
         default: return nullptr;
     }
 }
@@ -509,9 +585,9 @@ void theta_all_in_one(les_ctx* c, int jt) {
 // H spectra in spec3[0..2], d' in spec3[3] (FP64), FP32 p_hat in phf, and the
 // mean mode from FP64 plane sums.
 template <typename W>
-void press_stag_array(les_ctx* c) {
+void press_stag_array(les_ctx* c, int jt) {
     press_solve<W>(c, F(c, LES_F_U), F(c, LES_F_V), F(c, LES_F_W), WF<W>(c, LES_F_DIVTZ), WF<W>(c, LES_F_P),
-                   WF<W>(c, LES_F_DPDX), WF<W>(c, LES_F_DPDY), WF<W>(c, LES_F_DPDZ));
+                   WF<W>(c, LES_F_DPDX), WF<W>(c, LES_F_DPDY), WF<W>(c, LES_F_DPDZ), press_modi(c, jt));
 }
 
 // energy and d(ke)/dx_i added to the dudt diagnostics (103_flosol.f90:192-213,
@@ -591,7 +667,7 @@ void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     if (ovl) theta_post_overlap<W>(c, jt);
     else theta_post<W>(c, jt, tape);
     rhs_ab2_dev<W>(c, jt);
-    press_stag_array<W>(c);
+    press_stag_array<W>(c, jt);
     gradp_project_dev<W>(c, jt, !(ovl && theta_active(c, jt)));
     if (st) diagnostics_ke(c);
     filter_f<double>(c, F(c, LES_F_U), 0, nl);
@@ -684,8 +760,12 @@ int les_create(const les_params* prm, les_ctx** out) {
     if (!prm || !out) return LES_E_BADARG;
     *out = nullptr;
     if (!good_size(prm->nxt) || !good_size(prm->nyt) || prm->nzt < 4) return LES_E_BADARG;
-    if (prm->model != 5 || prm->ocean_flag || !prm->sgs || prm->lbc != 1 || prm->cs_count < 1) return LES_E_BADARG;
-    if (prm->ubc == 1 && prm->damping_method != 1) return LES_E_BADARG;
+    if (prm->model != 5 || !prm->sgs || prm->cs_count < 1) return LES_E_BADARG;
+    if (prm->lbc != 0 && prm->lbc != 1) return LES_E_BADARG;
+    // lbc = 0 reads ustar_avg and psi_h, which the ocean branch never sets on
+    // the surface ranks (010_scalars.f90:380-390, :696-716)
+    if (prm->lbc == 0 && prm->ocean_flag) return LES_E_BADARG;
+    if (prm->damping_method < 1 || prm->damping_method > 3) return LES_E_BADARG;
     if (3 * prm->nxt / 2 > 3072 || 3 * prm->nyt / 2 > 3072) return LES_E_BADARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return LES_E_NODEV;
@@ -724,8 +804,9 @@ int les_create(const les_params* prm, les_ctx** out) {
     const size_t F3 = c->plane * c->nlev;
     hipError_t e = hipSuccess;
     for (int f = 0; f < LES_N_FIELDS3D && e == hipSuccess; ++f) e = dalloc(c, &c->f3[f], F3);
-    double** surfs[5] = {&c->ustar, &c->psi_m, &c->phi_m, &c->psi_h, &c->phi_h};
-    for (int s = 0; s < 5 && e == hipSuccess; ++s) e = dalloc(c, surfs[s], c->plane);
+    double** surfs[6] = {&c->ustar, &c->psi_m, &c->phi_m, &c->psi_h, &c->phi_h, &c->ustar_top};
+    for (int s = 0; s < 6 && e == hipSuccess; ++s) e = dalloc(c, surfs[s], c->plane);
+    if (e == hipSuccess) e = dalloc(c, &c->dmean, 3 * c->nlev);
     const size_t B3 = c->plane2 * c->nlev;
     if (e == hipSuccess) e = dalloc(c, &c->u_big, B3);
     if (e == hipSuccess) e = dalloc(c, &c->v_big, B3);
@@ -771,8 +852,19 @@ int les_create(const les_params* prm, les_ctx** out) {
     les_note(c, hipMemsetAsync(c->ustar, 0, c->plane * 8, c->stream));
     fill_levels(c->stream, c->phi_m, c->plane, 0, 1, 1.0);
     fill_levels(c->stream, c->phi_h, c->plane, 0, 1, 1.0);
+    les_note(c, hipMemsetAsync(c->ustar_top, 0, c->plane * 8, c->stream));
+    les_note(c, hipMemsetAsync(c->dmean, 0, 3 * c->nlev * 8, c->stream));
     std::vector<double> sp = sponge_profile(c);
     les_note(c, hipMemcpyAsync(c->sponge, sp.data(), c->nlev * 8, hipMemcpyHostToDevice, c->stream));
+    std::vector<double> ust, vst;
+    if (!stokes_profile(c, ust, vst)) {
+        les_destroy(c);
+        return LES_E_BADARG;
+    }
+    les_note(c, hipMemcpyAsync(c->ust, ust.data(), c->nlev * 8, hipMemcpyHostToDevice, c->stream));
+    les_note(c, hipMemcpyAsync(c->vst, vst.data(), c->nlev * 8, hipMemcpyHostToDevice, c->stream));
+    c->wind_ustar = prm->u_star;      // 102_initial.f90:201, 246
+    c->wind_rad = prm->rad_stress;
     if (finish(c) != LES_OK) {
         les_destroy(c);
         return LES_E_HIP;
@@ -837,6 +929,22 @@ int les_set_sponge(les_ctx* c, const double* sp) {
     if (!c || !sp) return LES_E_BADARG;
     les_note(c, hipMemcpyAsync(c->sponge, sp, c->nlev * 8, hipMemcpyHostToDevice, c->stream));
     return finish(c);
+}
+
+int les_get_profiles(les_ctx* c, double* ust, double* vst, double* sponge) {
+    if (!c) return LES_E_BADARG;
+    if (ust) les_note(c, hipMemcpyAsync(ust, c->ust, c->nlev * 8, hipMemcpyDeviceToHost, c->stream));
+    if (vst) les_note(c, hipMemcpyAsync(vst, c->vst, c->nlev * 8, hipMemcpyDeviceToHost, c->stream));
+    if (sponge) les_note(c, hipMemcpyAsync(sponge, c->sponge, c->nlev * 8, hipMemcpyDeviceToHost, c->stream));
+    return finish(c);
+}
+
+// wallstress_bottom's dynamic wind stress (104_wallstress.f90:119-127)
+int les_set_wind_stress(les_ctx* c, double ustar, double agl_deg) {
+    if (!c) return LES_E_BADARG;
+    c->wind_ustar = ustar;
+    c->wind_rad = agl_deg / 180.0 * PI;   // deg2rad, 014_stokes_drift.f90:300-307
+    return LES_OK;
 }
 
 // FP32 work fields of the mixed mode are converted through an FP64 staging
@@ -1036,9 +1144,8 @@ int les_theta_all_in_one(les_ctx* c, int jt) {
     return finish(c);
 }
 int les_press_stag_array(les_ctx* c, int jt) {
-    (void)jt;
     if (!c) return LES_E_BADARG;
-    LES_DISPATCH(press_stag_array, c);
+    LES_DISPATCH(press_stag_array, c, jt);
     return finish(c);
 }
 // Admas_Bashforth (103_flosol.f90:319-389) on the device fields: Euler at

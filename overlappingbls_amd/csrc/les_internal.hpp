@@ -95,6 +95,9 @@ struct les_ctx {
     double* st_mean;                // plane means of u, v, theta (3 nlev)
     double *st_run, *st_win;        // LES_N_SLICES slices of nxt x (nzt-1)
     double *ustar, *psi_m, *phi_m, *psi_h, *phi_h;  // one plane each
+    double* ustar_top;              // ocean obukhov: ustar_avg of the top wall (one plane)
+    double wind_ustar, wind_rad;    // ocean wind stress (u_star / u_scale, radians)
+    double* dmean;                  // plane means of u, v, w for damping methods 2, 3 (3 nlev)
     double *u_big, *v_big, *w_big;                  // nlev planes of nx2*ny2
     // wavenumber tables (half spectrum, 002_fft.f90:143-167)
     double *kx2d, *ky2d, *k2, *Gtest, *Gtest2;
@@ -284,9 +287,9 @@ void step_scalar_dev(les_ctx* c);
 template <typename W> void press_rhs_prep_dev(les_ctx* c, W* rx, W* ry, W* rz);
 template <typename W>
 void press_rhs_prep_on(les_ctx* c, W* rx, W* ry, W* rz, const double* u, const double* v, const double* w,
-                       const W* divtz);
+                       const W* divtz, double modi);
 template <typename W> void dpdz_on(les_ctx* c, W* dpdz, const W* p);
-template <typename W> void press_hz_sums_dev(les_ctx* c, double* out);
+template <typename W> void press_hz_sums_dev(les_ctx* c, double* out, double modi);
 template <typename W> void dpdz_dev(les_ctx* c);
 // RHS assembly + Admas_Bashforth, and RHS -= grad p + project, each one pass
 template <typename W> void rhs_ab2_dev(les_ctx* c, int jt);
@@ -300,9 +303,11 @@ constexpr int MAX3_BLOCKS = 2048;  // partial maxima per component (c->red + 64)
 void max3_dev(les_ctx* c, double* out, bool lag = false);
 
 // ---- forward step and the linearised step (les_api.hip, les_lin.hip) ---------
+// modi: lbc = 0's surface-temperature term of the bottom row
+// (109_press_stag_array.f90:186-191), 0 otherwise
 template <typename W>
 void press_solve(les_ctx* c, const double* u, const double* v, const double* w, const W* divtz, W* pf, W* dpdx,
-                 W* dpdy, W* dpdz);
+                 W* dpdy, W* dpdz, double modi = 0.0);
 // one FP64 flosol step; tape != null records the values the linearised step
 // reads; stats = false skips the time-averaged slices
 void step_fp64(les_ctx* c, int jt, Tape* tape, bool stats);

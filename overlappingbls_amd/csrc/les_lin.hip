@@ -619,6 +619,8 @@ int lin_check_supported(const les_ctx* c, int jt0, int nsteps) {
     if (c->precision != LES_FP64) return LES_E_BADARG;
     if (!P.theta_flag || jt0 <= P.theta_init_time || jt0 < 2 || nsteps < 0) return LES_E_BADARG;
     if (P.lbc_mom != LES_BC_WALL || P.ubc_mom != LES_BC_STRESS_FREE) return LES_E_BADARG;
+    // linearised: the land step with a prescribed surface flux and damping method 1
+    if (P.ocean_flag || P.lbc != 1 || (P.ubc == 1 && P.damping_method != 1)) return LES_E_BADARG;
     return LES_OK;
 }
 

@@ -11,10 +11,12 @@ implicit none
 private
 
 integer(c_int), parameter, public :: LES_OK = 0, LES_E_BADARG = -1, LES_E_HIP = -2, &
-                                     LES_E_CFL = -3, LES_E_NOMEM = -4, LES_E_NODEV = -5
+                                     LES_E_CFL = -3, LES_E_NOMEM = -4, LES_E_NODEV = -5, &
+                                     LES_E_IO = -6
 integer(c_int), parameter, public :: LES_FP64 = 0, LES_MIXED = 1
 integer(c_int), parameter, public :: LES_WALL_ROUGH = 0, LES_WALL_SMOOTH = 1
 integer(c_int), parameter, public :: LES_BC_WALL = 0, LES_BC_STRESS_FREE = 1
+integer(c_int), parameter, public :: LES_STOKES_DP = 0, LES_STOKES_MONO = 1, LES_STOKES_DYNAMIC = 2
 
 ! field identifiers (enum les_field)
 integer(c_int), parameter, public :: LES_F_U = 0, LES_F_V = 1, LES_F_W = 2, LES_F_THETA = 3, &
@@ -27,7 +29,12 @@ integer(c_int), parameter, public :: LES_F_U = 0, LES_F_V = 1, LES_F_W = 2, LES_
     LES_F_P = 30, LES_F_DPDX = 31, LES_F_DPDY = 32, LES_F_DPDZ = 33, &
     LES_F_CS_OPT2 = 34, LES_F_F_LM = 35, LES_F_F_MM = 36, LES_F_F_QN = 37, LES_F_F_NN = 38, &
     LES_F_NU_T = 39, LES_F_U_LAG = 40, LES_F_V_LAG = 41, LES_F_W_LAG = 42, &
-    LES_F_SGS_T3 = 56, LES_F_PSI_M_SFC = 101
+    LES_F_S11 = 43, LES_F_S12 = 44, LES_F_S13 = 45, LES_F_S22 = 46, LES_F_S23 = 47, LES_F_S33 = 48, &
+    LES_F_CX = 49, LES_F_CY = 50, LES_F_CZ = 51, &
+    LES_F_DTDX = 52, LES_F_DTDY = 53, LES_F_DTDZ = 54, LES_F_BETA_SCAL = 55, LES_F_SGS_T3 = 56, &
+    LES_F_WORK0 = 57, LES_F_WORK1 = 58, LES_F_WORK2 = 59, &
+    LES_F_USTAR_SFC = 100, LES_F_PSI_M_SFC = 101, LES_F_PHI_M_SFC = 102, LES_F_PSI_H_SFC = 103, &
+    LES_F_PHI_H_SFC = 104, LES_F_USTAR_TOP_SFC = 105
 
 ! time-averaged slice identifiers (enum les_slice): flow_slice then theta_slice
 integer(c_int), parameter, public :: LES_S_U = 0, LES_S_V = 1, LES_S_W = 2, LES_S_P = 3, &
@@ -44,9 +51,12 @@ type, bind(C), public :: les_params
     integer(c_int32_t) :: nxt, nyt, nzt, model, cs_count, dyn_init, inilag, restart_vel, sgs, molec, &
                           walltype, lbc_mom, ubc_mom, ubc, damping_method, coriolis_forcing, &
                           use_mean_p_force, ocean_flag, theta_flag, restart_theta, theta_init_time, &
-                          passive_scalar, lbc, pad0
+                          passive_scalar, lbc, stokes_flag
     real(c_double) :: lx_tot, ly_tot, lz_tot, dt, z_i, u_scale, T_scale, nu_molec, zo, d0, Pr, wt_s, &
                       dTdz_top, coriol, ug, vg, mean_p_force, lz_npz
+    integer(c_int32_t) :: stokes_action, flag_crossvel
+    real(c_double) :: u_star, rad_stress, alpha_w, u_cross, v_cross, amp_w, lambda_w, agl_ust, &
+                      inv_strength, T_s, Co
 end type les_params
 
 public :: les_create, les_destroy, les_set_field, les_get_field, les_step, les_run, &
@@ -59,6 +69,8 @@ public :: les_create, les_destroy, les_set_field, les_get_field, les_step, les_r
           les_lin_set, les_lin_get, les_lin_zero, les_set_sgs_frozen, les_tlm_window, &
           les_adjoint_window, les_step_tlm, les_step_adj, &
           les_write_restart, les_read_restart, les_write_avgslice, &
+          les_lagrange_sdep, les_interpolag_sdep, les_set_sgs_init_flags, les_get_sgs_init_flags, &
+          les_set_sponge, les_set_wind_stress, les_get_profiles, les_sync, &
           les_check
 
 interface
@@ -80,6 +92,46 @@ interface
         import :: c_int, c_ptr, c_double
         type(c_ptr), value :: ctx
         real(c_double), intent(in) :: ust(*), vst(*)
+    end function
+    integer(c_int) function les_set_sponge(ctx, sponge) bind(C, name="les_set_sponge")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: ctx
+        real(c_double), intent(in) :: sponge(*)
+    end function
+    integer(c_int) function les_get_profiles(ctx, ust, vst, sponge) bind(C, name="les_get_profiles")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: ctx
+        real(c_double), intent(out) :: ust(*), vst(*), sponge(*)
+    end function
+    integer(c_int) function les_set_wind_stress(ctx, ustar, agl_deg) bind(C, name="les_set_wind_stress")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: ctx
+        real(c_double), value :: ustar, agl_deg
+    end function
+    integer(c_int) function les_set_sgs_init_flags(ctx, f_lm_mm_init, f_qn_nn_init) &
+            bind(C, name="les_set_sgs_init_flags")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: f_lm_mm_init, f_qn_nn_init
+    end function
+    integer(c_int) function les_get_sgs_init_flags(ctx, f_lm_mm_init, f_qn_nn_init) &
+            bind(C, name="les_get_sgs_init_flags")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), intent(out) :: f_lm_mm_init, f_qn_nn_init
+    end function
+    integer(c_int) function les_lagrange_sdep(ctx, jt) bind(C, name="les_lagrange_sdep")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: jt
+    end function
+    integer(c_int) function les_interpolag_sdep(ctx) bind(C, name="les_interpolag_sdep")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function les_sync(ctx) bind(C, name="les_sync")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
     end function
     integer(c_int) function les_set_field(ctx, f, h, ld, lev0, nlev) bind(C, name="les_set_field")
         import :: c_int, c_ptr, c_double

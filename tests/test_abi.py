@@ -47,11 +47,26 @@ def test_oracle_exports_the_same_surface(oracle_mod):
     assert not missing, missing
 
 
-def test_params_struct_layout():
+def test_params_struct_layout(tmp_path):
+    """The ctypes mirror has the C header's size and field offsets (checked
+    against gcc's layout of include/les_gpu.h)."""
+    import shutil
     from overlappingbls_amd.abi import LesParams
-    # 24 int32 then 18 doubles, no implicit padding
-    assert C.sizeof(LesParams) == 24 * 4 + 18 * 8
-    assert LesParams.lx_tot.offset == 96
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    names = [n for n, _ in LesParams._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "les_gpu.h"\nint main(void) {\n'
+                   '  printf("%zu\\n", sizeof(les_params));\n'
+                   + "".join(f'  printf("%zu\\n", offsetof(les_params, {n}));\n' for n in names)
+                   + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    vals = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert vals[0] == C.sizeof(LesParams)
+    for n, off in zip(names, vals[1:]):
+        assert getattr(LesParams, n).offset == off, n
 
 
 def test_config_nondimensionalisation():

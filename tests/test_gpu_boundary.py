@@ -57,15 +57,33 @@ def test_bind_upload_download(gpu_lib):
     assert np.array_equal(u, 3.0 * config.uniform(3, 0, sh))
 
 
-def test_namelist_options_the_step_does_not_support_are_rejected(gpu_lib):
-    """param.nml options outside the step's scope reach les_create, which
-    refuses them (the ocean branch, prescribed surface temperature, damping
-    methods 2 and 3 with ubc = 1)."""
-    from overlappingbls_amd import config
+def test_options_the_step_does_not_support_are_rejected(gpu_lib):
+    """les_create refuses the physics options it does not implement: other
+    SGS models, sgs = .false., the Donelan-Pierson Stokes spectrum, and lbc
+    = 0 in the ocean (whose surface ranks never set ustar_avg, psi_h)."""
     from overlappingbls_amd.abi import LES_E_BADARG, LesError
-    base = "&domain_param nxt = 16, nyt = 16, nzt = 9 /\n"
-    for extra in ("&flow_param ocean_flag = .true. /", "&temperature_param lbc = 0 /",
-                  "&flow_param ubc = 1, damping_method = 2 /"):
+    base = ("&domain_param nxt = 16, nyt = 16, nzt = 9 /\n&canopy_param flag_canopy = .false. /\n"
+            "&ocean_param stokes_flag = .false. /\n")
+    for extra in ("&flow_param model = 2 /", "&flow_param sgs = .false. /",
+                  "&ocean_param stokes_flag = .true., stokes_action = 'donelan.pierson1987' /\n"
+                  "&flow_param ubc_mom = 'stress free' /",
+                  "&flow_param ocean_flag = .true. /\n&temperature_param lbc = 0 /"):
         with pytest.raises(LesError) as e:
             gpu_lib.create(config.read_namelist(base + extra).params())
         assert e.value.status == LES_E_BADARG, extra
+
+
+def test_reference_configurations_are_accepted(gpu_lib):
+    """The options of the reference's own param.nml files (ocean branch,
+    wall top, damping method 3, monochromatic Stokes drift) and lbc = 0 on
+    land create a context."""
+    base = "&domain_param nxt = 16, nyt = 16, nzt = 9 /\n&canopy_param flag_canopy = .false. /\n"
+    for extra in ("&flow_param ocean_flag = .true., lbc_mom = 'stress free', ubc_mom = 'wall', ubc = 0,"
+                  " damping_method = 3 /\n&ocean_param stokes_flag = .true., stokes_action = 'mono',"
+                  " amp_w = 1.13d0, lambda_w = 60.d0 /",
+                  "&flow_param ubc_mom = 'stress free' /\n&ocean_param stokes_flag = .false. /\n"
+                  "&temperature_param lbc = 0, theta_flag = .true. /",
+                  "&flow_param ubc = 1, ubc_mom = 'stress free', damping_method = 2 /\n"
+                  "&ocean_param stokes_flag = .false. /"):
+        g = gpu_lib.create(config.read_namelist(base + extra).params())
+        g.close()

@@ -36,12 +36,15 @@ TEXT = """
   theta_flag = .true., restart_theta = .false., theta_init_time = 3,
   T_scale = 300.d0, wt_s = -0.005d0, dTdz_top = 0.003, lbc = 1
 /
+&canopy_param flag_canopy = .false. /
+&ocean_param stokes_flag = .false. /
 """
 
 
 def test_parse_groups_values_and_comments():
     g = config.parse_namelist(TEXT)
-    assert set(g) == {"mpi_param", "domain_param", "time_param", "flow_param", "temperature_param"}
+    assert set(g) == {"mpi_param", "domain_param", "time_param", "flow_param", "temperature_param",
+                      "canopy_param", "ocean_param"}
     assert g["mpi_param"] == {"use_mpi": True, "npy": 8, "npz": 4}
     assert g["domain_param"]["nyt"] == 96 and g["domain_param"]["ly_tot"] == 450.0
     assert g["flow_param"]["ubc_mom"] == "stress free" and g["flow_param"]["walltype"] == "smooth"
@@ -64,8 +67,65 @@ def test_read_namelist_into_the_parameter_block():
     assert math.isclose(p.lz_npz, 100.0 / 4 / 100.0 * 4)
 
 
-def test_unsupported_options_pass_through():
-    nl = config.read_namelist("&flow_param ocean_flag = .true., damping_method = 3 /\n"
-                              "&temperature_param lbc = 0 /")
+def test_options_outside_the_step_are_refused():
+    """Options that select physics outside the timestep raise at params():
+    the reference's defaults apply to groups a param.nml leaves out, and its
+    default flag_canopy is .true. (001_param.f90:205)."""
+    import pytest
+    base = "&domain_param nxt = 16, nyt = 16, nzt = 9 /\n"
+    with pytest.raises(config.UnsupportedOption, match="flag_canopy"):
+        config.read_namelist(base).params()
+    for extra in ("&flow_param dns_bc = .true. /", "&flow_param inflow = .true. /",
+                  "&flow_param flag_relaxForce = .true. /", "&con_param pcon_flag = .true. /",
+                  "&temperature_param flag_dynWT = .true. /", "&temperature_param spatial_flux_flag = .true. /",
+                  "&ocean_param stokes_action = 'read' /"):
+        with pytest.raises(config.UnsupportedOption):
+            config.read_namelist(base + "&canopy_param flag_canopy = .false. /\n" + extra).params()
+
+
+def test_model_sgs_and_physics_options_reach_the_parameter_block():
+    """model, sgs, the ocean branch, lbc, damping and Stokes options map into
+    les_params, where les_create accepts or rejects them."""
+    nl = config.read_namelist("&flow_param model = 2, sgs = .false., ocean_flag = .true., damping_method = 3,"
+                              " ubc = 1, u_star = 0.02, u_scale = 0.01, freq_coriolis = 1e-4 /\n"
+                              "&temperature_param lbc = 0, T_scale = 300.d0, theta_flag = .true. /\n"
+                              "&canopy_param flag_canopy = .false. /\n"
+                              "&ocean_param alpha_w = 2.0d-4, agl_stress = 30.d0, stokes_action = 'mono' /")
     p = nl.params()
-    assert (p.ocean_flag, p.damping_method, p.lbc) == (1, 3, 0)
+    assert (p.model, p.sgs, p.ocean_flag, p.damping_method, p.lbc, p.ubc) == (2, 0, 1, 3, 0, 1)
+    assert (p.stokes_flag, p.stokes_action) == (1, 1)       # the reference's default stokes_flag
+    assert math.isclose(p.u_star, 2.0)                       # u_star / u_scale (102_initial.f90:201)
+    assert math.isclose(p.alpha_w, 2.0e-4 * 300.0)           # alpha_w T_scale (:228-230)
+    assert math.isclose(p.rad_stress, math.pi / 6)           # agl_stress in radians (:246)
+    assert math.isclose(p.coriol, 1e-4 * nl.z_i / 0.01)      # the ocean always sets coriol (:235-238)
+    assert math.isclose(p.T_s, 288.0 / 300.0)                # theta_s1 / T_scale (008_bottombc.f90:113)
+
+
+def test_mean_pressure_force_default_and_zero():
+    """mean_p_force is 1/lz_tot only when the namelist leaves it unset
+    (float_missing, 102_initial.f90:214-219); an explicit 0 stays 0."""
+    base = "&domain_param lz_tot = 200.d0, z_i = 100.d0 /\n&canopy_param flag_canopy = .false. /\n"
+    p = config.read_namelist(base + "&flow_param use_mean_p_force = .true. /").params()
+    assert math.isclose(p.mean_p_force, 1.0 / 2.0)
+    p = config.read_namelist(base + "&flow_param use_mean_p_force = .true., mean_p_force = 0.0d0 /").params()
+    assert p.mean_p_force == 0.0
+    p = config.read_namelist(base + "&flow_param use_mean_p_force = .true., mean_p_force = 2.0d-4,"
+                                    " u_scale = 0.5 /").params()
+    assert math.isclose(p.mean_p_force, 2.0e-4 / 0.25 * 100.0)
+
+
+def test_reference_param_files():
+    """The three param.nml files the reference ships (its own configurations)
+    read into parameter blocks: the ocean branch with a wind-stressed
+    surface, a wall top, damping method 3 (inactive, ubc = 0) and, for SBL,
+    monochromatic Stokes drift."""
+    import os
+    import pytest
+    root = "/root/reference"
+    if not os.path.isdir(root):
+        pytest.skip("the reference tree is not present")
+    for case, stokes in (("SBL", 1), ("SBBL", 1), ("BBL", 0)):
+        p = config.read_namelist(os.path.join(root, case, "readin", "param.nml")).params()
+        assert (p.ocean_flag, p.lbc_mom, p.ubc_mom, p.ubc, p.damping_method) == (1, 1, 0, 0, 3), case
+        assert (p.stokes_flag, p.model, p.sgs) == (stokes, 5, 1), case
+        assert math.isclose(p.alpha_w, 2.0e-4 * 300.0), case
