@@ -282,6 +282,33 @@ def workload(name: str) -> Namelist:
                         coriolis_forcing=True, ug_dim=1.0, **base)
     if name == "small":
         return Namelist(nxt=32, nyt=32, nzt=17, cs_count=3, dyn_init=3, wt_s=-0.005 * 300.0, **base)
+    # the reference's own configurations (SBL/SBBL/BBL param.nml): the ocean
+    # branch with a wind-stressed surface and a rough wall top, monochromatic
+    # Stokes drift in its finite-depth form
+    if name == "ocean_sbl":
+        return Namelist(nxt=32, nyt=32, nzt=17, cs_count=3, dyn_init=3, ocean_flag=True, lbc_mom="stress free",
+                        ubc_mom="wall", u_star=1.0, agl_stress=20.0, freq_coriolis=1e-4, ug_dim=0.0,
+                        vg_dim=0.0, alpha_w=2.0e-4, stokes_flag=True, stokes_action="mono", amp_w=1.13,
+                        lambda_w=60.0, agl_Ust=10.0, wt_s=0.01 * 300.0, **base)
+    # deep ocean: stress-free floor, deep-water Stokes drift, damping method 3,
+    # cross flow
+    if name == "ocean_deep":
+        return Namelist(nxt=24, nyt=12, nzt=9, cs_count=2, dyn_init=2, ocean_flag=True, lbc_mom="stress free",
+                        ubc_mom="stress free", u_star=1.0, agl_stress=-30.0, freq_coriolis=1e-2, ug_dim=0.3,
+                        vg_dim=-0.2, alpha_w=2.0e-4, stokes_flag=True, stokes_action="mono", amp_w=0.5,
+                        lambda_w=40.0, agl_Ust=-15.0, ubc=1, damping_method=3, inv_strength=1.0,
+                        flag_crossVel=True, udim_cross=0.2, vdim_cross=0.1, wt_s=-0.005 * 300.0, **base)
+    # land: prescribed surface temperature (lbc = 0), smooth walls at both ends
+    if name == "land_lbc0":
+        return Namelist(nxt=16, nyt=16, nzt=9, cs_count=2, dyn_init=2, lbc=0, walltype="smooth",
+                        ubc_mom="wall", coriolis_forcing=True, ug_dim=1.0, **base)
+    # land: damping method 2 under a stress-free top, with deep-water Stokes drift
+    if name == "land_damp2":
+        return Namelist(nxt=16, nyt=24, nzt=9, cs_count=2, dyn_init=2, ubc=1, damping_method=2,
+                        stokes_flag=True, stokes_action="mono", amp_w=0.5, lambda_w=30.0, agl_Ust=40.0,
+                        wt_s=0.01 * 300.0, **base)
+    if name == "deep16":      # many levels on a small plane: the TDMA's LDS and HBM levels both run
+        return Namelist(nxt=16, nyt=16, nzt=129, cs_count=2, dyn_init=2, wt_s=-0.005 * 300.0, **base)
     if name == "wide":        # production line lengths 256 / 384, few levels
         return Namelist(nxt=256, nyt=256, nzt=5, cs_count=2, dyn_init=2, wt_s=-0.005 * 300.0, **base)
     if name == "wide_rect":   # 512 / 768 along x, 128 / 192 along y
@@ -325,13 +352,14 @@ def gaussian(seed: int, field_id: int, shape) -> np.ndarray:
     return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
 
 
-def initial_fields(nl: Namelist, seed: int = 1234) -> dict:
+def initial_fields(nl: Namelist, seed: int = 1234, vw_noise: bool = False) -> dict:
     """Seeded synthetic initial state, levels 0..nzt (ghost levels BOGUS as
     data_exchange leaves them, 202_parallel.f90:722-733).
 
-    u = (1/0.4) ln(z/z0) (1 + 0.1 U(-1,1)) for z < 0.5, v = w = 0 plus the
-    same-amplitude noise on v and w below z = 0.5; theta = 1 + gamma max(0,
-    z - 0.3) (+ N(0, 0.05) noise for a passive scalar).
+    u = (1/0.4) ln(z/z0) (1 + 0.1 U(-1,1)) for z < 0.5 and v = w = 0 (the
+    benchmark configurations); vw_noise adds noise of the same amplitude to v
+    and a tenth of it to w below z = 0.5.  theta = 1 + gamma max(0, z - 0.3)
+    (+ N(0, 0.05) noise for a passive scalar).
     """
     p = nl.params()
     nx, ny, nz = p.nxt, p.nyt, p.nzt
@@ -347,6 +375,9 @@ def initial_fields(nl: Namelist, seed: int = 1234) -> dict:
     u = prof[:, None, None] * (1.0 + 0.1 * low * uniform(seed, 1, shape))
     v = 0.1 * prof[:, None, None] * low * uniform(seed, 2, shape)
     w = 0.1 * np.maximum(prof, 0.0)[:, None, None] * loww * uniform(seed, 3, shape) * 0.1
+    if not vw_noise:
+        v = np.zeros(shape)
+        w = np.zeros(shape)
     gamma = p.dTdz_top / p.T_scale * p.z_i
     theta = 1.0 + gamma * np.maximum(0.0, zuvp - 0.3)[:, None, None] + 0.0 * u
     if nl.passive_scalar:

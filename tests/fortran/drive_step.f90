@@ -12,7 +12,8 @@ real(c_double), allocatable :: u(:,:,:)
 prm = les_params(nxt, nyt, nzt, 5, 2, 2, 1, 0, 1, 0, LES_WALL_ROUGH, LES_BC_WALL, LES_BC_STRESS_FREE, &
                  0, 1, 0, 0, 0, 1, 0, 1, 0, 1, 0, &
                  6.283185307179586d0, 6.283185307179586d0, 1d0, 2d-4, 100d0, 1d0, 300d0, 1.14d-5, &
-                 1d-4, 0d0, 0.4d0, 0d0, 0.03d0, 0d0, 0d0, 0d0, 0d0, 1d0)
+                 1d-4, 0d0, 0.4d0, 0d0, 0.03d0, 0d0, 0d0, 0d0, 0d0, 1d0, &
+                 LES_STOKES_MONO, 0, 0d0, 0d0, 2d-4, 0d0, 0d0, 0d0, 1d0, 0d0, 0d0, 0.96d0, 0.16d0)
 st = les_create(prm, ctx)
 call les_check(st, "les_create")
 allocate(u(ldx, nyt, 0:nzt))

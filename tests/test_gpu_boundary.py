@@ -18,7 +18,9 @@ def spectra(nz, ny, lhx, seed):
     return [config.uniform(seed, 2 * i, sh) + 1j * config.uniform(seed, 2 * i + 1, sh) for i in range(3)]
 
 
-@pytest.mark.parametrize("case", ["tiny_ragged", "mid", "wide"])
+# deep16 (nzt = 129) runs both the levels the TDMA keeps in LDS and the ones
+# whose d' goes through HBM, and the boundary between them
+@pytest.mark.parametrize("case", ["tiny_ragged", "mid", "wide", "deep16"])
 def test_tdma_solve(case, oracle_mod, gpu_lib):
     nl, o, g = pair(case, oracle_mod, gpu_lib)
     p = nl.params()

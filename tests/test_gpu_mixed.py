@@ -22,7 +22,9 @@ from test_gpu_parity import TOL, compare_state, pair, rand_field, rel, sync, PRO
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["tiny_ragged", "small", "mid", "mid_rect"]
+CASES = ["tiny_ragged", "small", "mid", "mid_rect", "wide", "wide_rect", "deep16"]
+# the ocean branch, lbc = 0 and damping methods 2/3 in the mixed step
+STEP_CASES = CASES + ["ocean_sbl", "ocean_deep", "land_lbc0", "land_damp2"]
 
 MIX_FFT = 1e-6     # FP32 unit roundoff 6e-8 times the log2 n growth of a transform pair
 MIX_WORK = 2e-5    # products and differences of FP32 fields (Cs_opt2 is a ratio of contractions)
@@ -142,7 +144,7 @@ def test_mixed_routines(case, oracle_mod, gpu_lib):
     assert not over, over
 
 
-@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("case", STEP_CASES)
 def test_mixed_steps(case, oracle_mod, gpu_lib):
     """2*cs_count+1 free-running mixed steps (two dynamic-SGS updates)
     against the FP64 oracle: prognostic state and mean profiles."""

@@ -28,6 +28,12 @@ TOL = 1e-12
 # mid_rect the specialised ones at line lengths 64..192, wide and wide_rect at
 # the production lengths 256..768 (few levels, so the oracle stays fast)
 CASES = ["tiny", "tiny_ragged", "small", "mid", "mid_rect", "wide", "wide_rect"]
+# the other branches: the reference's own ocean configuration (wind-stressed
+# surface, rough wall top, finite-depth Stokes drift), a deep ocean with
+# damping method 3 and cross flow, land with lbc = 0 between smooth walls,
+# land with damping method 2 and Stokes drift
+BRANCH_CASES = ["ocean_sbl", "ocean_deep", "land_lbc0", "land_damp2"]
+STEP_CASES = CASES + BRANCH_CASES
 
 
 def rel(a, b):
@@ -107,7 +113,7 @@ def compare_state(o, g, names, tol=TOL):
     return bad
 
 
-@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("case", STEP_CASES)
 def test_routines_in_flosol_order(case, oracle_mod, gpu_lib):
     """Drive flosol's call order routine by routine (103_flosol.f90:23-229)
     on both sides and compare each routine's outputs."""
@@ -125,7 +131,7 @@ def test_routines_in_flosol_order(case, oracle_mod, gpu_lib):
             ctx.filter_data(n)
     for ctx in (o, g):
         ctx.obukhov(jt)
-    for n in ("USTAR_SFC", "PSI_M_SFC", "PHI_M_SFC", "PSI_H_SFC", "PHI_H_SFC"):
+    for n in ("USTAR_SFC", "PSI_M_SFC", "PHI_M_SFC", "PSI_H_SFC", "PHI_H_SFC", "USTAR_TOP_SFC"):
         assert rel(g.get(n), o.get(n)) < TOL, n
     for ctx in (o, g):
         ctx.ddx("DUDX", "U"); ctx.ddy("DUDY", "U")
@@ -224,8 +230,8 @@ def sensitivity(oracle_mod, p, state, jt, eps_in=1e-10):
 class Noise:
     """The oracle's rounding variants, run alongside the oracle."""
 
-    def __init__(self, oracle_mod, p):
-        self.ctxs = [oracle_mod.create(p, variant=v) for v in ("fma", "r2")]
+    def __init__(self, oracle_mod, p, variants=("fma", "r2")):
+        self.ctxs = [oracle_mod.create(p, variant=v) for v in variants]
 
     def load(self, fields):
         for c in self.ctxs:
@@ -248,11 +254,11 @@ class Noise:
         return max(rel(get(c), ref) for c in self.ctxs)
 
 
-def noise_ctx(oracle_mod, p):
-    return Noise(oracle_mod, p)
+def noise_ctx(oracle_mod, p, variants=("fma", "r2")):
+    return Noise(oracle_mod, p, variants)
 
 
-@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("case", STEP_CASES)
 def test_steps_resynced(case, oracle_mod, gpu_lib):
     """flosol per step from the oracle's state, through two dynamic-SGS
     updates (cs_count = 2 or 3): state fields at TOL (or NOISE times the
@@ -285,7 +291,7 @@ def test_steps_resynced(case, oracle_mod, gpu_lib):
     assert o.sgs_init_flags() == g.sgs_init_flags() == (1, 1)
 
 
-@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("case", STEP_CASES)
 def test_steps_free_running(case, oracle_mod, gpu_lib):
     """Both sides run 2*cs_count+1 steps on their own; the prognostic state
     stays within 1e-10 (rounding growth over the window), or NOISE times the

@@ -508,3 +508,166 @@ def test_theta_uniform_advection_and_buoyancy(oracle_mod):
     # first step is Euler (jt == theta_init_time): theta += dt RHS
     th1 = o.get("THETA")
     assert np.abs(th1[3] - (th[3] + P.dt * exp)).max() < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# ocean branch, Stokes drift, damping profiles, lbc = 0
+# ---------------------------------------------------------------------------
+def test_stokes_drift_monochromatic(oracle_mod):
+    """init_stokes_drift (014_stokes_drift.f90:29-115).  Under a stress-free
+    top the deep-water drift of a monochromatic wave, computed here in
+    dimensional units: U_s = omega k a^2 exp(-2 k z), omega^2 = g k, then
+    divided by u_scale.  Under a wall top (mono_shallow_water) it tends to
+    the same profile when the depth is many wavelengths."""
+    amp, lam, agl = 0.8, 40.0, 25.0
+    nl, P, o = mk(oracle_mod, 16, 16, 17, ubc_mom="stress free", stokes_flag=True, stokes_action="mono",
+                  amp_w=amp, lambda_w=lam, agl_Ust=agl, u_scale=0.5)
+    ust, vst, _ = o.profiles()
+    k = 2 * math.pi / lam
+    us_dim = math.sqrt(G * k) * k * amp * amp
+    z = (np.arange(P.nzt + 1) - 0.5) * (nl.lz_tot / (P.nzt - 1))      # metres
+    prof = us_dim / nl.u_scale * np.exp(-2 * k * z)
+    assert np.allclose(ust, prof * math.cos(math.radians(agl)), rtol=1e-12, atol=0)
+    assert np.allclose(vst, prof * math.sin(math.radians(agl)), rtol=1e-12, atol=0)
+    # wall top at depth 100 m (k H = 15.7): the finite-depth form differs from
+    # the deep-water one by terms of relative size exp(-4 k (H - z)) and
+    # 1 - tanh(k H) ~ 5e-14, far below the surface value
+    nl2, P2, o2 = mk(oracle_mod, 16, 16, 17, ubc_mom="wall", stokes_flag=True, stokes_action="dynamic",
+                     amp_w=amp, lambda_w=lam, agl_Ust=agl, u_scale=0.5)
+    ust2, vst2, _ = o2.profiles()
+    assert np.allclose(ust2[1:-1], ust[1:-1], rtol=1e-12, atol=1e-12 * ust[1])
+    assert np.allclose(vst2[1:-1], vst[1:-1], rtol=1e-12, atol=1e-12 * abs(vst[1]))
+    # no drift without stokes_flag
+    nl3, P3, o3 = mk(oracle_mod, stokes_flag=False)
+    assert not any(a.any() for a in o3.profiles()[:2])
+
+
+def test_sponge_profiles(oracle_mod):
+    """setsponge (007_topbc.f90:15-85) at the uvp heights z = (k - 1/2) dz:
+    method 2 a raised cosine from 0.75 lz to lz reaching cfrdmp = 3.9;
+    method 3 ell^2 sigma_max above 0.8 lz, sigma_max = fac N / (2 pi) with N
+    the Brunt-Vaisala frequency of the inversion in units of u_scale/z_i
+    (dimensional here), fac = 1.5 on land and 0.22 in the ocean (N^2 = g
+    alpha_w dtheta/dz).  The reference writes fac = 0.22 and 8.0*atan(1.0)
+    as default-real (single precision) literals."""
+    gamma = 0.01
+    for method, ocean in ((2, False), (3, False), (3, True)):
+        nl, P, o = mk(oracle_mod, 16, 16, 33, ubc=1, ubc_mom="stress free", damping_method=method,
+                      inv_strength=gamma, ocean_flag=ocean, u_scale=0.4)
+        sp = o.profiles()[2]
+        L = P.lz_tot
+        z = (np.arange(P.nzt + 1) - 0.5) * (L / (P.nzt - 1))
+        exp = np.zeros(P.nzt + 1)
+        if method == 2:
+            top = (z >= 0.75 * L) & (z <= L)
+            exp[top] = 3.9 * 0.5 * (1 - np.cos(math.pi * (z[top] - 0.75 * L) / (0.25 * L)))
+            exp[P.nzt] = 0.0
+        else:
+            N_dim = math.sqrt(G * (nl.alpha_w if ocean else 1.0 / nl.T_scale) * gamma)
+            fac = float(np.float32(0.22)) if ocean else 1.5
+            two_pi = float(np.float32(8.0) * np.arctan(np.float32(1.0)))
+            smax = fac * N_dim * nl.z_i / nl.u_scale / two_pi
+            top = (z >= 0.8 * L) & (z <= L)
+            exp[top] = ((z[top] - 0.8 * L) / (0.2 * L)) ** 2 * smax
+        exp[0] = 0.0
+        assert np.allclose(sp, exp, rtol=1e-12, atol=1e-300), (method, ocean)
+        assert sp.max() > 0
+    nl, P, o = mk(oracle_mod, ubc=0, damping_method=3, inv_strength=gamma)
+    assert not o.profiles()[2].any()                 # ubc = 0: no sponge (102_initial.f90:45-50)
+
+
+def test_ocean_wallstress_bottom(oracle_mod):
+    """Ocean surface (104_wallstress.f90:118-136): tau = u*^2 along the wind
+    direction; dudz, dvdz of level 1 are level 2's shear projected on it;
+    les_set_wind_stress replaces u* and the angle."""
+    nl, P, o = mk(oracle_mod, ocean_flag=True, u_star=0.3, agl_stress=20.0, u_scale=0.5)
+    sh = (P.nzt + 1, P.nyt, P.nxt)
+    rng = np.random.default_rng(5)
+    du, dv = rng.random(sh), rng.random(sh) - 0.5
+    o.set("DUDZ", du); o.set("DVDZ", dv)
+    o.wallstress_bottom()
+    us, a = 0.3 / 0.5, math.radians(20.0)
+    assert np.allclose(o.get("TXZ", 1, 1), us * us * math.cos(a), rtol=1e-14)
+    assert np.allclose(o.get("TYZ", 1, 1), us * us * math.sin(a), rtol=1e-14)
+    proj = du[2] * math.cos(a) + dv[2] * math.sin(a)
+    assert np.allclose(o.get("DUDZ", 1, 1)[0], proj * math.cos(a), rtol=1e-13)
+    assert np.allclose(o.get("DVDZ", 1, 1)[0], proj * math.sin(a), rtol=1e-13)
+    o.set_wind_stress(0.25, -45.0)
+    o.wallstress_bottom()
+    assert np.allclose(o.get("TYZ", 1, 1), -0.25**2 * math.sqrt(0.5), rtol=1e-14)
+
+
+@pytest.mark.parametrize("smooth", [False, True])
+def test_ocean_obukhov_top_wall(oracle_mod, smooth):
+    """Ocean obukhov under a wall top (010_scalars.f90:696-716): the top
+    wall's u* from the plane-mean speed of level nz-1 by the log law
+    (neutral); the surface planes are not touched."""
+    nl, P, o = mk(oracle_mod, ocean_flag=True, ubc_mom="wall", lbc_mom="stress free",
+                  walltype="smooth" if smooth else "rough")
+    nz = P.nzt
+    sh = (nz + 1, P.nyt, P.nxt)
+    rng = np.random.default_rng(8)
+    u, v = 2 + rng.random(sh), rng.random(sh) - 0.5
+    o.set("U", u); o.set("V", v)
+    o.set("PSI_M_SFC", np.full((P.nyt, P.nxt), 0.25))
+    o.obukhov(5)
+    dz = P.lz_tot / (nz - 1)
+    spd = np.sqrt(u[nz - 1] ** 2 + v[nz - 1] ** 2).mean()
+    if smooth:
+        us = spd / (math.log(0.5 * dz / (P.nu_molec / P.u_scale / P.z_i)) / VONK + 5.2)
+    else:
+        us = spd * VONK / math.log(0.5 * dz / P.zo)
+    assert np.allclose(o.get("USTAR_TOP_SFC"), us, rtol=1e-12)
+    assert np.all(o.get("PSI_M_SFC") == 0.25) and np.all(o.get("USTAR_SFC") == 0.0)
+
+
+def test_ocean_buoyancy(oracle_mod):
+    """calc_beta's ocean form (010_scalars.f90:239-248): g alpha_w (theta -
+    <theta>) averaged to w-nodes, with alpha_w scaled by T_scale
+    (102_initial.f90:228-230): dimensionally g alpha (T - <T>) z_i/u_scale^2."""
+    nl, P, o = mk(oracle_mod, 16, 16, 10, ocean_flag=True, alpha_w=2e-4, u_scale=0.5, theta_init_time=1)
+    nz = P.nzt
+    sh = (nz + 1, P.nyt, P.nxt)
+    rng = np.random.default_rng(9)
+    th = 0.95 + 0.001 * rng.random(sh)
+    o.set("THETA", th)
+    for n in ("U", "V", "W", "NU_T"):
+        o.set(n, np.zeros(sh))
+    o.theta_all_in_one(2)
+    beta = o.get("BETA_SCAL")
+    # theta as theta_all_in_one filtered it; the mean taken in the reference's order
+    o2 = oracle_mod.create(P)
+    o2.set("THETA", th); o2.filter_data("THETA")
+    thf = o2.get("THETA")
+    T = thf * nl.T_scale
+    Tp = T - T.reshape(nz + 1, -1).mean(axis=1)[:, None, None]
+    b = G * nl.alpha_w * Tp * nl.z_i / nl.u_scale**2
+    exp = 0.5 * (b[2:nz] + b[1:nz - 1])
+    assert np.allclose(beta[2:nz], exp, rtol=1e-7, atol=1e-12 * np.abs(exp).max())
+    assert np.all(beta[:2] == 0) and np.all(beta[nz:] == 0)
+
+
+def test_surface_temperature_flux(oracle_mod):
+    """lbc = 0 (010_scalars.f90:380-390): the surface flux from the
+    Monin-Obukhov profile of theta between the surface (T_s = 288 K /
+    T_scale) and the first level, over the scalar roughness z_os = zo/10:
+    q = (T_s - theta_1) kappa u* / (ln((dz/2)/z_os) - psi_h); it becomes
+    sgs_t3 at the surface."""
+    nl, P, o = mk(oracle_mod, 16, 16, 10, lbc=0, theta_init_time=1)
+    nz = P.nzt
+    sh = (nz + 1, P.nyt, P.nxt)
+    rng = np.random.default_rng(2)
+    th = 0.97 + 0.01 * rng.random(sh)
+    o.set("THETA", th)
+    for n in ("U", "V", "W", "NU_T"):
+        o.set(n, np.zeros(sh))
+    ustar = 0.2 + 0.1 * rng.random((P.nyt, P.nxt))
+    psih = -0.3 * rng.random((P.nyt, P.nxt))
+    o.set("USTAR_SFC", ustar); o.set("PSI_H_SFC", psih)
+    o.theta_all_in_one(2)
+    o2 = oracle_mod.create(P)
+    o2.set("THETA", th); o2.filter_data("THETA")
+    th1 = o2.get("THETA")[1]
+    dz = P.lz_tot / (nz - 1)
+    q = (288.0 / nl.T_scale - th1) * VONK * ustar / (math.log(0.5 * dz / (P.zo / 10)) - psih)
+    assert np.allclose(o.get("SGS_T3")[1], q, rtol=1e-12)
