@@ -110,7 +110,7 @@ void orc_fft(const orc_plan* p, cplx* x, cplx* work, int sign) {
 
 void orc_r2c(const orc_plan* p, const double* in, ptrdiff_t istride, cplx* out) {
     const int n = p->n;
-    cplx buf[1536], work[1536];
+    cplx buf[ORC_NMAX], work[ORC_NMAX];
     for (int j = 0; j < n; ++j) buf[j] = in[j * istride];
     orc_fft(p, buf, work, -1);
     for (int k = 0; k <= n / 2; ++k) out[k] = buf[k];
@@ -118,7 +118,7 @@ void orc_r2c(const orc_plan* p, const double* in, ptrdiff_t istride, cplx* out) 
 
 void orc_c2r(const orc_plan* p, const cplx* in, double* out, ptrdiff_t ostride) {
     const int n = p->n;
-    cplx buf[1536], work[1536];
+    cplx buf[ORC_NMAX], work[ORC_NMAX];
     buf[0] = creal(in[0]);
     for (int k = 1; k < (n + 1) / 2; ++k) {
         buf[k] = in[k];

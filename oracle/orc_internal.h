@@ -15,6 +15,9 @@
 
 typedef double complex cplx;
 
+/* longest transform: the 3/2 grid of nxt, nyt <= 2048 (les_create's limit) */
+#define ORC_NMAX 3072
+
 #define ORC_BOGUS (-1234567890.0)   /* 001_param.f90:40 */
 #define ORC_VONK 0.4                /* 001_param.f90:32 */
 #define ORC_G 9.81                  /* 001_param.f90:33 */

@@ -138,6 +138,13 @@ static void orc_e14_5(FILE* f, double x) {
     fprintf(f, "%14s", field);
 }
 
+/* One record of fmt_5168: item i written, a new line after every 1400 items
+ * (format reversion) and at the end of the record */
+static void orc_record_item(FILE* f, double x, int i, int n) {
+    orc_e14_5(f, x);
+    if ((i + 1) % 1400 == 0 || i + 1 == n) fputc('\n', f);
+}
+
 int orc_write_avgslice(les_ctx* c, const char* path, int nums, int average_dim_num) {
     if (!path || !c->sl_run || (average_dim_num != 1 && average_dim_num != 2)) return LES_E_BADARG;
     const int nx = c->nx, nl = c->nz - 1, ns = c->prm.theta_flag ? LES_N_SLICES : LES_N_FLOW_SLICES;
@@ -154,18 +161,16 @@ int orc_write_avgslice(les_ctx* c, const char* path, int nums, int average_dim_n
         if (!f) { st = LES_E_IO; break; }
         if (average_dim_num == 1) {
             for (int k = 0; k < nl; ++k) {
-                orc_e14_5(f, t);
-                for (int i = 0; i < nx; ++i) orc_e14_5(f, a[(size_t)k * nx + i]);
-                fputc('\n', f);
+                orc_record_item(f, t, 0, nx + 1);
+                for (int i = 0; i < nx; ++i) orc_record_item(f, a[(size_t)k * nx + i], i + 1, nx + 1);
             }
         } else {
-            orc_e14_5(f, t);
+            orc_record_item(f, t, 0, nl + 1);
             for (int k = 0; k < nl; ++k) {
                 double sum = 0.0; /* sum(arr2d(:, k), dim=1) in x order */
                 for (int i = 0; i < nx; ++i) sum += a[(size_t)k * nx + i];
-                orc_e14_5(f, sum / nx);
+                orc_record_item(f, sum / nx, k + 1, nl + 1);
             }
-            fputc('\n', f);
         }
         if (fclose(f) != 0) st = LES_E_IO;
     }

@@ -199,6 +199,7 @@ static int init_stokes(les_ctx* c) {
 int orc_create(const les_params* prm, les_ctx** out) {
     *out = NULL;
     if (!good_size(prm->nxt) || !good_size(prm->nyt) || prm->nzt < 4) return LES_E_BADARG;
+    if (3 * prm->nxt / 2 > ORC_NMAX || 3 * prm->nyt / 2 > ORC_NMAX) return LES_E_BADARG;
     if (prm->model != 5 || !prm->sgs || prm->cs_count < 1) return LES_E_BADARG;
     if (prm->lbc != 0 && prm->lbc != 1) return LES_E_BADARG;
     /* lbc = 0 reads ustar_avg and psi_h, which the ocean branch never sets on

@@ -13,7 +13,7 @@
 /* fftn_mpi, 203_fftn_mpi.f90:4-35.  out is (lhx, nyt) complex, kx fastest. */
 void orc_fftn_plane(les_ctx* c, const double* in, cplx* out) {
     const int nx = c->nx, ny = c->ny, lhx = c->lhx;
-    cplx col[1536], work[1536];
+    cplx col[ORC_NMAX], work[ORC_NMAX];
     /* x r2c per row (:21-23) */
     for (int j = 0; j < ny; ++j) orc_r2c(&c->px, in + (size_t)j * nx, 1, out + (size_t)j * lhx);
     /* in_hat_x(lhx,:) = 0 (:25) */
@@ -32,7 +32,7 @@ void orc_fftn_plane(les_ctx* c, const double* in, cplx* out) {
 void orc_ifftn_plane(les_ctx* c, const cplx* in, double* out) {
     const int nx = c->nx, ny = c->ny, lhx = c->lhx;
     cplx* tmp = (cplx*)malloc(sizeof(cplx) * (size_t)lhx * ny);
-    cplx col[1536], work[1536];
+    cplx col[ORC_NMAX], work[ORC_NMAX];
     for (int i = 0; i < lhx; ++i) {
         for (int j = 0; j < ny; ++j) col[j] = in[(size_t)j * lhx + i];
         orc_fft(&c->py, col, work, +1);
@@ -56,8 +56,8 @@ void orc_test_filter_plane(les_ctx* c, double* f, const double* G) {
 void orc_ddx_plane(les_ctx* c, double* d, const double* f) {
     const int nx = c->nx, ny = c->ny, lhx = c->lhx;
     const double cst = 1.0 / nx;
-    double tmp[1536];
-    cplx arr[1536];
+    double tmp[ORC_NMAX];
+    cplx arr[ORC_NMAX];
     for (int j = 0; j < ny; ++j) {
         for (int i = 0; i < nx; ++i) tmp[i] = cst * f[(size_t)j * nx + i];
         orc_r2c(&c->px, tmp, 1, arr);
@@ -70,8 +70,8 @@ void orc_ddx_plane(les_ctx* c, double* d, const double* f) {
 void orc_ddy_plane(les_ctx* c, double* d, const double* f) {
     const int nx = c->nx, ny = c->ny, lhy = c->lhy;
     const double cst = 1.0 / ny;
-    double tmp[1536];
-    cplx arr[1536];
+    double tmp[ORC_NMAX];
+    cplx arr[ORC_NMAX];
     for (int i = 0; i < nx; ++i) {
         for (int j = 0; j < ny; ++j) tmp[j] = cst * f[(size_t)j * nx + i];
         orc_r2c(&c->py, tmp, 1, arr);
@@ -84,8 +84,8 @@ void orc_ddy_plane(les_ctx* c, double* d, const double* f) {
 void orc_pad_plane(les_ctx* c, const double* var, double* big) {
     const int nx = c->nx, ny = c->ny, nx2 = c->nx2, ny2 = c->ny2;
     double* xbig = (double*)malloc(sizeof(double) * (size_t)nx2 * ny);
-    double tmp[1536];
-    cplx hat[1536], hatbig[1536];
+    double tmp[ORC_NMAX];
+    cplx hat[ORC_NMAX], hatbig[ORC_NMAX];
     /* x direction: tmp = var/nxt, r2c(nxt), keep 1..nxt/2, c2r(nxt2) (:101-117) */
     for (int j = 0; j < ny; ++j) {
         for (int i = 0; i < nx; ++i) tmp[i] = var[(size_t)j * nx + i] / nx;
@@ -109,8 +109,8 @@ void orc_pad_plane(les_ctx* c, const double* var, double* big) {
 void orc_unpad_plane(les_ctx* c, const double* big, double* var) {
     const int nx = c->nx, ny = c->ny, nx2 = c->nx2, ny2 = c->ny2;
     double* xs = (double*)malloc(sizeof(double) * (size_t)nx * ny2);
-    double tmp[1536];
-    cplx hatbig[1536], hat[1536];
+    double tmp[ORC_NMAX];
+    cplx hatbig[ORC_NMAX], hat[ORC_NMAX];
     /* tmp = var_big/nxt2; r2c(nxt2); keep 1..nxt/2; Nyquist 0; c2r(nxt) (:157-172) */
     for (int j = 0; j < ny2; ++j) {
         for (int i = 0; i < nx2; ++i) tmp[i] = big[(size_t)j * nx2 + i] / nx2;

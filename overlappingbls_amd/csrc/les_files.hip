@@ -209,6 +209,13 @@ void put_line(std::string& line, double v) {
     line.append(b, 14);
 }
 
+// item i of an n-item record of fmt_5168 = "(1400(E14.5))": format reversion
+// starts a new line after every 1400 items; the record ends with a newline
+void put_item(std::string& text, double v, int i, int n) {
+    put_line(text, v);
+    if ((i + 1) % 1400 == 0 || i + 1 == n) text += "\n";
+}
+
 }  // namespace
 
 int les_write_avgslice(les_ctx* c, const char* path, int nums, int average_dim_num) {
@@ -220,22 +227,19 @@ int les_write_avgslice(les_ctx* c, const char* path, int nums, int average_dim_n
     for (int s = 0; s < ns; ++s) {
         const int st = les_get_avgslice(c, s, 1, a.data());
         if (st != LES_OK) return st;
-        std::string text, line;
+        std::string text;
         if (average_dim_num == 1) {
             for (int k = 0; k < nl; ++k) {
-                line.clear();
-                put_line(line, time);
-                for (int i = 0; i < nx; ++i) put_line(line, a[(size_t)k * nx + i]);
-                text += line + "\n";
+                put_item(text, time, 0, nx + 1);
+                for (int i = 0; i < nx; ++i) put_item(text, a[(size_t)k * nx + i], i + 1, nx + 1);
             }
         } else {
-            put_line(text, time);
+            put_item(text, time, 0, nl + 1);
             for (int k = 0; k < nl; ++k) {
                 double sum = 0.0;
                 for (int i = 0; i < nx; ++i) sum += a[(size_t)k * nx + i];
-                put_line(text, sum / nx);
+                put_item(text, sum / nx, k + 1, nl + 1);
             }
-            text += "\n";
         }
         const std::string fn = std::string(path) + "result/aver_" + SLICE_NAMES[s] + ".out";
         File f;

@@ -35,14 +35,20 @@ def e14_5(x):
     return s.rjust(14) if len(s) <= 14 else "*" * 14
 
 
+def record_lines(values):
+    """One write of fmt_5168: format reversion starts a new line after every
+    1400 items."""
+    return ["".join(e14_5(v) for v in values[i:i + 1400]) for i in range(0, len(values), 1400)]
+
+
 def records(slice2d, time, average_dim_num):
     """The lines write_avgslice appends for one (nzt-1, nxt) window slice."""
     if average_dim_num == 1:
-        return ["".join(e14_5(v) for v in [time] + list(row)) for row in slice2d]
+        return [line for row in slice2d for line in record_lines([time] + list(row))]
     means = []
     for row in slice2d:
         s = 0.0
         for v in row:      # sum(arr2d, dim=1) in x order
             s += v
         means.append(s / len(row))
-    return ["".join(e14_5(v) for v in [time] + means)]
+    return record_lines([time] + means)

@@ -100,3 +100,24 @@ def test_gpu_writes_the_avgslice_records(gpu_lib, tmp_path, dim):
     for name, stem in NAMES.items():
         lines = (tmp_path / "result" / f"aver_{stem}.out").read_text().splitlines()
         assert lines == records(g.avgslice(name, 1), 2 * p.dt, dim), name
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_gpu_avgslice_records_wrap_after_1400_values(gpu_lib, tmp_path, dim):
+    """Records of more than 1400 values continue on a new line (fmt_5168's
+    format reversion, 011_scalars2.f90:21), as the oracle writes them."""
+    from avgslice_text import records
+    shape = dict(nxt=1536, nyt=8, nzt=5) if dim == 1 else dict(nxt=8, nyt=8, nzt=1537)
+    # grid spacing of the 64^3 workload in x and y, so the one step stays below CFL 0.6
+    nl = config.Namelist(cs_count=2, dyn_init=2, lx_tot=shape["nxt"] / 64 * 200.0 * 3.14159,
+                         ly_tot=shape["nyt"] / 64 * 200.0 * 3.14159, lz_tot=100.0, z_i=100.0, **shape)
+    p = nl.params()
+    g = gpu_lib.create(p)
+    config.load_state(g, config.initial_fields(nl, seed=3))
+    g.set_avgslice(True, 1, 1)
+    g.run(1, 1)
+    (tmp_path / "result").mkdir()
+    g.write_avgslice(str(tmp_path) + "/", 1, dim)
+    lines = (tmp_path / "result" / "aver_u.out").read_text().splitlines()
+    assert [len(l) for l in lines[:2]] == [14 * 1400, 14 * 137]
+    assert lines == records(g.avgslice("U", 1), p.dt, dim)

@@ -125,3 +125,27 @@ def test_oracle_writes_the_avgslice_records(oracle_mod, tmp_path, dim):
         assert all(len(l) == 14 * (1 + (p.nxt if dim == 1 else p.nzt - 1)) for l in lines), name
         assert lines[per:] == records(o.avgslice(name, 1), 4 * p.dt, dim), name
     assert THETA_SLICES[0] in NAMES
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_avgslice_records_wrap_after_1400_values(oracle_mod, tmp_path, dim):
+    """fmt_5168 = "(1400(E14.5))" (011_scalars2.f90:21): a record of more
+    than 1400 items continues on a new line (format reversion): 1536 + 1
+    items make a line of 1400 fields and one of 137."""
+    from avgslice_text import records
+    shape = dict(nxt=1536, nyt=8, nzt=5) if dim == 1 else dict(nxt=8, nyt=8, nzt=1537)
+    # grid spacing of the 64^3 workload in x and y, so the one step stays below CFL 0.6
+    nl = config.Namelist(cs_count=2, dyn_init=2, lx_tot=shape["nxt"] / 64 * 200.0 * 3.14159,
+                         ly_tot=shape["nyt"] / 64 * 200.0 * 3.14159, lz_tot=100.0, z_i=100.0, **shape)
+    p = nl.params()
+    o = oracle_mod.create(p)
+    config.load_state(o, config.initial_fields(nl, seed=3))
+    o.set_avgslice(True, 1, 1)
+    o.run(1, 1)
+    (tmp_path / "result").mkdir()
+    o.write_avgslice(str(tmp_path) + "/", 1, dim)
+    lines = (tmp_path / "result" / "aver_u.out").read_text().splitlines()
+    nrec = (p.nzt - 1) if dim == 1 else 1
+    assert len(lines) == 2 * nrec
+    assert [len(l) for l in lines[:2]] == [14 * 1400, 14 * 137]
+    assert lines == records(o.avgslice("U", 1), p.dt, dim)
