@@ -66,3 +66,55 @@ def test_fortran_window_matches_ctypes(tmp_path, gpu_lib):
     assert np.array_equal(g.lin_get("THETA", 1, nz - 1), gth_f)
     assert np.array_equal(g.lin_get("U", 1, nz - 1), gu_f)
     assert np.abs(gu_f).max() > 0.0 and np.abs(gth_f).max() > 0.0
+
+
+def build_routines_driver(tmp):
+    from overlappingbls_amd import lib
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        pytest.skip("amdflang not available")
+    srcs = [("m.o", os.path.join(ROOT, "overlappingbls_amd", "fortran", "les_gpu_mod.f90")),
+            ("mm.o", os.path.join(ROOT, "tests", "fortran", "mock_modules.f90")),
+            ("di.o", os.path.join(ROOT, "overlappingbls_amd", "fortran", "les_dropin.f90")),
+            ("dr.o", os.path.join(ROOT, "tests", "fortran", "drive_routines.f90"))]
+    for obj, src in srcs:
+        subprocess.run([fc, "-c", src, "-o", obj, "-I", ".", "-module-dir", "."], check=True, cwd=tmp)
+    subprocess.run([fc, "dr.o", "di.o", "mm.o", "m.o", "-o", "drive_routines", lib.LIB,
+                    f"-Wl,-rpath,{os.path.dirname(lib.LIB)}"], check=True, cwd=tmp)
+    return os.path.join(tmp, "drive_routines")
+
+
+def test_fortran_dropins_in_flosol_order_match_the_whole_step(tmp_path, gpu_lib):
+    """overlappingbls_amd/fortran/les_dropin.f90's routines, with the
+    reference's interfaces, called in flosol's order from Fortran on the
+    reference's (ldx, nynpy, 0:nz) arrays (tests/fortran/drive_routines.f90,
+    theta off), reproduce the library's whole step (les_run) on the same
+    fields over three steps, the second a dynamic-SGS update."""
+    from test_gpu_parity import rel
+    tmp = str(tmp_path)
+    exe = build_routines_driver(tmp)
+    nl = config.workload("tiny")
+    nl.theta_flag = False
+    p = nl.params()
+    nsteps = 3
+    fields = config.initial_fields(nl, seed=91)
+    with open(os.path.join(tmp, "params.bin"), "wb") as f:
+        f.write(bytes(p))
+        f.write(np.int32(nsteps).tobytes())
+    with open(os.path.join(tmp, "fields.bin"), "wb") as f:
+        for n in ("U", "V", "W"):
+            f.write(np.ascontiguousarray(fields[n], dtype=np.float64).tobytes())
+    r = subprocess.run([exe, tmp], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "fortran routines ok" in r.stdout
+    nz, ny, nx = p.nzt, p.nyt, p.nxt
+    out = np.fromfile(os.path.join(tmp, "out.bin"), dtype=np.float64).reshape(5, nz + 1, ny, nx)
+    g = gpu_lib.create(p)
+    config.load_state(g, fields)
+    g.run(1, nsteps)
+    errs = {}
+    for k, (n, sl) in enumerate((("U", slice(1, nz)), ("V", slice(1, nz)), ("W", slice(1, nz + 1)),
+                                 ("RHSX", slice(1, nz)), ("P", slice(0, nz)))):
+        errs[n] = rel(out[k][sl], g.get(n)[sl])
+    print("drop-ins against les_run", errs)
+    assert all(e < 1e-12 for e in errs.values()), errs
