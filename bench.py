@@ -13,12 +13,20 @@ FP64 step, and its error against the FP64 run from the same initial fields
 after the W + K steps (relative L2 of u, v, w, theta and of the plane-mean u
 and theta profiles).
 
-roofline: the dominant kernel family (by device time in the step) timed live
-with HIP events on the library's stream; `achieved` = algorithmic bytes per
-launch / launch time.  cpu_baseline: the FP64 CPU restatement (oracle) on the
-host cores, two steps of the same 256^3 workload (one plain, one dynamic-SGS
-update) weighted 4:1 as in a cs_count = 5 cycle.  adjoint: the discrete
-adjoint of the FP64 step at 128^3 (see adjoint_leg).
+roofline: after the timed steps, one plain step and one dynamic-SGS update
+step run with every launch of the library's kernel families bracketed by HIP
+events on the stream it runs on (les_profile_step); each family's device
+time per step is (cs_count - 1) plain + 1 update over cs_count.  The family
+with the most device time per step is the roofline kernel: `achieved` = its
+algorithmic bytes per launch / its average launch time.  `step` adds the
+whole step's fraction: the algorithmic bytes of the stage list below (frozen
+here) over the timed ms per step.  `traffic` is the HBM bytes per launch of
+that family from the committed rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE runs
+(profiles/traffic.json), null when none covers it.  cpu_baseline: the FP64
+CPU restatement (oracle) on the host cores, three plain steps (median) and
+one dynamic-SGS update of the same 256^3 workload, weighted 4:1 as in a
+cs_count = 5 cycle.  adjoint: the discrete adjoint of the FP64 step at 128^3
+(see adjoint_leg).
 """
 from __future__ import annotations
 
@@ -32,6 +40,34 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E datasheet
+
+# Compulsory HBM traffic of one FP64 step with theta on, in units of N*s
+# bytes (N = nxt*nyt*(nzt-1) points, s = 8): each stage reads its input 3-D
+# fields once and writes its outputs once; a half-spectrum field counts 1 N,
+# a 3/2-padded one 2.25 N.  Every cs_count steps the Lagrangian update adds
+# LAGRANGE_NS (interpolag_Sdep 14 + lagrange_Sdep 18).
+STAGES_NS = {
+    "ddx/ddy of u, v, w": 12.0,
+    "ddz of u, v, w": 6.0,
+    "sgs_stag": 26.0,
+    "divstress x3": 12.0,
+    "convec (with the padded u, v, w kept for theta)": 18.75,
+    "theta: filter_data + 3 gradients": 8.0,
+    "scalar_RHS_calc": 12.75,
+    "calc_beta + step_scalar": 5.0,
+    "RHS assembly": 10.0,
+    "Admas_Bashforth": 12.0,
+    "press_stag_array I/O": 6.0,
+    "dpdz + RHS -= grad p": 10.0,
+    "project": 9.0,
+    "filter_data of u, v, w": 6.0,
+}
+LAGRANGE_NS = 32.0
+
+
+def step_bytes(n_points, cs_count, s=8.0):
+    """Algorithmic bytes of one step averaged over a cs_count cycle."""
+    return n_points * s * (sum(STAGES_NS.values()) + LAGRANGE_NS / cs_count)
 
 
 def parse():
@@ -239,26 +275,58 @@ def adjoint_leg(workload="sbl128", nsteps=20, k_ckpt=5, obj_steps=5, ndir=8, eps
     return res
 
 
-def traffic_from_profiles(kernel_name: str):
-    """HBM bytes per launch measured with rocprofv3 --pmc (committed summary)."""
+def traffic_from_profiles(family: str):
+    """HBM bytes per launch of a kernel family measured with rocprofv3 --pmc
+    (committed summary profiles/traffic.json, keyed by family)."""
     path = os.path.join(HERE, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         data = json.load(f)
-    return data.get(kernel_name)
+    return data.get(family)
+
+
+def family_profile(ctx, jt, cs_count):
+    """Per-family device time, algorithmic bytes and launches per step from
+    step jt on: the next plain step and the next dynamic-SGS update step
+    profiled (les_profile_step; the steps between run normally), weighted
+    (cs_count - 1):1."""
+    while jt % cs_count == 0:
+        ctx.run(jt, 1)
+        jt += 1
+    a = ctx.profile_step(jt)
+    jt += 1
+    while jt % cs_count != 0:
+        ctx.run(jt, 1)
+        jt += 1
+    b = ctx.profile_step(jt)
+    w = (cs_count - 1) / cs_count
+    out = {}
+    for fam in a:
+        cnt = w * a[fam][2] + (1 - w) * b[fam][2]
+        if cnt > 0:
+            ms = w * a[fam][0] + (1 - w) * b[fam][0]
+            nb = w * a[fam][1] + (1 - w) * b[fam][1]
+            out[fam] = {"ms_per_step": ms, "bytes_per_step": nb, "launches_per_step": cnt,
+                        "GBs": nb / (ms * 1e-3) / 1e9 if ms > 0 else None}
+    return out
 
 
 def cpu_baseline(workload: str):
     from oracle import oracle  # checker / baseline only
     from overlappingbls_amd import config
+    import statistics
     nl = config.workload(workload)
     p = nl.params()
     o = oracle.create(p)
     config.load_state(o, config.initial_fields(nl))
-    jt_plain, jt_upd = nl.cs_count - 1, nl.cs_count
-    t0 = time.perf_counter(); o.step(jt_plain); t_plain = time.perf_counter() - t0
-    t0 = time.perf_counter(); o.step(jt_upd); t_upd = time.perf_counter() - t0
+    plain = []
+    for jt in range(1, nl.cs_count):         # plain steps 1 .. cs_count-1 (at most 3 timed)
+        t0 = time.perf_counter(); o.step(jt); dt = time.perf_counter() - t0
+        if jt >= nl.cs_count - 3:
+            plain.append(dt)
+    t0 = time.perf_counter(); o.step(nl.cs_count); t_upd = time.perf_counter() - t0
+    t_plain = statistics.median(plain)
     per_step = ((nl.cs_count - 1) * t_plain + t_upd) / nl.cs_count
     n = p.nxt * p.nyt * (p.nzt - 1)
     return {
@@ -266,8 +334,9 @@ def cpu_baseline(workload: str):
         "unit": "Mgridpts/s",
         "cores": oracle.threads(),
         "kind": "port",
-        "sample": f"{workload}: steps {jt_plain} (plain, {t_plain:.2f} s) and {jt_upd} (dynamic-SGS update, "
-                  f"{t_upd:.2f} s), weighted {nl.cs_count - 1}:1",
+        "sample": f"{workload}: median of {len(plain)} plain steps ({t_plain:.2f} s) and the dynamic-SGS update "
+                  f"step {nl.cs_count} ({t_upd:.2f} s), weighted {nl.cs_count - 1}:1; the oracle is the C "
+                  f"restatement built with gcc -O2, no FMA contraction, its own radix-2/3/4 FFT",
     }
 
 
@@ -325,7 +394,7 @@ def main():
     n = p.nxt * p.nyt * (p.nzt - 1)
     fields = config.initial_fields(nl, seed=2024)
     fams = {"tdma": K_TDMA, "xline_deriv": K_XDERIV, "yline_deriv": K_YDERIV, "pad32": K_PAD, "copy": K_COPY}
-    runs, kts, states = {}, {}, {}
+    runs, kts, states, profs = {}, {}, {}, {}
     for prec in (("fp64", "mixed") if args.precision == "both" else (args.precision,)):
         # One live context at a time: a context's main and auxiliary streams
         # overlap only when they get hardware queues of their own, and a
@@ -338,9 +407,11 @@ def main():
         t = time_steps(ctx, args.warmup, args.steps)
         runs[prec] = {"ms_per_step": 1e3 * t / args.steps, "value": n / (t / args.steps) / 1e6}
         if not args.profile_only:
-            kts[prec] = kernel_table(ctx, fams)
             states[prec] = {name: ctx.get(name)[config.physical_levels(name, p.nzt)]
                             for name in ("U", "V", "W", "THETA")}
+            # a plain step and an update step with the kernel families timed
+            profs[prec] = family_profile(ctx, args.warmup + args.steps + 1, nl.cs_count)
+            kts[prec] = kernel_table(ctx, fams)
         ctx.close()
     if args.profile_only:
         print(json.dumps({k: v["ms_per_step"] for k, v in runs.items()}))
@@ -348,16 +419,26 @@ def main():
     head = "fp64" if "fp64" in runs else "mixed"
     kt = kts[head]
     value, ms_step = runs[head]["value"], runs[head]["ms_per_step"]
-    dom = os.environ.get("LES_ROOFLINE_KERNEL", "yline_deriv")
-    d = kt[dom]
-    traffic = traffic_from_profiles(dom)
-    roofline = {"bound": "hbm", "achieved": d["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": d["GBs"] / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
-                "bytes_per_launch": d["bytes"], "ms_per_launch": d["ms"]}
+    prof = profs[head]
+    dom = max(prof, key=lambda f: prof[f]["ms_per_step"])     # most device time per step
+    d = prof[dom]
+    achieved = d["bytes_per_step"] / (d["ms_per_step"] * 1e-3) / 1e9
+    sb = step_bytes(n, nl.cs_count)
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(dom),
+                "traffic_source": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, profiles/traffic.json",
+                "kernel": dom,
+                "bytes_per_launch": d["bytes_per_step"] / d["launches_per_step"],
+                "ms_per_launch": d["ms_per_step"] / d["launches_per_step"],
+                "share_of_step": d["ms_per_step"] / ms_step,
+                "step": {"bytes": sb, "GBs": sb / (ms_step * 1e-3) / 1e9,
+                         "frac": sb / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "stages_Ns": sum(STAGES_NS.values()), "update_Ns_per_cs_count": LAGRANGE_NS}}
     mixed = None
     if "mixed" in runs:
         mixed = dict(runs["mixed"])
         mixed["kernels"] = kts["mixed"]
+        mixed["families"] = {f: round(x["ms_per_step"], 5) for f, x in profs["mixed"].items()}
         if "fp64" in runs:
             mixed["speedup_vs_fp64"] = runs["fp64"]["ms_per_step"] / runs["mixed"]["ms_per_step"]
             err = {"after_steps": args.warmup + args.steps}
@@ -386,6 +467,8 @@ def main():
                                f"(cs_count={nl.cs_count}), rough log-law wall, stress-free top, "
                                f"dt={p.dt:g}", "precision": head, "parallelism": "1 GPU"},
         "roofline": roofline,
+        "families": {f: {k: (round(v, 5) if isinstance(v, float) else v) for k, v in x.items()}
+                     for f, x in sorted(prof.items(), key=lambda kv: -kv[1]["ms_per_step"])},
         "kernels": kt,
     }
     if mixed is not None:

@@ -325,15 +325,24 @@ int les_write_avgslice(les_ctx* ctx, const char* path, int nums, int average_dim
  * forward values ("frozen SGS"; Nu_t = Cs_opt2 Delta^2 |S| still varies with
  * the strain).  The wall model, obukhov's stability functions, advection,
  * the pressure projection and the scalar with calc_beta are all linearised.
- * Supported: LES_FP64, theta_flag = 1, lbc_mom = wall, ubc_mom = stress
- * free, steps jt >= 2 with jt > theta_init_time; else LES_E_BADARG.
+ * Supported: LES_FP64, theta_flag = 1, the land branch (ocean_flag = 0)
+ * with lbc = 0 or 1, lbc_mom = wall, ubc_mom = stress free, damping method 1,
+ * steps jt >= 2 with jt > theta_init_time; else LES_E_BADARG.
  *
  * The linear state ("lin") is one tangent or adjoint of the step's state:
  * LES_F_U, LES_F_V, LES_F_W, LES_F_THETA, LES_F_RHSX, LES_F_RHSY,
  * LES_F_RHSZ, LES_F_RHST (levels 1..nzt-1; other levels are ignored and
- * read back as 0) and LES_F_PSI_M_SFC (one plane).  The objective J of the
- * windows is the mean over the plane and over the last obj_steps steps of
- * ustar^2 (the surface momentum flux |tau_w|). */
+ * read back as 0), LES_F_PSI_M_SFC (one plane) and the surface scalar flux
+ * sgs_t3(:,:,1), addressed as LES_F_SGS_T3 with lev0 = 1, nlev = 1 (the next
+ * step's obukhov reads its plane mean; with lbc = 0 it is the bulk flux
+ * (T_s - theta(1)) kappa ustar / (log(dz/(2 z_os)) - psi_h), 010_scalars.f90
+ * :359-394, a function of the state).  The objective J of the windows is the
+ * mean over the plane and over the last obj_steps steps of ustar^2 (the
+ * surface momentum flux |tau_w|, LES_OBJ_USTAR2, the default) or of the
+ * surface scalar flux sgs_t3(:,:,1) (LES_OBJ_T_FLUX). */
+#define LES_OBJ_USTAR2 0
+#define LES_OBJ_T_FLUX 1
+int les_lin_set_objective(les_ctx* ctx, int kind);
 int les_lin_set(les_ctx* ctx, int field, const double* host, int ld, int lev0, int nlev);
 int les_lin_get(les_ctx* ctx, int field, double* host, int ld, int lev0, int nlev);
 int les_lin_zero(les_ctx* ctx);
@@ -384,6 +393,18 @@ int les_lin_stage_times(les_ctx* ctx, int jt, double* tl_ms, double* adj_ms);
  * context's stream, bracketed by HIP events, and the algorithmic HBM bytes
  * one launch must move.  Operates on scratch fields only. */
 int les_time_kernel(les_ctx* ctx, int kernel, int reps, double* ms, double* bytes);
+
+/* Per-family device time of one step (no reference counterpart).  Runs
+ * flosol step jt as les_run does (the state advances one step) with every
+ * launch of an instrumented kernel family bracketed by HIP events on the
+ * stream it is enqueued on; ms[f] receives family f's summed device time,
+ * bytes[f] its algorithmic HBM bytes (each launch's inputs read once and its
+ * outputs written once) and launches[f] its launch count; arrays of
+ * les_nfamilies() entries, names from les_family_name.  One profiled step
+ * at a time per process. */
+int les_nfamilies(void);
+const char* les_family_name(int fam);
+int les_profile_step(les_ctx* ctx, int jt, double* ms, double* bytes, int* launches);
 
 /* HIP stream the context enqueues on (hipStream_t as void*). */
 void* les_stream(les_ctx* ctx);

@@ -20,6 +20,7 @@ LES_E_NODEV = -5
 LES_E_IO = -6
 
 K_TDMA, K_XDERIV, K_YDERIV, K_PAD, K_COPY = 0, 1, 2, 3, 4
+LES_OBJ_USTAR2, LES_OBJ_T_FLUX = 0, 1
 
 LES_FP64 = 0
 LES_MIXED = 1
@@ -164,8 +165,11 @@ def bind(lib: C.CDLL, prefix: str):
         "adjoint_window": [P, i, i, i, i, C.POINTER(C.c_double)],
         "step_adj": [P, i],
         "lin_check": [P, i, i, C.c_ulonglong, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+        "lin_set_objective": [P, i],
         "lin_nstages": [],
         "lin_stage_times": [P, i, _PD, _PD],
+        "profile_step": [P, i, _PD, _PD, C.POINTER(i)],
+        "nfamilies": [],
     }
     fns = {}
     for name, argtypes in sig.items():
@@ -175,11 +179,12 @@ def bind(lib: C.CDLL, prefix: str):
         f.argtypes = argtypes
         f.restype = C.c_int
         fns[name] = f
-    f = getattr(lib, prefix + "lin_stage_name", None)
-    if f is not None:
-        f.argtypes = [i]
-        f.restype = C.c_char_p
-        fns["lin_stage_name"] = f
+    for name in ("lin_stage_name", "family_name"):
+        f = getattr(lib, prefix + name, None)
+        if f is not None:
+            f.argtypes = [i]
+            f.restype = C.c_char_p
+            fns[name] = f
     return fns
 
 
@@ -426,10 +431,20 @@ class LesContext:
         if "sync" in self._f:
             self._call("sync", self._h)
 
+    def profile_step(self, jt):
+        """Run step jt with the kernel families timed (les_profile_step):
+        {family: (device ms, algorithmic bytes, launches)}."""
+        n = self._f["nfamilies"]()
+        ms, nb = np.zeros(n), np.zeros(n)
+        cnt = (C.c_int * n)()
+        self._call("profile_step", self._h, int(jt), _pd(ms), _pd(nb), cnt)
+        return {self._f["family_name"](k).decode(): (float(ms[k]), float(nb[k]), int(cnt[k])) for k in range(n)}
+
     # ---- tangent-linear model and adjoint (FP64, frozen SGS) -------------
     def lin_set(self, name, arr, lev0=0):
         """Set the linear state (tangent or adjoint seed) of U, V, W, THETA,
-        RHSX, RHSY, RHSZ, RHST (levels) or PSI_M_SFC (one plane)."""
+        RHSX, RHSY, RHSZ, RHST (levels), PSI_M_SFC (one plane) or SGS_T3
+        (the surface flux plane: lev0 = 1, one level)."""
         a = np.ascontiguousarray(arr, dtype=np.float64)
         if name in SURFACE:
             a = a.reshape(1, self.ny, self.nx)
@@ -448,6 +463,11 @@ class LesContext:
         out = np.empty((nlev, self.ny, self.nx))
         self._call("lin_get", self._h, FIELD[name], _pd(out), self.nx, lev0, nlev)
         return out
+
+    def set_objective(self, kind):
+        """Objective of the derivative windows: LES_OBJ_USTAR2 (mean ustar^2)
+        or LES_OBJ_T_FLUX (mean surface flux sgs_t3(:,:,1))."""
+        self._call("lin_set_objective", self._h, int(kind))
 
     def lin_zero(self):
         self._call("lin_zero", self._h)

@@ -12,7 +12,7 @@ LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "liblesgpu.so")
 SOURCES = [
     "k_lines.hip", "k_lines_rg.hip", "k_tdma.hip", "k_point.hip", "k_stats.hip", "k_lin.hip", "les_api.hip",
-    "les_lin.hip", "les_files.hip",
+    "les_lin.hip", "les_files.hip", "les_prof.hip",
 ]
 ARCH = os.environ.get("LES_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

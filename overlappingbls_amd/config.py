@@ -307,8 +307,26 @@ def workload(name: str) -> Namelist:
         return Namelist(nxt=16, nyt=24, nzt=9, cs_count=2, dyn_init=2, ubc=1, damping_method=2,
                         stokes_flag=True, stokes_action="mono", amp_w=0.5, lambda_w=30.0, agl_Ust=40.0,
                         wt_s=0.01 * 300.0, **base)
-    if name == "deep16":      # many levels on a small plane: the TDMA's LDS and HBM levels both run
-        return Namelist(nxt=16, nyt=16, nzt=129, cs_count=2, dyn_init=2, wt_s=-0.005 * 300.0, **base)
+    # prescribed surface temperature (lbc = 0) under the linearised step:
+    # a cooled surface (stable obukhov branch) and a warmed one (unstable)
+    if name == "tiny_lbc0":
+        return Namelist(nxt=16, nyt=16, nzt=9, cs_count=2, dyn_init=2, lbc=0, theta_s1=299.0, **base)
+    if name == "ragged_lbc0":
+        return Namelist(nxt=24, nyt=12, nzt=7, cs_count=2, dyn_init=2, lbc=0, theta_s1=301.5,
+                        coriolis_forcing=True, ug_dim=1.0, vg_dim=0.2, ubc=1, **base)
+    # the 128^3 configuration with the surface temperature prescribed 0.15 K
+    # above the air (a surface heat flux near 0.01 K m/s that depends on the
+    # state), for the heat-flux objective of the adjoint
+    if name == "sbl128_hf":
+        return Namelist(nxt=128, nyt=128, nzt=129, lbc=0, theta_s1=300.15, passive_scalar=True,
+                        coriolis_forcing=True, freq_coriolis=1e-4, ug_dim=1.0, vg_dim=0.0, **base)
+    # many levels on a small plane: the TDMA's LDS and HBM levels both run.
+    # dz is 1/50 of dx here, so the explicit vertical SGS diffusion near the
+    # wall (Nu_t dt / dz^2) needs a tenth of the usual time step to stay
+    # stable
+    if name == "deep16":
+        return Namelist(nxt=16, nyt=16, nzt=129, cs_count=2, dyn_init=2, wt_s=-0.005 * 300.0,
+                        **dict(base, dt=2e-5 * 100.0))
     if name == "wide":        # production line lengths 256 / 384, few levels
         return Namelist(nxt=256, nyt=256, nzt=5, cs_count=2, dyn_init=2, wt_s=-0.005 * 300.0, **base)
     if name == "wide_rect":   # 512 / 768 along x, 128 / 192 along y

@@ -47,6 +47,12 @@ __global__ void k_axpy(double* y, const double* __restrict__ x, double a, long n
 void lin_axpy(les_ctx* c, double* y, const double* x, double a, long n) {
     hipLaunchKernelGGL(k_axpy, lgrid(n), dim3(256), 0, c->stream, y, x, a, n);
 }
+__global__ void k_addc(double* y, double a, long n) {
+    LGRID(i, n) y[i] = y[i] + a;
+}
+void lin_addc(les_ctx* c, double* y, double a, long n) {
+    hipLaunchKernelGGL(k_addc, lgrid(n), dim3(256), 0, c->stream, y, a, n);
+}
 
 // counter-based uniform(-1, 1): splitmix64 of (seed, salt, index)
 __device__ __forceinline__ double hash_unif(unsigned long long seed, unsigned long long salt, unsigned long long i) {
@@ -106,7 +112,7 @@ void lin_acc(les_ctx* c, const double* part, double w, double* acc) {
 
 // ---- obukhov (theta-active branch, 010_scalars.f90:758-843) ------------------
 struct ObkPt {
-    double D, us, L, th, arg, x, zeta, pm, fm;
+    double D, us, L, th, wt, arg, x, zeta, pm, fm, ph;
     int br;  // 1: L < 0, 2: L > 0, 0: neutral
 };
 
@@ -117,6 +123,7 @@ __device__ ObkPt obk_point(const double* s, const ObkArgs& a) {
     const double wt_ = s[2] / a.nn;
     const double U = s[0] / a.nn;
     const double ps = s[1] / a.nn;
+    o.wt = wt_;
     o.th = s[3] / a.nn;
     o.D = log(0.5 * a.dz / a.zo_) - ps;
     o.us = U * VONK / o.D;
@@ -128,15 +135,18 @@ __device__ ObkPt obk_point(const double* s, const ObkArgs& a) {
         o.x = pow(o.arg, 0.25);
         o.pm = 2.0 * log((1.0 + o.x) / 2.0) + log((1.0 + o.x * o.x) / 2.0) - 2.0 * atan(o.x) + PI / 2.0;
         o.fm = 1.0 / o.x;
+        o.ph = 2.0 * log((1.0 + o.x * o.x) / 2.0);
     } else if (o.L > 0.0 && wt_ != 0.0) {
         o.br = 2;
         o.zeta = 0.5 * a.dz / o.L;
         o.fm = 1.0 + 4.8 * o.zeta;
         o.pm = -1.0 * 4.8 * o.zeta;
+        o.ph = -1.0 * 7.8 * o.zeta;
     } else {
         o.br = 0;
         o.pm = 0.0;
         o.fm = 1.0;
+        o.ph = 0.0;
     }
     return o;
 }
@@ -144,19 +154,23 @@ __device__ ObkPt obk_point(const double* s, const ObkArgs& a) {
 __device__ __forceinline__ double dpm_dx(double x) {
     return 2.0 / (1.0 + x) + 2.0 * x / (1.0 + x * x) - 2.0 / (1.0 + x * x);
 }
+__device__ __forceinline__ double dph_dx(double x) { return 4.0 * x / (1.0 + x * x); }
 
-// block 0: sum (u du + v dv)/|u|, block 1: sum dpsi, block 2: sum dtheta(1)
+// block 0: sum (u du + v dv)/|u|, block 1: sum dpsi, block 2: sum dtheta(1),
+// block 3: sum dq (the surface flux obukhov averages into wt_)
 __global__ __launch_bounds__(256) void k_obk_tl_sums(const double* __restrict__ u1, const double* __restrict__ v1,
                                                       const double* __restrict__ du, const double* __restrict__ dv,
                                                       const double* __restrict__ dpsi, const double* __restrict__ dth,
-                                                      long plane, double* __restrict__ out) {
+                                                      const double* __restrict__ dq, long plane,
+                                                      double* __restrict__ out) {
     __shared__ double sh[256];
     const int which = blockIdx.x;
     double s = 0.0;
     for (long i = threadIdx.x; i < plane; i += 256) {
         if (which == 0) s += (u1[i] * du[i] + v1[i] * dv[i]) / sqrt(u1[i] * u1[i] + v1[i] * v1[i]);
         else if (which == 1) s += dpsi[i];
-        else s += dth[i];
+        else if (which == 2) s += dth[i];
+        else s += dq[i];
     }
     sh[threadIdx.x] = s;
     __syncthreads();
@@ -167,44 +181,49 @@ __global__ __launch_bounds__(256) void k_obk_tl_sums(const double* __restrict__ 
     if (threadIdx.x == 0) out[which] = sh[0];
 }
 
+// L = -us^3 theta / (kappa g wt): dL/L = 3 dus/us + dtheta/theta - dwt/wt
 __global__ void k_obk_tl_finish(const double* __restrict__ sums, const double* __restrict__ ds, ObkArgs a,
-                                double* dpsi, double* dphi, long plane) {
+                                double* dpsi, double* dphi, double* dpsh, long plane) {
     const ObkPt o = obk_point(sums, a);
-    const double dU = ds[0] / a.nn, dps = ds[1] / a.nn, dth = ds[2] / a.nn;
+    const double dU = ds[0] / a.nn, dps = ds[1] / a.nn, dth = ds[2] / a.nn, dwt = ds[3] / a.nn;
     const double dD = -dps;
     const double dus = VONK * dU / o.D - o.us * dD / o.D;
-    const double dL = (o.br != 0) ? o.L * (3.0 * dus / o.us + dth / o.th) : 0.0;
-    double dpm = 0.0, dfm = 0.0;
+    const double dL = (o.br != 0) ? o.L * (3.0 * dus / o.us + dth / o.th - dwt / o.wt) : 0.0;
+    double dpm = 0.0, dfm = 0.0, dph = 0.0;
     if (o.br == 1) {
         const double darg = 8.0 * a.dz * dL / (o.L * o.L);
         const double dx = 0.25 * o.x * darg / o.arg;
         dpm = dpm_dx(o.x) * dx;
         dfm = -dx / (o.x * o.x);
+        dph = dph_dx(o.x) * dx;
     } else if (o.br == 2) {
         const double dz = -o.zeta * dL / o.L;
         dfm = 4.8 * dz;
         dpm = -4.8 * dz;
+        dph = -7.8 * dz;
     }
     LGRID(i, plane) {
         dpsi[i] = dpm;
         dphi[i] = dfm;
+        dpsh[i] = dph;
     }
 }
 
 void obk_tl(les_ctx* c, const Tape& t, LinBufs& b, const ObkArgs& a) {
     const long np = (long)c->plane;
-    hipLaunchKernelGGL(k_obk_tl_sums, dim3(3), dim3(256), 0, c->stream, (const double*)t.u1, (const double*)t.v1,
+    hipLaunchKernelGGL(k_obk_tl_sums, dim3(4), dim3(256), 0, c->stream, (const double*)t.u1, (const double*)t.v1,
                        (const double*)(b.v[L_U] + np), (const double*)(b.v[L_V] + np), (const double*)b.v[L_PSI],
-                       (const double*)(b.v[L_TH] + np), np, b.red);
+                       (const double*)(b.v[L_TH] + np), (const double*)b.v[L_Q], np, b.red);
     hipLaunchKernelGGL(k_obk_tl_finish, lgrid(np), dim3(256), 0, c->stream, (const double*)t.obk,
-                       (const double*)b.red, a, b.v[L_PSI], b.v[L_PHI], np);
+                       (const double*)b.red, a, b.v[L_PSI], b.v[L_PHI], b.v[L_PSH], np);
 }
 
-// block 0: sum lambda_psi, block 1: sum lambda_phi
+// block 0: sum lambda_psi, block 1: sum lambda_phi, block 2: sum lambda_psi_h
 __global__ __launch_bounds__(256) void k_obk_adj_sums(const double* __restrict__ lpsi, const double* __restrict__ lphi,
-                                                       long plane, double* __restrict__ out) {
+                                                       const double* __restrict__ lpsh, long plane,
+                                                       double* __restrict__ out) {
     __shared__ double sh[256];
-    const double* p = blockIdx.x == 0 ? lpsi : lphi;
+    const double* p = blockIdx.x == 0 ? lpsi : blockIdx.x == 1 ? lphi : lpsh;
     double s = 0.0;
     for (long i = threadIdx.x; i < plane; i += 256) s += p[i];
     sh[threadIdx.x] = s;
@@ -218,41 +237,45 @@ __global__ __launch_bounds__(256) void k_obk_adj_sums(const double* __restrict__
 
 __global__ void k_obk_adj_finish(const double* __restrict__ sums, const double* __restrict__ ls, ObkArgs a,
                                  const double* __restrict__ u1, const double* __restrict__ v1, double* lu,
-                                 double* lv, double* lpsi, double* lphi, double* lth, long plane) {
+                                 double* lv, double* lpsi, double* lphi, double* lpsh, double* lth, double* lq,
+                                 long plane) {
     const ObkPt o = obk_point(sums, a);
-    const double lpm = ls[0], lfm = ls[1];
+    const double lpm = ls[0], lfm = ls[1], lph = ls[2];
     double lL = 0.0;
     if (o.br == 1) {
-        const double lx = dpm_dx(o.x) * lpm - lfm / (o.x * o.x);
+        const double lx = dpm_dx(o.x) * lpm - lfm / (o.x * o.x) + dph_dx(o.x) * lph;
         const double larg = 0.25 * o.x / o.arg * lx;
         lL = larg * 8.0 * a.dz / (o.L * o.L);
     } else if (o.br == 2) {
-        const double lz = -4.8 * lpm + 4.8 * lfm;
+        const double lz = -4.8 * lpm + 4.8 * lfm - 7.8 * lph;
         lL = lz * (-o.zeta / o.L);
     }
-    // neutral branch (wt = 0: L infinite): psi_m, phi_m are constants
+    // neutral branch (wt = 0: L infinite): psi_m, phi_m, psi_h are constants
     const double lus = (o.br != 0) ? lL * 3.0 * o.L / o.us : 0.0;
     const double lthb = (o.br != 0) ? lL * o.L / o.th : 0.0;
+    const double lwt = (o.br != 0) ? -lL * o.L / o.wt : 0.0;
     const double lU = lus * VONK / o.D;
     const double lD = -lus * o.us / o.D;
-    const double l0 = lU / a.nn, l1 = -lD / a.nn, l3 = lthb / a.nn;
+    const double l0 = lU / a.nn, l1 = -lD / a.nn, l3 = lthb / a.nn, l4 = lwt / a.nn;
     LGRID(i, plane) {
         const double m = sqrt(u1[i] * u1[i] + v1[i] * v1[i]);
         lu[i] = lu[i] + l0 * u1[i] / m;
         lv[i] = lv[i] + l0 * v1[i] / m;
         lpsi[i] = l1;
         lth[i] = lth[i] + l3;
+        lq[i] = lq[i] + l4;
         lphi[i] = 0.0;
+        lpsh[i] = 0.0;
     }
 }
 
 void obk_adj(les_ctx* c, const Tape& t, LinBufs& b, const ObkArgs& a) {
     const long np = (long)c->plane;
-    hipLaunchKernelGGL(k_obk_adj_sums, dim3(2), dim3(256), 0, c->stream, (const double*)b.v[L_PSI],
-                       (const double*)b.v[L_PHI], np, b.red);
+    hipLaunchKernelGGL(k_obk_adj_sums, dim3(3), dim3(256), 0, c->stream, (const double*)b.v[L_PSI],
+                       (const double*)b.v[L_PHI], (const double*)b.v[L_PSH], np, b.red);
     hipLaunchKernelGGL(k_obk_adj_finish, lgrid(np), dim3(256), 0, c->stream, (const double*)t.obk,
                        (const double*)b.red, a, (const double*)t.u1, (const double*)t.v1, b.v[L_U] + np,
-                       b.v[L_V] + np, b.v[L_PSI], b.v[L_PHI], b.v[L_TH] + np, np);
+                       b.v[L_V] + np, b.v[L_PSI], b.v[L_PHI], b.v[L_PSH], b.v[L_TH] + np, b.v[L_Q], np);
 }
 
 // ---- wall stress, bottom (104_wallstress.f90:98-182) ---------------------------
@@ -277,7 +300,7 @@ __global__ void k_wall_tl(const double* __restrict__ obk, ObkArgs oa, WallArgs w
                           const double* __restrict__ duv1f, const double* __restrict__ du,
                           const double* __restrict__ dv, const double* __restrict__ dpsi,
                           const double* __restrict__ dphi, double* dtxz, double* dtyz, double* ddudz, double* ddvdz,
-                          double* dus2, long plane) {
+                          double* ddus, double* dus2, long plane) {
     const ObkPt o = obk_point(obk, oa);
     LGRID(i, plane) {
         const double uf = uv1f[i], vf = uv1f[plane + i];
@@ -293,6 +316,7 @@ __global__ void k_wall_tl(const double* __restrict__ obk, ObkArgs oa, WallArgs w
         const double g = dus * p.phi / p.ua - p.us * p.phi * dua / (p.ua * p.ua) + p.us * dphi[i] / p.ua;
         ddudz[i] = (u == 0.0) ? 0.0 : p.C * (g * u + p.us * p.phi / p.ua * du[i]);
         ddvdz[i] = (v == 0.0) ? 0.0 : p.C * (g * v + p.us * p.phi / p.ua * dv[i]);
+        ddus[i] = dus;
         if (dus2) dus2[i] = 2.0 * p.us * dus;
     }
 }
@@ -300,8 +324,8 @@ __global__ void k_wall_tl(const double* __restrict__ obk, ObkArgs oa, WallArgs w
 // wobj: weight of ustar^2 in the objective at this step
 __global__ void k_wall_adj(const double* __restrict__ obk, ObkArgs oa, WallArgs wa, const double* __restrict__ uv1f,
                            const double* __restrict__ u1, const double* __restrict__ v1, double* ltxz, double* ltyz,
-                           double* ldudz, double* ldvdz, double* luv1f, double* lu, double* lv, double* lpsi,
-                           double* lphi, double wobj, long plane) {
+                           double* ldudz, double* ldvdz, double* ldus, double* luv1f, double* lu, double* lv,
+                           double* lpsi, double* lphi, double wobj, long plane) {
     const ObkPt o = obk_point(obk, oa);
     LGRID(i, plane) {
         const double uf = uv1f[i], vf = uv1f[plane + i];
@@ -313,7 +337,7 @@ __global__ void k_wall_adj(const double* __restrict__ obk, ObkArgs oa, WallArgs 
         const double lcst = lt * uf + ly * vf;
         double luf = p.cst * lt, lvf = p.cst * ly;
         const double q = u * ldu + v * ldv;
-        const double lus = -2.0 * p.us / p.ua * lcst + p.C * p.phi * q / p.ua + 2.0 * p.us * wobj;
+        const double lus = -2.0 * p.us / p.ua * lcst + p.C * p.phi * q / p.ua + 2.0 * p.us * wobj + ldus[i];
         double lua = p.us * p.us / (p.ua * p.ua) * lcst - p.C * p.us * p.phi * q / (p.ua * p.ua);
         lu[i] = lu[i] + p.C * p.us * p.phi / p.ua * ldu;
         lv[i] = lv[i] + p.C * p.us * p.phi / p.ua * ldv;
@@ -325,7 +349,7 @@ __global__ void k_wall_adj(const double* __restrict__ obk, ObkArgs oa, WallArgs 
         lvf += vf / p.ua * lua;
         luv1f[i] = luf;
         luv1f[plane + i] = lvf;
-        ltxz[i] = 0.0; ltyz[i] = 0.0; ldudz[i] = 0.0; ldvdz[i] = 0.0;
+        ltxz[i] = 0.0; ltyz[i] = 0.0; ldudz[i] = 0.0; ldvdz[i] = 0.0; ldus[i] = 0.0;
     }
 }
 
@@ -335,14 +359,14 @@ void wall_tl(les_ctx* c, const Tape& t, LinBufs& b, const ObkArgs& oa, const Wal
                        (const double*)t.uv1f, (const double*)t.u1, (const double*)t.v1, (const double*)b.uv1f,
                        (const double*)(b.v[L_U] + np), (const double*)(b.v[L_V] + np), (const double*)b.v[L_PSI],
                        (const double*)b.v[L_PHI], b.v[L_TXZ] + np, b.v[L_TYZ] + np, b.v[L_UZ] + np,
-                       b.v[L_VZ] + np, dus2, np);
+                       b.v[L_VZ] + np, b.v[L_DUS], dus2, np);
 }
 void wall_adj(les_ctx* c, const Tape& t, LinBufs& b, const ObkArgs& oa, const WallArgs& wa, double wobj) {
     const long np = (long)c->plane;
     hipLaunchKernelGGL(k_wall_adj, lgrid(np), dim3(256), 0, c->stream, (const double*)t.obk, oa, wa,
                        (const double*)t.uv1f, (const double*)t.u1, (const double*)t.v1, b.v[L_TXZ] + np,
-                       b.v[L_TYZ] + np, b.v[L_UZ] + np, b.v[L_VZ] + np, b.uv1f, b.v[L_U] + np, b.v[L_V] + np,
-                       b.v[L_PSI], b.v[L_PHI], wobj, np);
+                       b.v[L_TYZ] + np, b.v[L_UZ] + np, b.v[L_VZ] + np, b.v[L_DUS], b.uv1f, b.v[L_U] + np,
+                       b.v[L_V] + np, b.v[L_PSI], b.v[L_PHI], wobj, np);
 }
 
 // ---- vertical differences --------------------------------------------------------
@@ -970,6 +994,60 @@ void sflux_adj(les_ctx* c, const Tape& t, LinBufs& b) {
     clear_levels(c, b, b.v[L_FZ], 2, p.nz);
 }
 
+// ---- bulk surface flux of lbc = 0 (010_scalars.f90:359-394) -------------------------
+// q = (T_s - theta(1)) kappa ustar / (log_zos - psi_h), with theta(1) the
+// filtered theta, ustar the wall model's per-point value and psi_h obukhov's;
+// sgs_t3(:,:,1) = q and the z flux at level 1 is -q.  lbc = 1 prescribes q.
+__global__ void k_sfc_tl(const double* __restrict__ obk, ObkArgs oa, WallArgs wa, SfcArgs sa,
+                         const double* __restrict__ uv1f, const double* __restrict__ th1,
+                         const double* __restrict__ dth1, const double* __restrict__ dus,
+                         const double* __restrict__ dpsh, double* dq, double* fz1, long plane) {
+    const ObkPt o = obk_point(obk, oa);
+    LGRID(i, plane) {
+        double d = 0.0;
+        if (sa.bulk) {
+            const WallPt w = wall_point(uv1f[i], uv1f[plane + i], o, wa);
+            const double den = sa.log_zos - o.ph;
+            const double dT = sa.T_s - th1[i];
+            const double q = dT * VONK * w.us / den;
+            d = VONK / den * (-dth1[i] * w.us + dT * dus[i]) + q / den * dpsh[i];
+        }
+        dq[i] = d;
+        fz1[i] = -1.0 * d;
+    }
+}
+__global__ void k_sfc_adj(const double* __restrict__ obk, ObkArgs oa, WallArgs wa, SfcArgs sa,
+                          const double* __restrict__ uv1f, const double* __restrict__ th1, double* lth1,
+                          double* lus, double* lpsh, double* lq, double* lfz1, long plane) {
+    const ObkPt o = obk_point(obk, oa);
+    LGRID(i, plane) {
+        if (sa.bulk) {
+            const WallPt w = wall_point(uv1f[i], uv1f[plane + i], o, wa);
+            const double den = sa.log_zos - o.ph;
+            const double dT = sa.T_s - th1[i];
+            const double q = dT * VONK * w.us / den;
+            const double l = lq[i] - lfz1[i];
+            lth1[i] = lth1[i] - VONK / den * w.us * l;
+            lus[i] = lus[i] + VONK / den * dT * l;
+            lpsh[i] = lpsh[i] + q / den * l;
+        }
+        lq[i] = 0.0;
+        lfz1[i] = 0.0;
+    }
+}
+void sfc_tl(les_ctx* c, const Tape& t, LinBufs& b, const ObkArgs& oa, const WallArgs& wa, const SfcArgs& sa) {
+    const long np = (long)c->plane;
+    hipLaunchKernelGGL(k_sfc_tl, lgrid(np), dim3(256), 0, c->stream, (const double*)t.obk, oa, wa, sa,
+                       (const double*)t.uv1f, (const double*)(t.thf + np), (const double*)(b.v[L_TH] + np),
+                       (const double*)b.v[L_DUS], (const double*)b.v[L_PSH], b.v[L_Q], b.v[L_FZ] + np, np);
+}
+void sfc_adj(les_ctx* c, const Tape& t, LinBufs& b, const ObkArgs& oa, const WallArgs& wa, const SfcArgs& sa) {
+    const long np = (long)c->plane;
+    hipLaunchKernelGGL(k_sfc_adj, lgrid(np), dim3(256), 0, c->stream, (const double*)t.obk, oa, wa, sa,
+                       (const double*)t.uv1f, (const double*)(t.thf + np), b.v[L_TH] + np, b.v[L_DUS],
+                       b.v[L_PSH], b.v[L_Q], b.v[L_FZ] + np, np);
+}
+
 // RT(k) = -adv + t1 + t2 + (fz(k+1) - fz(k))/dz on 1..nz-1; zero elsewhere
 __global__ void k_sdiv_tl(double* rt, const double* __restrict__ adv, const double* __restrict__ t1,
                           const double* __restrict__ t2, const double* __restrict__ fz, LP p) {
@@ -992,7 +1070,7 @@ __global__ void k_sdiv_adj(double* lrt, double* ladv, double* lfx, double* lfy, 
             lfx[i] += -t1[i];
             lfy[i] += -t2[i];
         }
-        if (j >= 2) lfz[i] += (lrt[i - p.plane] - ((j <= nz - 1) ? lrt[i] : 0.0)) / p.dz;
+        lfz[i] += (((j >= 2) ? lrt[i - p.plane] : 0.0) - ((j <= nz - 1) ? lrt[i] : 0.0)) / p.dz;
     }
 }
 void sdiv_tl(les_ctx* c, LinBufs& b, const double* t1, const double* t2) {

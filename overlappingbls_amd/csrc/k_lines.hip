@@ -293,9 +293,19 @@ static inline int lines_per_block(int L) {
 
 int g_use_ct = 1;
 
+static int line_fam(LineOp op, bool y) {
+    switch (op) {
+        case OP_DERIV: return y ? FAM_YDERIV : FAM_XDERIV;
+        case OP_NYQ: return y ? FAM_YFILTER : FAM_XFILTER;
+        case OP_PAD: return y ? FAM_YPAD : FAM_XPAD;
+        default: return y ? FAM_YUNPAD : FAM_XUNPAD;
+    }
+}
+
 template <typename T, typename TI>
 void xline_real(hipStream_t s, const TI* in, T* out, long nrows, int ld_in, int ld_out, const Twiddle& tin,
                 const Twiddle& tout, LineOp op, double kfac) {
+    FamScope fs(s, line_fam(op, false), (double)nrows * ((double)tin.n * sizeof(TI) + (double)tout.n * sizeof(T)));
     if (g_use_ct && xline_real_ct<T, TI>(s, in, out, nrows, ld_in, ld_out, tin, tout, op, kfac)) return;
     const int L = tin.n > tout.n ? tin.n : tout.n;
     const int P = lines_per_block(L);
@@ -308,6 +318,8 @@ void xline_real(hipStream_t s, const TI* in, T* out, long nrows, int ld_in, int 
 template <typename T, typename TI>
 void yline_real(hipStream_t s, const TI* in, T* out, int nplanes, int ncols, int ld_in, int ld_out, long ps_in,
                 long ps_out, const Twiddle& tin, const Twiddle& tout, LineOp op, double kfac) {
+    FamScope fs(s, line_fam(op, true),
+                (double)nplanes * ncols * ((double)tin.n * sizeof(TI) + (double)tout.n * sizeof(T)));
     if (g_use_ct && yline_real_ct<T, TI>(s, in, out, nplanes, ncols, ld_in, ld_out, ps_in, ps_out, tin, tout, op, kfac)) return;
     const int L = tin.n > tout.n ? tin.n : tout.n;
     const int C = lines_per_block(L);
@@ -321,6 +333,7 @@ template <typename T, typename TI>
 void x_r2c(hipStream_t s, const TI* in, cx<T>* out, long nrows, int ld_in, int ldc, const Twiddle& tw,
            double scale, int kst) {
     if (kst < 0) kst = tw.n / 2 + 1;
+    FamScope fs(s, FAM_XR2C, (double)nrows * ((double)tw.n * sizeof(TI) + 2.0 * kst * sizeof(T)));
     if (g_use_ct && x_r2c_ct<T, TI>(s, in, out, nrows, ld_in, ldc, tw, scale, kst)) return;
     const int P = lines_per_block(tw.n);
     const long nb = (nrows + 2 * P - 1) / (2 * P);
@@ -333,6 +346,7 @@ template <typename T>
 void x_c2r(hipStream_t s, const cx<T>* in, T* out, long nrows, int ldc, int ld_out, const Twiddle& tw, int deriv,
            double kfac, int kst) {
     if (kst < 0) kst = tw.n / 2 + 1;
+    FamScope fs(s, FAM_XC2R, (double)nrows * (2.0 * kst * sizeof(T) + (double)tw.n * sizeof(T)));
     if (g_use_ct && x_c2r_ct<T>(s, in, out, nrows, ldc, ld_out, tw, deriv, kfac, kst)) return;
     const int P = lines_per_block(tw.n);
     const long nb = (nrows + 2 * P - 1) / (2 * P);
@@ -347,6 +361,11 @@ void yline_spec(hipStream_t s, const cx<T>* in, cx<T>* out, int nplanes, int nco
     if (ld < 0) ld = ncols;
     if (ldG < 0) ldG = ld;
     if (mc < 0) mc = tw.n;
+    // rows read and written: the pruned forward stores, and the pruned
+    // multiply-inverse reads, only the rows inside the filter support
+    const double kept = (2 * mc - 1 < tw.n) ? (double)(2 * mc - 1) : (double)tw.n;
+    const double rin = (op == SP_MULINV) ? kept : (double)tw.n, rout = (op == SP_FWD) ? kept : (double)tw.n;
+    FamScope fs(s, FAM_YSPEC, (double)nplanes * ncols * 2.0 * sizeof(T) * (rin + rout));
     if (g_use_ct && yline_spec_ct<T>(s, in, out, nplanes, ncols, ps, tw, op, G, ld, ldG, mc)) return;
     const int C = lines_per_block(tw.n);
     const int nbx = (ncols + C - 1) / C;

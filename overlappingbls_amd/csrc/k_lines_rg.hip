@@ -1047,6 +1047,11 @@ bool cross_unpad_x(hipStream_t s, const ConvIn<T>& a, int scalar, T* o0, T* o1, 
                    const Twiddle& tin, const Twiddle& tout) {
     const long plane2 = (long)tin.n * ny2;
     const long nrp = (long)(nz - 1) * ny2 / 2;
+    if (ny2 % 2 != 0 || tin.n * 2 != tout.n * 3 || (tout.n != 128 && tout.n != 256 && tout.n != 512)) return false;
+    // six padded inputs (levels 1..nz) read, one or three x-unpadded outputs
+    // (levels 1..nz-1) written
+    FamScope fs(s, scalar ? FAM_SCALAR_UNPAD_X : FAM_CROSS_UNPAD_X,
+                sizeof(T) * (6.0 * nz * plane2 + (scalar ? 1.0 : 3.0) * (nz - 1) * ny2 * (double)tout.n));
 #define FCASE(NN_)                                                                                          \
     if (tin.n == 3 * NN_ / 2 && tout.n == NN_ && ny2 % 2 == 0) {                                              \
         if (scalar)                                                                                       \
@@ -1065,6 +1070,11 @@ bool vort_pad_x(hipStream_t s, const VortIn<T>& g, T* o0, T* o1, T* o2, int nz, 
                 const Twiddle& tout) {
     const long plane = (long)tin.n * ny;
     const long nrp = (long)nz * ny / 2;
+#define VOK(NN_) || (tin.n == NN_ && tout.n == 3 * NN_ / 2)
+    if (ny % 2 != 0 || !(false LES_SIZES(VOK))) return false;
+#undef VOK
+    // six velocity gradients read, three x-padded vorticity components written
+    FamScope fs(s, FAM_VORT_PAD_X, sizeof(T) * (6.0 * nz * plane + 3.0 * nz * ny * (double)tout.n));
 #define VCASE(NN_)                                                                                           \
     if (tin.n == NN_ && tout.n == 3 * NN_ / 2 && ny % 2 == 0) {                                                \
         launch_fused<T, NN_, 3 * NN_ / 2, OP_PAD, 3>(s, g, 0, o0, o1, o2, nrp, ny, plane, nz, tin, tout);  \
@@ -1078,7 +1088,11 @@ template <typename T>
 bool lag_contract_x(hipStream_t s, int scale, const cx<T>* spt, int ldc, int kst, int nz, int ny, double* lm,
                     double* mm, const LagUpdate& u, double cst, const Twiddle& tw) {
     const long nrp = (long)nz * ny / 2;
-    if (ny % 2 != 0) return false;
+    if (ny % 2 != 0 || (tw.n != 64 && tw.n != 128 && tw.n != 256)) return false;
+    // the 21 filtered spectra read; L M, M M (2 delta) or the F_* update and
+    // Cs_opt2 (4 delta) written, FP64
+    FamScope fs(s, FAM_LAG_CONTRACT_X,
+                21.0 * nz * ny * ldc * 2.0 * sizeof(T) + (scale == 2 ? 2.0 : 7.0) * nz * ny * (double)tw.n * 8.0);
 #define LCCASE(NN_)                                                                                         \
     if (tw.n == NN_) {                                                                                      \
         const dim3 g((unsigned)nrp), b(lag_threads<T, NN_>());                                              \

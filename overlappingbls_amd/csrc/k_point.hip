@@ -181,6 +181,7 @@ __global__ __launch_bounds__(64) void k_level_means_seq(const double* __restrict
 }
 
 void level_means_seq(hipStream_t s, const double* f, size_t plane, int lev0, int nlev, double nn, double* out) {
+    FamScope fs(s, FAM_PLANE_SUMS, 8.0 * (double)plane * nlev);
     hipLaunchKernelGGL(k_level_means_seq, dim3(nlev), dim3(64), 0, s, f + (size_t)lev0 * plane, (long)plane, nn,
                        out);
 }
@@ -257,6 +258,7 @@ __global__ void k_obukhov_ocean(double* ustar_top, const double* __restrict__ su
 
 // obukhov's plane sums (and the fills that precede them) on stream s
 void obukhov_sums_dev(les_ctx* c, int jt, hipStream_t s) {
+    FamScope fs(s, FAM_PLANE_SUMS, 8.0 * 5.0 * (double)c->plane);
     const les_params& P = c->prm;
     const long np = (long)c->plane;
     const double* u1 = c->f3[LES_F_U] + np;
@@ -452,6 +454,7 @@ static Strain<W> strains(les_ctx* c) {
 
 template <typename W>
 void strain_dev(les_ctx* c) {
+    FamScope fs(c->stream, FAM_SGS_STRESS, 15.0 * sizeof(W) * (double)c->plane * (double)c->nlev);
     KP p = kp(c);
     hipLaunchKernelGGL(k_strain<W>, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, grads<W>(c), strains<W>(c),
                        p);
@@ -467,6 +470,7 @@ __global__ void k_lag_acc(double* ul, double* vl, double* wl, const double* __re
 }
 
 void lag_accumulate_dev(les_ctx* c) {
+    FamScope fs(c->stream, FAM_LAGRANGE_POINT, 9.0 * 8.0 * (double)c->plane * (double)c->nlev);
     const long n = (long)(c->plane * c->nlev);
     hipLaunchKernelGGL(k_lag_acc, grid_for(n), dim3(256), 0, c->stream, c->f3[LES_F_U_LAG], c->f3[LES_F_V_LAG],
                        c->f3[LES_F_W_LAG], (const double*)c->f3[LES_F_U], (const double*)c->f3[LES_F_V],
@@ -599,6 +603,7 @@ __global__ void k_strain_stress(Grad<W> g, const double* __restrict__ Cs, W* Nu,
 
 template <typename W>
 void strain_stress_dev(les_ctx* c) {
+    FamScope fs(c->stream, FAM_SGS_STRESS, (16.0 * sizeof(W) + 8.0) * (double)c->plane * (double)c->nlev);
     KP p = kp(c);
     hipLaunchKernelGGL(k_strain_stress<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, grads<W>(c),
                        (const double*)c->f3[LES_F_CS_OPT2], WF<W>(c, LES_F_NU_T), WF<W>(c, LES_F_TXX),
@@ -608,6 +613,7 @@ void strain_stress_dev(les_ctx* c) {
 
 template <typename W>
 void nu_t_stress_dev(les_ctx* c) {
+    FamScope fs(c->stream, FAM_SGS_STRESS, (13.0 * sizeof(W) + 8.0) * (double)c->plane * (double)c->nlev);
     KP p = kp(c);
     hipLaunchKernelGGL(k_stress<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, strains<W>(c),
                        (const double*)c->f3[LES_F_CS_OPT2], WF<W>(c, LES_F_NU_T), WF<W>(c, LES_F_TXX),
@@ -682,6 +688,7 @@ __global__ void k_interpolag(const double* __restrict__ ul, const double* __rest
 }
 
 void interpolag_dev(les_ctx* c) {
+    FamScope fs(c->stream, FAM_LAGRANGE_POINT, 14.0 * 8.0 * (double)c->plane * (double)c->nlev);
     KP p = kp(c);
     const int ids[4] = {LES_F_F_LM, LES_F_F_MM, LES_F_F_QN, LES_F_F_NN};
     // the old F_* become the interpolation source by a buffer exchange; the
@@ -744,6 +751,7 @@ __global__ void k_lag_prep(const double* __restrict__ u, const double* __restric
 
 template <typename W>
 void lagrange_prep_dev(les_ctx* c, W* buf21) {
+    FamScope fs(c->stream, FAM_LAGRANGE_POINT, (24.0 + 27.0 * sizeof(W)) * (double)c->plane * c->nz);
     KP p = kp(c);
     const double *um = nullptr, *vm = nullptr;
     if (sizeof(W) < sizeof(double)) {
@@ -780,6 +788,7 @@ __global__ __launch_bounds__(256, 2) void k_lag_contract(const W* __restrict__ b
 
 template <typename W>
 void lagrange_contract_dev(les_ctx* c, const W* b2, const W* b4, int init_lm, int init_qn) {
+    FamScope fs(c->stream, FAM_LAGRANGE_POINT, (42.0 * sizeof(W) + 56.0) * (double)c->plane * c->nz);
     KP p = kp(c);
     const double bexp = log(2.0) / (log(4.0) - log(2.0));
     const LagUpdate u{c->f3[LES_F_F_LM], c->f3[LES_F_F_MM], c->f3[LES_F_F_QN], c->f3[LES_F_F_NN],
@@ -814,11 +823,13 @@ __global__ void k_ddz_w(W* d, const TI* __restrict__ f, KP p) {
 
 template <typename W, typename TI>
 void ddz_uv_dev(les_ctx* c, W* d, const TI* f) {
+    FamScope fs(c->stream, FAM_DDZ, (double)(sizeof(TI) + sizeof(W)) * (double)c->plane * (double)c->nlev);
     KP p = kp(c);
     hipLaunchKernelGGL((k_ddz_uv<W, TI>), grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, d, f, p);
 }
 template <typename W, typename TI>
 void ddz_w_dev(les_ctx* c, W* d, const TI* f) {
+    FamScope fs(c->stream, FAM_DDZ, (double)(sizeof(TI) + sizeof(W)) * (double)c->plane * (double)c->nlev);
     KP p = kp(c);
     hipLaunchKernelGGL((k_ddz_w<W, TI>), grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, d, f, p);
 }
@@ -845,6 +856,7 @@ __global__ void k_divstress(W* out, const W* __restrict__ a, const W* __restrict
 
 template <typename W>
 void divstress_combine_dev(les_ctx* c, W* out, const W* a, const W* b, const W* d, int wver) {
+    FamScope fs(c->stream, FAM_DIVSTRESS, 4.0 * sizeof(W) * (double)c->plane * (double)c->nlev);
     KP p = kp(c);
     hipLaunchKernelGGL(k_divstress<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, out, a, b, d, p, wver);
 }
@@ -919,6 +931,7 @@ __global__ void k_theta_prep(W* dTdz, double* rhsf, const double* __restrict__ r
 
 template <typename W>
 void theta_prep_dev(les_ctx* c) {
+    FamScope fs(c->stream, FAM_THETA_POINT, (2.0 * sizeof(W) + 16.0) * (double)c->plane * (double)c->nlev);
     KP p = kp(c);
     const les_params& P = c->prm;
     hipLaunchKernelGGL(k_theta_prep<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, WF<W>(c, LES_F_DTDZ),
@@ -995,6 +1008,7 @@ __global__ void k_scalar_flux(W* temp, double* sgs_t3, const W* __restrict__ Nu,
 
 template <typename W>
 void scalar_flux_dev(les_ctx* c, W* temp, int comp) {
+    FamScope fs(c->stream, FAM_SCALAR_FLUX, (3.0 * sizeof(W) + (comp == 2 ? 8.0 : 0.0)) * (double)c->plane * (double)c->nlev);
     KP p = kp(c);
     const les_params& P = c->prm;
     const W* ds = WF<W>(c, comp == 0 ? LES_F_DTDX : comp == 1 ? LES_F_DTDY : LES_F_DTDZ);
@@ -1036,6 +1050,7 @@ __global__ void k_ddz_w_accum(double* rhs, const W* __restrict__ f, KP p) {
 
 template <typename W>
 void ddz_w_accum_dev(les_ctx* c, const W* f) {
+    FamScope fs(c->stream, FAM_SCALAR_FLUX, (16.0 + sizeof(W)) * (double)c->plane * (double)c->nlev);
     KP p = kp(c);
     hipLaunchKernelGGL(k_ddz_w_accum<W>, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream,
                        c->f3[LES_F_RHST], f, p);
@@ -1043,6 +1058,7 @@ void ddz_w_accum_dev(les_ctx* c, const W* f) {
 
 template <typename W>
 void scalar_accum_dev(les_ctx* c, const W* dtemp, const W* adv) {
+    FamScope fs(c->stream, FAM_SCALAR_FLUX, (16.0 + (adv ? 2.0 : 1.0) * sizeof(W)) * (double)c->plane * (double)c->nlev);
     KP p = kp(c);
     hipLaunchKernelGGL(k_scalar_accum<W>, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream,
                        c->f3[LES_F_RHST], dtemp, adv, p);
@@ -1074,6 +1090,7 @@ __global__ void k_calc_beta(W* beta, const double* __restrict__ th, const double
 
 template <typename W>
 void calc_beta_dev(les_ctx* c, const double* means) {
+    FamScope fs(c->stream, FAM_THETA_POINT, (8.0 + sizeof(W)) * (double)c->plane * (double)c->nlev);
     KP p = kp(c);
     const les_params& P = c->prm;
     hipLaunchKernelGGL(k_calc_beta<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream,
@@ -1100,6 +1117,7 @@ __global__ void k_scalar_top(double* th, KP p, int above1, double topgrad) {
 }
 
 void step_scalar_dev(les_ctx* c) {
+    FamScope fs(c->stream, FAM_THETA_POINT, 32.0 * (double)c->plane * (double)c->nlev);
     KP p = kp(c);
     const les_params& P = c->prm;
     const double topgrad = P.dTdz_top / P.T_scale * P.z_i * c->dz;
@@ -1144,6 +1162,7 @@ __global__ void k_press_prep(W* rx, W* ry, W* rz, const double* __restrict__ u, 
 template <typename W>
 void press_rhs_prep_on(les_ctx* c, W* rx, W* ry, W* rz, const double* u, const double* v, const double* w,
                        const W* divtz, double modi) {
+    FamScope fs(c->stream, FAM_PRESS_PREP, (24.0 + 3.0 * sizeof(W)) * (double)c->plane * (double)c->nlev);
     KP p = kp(c);
     hipLaunchKernelGGL(k_press_prep<W>, grid_for(p.plane * (p.nz + 1)), dim3(256), 0, c->stream, rx, ry, rz, u, v, w,
                        divtz, p, 1.0 / (double)(c->nx * c->ny), (int)(sizeof(W) < sizeof(double)), modi);
@@ -1197,6 +1216,7 @@ __global__ void k_hz_fold(const double* __restrict__ part, int nlev, double* __r
 
 template <typename W>
 void press_hz_sums_dev(les_ctx* c, double* out, double modi) {
+    FamScope fs(c->stream, FAM_PRESS_PREP, (8.0 + sizeof(W)) * (double)c->plane * (double)c->nlev);
     KP p = kp(c);
     double* part = c->work3[5];  // (nz+1) * HZ_SPLIT partial sums
     hipLaunchKernelGGL(k_hz_part<W>, dim3((p.nz + 1) * HZ_SPLIT), dim3(256), 0, c->stream,
@@ -1217,6 +1237,7 @@ __global__ void k_dpdz(W* dpdz, const W* __restrict__ pr, KP p) {
 
 template <typename W>
 void dpdz_on(les_ctx* c, W* dpdz, const W* pr) {
+    FamScope fs(c->stream, FAM_PRESS_PREP, 2.0 * sizeof(W) * (double)c->plane * (double)c->nlev);
     KP p = kp(c);
     hipLaunchKernelGGL(k_dpdz<W>, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, dpdz, pr, p);
 }
@@ -1318,6 +1339,7 @@ __global__ void k_rhs_ab2(RhsArgs<W> a, double* u, double* v, double* w, KP p, R
 
 template <typename W>
 void rhs_ab2_dev(les_ctx* c, int jt) {
+    FamScope fs(c->stream, FAM_RHS_AB2, (7.0 * sizeof(W) + 96.0) * (double)c->plane * c->nz);
     KP p = kp(c);
     const les_params& P = c->prm;
     RhsArgs<W> a;
@@ -1463,6 +1485,7 @@ void theta_rows_dev(les_ctx* c) {
 // theta_rows = false: theta's rows are left to theta_rows_dev
 template <typename W>
 void gradp_project_dev(les_ctx* c, int jt, bool theta_rows) {
+    FamScope fs(c->stream, FAM_GRADP_PROJECT, (96.0 + 3.0 * sizeof(W)) * (double)c->plane * (c->nz - 1));
     KP p = kp(c);
     const les_params& P = c->prm;
     const bool st = stats_step(c, jt);
@@ -1569,6 +1592,7 @@ __global__ __launch_bounds__(256) void k_max3_final(const double* __restrict__ p
 }
 
 void max3_dev(les_ctx* c, double* out, bool lag) {
+    FamScope fs(c->stream, FAM_CFL_MAX, (lag ? 9.0 : 3.0) * 8.0 * (double)c->plane * (double)c->nlev);
     const long lo = (long)c->plane, hi = (long)c->plane * c->nz;
     const long n = lag ? (long)c->plane * (long)c->nlev : hi - lo;
     long nb = (n + 255) / 256;

@@ -197,6 +197,10 @@ void tdma_press(les_ctx* c, const cx<T>* Hx, const cx<T>* Hy, const cx<T>* Hz, c
                 const double* hzm) {
     const long ns = (long)c->ns;
     const int nb = (int)((2 * ns + 127) / 128);
+    // Hx, Hy (levels 1..nz-1) and Hz (0..nz) read, p_hat (0..nz-1) written
+    FamScope fs(c->stream, FAM_TDMA,
+                (double)ns * 2.0 * sizeof(T) * (double)(2 * (c->nz - 1) + (c->nz + 1) + c->nz) +
+                    (hzm ? 8.0 * (c->nz + 1) : 0.0));
     hipLaunchKernelGGL(k_tdma<T>, dim3(nb), dim3(128), 0, c->stream, reinterpret_cast<const T*>(Hx),
                        reinterpret_cast<const T*>(Hy), reinterpret_cast<const T*>(Hz), dsc,
                        reinterpret_cast<T*>(ph), hzm, c->kx2d, c->ky2d, c->k2, c->notnyq, c->coef_c, ns,

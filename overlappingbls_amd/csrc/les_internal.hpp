@@ -126,6 +126,7 @@ struct les_ctx {
     int cfl_cap;
     int f_lm_mm_init, f_qn_nn_init;
     int sgs_frozen;            // skip the Lagrangian SGS update (Cs_opt2, F_* held)
+    int lin_obj;               // objective of the derivative windows (LES_OBJ_*)
     int fuse;                  // fused kernels (LES_FUSE=0 at creation: the separate ones)
     int overlap;               // theta's RHS and update on aux (LES_OVERLAP=0: in order)
     int lag_done;              // this step's u_lag accumulation already done (k_max3_lag)
@@ -150,6 +151,29 @@ inline W* WF(les_ctx* c, int id) {
 }
 
 namespace les {
+
+// ---- per-family device time (les_prof.hip) ----------------------------------
+// Kernel families of the step.  While les_profile_step runs a step, every
+// launch of an instrumented family is bracketed by two HIP events on the
+// stream it is enqueued on, and its algorithmic bytes (inputs read once,
+// outputs written once) are recorded.
+enum Fam : int {
+    FAM_XDERIV = 0, FAM_YDERIV, FAM_XFILTER, FAM_YFILTER, FAM_XPAD, FAM_YPAD, FAM_XUNPAD, FAM_YUNPAD,
+    FAM_XR2C, FAM_XC2R, FAM_YSPEC, FAM_TDMA, FAM_VORT_PAD_X, FAM_CROSS_UNPAD_X, FAM_SCALAR_UNPAD_X,
+    FAM_LAG_CONTRACT_X, FAM_RHS_AB2, FAM_GRADP_PROJECT, FAM_SGS_STRESS, FAM_SCALAR_FLUX, FAM_DDZ,
+    FAM_DIVSTRESS, FAM_CFL_MAX, FAM_PLANE_SUMS, FAM_PRESS_PREP, FAM_LAGRANGE_POINT, FAM_THETA_POINT,
+    FAM_N
+};
+const char* fam_name(int f);
+struct FamProf;
+extern FamProf* g_prof;   // non-null only inside les_profile_step
+// RAII bracket of one launch (or a dispatcher's launches) of family fam
+struct FamScope {
+    FamScope(hipStream_t s, int fam, double bytes);
+    ~FamScope();
+    int slot = -1;
+    hipStream_t s = nullptr;
+};
 
 // ---- line transforms (k_lines.hip) ---------------------------------------
 // Real x-lines: nrows rows, row r at in + r*ld_in (length nin) -> out + r*ld_out.

@@ -175,13 +175,14 @@ struct Run {
     les_ctx* c;
     LinBufs& b;
     const Tape& t;
-    double wobj;      // weight of ustar^2 in the objective at this step
-    double* dus2;     // tlm: 2 us d(us) plane (objective steps), or null
+    double wobj;      // weight of the objective at this step
+    double* dus2;     // tlm: 2 us d(us) plane (ustar^2 objective steps), or null
     ObkArgs oa;
     WallArgs wa;
     RhsFlags rf;
     double gbeta;
     int sf;
+    SfcArgs sa;
 };
 
 ObkArgs obk_args(const les_ctx* c) {
@@ -205,6 +206,14 @@ WallArgs wall_args(const les_ctx* c) {
     w.d0 = P.d0;
     w.zo = P.zo;
     return w;
+}
+SfcArgs sfc_args(const les_ctx* c) {
+    const les_params& P = c->prm;
+    SfcArgs a;
+    a.bulk = P.lbc == 0;
+    a.T_s = P.T_s;
+    a.log_zos = log(c->dz / (2.0 * ((1.0 / 10.0) * P.zo)));   // z_os = zo/10, 008_bottombc.f90:126
+    return a;
 }
 RhsFlags rhs_flags(const les_ctx* c) {
     const les_params& P = c->prm;
@@ -320,7 +329,7 @@ void s_wall_tl(Run& r) {
 void s_wall_adj(Run& r) {
     les_ctx* c = r.c;
     const size_t np = NP(r);
-    wall_adj(c, r.t, r.b, r.oa, r.wa, r.wobj);
+    wall_adj(c, r.t, r.b, r.oa, r.wa, r.c->lin_obj == LES_OBJ_USTAR2 ? r.wobj : 0.0);
     test_filter_f<double>(c, r.b.uv1f, 2, c->Gtest);
     lin_axpy(c, V(r, L_U) + np, r.b.uv1f, 1.0, (long)np);
     lin_axpy(c, V(r, L_V) + np, r.b.uv1f + np, 1.0, (long)np);
@@ -416,6 +425,8 @@ void s_sunpad_adj(Run& r) {
 }
 void s_sflux_tl(Run& r) { sflux_tl(r.c, r.t, r.b); }
 void s_sflux_adj(Run& r) { sflux_adj(r.c, r.t, r.b); }
+void s_sfc_tl(Run& r) { sfc_tl(r.c, r.t, r.b, r.oa, r.wa, r.sa); }
+void s_sfc_adj(Run& r) { sfc_adj(r.c, r.t, r.b, r.oa, r.wa, r.sa); }
 void s_sdiv_tl(Run& r) {
     ddx_f<double, double>(r.c, W3(r, 0), V(r, L_FX), 0, NL(r));
     ddy_f<double, double>(r.c, W3(r, 1), V(r, L_FY), 0, NL(r));
@@ -476,8 +487,8 @@ std::vector<Stage> stages(int nz) {
     auto R = [](int v, int l0, int l1) { return Reg{v, l0, l1}; };
     std::vector<Stage> s;
     s.push_back({"obukhov", s_obk_tl, s_obk_adj,
-                 {R(L_U, 1, 1), R(L_V, 1, 1), R(L_PSI, 0, 0), R(L_TH, 1, 1)},
-                 {R(L_PSI, 0, 0), R(L_PHI, 0, 0)}});
+                 {R(L_U, 1, 1), R(L_V, 1, 1), R(L_PSI, 0, 0), R(L_TH, 1, 1), R(L_Q, 0, 0)},
+                 {R(L_PSI, 0, 0), R(L_PHI, 0, 0), R(L_PSH, 0, 0)}});
     s.push_back({"velocity_gradients", s_deriv_tl, s_deriv_adj,
                  {R(L_U, a, b), R(L_V, a, b), R(L_W, a, b)},
                  {R(L_UX, a, b), R(L_UY, a, b), R(L_VX, a, b), R(L_VY, a, b), R(L_WX, a, b), R(L_WY, a, b),
@@ -490,7 +501,7 @@ std::vector<Stage> stages(int nz) {
                  {R(L_TB1, a, b), R(L_TB2, a, b), R(L_TB3, 2, b)}});
     s.push_back({"wallstress", s_wall_tl, s_wall_adj,
                  {R(L_U, 1, 1), R(L_V, 1, 1), R(L_PSI, 0, 0), R(L_PHI, 0, 0)},
-                 {R(L_TXZ, 1, 1), R(L_TYZ, 1, 1), R(L_UZ, 1, 1), R(L_VZ, 1, 1)}});
+                 {R(L_TXZ, 1, 1), R(L_TYZ, 1, 1), R(L_UZ, 1, 1), R(L_VZ, 1, 1), R(L_DUS, 0, 0)}});
     s.push_back({"strain", s_strain_tl, s_strain_adj,
                  {R(L_UX, a, b), R(L_UY, a, b), R(L_UZ, a, b), R(L_VX, a, b), R(L_VY, a, b), R(L_VZ, a, b),
                   R(L_WX, a, b), R(L_WY, a, b), R(L_WZ, a, b)},
@@ -523,8 +534,10 @@ std::vector<Stage> stages(int nz) {
     s.push_back({"scalar_sgs_flux", s_sflux_tl, s_sflux_adj,
                  {R(L_NU, 1, nz), R(L_TX, a, b), R(L_TY, a, b), R(L_TZ, 2, b)},
                  {R(L_FX, a, b), R(L_FY, a, b), R(L_FZ, 2, nz)}});
+    s.push_back({"surface_flux", s_sfc_tl, s_sfc_adj, {R(L_TH, 1, 1), R(L_DUS, 0, 0), R(L_PSH, 0, 0)},
+                 {R(L_Q, 0, 0), R(L_FZ, 1, 1)}});
     s.push_back({"scalar_rhs", s_sdiv_tl, s_sdiv_adj,
-                 {R(L_ADV, a, b), R(L_FX, a, b), R(L_FY, a, b), R(L_FZ, 2, nz)}, {R(L_RT, a, b)}});
+                 {R(L_ADV, a, b), R(L_FX, a, b), R(L_FY, a, b), R(L_FZ, 1, nz)}, {R(L_RT, a, b)}});
     s.push_back({"calc_beta", s_beta_tl, s_beta_adj, {R(L_TH, a, b)}, {R(L_BETA, 2, b)}});
     s.push_back({"step_scalar", s_sstep_tl, s_sstep_adj, {R(L_TH, a, b), R(L_RT, a, b), R(L_RTF, a, b)},
                  {R(L_TH, a, b)}});
@@ -554,11 +567,12 @@ std::vector<Reg> state_regs(int nz) {
     std::vector<Reg> r;
     for (int v = L_U; v <= L_RT; ++v) r.push_back({v, 1, nz - 1});
     r.push_back({L_PSI, 0, 0});
+    r.push_back({L_Q, 0, 0});
     return r;
 }
 
 Run make_run(les_ctx* c, const Tape& t, double wobj, double* dus2) {
-    Run r{c, c->lin->b, t, wobj, dus2, c->lin->oa, wall_args(c), rhs_flags(c), 0.0, 0};
+    Run r{c, c->lin->b, t, wobj, dus2, c->lin->oa, wall_args(c), rhs_flags(c), 0.0, 0, sfc_args(c)};
     const les_params& P = c->prm;
     r.gbeta = GRAV * (P.z_i / (P.u_scale * P.u_scale));
     r.sf = P.ubc_mom == LES_BC_STRESS_FREE;
@@ -569,14 +583,15 @@ Run make_run(les_ctx* c, const Tape& t, double wobj, double* dus2) {
 void tlm_step(les_ctx* c, const Tape& t, double wobj) {
     LinBufs& b = c->lin->b;
     for (int m = 0; m < 3; ++m) std::swap(b.v[L_RX + m], b.v[L_RXF + m]);
-    Run r = make_run(c, t, wobj, wobj != 0.0 ? b.us2 : nullptr);
+    const bool us2 = c->lin_obj == LES_OBJ_USTAR2;
+    Run r = make_run(c, t, wobj, (wobj != 0.0 && us2) ? b.us2 : nullptr);
     const std::vector<Stage> st = stages(c->nz);
     for (const Stage& s : st) {
         s.tl(r);
         lin_flush(c, b);
     }
     if (wobj != 0.0) {
-        lin_dot_part(c, b.us2, nullptr, (long)c->plane, 0, 0, 0, 2, b.part);
+        lin_dot_part(c, us2 ? b.us2 : b.v[L_Q], nullptr, (long)c->plane, 0, 0, 0, 2, b.part);
         lin_acc(c, b.part, wobj, b.acc);
     }
 }
@@ -586,6 +601,8 @@ void adj_step(les_ctx* c, const Tape& t, double wobj) {
     LinBufs& b = c->lin->b;
     Run r = make_run(c, t, wobj, nullptr);
     const std::vector<Stage> st = stages(c->nz);
+    // the surface flux objective reads this step's output sgs_t3(:,:,1)
+    if (wobj != 0.0 && c->lin_obj == LES_OBJ_T_FLUX) lin_addc(c, b.v[L_Q], wobj, (long)c->plane);
     for (int i = (int)st.size() - 1; i >= 0; --i) {
         st[i].adj(r);
         lin_flush(c, b);
@@ -607,10 +624,14 @@ void clear_intermediates(les_ctx* c) {
     for (int v = L_NSTATE; v < L_COUNT; ++v) fill_levels(c->stream, b.v[v], lv_size(c, v), 0, 1, 0.0);
 }
 
-// the objective: J += wobj * sum over the plane of ustar^2
+// the objective: J += wobj * sum over the plane of ustar^2, or of the
+// surface flux sgs_t3(:,:,1)
 void obj_accumulate(les_ctx* c, double wobj) {
     LinBufs& b = c->lin->b;
-    lin_dot_part(c, c->ustar, nullptr, (long)c->plane, 0, 0, 0, 1, b.part);
+    if (c->lin_obj == LES_OBJ_T_FLUX)
+        lin_dot_part(c, c->f3[LES_F_SGS_T3] + c->plane, nullptr, (long)c->plane, 0, 0, 0, 2, b.part);
+    else
+        lin_dot_part(c, c->ustar, nullptr, (long)c->plane, 0, 0, 0, 1, b.part);
     lin_acc(c, b.part, wobj, b.acc + 1);
 }
 
@@ -619,8 +640,8 @@ int lin_check_supported(const les_ctx* c, int jt0, int nsteps) {
     if (c->precision != LES_FP64) return LES_E_BADARG;
     if (!P.theta_flag || jt0 <= P.theta_init_time || jt0 < 2 || nsteps < 0) return LES_E_BADARG;
     if (P.lbc_mom != LES_BC_WALL || P.ubc_mom != LES_BC_STRESS_FREE) return LES_E_BADARG;
-    // linearised: the land step with a prescribed surface flux and damping method 1
-    if (P.ocean_flag || P.lbc != 1 || (P.ubc == 1 && P.damping_method != 1)) return LES_E_BADARG;
+    // linearised: the land step (prescribed surface flux or temperature) with damping method 1
+    if (P.ocean_flag || (P.ubc == 1 && P.damping_method != 1)) return LES_E_BADARG;
     return LES_OK;
 }
 
@@ -635,9 +656,12 @@ int lin_var(int id) {
         case LES_F_RHSZ: return L_RZ;
         case LES_F_RHST: return L_RT;
         case LES_F_PSI_M_SFC: return L_PSI;
+        case LES_F_SGS_T3: return L_Q;
         default: return -1;
     }
 }
+// the level a one-plane linear variable is addressed at (sgs_t3: level 1)
+int lin_plane_lev(int id) { return id == LES_F_SGS_T3 ? 1 : 0; }
 
 // random values on a region of variable v (zeros elsewhere in the variable
 // when fresh), or the inner product of the region with the same values
@@ -733,10 +757,11 @@ int les_set_sgs_frozen(les_ctx* c, int frozen) {
 int les_lin_set(les_ctx* c, int id, const double* h, int ld, int lev0, int nlev) {
     const int v = lin_var(id);
     if (!c || !h || v < 0 || ld < c->nx) return LES_E_BADARG;
-    if (lin_plane_only(v) ? (lev0 != 0 || nlev != 1) : (lev0 < 0 || nlev < 0 || lev0 + nlev > (int)c->nlev))
+    const bool pl = lin_plane_only(v);
+    if (pl ? (lev0 != lin_plane_lev(id) || nlev != 1) : (lev0 < 0 || nlev < 0 || lev0 + nlev > (int)c->nlev))
         return LES_E_BADARG;
     if (lin_alloc(c) != hipSuccess) return LES_E_NOMEM;
-    double* d = c->lin->b.v[v] + (size_t)lev0 * c->plane;
+    double* d = c->lin->b.v[v] + (pl ? 0 : (size_t)lev0 * c->plane);
     les_note(c, hipMemcpy2DAsync(d, (size_t)c->nx * 8, h, (size_t)ld * 8, (size_t)c->nx * 8, (size_t)nlev * c->ny,
                                  hipMemcpyHostToDevice, c->stream));
     return finish(c);
@@ -745,13 +770,20 @@ int les_lin_set(les_ctx* c, int id, const double* h, int ld, int lev0, int nlev)
 int les_lin_get(les_ctx* c, int id, double* h, int ld, int lev0, int nlev) {
     const int v = lin_var(id);
     if (!c || !h || v < 0 || ld < c->nx) return LES_E_BADARG;
-    if (lin_plane_only(v) ? (lev0 != 0 || nlev != 1) : (lev0 < 0 || nlev < 0 || lev0 + nlev > (int)c->nlev))
+    const bool pl = lin_plane_only(v);
+    if (pl ? (lev0 != lin_plane_lev(id) || nlev != 1) : (lev0 < 0 || nlev < 0 || lev0 + nlev > (int)c->nlev))
         return LES_E_BADARG;
     if (lin_alloc(c) != hipSuccess) return LES_E_NOMEM;
-    const double* d = c->lin->b.v[v] + (size_t)lev0 * c->plane;
+    const double* d = c->lin->b.v[v] + (pl ? 0 : (size_t)lev0 * c->plane);
     les_note(c, hipMemcpy2DAsync(h, (size_t)ld * 8, d, (size_t)c->nx * 8, (size_t)c->nx * 8, (size_t)nlev * c->ny,
                                  hipMemcpyDeviceToHost, c->stream));
     return finish(c);
+}
+
+int les_lin_set_objective(les_ctx* c, int kind) {
+    if (!c || (kind != LES_OBJ_USTAR2 && kind != LES_OBJ_T_FLUX)) return LES_E_BADARG;
+    c->lin_obj = kind;
+    return LES_OK;
 }
 
 int les_lin_zero(les_ctx* c) {
@@ -763,7 +795,8 @@ int les_lin_zero(les_ctx* c) {
 
 // tangent-linear window: state advances nsteps from jt0; lin (tangent of
 // the state at jt0) becomes the tangent at jt0 + nsteps.  dJ: tangent of
-// the objective (mean of ustar^2 over the plane and the last obj_steps steps)
+// the objective (mean over the plane and the last obj_steps steps of ustar^2
+// or of the surface flux, les_lin_set_objective)
 int les_tlm_window(les_ctx* c, int jt0, int nsteps, int obj_steps, double* dJ) {
     if (!c || obj_steps < 0 || obj_steps > nsteps) return LES_E_BADARG;
     int st = lin_check_supported(c, jt0, nsteps);
@@ -792,8 +825,8 @@ int les_step_tlm(les_ctx* c, int jt) { return les_tlm_window(c, jt, 1, 0, nullpt
 // adjoint of the state at jt0 + nsteps (zero for a pure objective).  On
 // return the state is at jt0 + nsteps, lin holds the gradient with respect
 // to the state at jt0 of <lin seed, x(jt0 + nsteps)> + J, and *J the
-// objective (mean of ustar^2 over the plane and the last obj_steps steps).
-// Checkpoints every k_ckpt steps; the last segment is taped during the
+// objective (mean over the plane and the last obj_steps steps of ustar^2 or
+// of the surface flux).  Checkpoints every k_ckpt steps; the last segment is taped during the
 // forward sweep, every earlier one recomputed once with tapes.  k_ckpt = 0
 // picks the spacing from free device memory: as many tapes as fit with a
 // sixteenth of the device left free, up to the whole window (then nothing

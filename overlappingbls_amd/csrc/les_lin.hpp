@@ -7,11 +7,14 @@ namespace les {
 
 // Linear variables: the tangent (or adjoint) of each forward field the step
 // reads or writes.  L_U .. L_PSI are the step's state; the rest are the
-// step's intermediates.  Plain fields: nlev planes; L_PSI, L_PHI: one plane;
-// L_UB .. L_RM: nlev planes of the 3/2 grid.
+// step's intermediates.  Plain fields: nlev planes; L_PSI, L_Q, L_PHI, L_PSH,
+// L_DUS: one plane; L_UB .. L_RM: nlev planes of the 3/2 grid.  L_Q is the
+// surface scalar flux sgs_t3(:,:,1), which the next step's obukhov reads
+// (its plane mean); L_PSH is psi_h, L_DUS the per-point ustar of the wall
+// model.
 enum LinVar : int {
-    L_U, L_V, L_W, L_TH, L_RX, L_RY, L_RZ, L_RT, L_PSI,
-    L_RXF, L_RYF, L_RZF, L_RTF, L_PHI,
+    L_U, L_V, L_W, L_TH, L_RX, L_RY, L_RZ, L_RT, L_PSI, L_Q,
+    L_RXF, L_RYF, L_RZF, L_RTF, L_PHI, L_PSH, L_DUS,
     L_UX, L_UY, L_UZ, L_VX, L_VY, L_VZ, L_WX, L_WY, L_WZ,
     L_TX, L_TY, L_TZ,
     L_S11, L_S12, L_S13, L_S22, L_S23, L_S33, L_NU,
@@ -23,8 +26,8 @@ enum LinVar : int {
     L_UB, L_VB, L_WB, L_OB1, L_OB2, L_OB3, L_CB1, L_CB2, L_CB3, L_TB1, L_TB2, L_TB3, L_RM,
     L_COUNT
 };
-constexpr int L_NSTATE = L_PSI + 1;
-inline bool lin_plane_only(int v) { return v == L_PSI || v == L_PHI; }
+constexpr int L_NSTATE = L_Q + 1;
+inline bool lin_plane_only(int v) { return v == L_PSI || v == L_Q || v == L_PHI || v == L_PSH || v == L_DUS; }
 inline bool lin_padded(int v) { return v >= L_UB; }
 
 constexpr int LIN_NPART = 256;  // partial sums per reduction
@@ -57,6 +60,11 @@ LP lin_params(const les_ctx* c);
 
 struct ObkArgs {
     double nn, dz, zo_, g_;
+};
+// the bulk surface flux of lbc = 0: (T_s - theta(1)) kappa ustar / (log_zos - psi_h)
+struct SfcArgs {
+    int bulk;
+    double T_s, log_zos;
 };
 struct WallArgs {
     int smooth;
@@ -140,6 +148,9 @@ void sadv_tl(les_ctx* c, const Tape& t, LinBufs& b);
 void sadv_adj(les_ctx* c, const Tape& t, LinBufs& b);
 void sflux_tl(les_ctx* c, const Tape& t, LinBufs& b);
 void sflux_adj(les_ctx* c, const Tape& t, LinBufs& b);
+void sfc_tl(les_ctx* c, const Tape& t, LinBufs& b, const ObkArgs& oa, const WallArgs& wa, const SfcArgs& sa);
+void sfc_adj(les_ctx* c, const Tape& t, LinBufs& b, const ObkArgs& oa, const WallArgs& wa, const SfcArgs& sa);
+void lin_addc(les_ctx* c, double* y, double a, long n);
 void sdiv_tl(les_ctx* c, LinBufs& b, const double* t1, const double* t2);
 void sdiv_adj(les_ctx* c, LinBufs& b, const double* t1, const double* t2);
 void beta_tl(les_ctx* c, const Tape& t, LinBufs& b, double g);

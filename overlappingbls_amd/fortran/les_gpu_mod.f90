@@ -17,6 +17,7 @@ integer(c_int), parameter, public :: LES_FP64 = 0, LES_MIXED = 1
 integer(c_int), parameter, public :: LES_WALL_ROUGH = 0, LES_WALL_SMOOTH = 1
 integer(c_int), parameter, public :: LES_BC_WALL = 0, LES_BC_STRESS_FREE = 1
 integer(c_int), parameter, public :: LES_STOKES_DP = 0, LES_STOKES_MONO = 1, LES_STOKES_DYNAMIC = 2
+integer(c_int), parameter, public :: LES_OBJ_USTAR2 = 0, LES_OBJ_T_FLUX = 1
 
 ! field identifiers (enum les_field)
 integer(c_int), parameter, public :: LES_F_U = 0, LES_F_V = 1, LES_F_W = 2, LES_F_THETA = 3, &
@@ -66,7 +67,7 @@ public :: les_create, les_destroy, les_set_field, les_get_field, les_step, les_r
           les_obukhov, les_wallstress_bottom, les_wallstress_top, les_divstress_uv, &
           les_divstress_w, les_set_stokes, les_get_cfl, les_set_avgslice, les_get_avgslice, &
           les_ab2, les_tdma_solve, les_bind_field, les_upload, les_download, &
-          les_lin_set, les_lin_get, les_lin_zero, les_set_sgs_frozen, les_tlm_window, &
+          les_lin_set, les_lin_get, les_lin_zero, les_lin_set_objective, les_set_sgs_frozen, les_tlm_window, &
           les_adjoint_window, les_step_tlm, les_step_adj, &
           les_write_restart, les_read_restart, les_write_avgslice, &
           les_lagrange_sdep, les_interpolag_sdep, les_set_sgs_init_flags, les_get_sgs_init_flags, &
@@ -338,6 +339,11 @@ interface
     integer(c_int) function les_lin_zero(ctx) bind(C, name="les_lin_zero")
         import :: c_int, c_ptr
         type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function les_lin_set_objective(ctx, kind) bind(C, name="les_lin_set_objective")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: kind
     end function
     integer(c_int) function les_set_sgs_frozen(ctx, frozen) bind(C, name="les_set_sgs_frozen")
         import :: c_int, c_ptr

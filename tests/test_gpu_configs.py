@@ -7,7 +7,9 @@
     and the mixed-precision step for 25 steps against the FP64 GPU run
     (mean u and theta profiles within 1e-5, per-field errors printed);
   - sbl128 adjoint (C3): a 20-step window with a checkpoint every 5 steps,
-    J over the last 5: dot-product test and central differences.
+    J over the last 5 (mean ustar^2 under the prescribed surface flux; the
+    mean surface heat flux under a prescribed surface temperature):
+    dot-product test and central differences.
 Free-running bar as the free-running parity test: max(1e-10, NOISE times the
 spread of the oracle's rounding variants over the same window)."""
 import numpy as np
@@ -107,22 +109,29 @@ def test_sbl256_mixed_against_fp64(gpu_lib):
     assert all(v < 1e-3 for v in errs.values()), errs
 
 
-def test_sbl128_adjoint_window(gpu_lib):
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("case,objective", [("sbl128", "ustar2"), ("sbl128_hf", "t_flux")])
+def test_sbl128_adjoint_window(case, objective, gpu_lib):
     """C3: 128^3 FP64, SGS frozen, a 20-step window from jt = 3 with a
-    checkpoint every 5 steps (each earlier segment recomputed), J = mean
-    ustar^2 over the last 5 steps.  Dot-product test of the window's tangent
-    and adjoint <= 1e-10; <grad J, d> against central differences of J
-    (eps = 1e-6 relative to each field's rms) along 2 seeded directions
-    <= 1e-6."""
+    checkpoint every 5 steps (each earlier segment recomputed), J over the
+    last 5 steps: mean ustar^2 (sbl128, prescribed surface flux) or the mean
+    surface heat flux sgs_t3(:,:,1) (sbl128_hf, prescribed surface
+    temperature, so the flux depends on theta(1), ustar and psi_h).
+    Dot-product test of the window's tangent and adjoint <= 1e-10; <grad J,
+    d> against central differences of J (eps = 1e-6 relative to each
+    field's rms) along 2 seeded directions <= 1e-6."""
     import test_gpu_adjoint as ta
+    from overlappingbls_amd.abi import LES_OBJ_T_FLUX, LES_OBJ_USTAR2
+    kind = LES_OBJ_T_FLUX if objective == "t_flux" else LES_OBJ_USTAR2
     nsteps, k, obj, jt0 = 20, 5, 5, ta.JT0
 
     def spun():
-        nl = config.workload("sbl128")
+        nl = config.workload(case)
         g = gpu_lib.create(nl.params())
         config.load_state(g, config.initial_fields(nl, seed=2025))
         g.run(1, jt0 - 1)
         g.set_sgs_frozen(True)
+        g.set_objective(kind)
         return g
 
     g = spun()
@@ -136,6 +145,8 @@ def test_sbl128_adjoint_window(gpu_lib):
                 a = a.copy()
                 if n == "PSI_M_SFC":
                     a = a + eps * d[n]
+                elif n == "SGS_T3":
+                    a[1] += eps * d[n]
                 else:
                     a[1:g.nz] += eps * d[n][1:g.nz]
             g.set(n, a)
@@ -148,12 +159,11 @@ def test_sbl128_adjoint_window(gpu_lib):
     reset(); ta.lin_load(g, y)
     g.adjoint_window(jt0, nsteps, k, obj_steps=obj)
     grad_y = ta.lin_read(g)
-    cut = lambda d: {n: (d[n][1:g.nz] if n != "PSI_M_SFC" else d[n]) for n in tl}
-    lhs = ta.inner(tl, cut(y)) + dJ
-    rhs = ta.inner(cut(dx), grad_y)
+    lhs = ta.inner(tl, ta.cut(y, g.nz)) + dJ
+    rhs = ta.inner(ta.cut(dx, g.nz), grad_y)
     dot = ta.rel_diff(lhs, rhs)
     reset(); g.lin_zero()
-    g.adjoint_window(jt0, nsteps, k, obj_steps=obj)
+    Jw = g.adjoint_window(jt0, nsteps, k, obj_steps=obj)
     gJ = ta.lin_read(g)
     reset()
     sc = ta.state_scale(g)
@@ -164,15 +174,16 @@ def test_sbl128_adjoint_window(gpu_lib):
         for i in range(nsteps):
             g.step(jt0 + i)
             if i >= nsteps - obj:
-                s += float(np.mean(g.get("USTAR_SFC") ** 2))
+                s += ta.objective_value(g, kind)
         return s / obj
 
+    assert ta.rel_diff(Jw, J_at(None, 0.0)) < 1e-12
     errs = []
     for s in range(2):
         d = ta.direction(g, 30 + s, sc)
-        a = ta.inner(cut(d), gJ)
+        a = ta.inner(ta.cut(d, g.nz), gJ)
         f = (J_at(d, 1e-6) - J_at(d, -1e-6)) / 2e-6
         errs.append(ta.rel_diff(a, f))
-    print("sbl128 adjoint window: dot", dot, "fd", errs)
+    print(case, objective, "adjoint window: J", Jw, "dot", dot, "fd", errs)
     assert dot < ta.DOT_TOL
     assert all(e < ta.FD_TOL for e in errs), errs
