@@ -40,6 +40,7 @@ extern "C" {
 #define LES_E_NOMEM    -4   /* device allocation failed                    */
 #define LES_E_NODEV    -5   /* no GPU visible                              */
 #define LES_E_IO       -6   /* restart file missing, short or not writable */
+#define LES_E_SIZE     -7   /* request beyond the allocation-free entry point's fixed scratch */
 
 /* ---- precision --------------------------------------------------------- */
 #define LES_FP64   0
@@ -224,6 +225,18 @@ int les_fftn(les_ctx* ctx, const double* in, int ld, double* out_c);
 int les_ifftn(les_ctx* ctx, const double* in_c, double* out, int ld);
 /* zero_padding, 108_01_padd.f90:73-132: (ld x nyt) -> (3nxt/2 x 3nyt/2). */
 int les_zero_padding(les_ctx* ctx, const double* var, int ld, double* var_big);
+/* No counterpart in the reference.  zero_padding (as les_zero_padding) of
+ * nplanes planes already in device memory, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc): nothing is allocated; the intermediate of the two
+ * passes lives in a static device array of the library, 48 MiB per GPU held
+ * until the process ends.  d_var: nplanes planes of nyt rows of row stride
+ * ld >= nxt (plane stride ld*nyt); d_var_big: nplanes planes of
+ * (3nxt/2) x (3nyt/2).  Requires nplanes * nyt * 3nxt/2 <= 6*2^20 (48 MiB of
+ * doubles), else LES_E_SIZE.  Synchronous on the context's stream.  All calls
+ * on one GPU share the array: calls that overlap on one GPU (other threads,
+ * other contexts) race. */
+int les_zero_padding_noalloc(les_ctx* ctx, const double* d_var, int ld, double* d_var_big, int nplanes);
 /* zero_unpadding, 108_01_padd.f90:136-197: (3nxt/2 x 3nyt/2) -> (ld x nyt). */
 int les_zero_unpadding(les_ctx* ctx, const double* var_big, double* var, int ld);
 

@@ -18,6 +18,7 @@ LES_E_CFL = -3
 LES_E_NOMEM = -4
 LES_E_NODEV = -5
 LES_E_IO = -6
+LES_E_SIZE = -7
 
 K_TDMA, K_XDERIV, K_YDERIV, K_PAD, K_COPY = 0, 1, 2, 3, 4
 LES_OBJ_USTAR2, LES_OBJ_T_FLUX = 0, 1
@@ -134,6 +135,7 @@ def bind(lib: C.CDLL, prefix: str):
         "ifftn": [P, _PD, _PD, i],
         "zero_padding": [P, _PD, i, _PD],
         "zero_unpadding": [P, _PD, _PD, i],
+        "zero_padding_noalloc": [P, C.c_void_p, i, C.c_void_p, i],
         "obukhov": [P, i],
         "wallstress_bottom": [P], "wallstress_top": [P],
         "sgs_stag": [P, i],

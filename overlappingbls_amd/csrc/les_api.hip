@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <utility>
 #include <vector>
 #include "les_lag.hpp"
@@ -751,6 +752,33 @@ int les::finish(les_ctx* c) {
     return first == hipSuccess ? LES_OK : LES_E_HIP;
 }
 
+// ---- scratch of the allocation-free entry point -----------------------------
+// les_zero_padding_noalloc keeps the x-padded intermediate of zero_padding in
+// this array instead of context memory.  Every GPU the library runs on holds
+// its own copy until the process ends.  Host code never uses the array's name
+// as an address: les_create looks up each device's copy once
+// (hipGetSymbolAddress) and the entry point passes that pointer.
+constexpr size_t NOALLOC_DOUBLES = (size_t)6 << 20;   // 48 MiB
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+constexpr int NOALLOC_MAX_DEV = 64;
+static double* g_scratch_dev[NOALLOC_MAX_DEV];        // per device, set once
+
+static std::mutex g_scratch_mu;
+
+// the device copy of scratch for the current device (looked up once)
+static hipError_t noalloc_scratch(int dev, double** out) {
+    if (dev < 0 || dev >= NOALLOC_MAX_DEV) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_dev[dev] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_dev[dev];
+    return hipSuccess;
+}
+
 // ============================================================================
 // C-ABI
 // ============================================================================
@@ -792,6 +820,7 @@ int les_create(const les_params* prm, les_ctx** out) {
     c->dz = prm->lz_tot / (c->nz - 1);
     c->precision = LES_FP64;
     if (hipGetDevice(&c->dev) != hipSuccess) { delete c; return LES_E_NODEV; }
+    if (noalloc_scratch(c->dev, &c->noalloc) != hipSuccess) { delete c; return LES_E_HIP; }
     c->err = hipSuccess;
     if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return LES_E_HIP; }
     if (hipStreamCreate(&c->aux) != hipSuccess || hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
@@ -1088,6 +1117,19 @@ int les_zero_padding(les_ctx* c, const double* var, int ld, double* big) {
                      c->stream));
     pad_f<double, double>(c, w, c->big3[0], 1);
     les_note(c, hipMemcpyAsync(big, c->big3[0], c->plane2 * 8, hipMemcpyDeviceToHost, c->stream));
+    return finish(c);
+}
+
+// zero_padding of nplanes device planes with the intermediate in the
+// library's static scratch: no device memory is allocated
+int les_zero_padding_noalloc(les_ctx* c, const double* d_var, int ld, double* d_big, int nplanes) {
+    if (!c || !d_var || !d_big || ld < c->nx || nplanes < 1) return LES_E_BADARG;
+    if ((size_t)nplanes * c->ny * c->nx2 > NOALLOC_DOUBLES) return LES_E_SIZE;
+    double* tmp = c->noalloc;
+    xline_real<double, double>(c->stream, d_var, tmp, (long)nplanes * c->ny, ld, c->nx2, c->tw_x, c->tw_x2, OP_PAD,
+                               0.0);
+    yline_real<double>(c->stream, tmp, d_big, nplanes, c->nx2, c->nx2, c->nx2, (long)c->ny * c->nx2,
+                       (long)c->plane2, c->tw_y, c->tw_y2, OP_PAD, 0.0);
     return finish(c);
 }
 

@@ -98,6 +98,7 @@ struct les_ctx {
     double* ustar_top;              // ocean obukhov: ustar_avg of the top wall (one plane)
     double wind_ustar, wind_rad;    // ocean wind stress (u_star / u_scale, radians)
     double* dmean;                  // plane means of u, v, w for damping methods 2, 3 (3 nlev)
+    double* noalloc;                // this device's copy of the allocation-free entry point's scratch
     double *u_big, *v_big, *w_big;                  // nlev planes of nx2*ny2
     // wavenumber tables (half spectrum, 002_fft.f90:143-167)
     double *kx2d, *ky2d, *k2, *Gtest, *Gtest2;
