@@ -85,6 +85,10 @@ def parse():
                     help="run only the adjoint leg's timed windows (for rocprofv3)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="run only the kernel-family timing loops, FP64 (for rocprofv3 --pmc)")
+    ap.add_argument("--pmc-run", action="store_true",
+                    help="the workload of the counter summary (run under rocprofv3 --pmc)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="per-family HBM traffic from two rocprofv3 --pmc runs of --pmc-run")
     ap.add_argument("--mixed-drift", type=int, default=0, metavar="N",
                     help="only: run FP64 and mixed N steps side by side and print their difference every N/10 steps")
     return ap.parse_args()
@@ -275,6 +279,142 @@ def adjoint_leg(workload="sbl128", nsteps=20, k_ckpt=5, obj_steps=5, ndir=8, eps
     return res
 
 
+# Kernel families of les_profile_step by kernel name (the library's FamScope
+# brackets), for the counter summary.  Line kernels carry their operation as
+# the last template argument: 0 derivative, 1 Nyquist filter, 2 pad, 3 unpad.
+_LINE_OPS = {"0": "deriv", "1": "filter", "2": "pad", "3": "unpad"}
+_POINT_FAMILIES = [
+    (("k_obukhov_sums", "k_level_means_seq"), "plane_sums"),
+    (("k_rhs_ab2",), "rhs_ab2"),
+    (("k_gradp_project", "k_project_bc"), "gradp_project"),
+    (("k_strain_stress", "k_strain<", "k_stress<"), "sgs_stress"),
+    (("k_scalar_flux", "k_scalar_accum", "k_ddz_w_accum"), "scalar_flux"),
+    (("k_ddz_uv", "k_ddz_w<"), "ddz"),
+    (("k_divstress",), "divstress"),
+    (("k_max3_lag", "k_max3_part", "k_max3_final"), "cfl_max"),
+    (("k_press_prep", "k_dpdz", "k_hz_part", "k_hz_fold"), "press_prep"),
+    (("k_lag_prep", "k_interpolag", "k_lag_contract<", "k_lag_acc"), "lagrange_point"),
+    (("k_theta_prep", "k_step_scalar", "k_calc_beta", "k_scalar_top"), "theta_point"),
+    (("k_tdma",), "tdma"),
+    (("kr_lag_contract",), "lag_contract_x"),
+    (("kr_x_r2c", "k_x_r2c"), "x_r2c"),
+    (("kr_x_c2r", "k_x_c2r"), "x_c2r"),
+    (("kr_yspec", "k_yline_spec"), "yline.spec"),
+    (("k_vort<",), "vort_pad_x"),
+]
+# the first kernel of every step: the CFL pass (les_run)
+_STEP_START = ("k_max3_lag", "k_max3_part")
+
+
+def kernel_family(name: str) -> str:
+    import re
+    m = re.search(r"kr_([xy])line<[^>]*?, (\d+)>", name)
+    if m:
+        return f"{m.group(1)}line.{_LINE_OPS[m.group(2)]}"
+    if "kr_fused_x" in name:
+        if "VortIn" in name:
+            return "vort_pad_x"
+        m = re.search(r"kr_fused_x<[^,]+, \d+, \d+, \d+, (\d+),", name)
+        return "scalar_unpad_x" if (m and m.group(1) == "1") else "cross_unpad_x"
+    for keys, fam in _POINT_FAMILIES:
+        if any(k in name for k in keys):
+            return fam
+    return "other"
+
+
+def pmc_run(workload: str):
+    """The workload the counter summary reads, under rocprofv3 --pmc: a
+    device copy of one field (calibration), steps 1-3 (warm-up), then the
+    plain step 4 and the dynamic-SGS update step 5 (cs_count = 5)."""
+    from overlappingbls_amd import config, lib
+    from overlappingbls_amd.abi import K_COPY
+    nl = config.workload(workload)
+    ctx = lib.create(nl.params())
+    config.load_state(ctx, config.initial_fields(nl, seed=2024))
+    ctx.time_kernel(K_COPY, 1)
+    ctx.run(1, 5)
+    ctx.sync()
+    print(json.dumps({"pmc_run": workload, "steps": [4, 5], "cs_count": nl.cs_count}))
+
+
+def _pmc_rows(d: str, counter: str):
+    import csv
+    import glob
+    files = sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True))
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {d}")
+    rows = []
+    with open(files[0]) as f:
+        for r in csv.DictReader(f):
+            if r.get("Counter_Name") == counter:
+                rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    return rows
+
+
+def pmc_summary(fetch_dir: str, write_dir: str, workload: str):
+    """HBM traffic per family of the plain step 4 and the update step 5 of
+    pmc_run, from the FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch),
+    scaled so that the calibration copy reads its known size; writes
+    profiles/traffic.json and prints a markdown table."""
+    from overlappingbls_amd import config
+    nl = config.workload(workload)
+    p = nl.params()
+    field_bytes = 8.0 * p.nxt * p.nyt * (p.nzt + 1)
+    out = {}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        rows = _pmc_rows(d, counter)
+        copies = [v for _, n, v in rows if "copyBuffer" in n and v > 0.5 * field_bytes / 1024.0]
+        if not copies:
+            raise SystemExit("no calibration copy in " + d)
+        factor = field_bytes / (copies[0] * 1024.0)
+        starts = [i for i, (_, n, _) in enumerate(rows) if any(k in n for k in _STEP_START)]
+        if len(starts) != 5:
+            raise SystemExit(f"expected 5 steps in {d}, found {len(starts)}")
+        steps = {4: rows[starts[3]:starts[4]], 5: rows[starts[4]:]}
+        out[counter] = {"factor": factor, "steps": steps}
+    fams = {}
+    w_plain, w_upd = (nl.cs_count - 1) / nl.cs_count, 1.0 / nl.cs_count
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        fac = out[counter]["factor"]
+        for jt, w in ((4, w_plain), (5, w_upd)):
+            for _, name, v in out[counter]["steps"][jt]:
+                f = fams.setdefault(kernel_family(name), {"FETCH_SIZE": 0.0, "WRITE_SIZE": 0.0, "launches": 0.0})
+                f[counter] += w * v * 1024.0 * fac
+                if counter == "FETCH_SIZE":
+                    f["launches"] += w
+    data = {"_unit": "HBM bytes per launch of each kernel family (FETCH_SIZE + WRITE_SIZE, KiB per dispatch, "
+                     "scaled by a device copy of one field), averaged over a cs_count cycle",
+            "_calibration": {"copy_bytes_each_way": field_bytes, "fetch_factor": out["FETCH_SIZE"]["factor"],
+                             "write_factor": out["WRITE_SIZE"]["factor"]},
+            "_workload": f"{workload}, FP64, bench.py --pmc-run (plain step 4, update step 5)"}
+    print(f"| family | launches/step | fetched MB/step | written MB/step | traffic MB/launch |")
+    print("|---|---|---|---|---|")
+    for fam, f in sorted(fams.items(), key=lambda kv: -(kv[1]["FETCH_SIZE"] + kv[1]["WRITE_SIZE"])):
+        tot = f["FETCH_SIZE"] + f["WRITE_SIZE"]
+        per = tot / f["launches"] if f["launches"] else 0.0
+        if fam != "other":
+            data[fam] = per
+            data[fam + "_detail"] = {"fetch_bytes_per_step": f["FETCH_SIZE"], "write_bytes_per_step": f["WRITE_SIZE"],
+                                     "launches_per_step": f["launches"]}
+        print(f"| {fam} | {f['launches']:.1f} | {f['FETCH_SIZE'] / 1e6:.1f} | {f['WRITE_SIZE'] / 1e6:.1f} | "
+              f"{per / 1e6:.1f} |")
+    # the name mapping checked against the library's own launch counts per
+    # family (les_profile_step, recorded by the bench run in profiles/)
+    bench_json = os.path.join(HERE, "profiles", f"bench_{workload}.json")
+    if os.path.exists(bench_json):
+        with open(bench_json) as fh:
+            lib_fams = json.load(fh).get("families", {})
+        bad = {f: (round(fams.get(f, {}).get("launches", 0.0), 3), x["launches_per_step"])
+               for f, x in lib_fams.items()
+               if abs(fams.get(f, {}).get("launches", 0.0) - x["launches_per_step"]) > 1e-6}
+        data["_launch_check"] = {"against": os.path.basename(bench_json), "mismatches": bad}
+        print(f"launch counts per family against {os.path.basename(bench_json)}: "
+              + ("all match" if not bad else f"mismatches {bad}"))
+    with open(os.path.join(HERE, "profiles", "traffic.json"), "w") as fh:
+        json.dump(data, fh, indent=1)
+
+
 def traffic_from_profiles(family: str):
     """HBM bytes per launch of a kernel family measured with rocprofv3 --pmc
     (committed summary profiles/traffic.json, keyed by family)."""
@@ -376,6 +516,12 @@ def main():
     from overlappingbls_amd import config, lib
     from overlappingbls_amd.abi import K_TDMA, K_XDERIV, K_YDERIV, K_PAD, K_COPY, LES_MIXED
 
+    if args.pmc_run:
+        pmc_run(args.workload)
+        return
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.workload)
+        return
     if args.adjoint_only:
         print(json.dumps(adjoint_leg(timed_only=True)))
         return

@@ -128,53 +128,84 @@ void level_means(hipStream_t s, const double* f, size_t plane, int lev0, int nle
 // n times the value), and the difference is amplified where the mean feeds
 // a well-resolved quantity (ustar, psi_m, the wall stress) or a cancellation
 // (theta - mean(theta) in calc_beta and the fluctuation statistics), so these
-// sums keep the reference's order.  One wavefront per sum: its lanes load
-// stages of SEQ_STAGE values (coalesced) into LDS while the running sum walks
-// the previous stage, one broadcast LDS read and one FP64 add per value.
+// sums keep the reference's order.  One wavefront per sum walks the values in
+// index order, one FP64 add per value (the add's latency bounds the walk at
+// about 4 ns per value on this GPU).  The values reach it through LDS: the
+// lanes load stages of SEQ_STAGE values (coalesced) two stages ahead of the
+// walk, into a ring of three LDS stages, so a load has two stages' walking
+// time to arrive even when other kernels load the memory system.
 constexpr int SEQ_STAGE = 1024;  // values per stage, 16 per lane
 
 template <typename F>
 __device__ __forceinline__ double wave_chain(long n, const F& val, double* lds) {
     const int lane = threadIdx.x & 63;
     constexpr int PER = SEQ_STAGE / 64;
-    double r[PER];
-    auto gload = [&](long b) {
+    double r0[PER], r1[PER];
+    auto gload = [&](double* r, long b) {
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
             const long idx = b + j * 64 + lane;
             r[j] = val(idx < n ? idx : n - 1);   // in bounds; values past n are never added
         }
     };
-    double s = 0.0;
-    gload(0);
-    int buf = 0;
-    for (long base = 0; base < n; base += SEQ_STAGE) {
-        double* L = lds + buf * SEQ_STAGE;
+    auto put = [&](const double* r, int slot) {
+        double* L = lds + slot * SEQ_STAGE;
 #pragma unroll
         for (int j = 0; j < PER; ++j) L[j * 64 + lane] = r[j];
-        if (base + SEQ_STAGE < n) gload(base + SEQ_STAGE);   // next stage in flight during the adds
-        __syncthreads();
+    };
+    const long nst = (n + SEQ_STAGE - 1) / SEQ_STAGE;
+    double s = 0.0;
+    // the walk of stage st from LDS slot st % 3
+    auto walk = [&](long st) {
+        const double* L = lds + (int)(st % 3) * SEQ_STAGE;
+        const long base = st * SEQ_STAGE;
         const int lim = (n - base < SEQ_STAGE) ? (int)(n - base) : SEQ_STAGE;
+        const double2* L2 = reinterpret_cast<const double2*>(L);
         int j = 0;
-        for (; j + 32 <= lim; j += 32) {
-            double t[32];
+        for (; j + 64 <= lim; j += 64) {
+            double2 t[32];
 #pragma unroll
-            for (int q = 0; q < 32; ++q) t[q] = L[j + q];
+            for (int q = 0; q < 32; ++q) t[q] = L2[(j >> 1) + q];
             // all 32 broadcast reads are issued before the first add consumes one
             asm volatile("" ::: "memory");
 #pragma unroll
-            for (int q = 0; q < 32; ++q) s += t[q];
+            for (int q = 0; q < 32; ++q) {
+                s += t[q].x;
+                s += t[q].y;
+            }
         }
         for (; j < lim; ++j) s += L[j];
-        buf ^= 1;
+    };
+    gload(r0, 0);
+    if (nst > 1) gload(r1, SEQ_STAGE);
+    put(r0, 0);
+    if (nst > 2) gload(r0, 2 * SEQ_STAGE);
+    __syncthreads();
+    // stage st + 1 goes to LDS while stage st is walked; the register set
+    // just written takes stage st + 3
+    for (long st = 0; st < nst; st += 2) {
+        if (st + 1 < nst) put(r1, (int)((st + 1) % 3));
+        if (st + 3 < nst) gload(r1, (st + 3) * SEQ_STAGE);
+        __syncthreads();
+        walk(st);
+        if (st + 1 >= nst) break;
+        if (st + 2 < nst) put(r0, (int)((st + 2) % 3));
+        if (st + 4 < nst) gload(r0, (st + 4) * SEQ_STAGE);
+        __syncthreads();
+        walk(st + 1);
     }
     return s;
 }
 
+// The walks run beside other kernels (on the auxiliary stream): at the
+// highest wave priority their dependent adds issue ahead of the other waves on
+// the SIMD instead of waiting their turn.
+
 // plane means, one wavefront (block) per plane
 __global__ __launch_bounds__(64) void k_level_means_seq(const double* __restrict__ f, long plane, double nn,
                                                          double* __restrict__ out) {
-    __shared__ double lds[2 * SEQ_STAGE];
+    __builtin_amdgcn_s_setprio(3);
+    __shared__ __align__(16) double lds[3 * SEQ_STAGE];
     const double* p = f + (long)blockIdx.x * plane;
     const double s = wave_chain(plane, [&](long i) { return p[i]; }, lds);
     if (threadIdx.x == 0) out[blockIdx.x] = s / nn;
@@ -192,7 +223,8 @@ __global__ __launch_bounds__(64) void k_obukhov_sums(const double* __restrict__ 
                                                       const double* __restrict__ psi, const double* __restrict__ t3,
                                                       const double* __restrict__ th1, long plane,
                                                       double* __restrict__ out) {
-    __shared__ double lds[2 * SEQ_STAGE];
+    __builtin_amdgcn_s_setprio(3);
+    __shared__ __align__(16) double lds[3 * SEQ_STAGE];
     const int which = blockIdx.x;
     double s;
     if (which == 0) s = wave_chain(plane, [&](long i) { return sqrt(u1[i] * u1[i] + v1[i] * v1[i]); }, lds);
@@ -884,6 +916,7 @@ __global__ void k_vort(W* v1, W* v2, W* v3, Grad<W> g, KP p) {
 
 template <typename W>
 void vorticity_dev(les_ctx* c, W* v1, W* v2, W* v3) {
+    FamScope fs(c->stream, FAM_VORT_PAD_X, 9.0 * sizeof(W) * (double)c->plane * (double)c->nz);
     KP p = kp(c);
     hipLaunchKernelGGL(k_vort<W>, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, v1, v2, v3, grads<W>(c), p);
 }

@@ -461,6 +461,16 @@ bool theta_active(const les_ctx* c, int jt) {
     return c->prm.theta_flag && jt >= c->prm.theta_init_time;
 }
 
+// theta's gradients on the 3/2 grid for the advection (:466-470)
+template <typename W>
+void theta_pads(les_ctx* c, W* dtdx, W* dtdy, W* dtdz) {
+    const int nz = c->nz;
+    const size_t np = c->plane, np2 = c->plane2;
+    pad_f<W, W>(c, dtdx + np, scr<W>(c->tbig[0]) + np2, nz - 1);
+    pad_f<W, W>(c, dtdy + np, scr<W>(c->tbig[1]) + np2, nz - 1);
+    pad_f<W, W>(c, dtdz + 2 * np, scr<W>(c->tbig[2]) + 2 * np2, nz - 1);
+}
+
 // filter_data(theta), dTdx, dTdy, dTdz, the theta RHS history (:111-140) and
 // the padded gradients (:466-470); calc_beta's plane means of theta start on
 // the auxiliary stream
@@ -481,9 +491,35 @@ void theta_pre(les_ctx* c, int jt) {
     ddy_f<W, double>(c, dtdy, th, 0, (int)c->nlev);
     ddz_uv_dev<W, double>(c, dtdz, th);
     theta_prep_dev<W>(c);
-    pad_f<W, W>(c, dtdx + np, scr<W>(c->tbig[0]) + np2, nz - 1);
-    pad_f<W, W>(c, dtdy + np, scr<W>(c->tbig[1]) + np2, nz - 1);
-    pad_f<W, W>(c, dtdz + 2 * np, scr<W>(c->tbig[2]) + 2 * np2, nz - 1);
+    theta_pads<W>(c, dtdx, dtdy, dtdz);
+}
+
+// theta_pre on the auxiliary stream, after obukhov's plane sums there.  The
+// momentum path reads none of theta's fields until calc_beta, which waits for
+// ev_th (recorded after the plane means); the x passes of the pads take the
+// auxiliary scratch, so the momentum pads on the context's stream run
+// concurrently.  Same kernels in the same order as theta_pre.
+template <typename W>
+void theta_pre_overlap(les_ctx* c, int jt) {
+    if (!theta_active(c, jt)) return;
+    const int nz = c->nz;
+    const size_t np = c->plane;
+    std::swap(c->stream, c->aux);
+    std::swap(c->work3[6], c->tscr[3]);
+    double* th = F(c, LES_F_THETA);
+    W* dtdx = WF<W>(c, LES_F_DTDX);
+    W* dtdy = WF<W>(c, LES_F_DTDY);
+    W* dtdz = WF<W>(c, LES_F_DTDZ);
+    filter_f<double>(c, th, 0, (int)c->nlev);
+    ddx_f<W, double>(c, dtdx, th, 0, (int)c->nlev);
+    ddy_f<W, double>(c, dtdy, th, 0, (int)c->nlev);
+    ddz_uv_dev<W, double>(c, dtdz, th);
+    theta_prep_dev<W>(c);
+    theta_pads<W>(c, dtdx, dtdy, dtdz);
+    level_means_seq(c->stream, th, np, 0, nz, (double)(c->nx * c->ny), c->th_mean);   // calc_hor_avg order
+    les_note(c, hipEventRecord(c->ev_th, c->stream));
+    std::swap(c->work3[6], c->tscr[3]);
+    std::swap(c->stream, c->aux);
 }
 
 // advection, SGS fluxes, calc_beta and step_scalar (:466-671)
@@ -541,6 +577,9 @@ void theta_post_overlap(les_ctx* c, int jt) {
     std::swap(c->stream, c->aux);
 }
 
+template <typename W>
+void theta_fluxes(les_ctx* c, const W* adv);
+
 // theta's RHS: advection and the SGS flux divergence (:466-594)
 template <typename W>
 void theta_rhs(les_ctx* c) {
@@ -559,6 +598,12 @@ void theta_rhs(les_ctx* c) {
         scalar_adv_dev<W>(c, scr<W>(c->tbig[0]), scr<W>(c->tbig[1]), scr<W>(c->tbig[2]), rm);
         unpad_f<W>(c, rm + np2, adv + np, nz - 1);
     }
+    theta_fluxes<W>(c, adv);
+}
+
+// the SGS flux divergence added to the advection adv (:513-594)
+template <typename W>
+void theta_fluxes(les_ctx* c, const W* adv) {
     W* temp = scr<W>(c->work3[0]);
     W* dtemp = scr<W>(c->work3[1]);
     scalar_flux_dev<W>(c, temp, 0);
@@ -638,6 +683,9 @@ void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     fork(c);
     obukhov_sums_dev(c, jt, c->aux);
     les_note(c, hipEventRecord(c->ev_obk, c->aux));
+    const bool st = stats && stats_step(c, jt);
+    const bool ovl = c->overlap && !tape && !st;  // taped and statistics steps stay in order
+    if (ovl) theta_pre_overlap<W>(c, jt);        // theta's filter, gradients and pads on aux
     const int nl = (int)c->nlev;
     const double *u = F(c, LES_F_U), *v = F(c, LES_F_V), *w = F(c, LES_F_W);
     ddx_f<W, double>(c, WF<W>(c, LES_F_DUDX), u, 0, nl);
@@ -649,7 +697,7 @@ void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     ddz_uv_dev<W, double>(c, WF<W>(c, LES_F_DUDZ), u);
     ddz_uv_dev<W, double>(c, WF<W>(c, LES_F_DVDZ), v);
     ddz_w_dev<W, double>(c, WF<W>(c, LES_F_DWDZ), w);
-    theta_pre<W>(c, jt);
+    if (!ovl) theta_pre<W>(c, jt);
     les_note(c, hipStreamWaitEvent(c->stream, c->ev_obk, 0));
     obukhov_finish_dev(c, jt);
     if (tape) tape_save(c, tape, TAPE_OBUKHOV);
@@ -663,8 +711,6 @@ void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     divstress<W>(c, WF<W>(c, LES_F_DIVTY), txy, tyy, tyz, 0);
     divstress<W>(c, WF<W>(c, LES_F_DIVTZ), txz, tyz, tzz, 1);
     convec<W>(c);
-    const bool st = stats && stats_step(c, jt);
-    const bool ovl = c->overlap && !tape && !st;  // taped and statistics steps stay in order
     if (ovl) theta_post_overlap<W>(c, jt);
     else theta_post<W>(c, jt, tape);
     rhs_ab2_dev<W>(c, jt);
