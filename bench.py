@@ -304,6 +304,9 @@ _POINT_FAMILIES = [
 ]
 # the first kernel of every step: the CFL pass (les_run)
 _STEP_START = ("k_max3_lag", "k_max3_part")
+# kernels that follow another one inside the same bracket of les_profile_step,
+# so that a family's launches count brackets, as the library's profiler does
+_SECONDARY = ("k_project_bc", "k_max3_final", "k_scalar_top", "k_hz_fold")
 
 
 def kernel_family(name: str) -> str:
@@ -381,7 +384,7 @@ def pmc_summary(fetch_dir: str, write_dir: str, workload: str):
             for _, name, v in out[counter]["steps"][jt]:
                 f = fams.setdefault(kernel_family(name), {"FETCH_SIZE": 0.0, "WRITE_SIZE": 0.0, "launches": 0.0})
                 f[counter] += w * v * 1024.0 * fac
-                if counter == "FETCH_SIZE":
+                if counter == "FETCH_SIZE" and not any(k in name for k in _SECONDARY):
                     f["launches"] += w
     data = {"_unit": "HBM bytes per launch of each kernel family (FETCH_SIZE + WRITE_SIZE, KiB per dispatch, "
                      "scaled by a device copy of one field), averaged over a cs_count cycle",

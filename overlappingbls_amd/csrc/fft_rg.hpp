@@ -204,6 +204,7 @@ constexpr int rg_nst(int n) {
 }
 constexpr int rg_radix(int n, int s) {
     switch (n) {
+        case 32: return s == 0 ? 8 : 4;
         case 64: return 8;
         case 96: return s == 0 ? 8 : 12;
         case 128: return s == 0 ? 16 : 8;

@@ -662,10 +662,10 @@ inline long xcd_blocks(long ngroups) {
 // rounds it), then runs the x pass from LDS: the product or vorticity field
 // is never written to and re-read from memory.
 //
-// A block takes one component (blockIdx.x % NC; consecutive blocks take the
+// A block takes one component (consecutive blocks of an XCD take the
 // components of the same rows, so the shared inputs are re-read from cache)
-// and NL row pairs: line l is rows 2 rp and 2 rp + 1, rp = (blockIdx.x / NC)
-// NL + l, row 0 being the first row of level 1.  The evaluation is branch
+// and one group of FusedWork: line l is the row pair of its column on level
+// k0 + 1 + l.  The evaluation is branch
 // free (every input loaded, the wall-level formula chosen by a select), so
 // the loads of all of a thread's elements can be in flight together.
 template <typename T, int COMP>
@@ -715,9 +715,11 @@ __device__ __forceinline__ double fused_value(const VortIn<T>& g, int k, long i,
     }
 }
 
-// inputs of a block's NL lines into the line buffer (component COMP)
+// inputs of a block's NL lines into the line buffer (component COMP): line l
+// is the row pair (rows 2 col, 2 col + 1) of level 1 + k0 + l; lines past the
+// last of the nlz levels read the last level and are not stored
 template <typename T, int NIN, int NL, int NTH, int LS, int COMP, typename IN>
-__device__ __forceinline__ void fused_inputs(const IN& a, cx<T>* buf, long rp0, long nrp, int ny_in, long plane_in,
+__device__ __forceinline__ void fused_inputs(const IN& a, cx<T>* buf, int k0, int col, int nlz, long plane_in,
                                              int nz) {
     static_assert(NIN % 64 == 0 && NTH % 64 == 0, "a wave's run of positions lies in one line");
     constexpr int PER = (NL * NIN + NTH - 1) / NTH;
@@ -725,23 +727,41 @@ __device__ __forceinline__ void fused_inputs(const IN& a, cx<T>* buf, long rp0, 
     for (int j = 0; j < PER; ++j) {
         const int e = (int)threadIdx.x + j * NTH;
         if (NL * NIN % NTH != 0 && e >= NL * NIN) break;
-        // a wave's run of positions lies in one line: the line, its rows and
-        // level are wave-uniform (scalar arithmetic, 32-bit row numbers)
+        // a wave's run of positions lies in one line: the line and its level
+        // are wave-uniform (scalar arithmetic)
         const int l = __builtin_amdgcn_readfirstlane(e / NIN), p = e - l * NIN;
-        const long rp = rp0 + l;
-        const int rpc = (int)((rp < nrp) ? rp : nrp - 1);  // rows past the end read the last pair
-        const int ra = 2 * rpc;
-        const int km = ra / ny_in;
-        const int k = 1 + km;
-        const long ia = (long)k * plane_in + (long)(ra - km * ny_in) * NIN + p;
+        const int kl = k0 + l;
+        const int k = 1 + ((kl < nlz) ? kl : nlz - 1);
+        const long ia = (long)k * plane_in + (long)(2 * col) * NIN + p;
         const double va = fused_value<T, COMP>(a, k, ia, plane_in, nz);
         const double vb = fused_value<T, COMP>(a, k, ia + NIN, plane_in, nz);
-        buf[l * LS + rg_pad(p)] = (rp < nrp) ? cx<T>{(T)va, (T)vb} : cx<T>{T(0), T(0)};
+        buf[l * LS + rg_pad(p)] = (kl < nlz) ? cx<T>{(T)va, (T)vb} : cx<T>{T(0), T(0)};
     }
 }
 
+// Work of the fused kernels: a group is NL consecutive levels (a chunk) of
+// one row pair (a column), so the level-neighbour rows that the products
+// read (w, the vorticity at k + 1; u, v at k - 1; the vorticity's w
+// gradients at k +- 1) are rows of the same group.  The chunks of a column
+// are consecutive groups on one XCD (xcd_item), so the rows at the chunk
+// boundaries come from that XCD's L2.  Columns are padded to a multiple of
+// N_XCD; groups of padding columns return at once.
+template <int NL>
+struct FusedWork {
+    int ycols, nlz, nch;
+    __host__ __device__ FusedWork(long nrp, int ny_in)
+        : ycols(ny_in / 2), nlz((int)(nrp / (ny_in / 2))), nch((nlz + NL - 1) / NL) {}
+    __host__ long groups() const { return (long)nch * (((long)ycols + N_XCD - 1) / N_XCD) * N_XCD; }
+    __device__ void item(long rg, int& col, int& k0) const {
+        const long q = rg / N_XCD;
+        col = (int)((q / nch) * N_XCD + rg % N_XCD);
+        k0 = (int)(q % nch) * NL;
+    }
+};
+
 // NC components per row pair; comp0 is the first component's index (3 for
-// the scalar advection).  ny_in rows per level, plane_in values per plane.
+// the scalar advection).  ny_in rows per level, plane_in values per plane;
+// nrp = levels * ny_in / 2 row pairs.
 template <typename T, int NIN, int NOUT, int OP, int NC, typename IN>
 __global__ __launch_bounds__((rg_threads<T, NIN, NOUT, false>())) void kr_fused_x(
     IN a, int comp0, T* o0, T* o1, T* o2, long nrp, int ny_in, long plane_in, int nz,
@@ -764,94 +784,100 @@ __global__ __launch_bounds__((rg_threads<T, NIN, NOUT, false>())) void kr_fused_
     long rg;
     int cb;
     xcd_item<NC>(rg, cb);
-    const long rp0 = rg * NL;
-    if (rp0 >= nrp) return;  // whole workgroup: past the last group
+    const FusedWork<NL> wk(nrp, ny_in);
+    int col, k0;
+    wk.item(rg, col, k0);
+    if (col >= wk.ycols) return;  // whole workgroup: a padding column
     const int comp = comp0 + cb;
-    if (comp == 0) fused_inputs<T, NIN, NL, NTH, LS, 0>(a, buf, rp0, nrp, ny_in, plane_in, nz);
-    else if (comp == 1) fused_inputs<T, NIN, NL, NTH, LS, 1>(a, buf, rp0, nrp, ny_in, plane_in, nz);
-    else if (comp == 2) fused_inputs<T, NIN, NL, NTH, LS, 2>(a, buf, rp0, nrp, ny_in, plane_in, nz);
-    else fused_inputs<T, NIN, NL, NTH, LS, 3>(a, buf, rp0, nrp, ny_in, plane_in, nz);
+    if (comp == 0) fused_inputs<T, NIN, NL, NTH, LS, 0>(a, buf, k0, col, wk.nlz, plane_in, nz);
+    else if (comp == 1) fused_inputs<T, NIN, NL, NTH, LS, 1>(a, buf, k0, col, wk.nlz, plane_in, nz);
+    else if (comp == 2) fused_inputs<T, NIN, NL, NTH, LS, 2>(a, buf, k0, col, wk.nlz, plane_in, nz);
+    else fused_inputs<T, NIN, NL, NTH, LS, 3>(a, buf, k0, col, wk.nlz, plane_in, nz);
     __syncthreads();
     T* o = (cb == 0) ? o0 : ((cb == 1) ? o1 : o2);
     auto ld = [&](int line, int p) { return buf[line * LS + rg_pad(p)]; };
     auto st = [&](int line, int p, cx<T> z) {
-        const long rp = rp0 + line;
-        if (rp >= nrp) return;
-        o[(2 * rp) * NOUT + p] = z.x;
-        o[(2 * rp + 1) * NOUT + p] = z.y;
+        const int kl = k0 + line;
+        if (kl >= wk.nlz) return;
+        const long r = (long)kl * (2 * wk.ycols) + 2 * col;  // first row of the pair
+        o[r * NOUT + p] = z.x;
+        o[(r + 1) * NOUT + p] = z.y;
     };
     line_pipeline<T, NIN, NOUT, OP, NL, NTH, false, true>(ld, st, buf, twi, twop, T(0));
 }
 
 // ---- lagrange_Sdep: filtered fields contracted as they are transformed back ----
-// (106_03_lagrange_Sdep.f90:132-331).  A workgroup takes one row pair (rows
-// 2 rp, 2 rp + 1 of levels 1..nz) of all 21 test-filtered fields: the x c2r
-// of the 21 lines (pruned half spectra, as kr_x_c2r computes it), the real
-// rows to LDS, then the contraction at each of the 2 N points.  The 21
-// filtered fields are never written and re-read.
+// (106_03_lagrange_Sdep.f90:132-331).  A workgroup takes one row (level k,
+// row y) of all 21 test-filtered fields: the c2r of each row as a complex
+// transform of half the length, the 21 real rows to LDS, then the
+// contraction at each of the N points.  The 21 filtered fields are never
+// written and re-read.
 //  SCALE 2: L_ij M_ij and M_ij M_ij of the 2 delta filter to lm, mm (FP64).
 //  SCALE 4: Q_ij N_ij and N_ij N_ij of the 4 delta filter, then with lm, mm
 //           the relaxation of F_LM .. F_NN and Cs^2 (lag_update).
+// The half-length c2r: with X the half spectrum (X[k] = 0 for k >= kst) of
+// a real row x of length N, H = N/2 and E_k = exp(2 pi i k / N),
+//   Z[k] = (X[k] + conj X[H-k]) + i E_k (X[k] - conj X[H-k]),  k < H,
+// has the unnormalised inverse transform z[m] = x[2m] + i x[2m+1].  One row
+// per workgroup needs 21 lines of H points of LDS (about 46 KB in FP64), so
+// three workgroups share a CU.
 constexpr int LAG_NF = 21;
 template <typename T, int N>
 constexpr int lag_threads() {
-    return round64(LAG_NF * rg_maxbf(N));
+    return round64(LAG_NF * rg_maxbf(N / 2));
 }
 
 template <typename T, int N, int SCALE>
 __global__ __launch_bounds__((lag_threads<T, N>())) void kr_lag_contract(const cx<T>* __restrict__ spt, int ldc,
-                                                                        int kst, long nrp, int ny, long plane,
+                                                                        int kst, int nrows, int ny, long plane,
                                                                         double* lm, double* mm, LagUpdate u,
                                                                         double cst,
                                                                         const cx<T>* __restrict__ tw_g) {
+    constexpr int H = N / 2;
     constexpr int NL = LAG_NF;
     constexpr int NTH = lag_threads<T, N>();
-    constexpr int LS = rg_ls(N);
-    constexpr int SI = rg_nst(N);
-    static_assert(LAG_NF * 2 * N * (int)sizeof(T) <= NL * LS * (int)sizeof(cx<T>), "rows fit the line buffer");
+    constexpr int LS = rg_ls(H);
+    static_assert(rg_nst(H) == 2, "two stages: load, LDS exchange, store");
+    static_assert(LAG_NF * N * (int)sizeof(T) <= NL * LS * (int)sizeof(cx<T>), "rows fit the line buffer");
     __shared__ cx<T> buf[NL * LS];
-    __shared__ cx<T> tw[N];
-    load_twr(tw, tw_g, N);
-    const long rp = blockIdx.x;
-    const long nzny = 2 * nrp;            // rows per field (levels 1..nz)
+    __shared__ cx<T> tw[H];  // exp(-2 pi i j / H)
+    for (int m = threadIdx.x; m < H; m += NTH) tw[m] = tw_g[2 * m];
+    const int row = blockIdx.x;  // level 1 + row / ny, row y = row % ny
     __syncthreads();
-    Stage<T, N, 0, +1, NL, NTH, false> i0;
-    i0.load([&](int line, int k) {        // field `line`, rows 2 rp, 2 rp + 1
-        const int kk = (2 * k <= N) ? k : N - k;
-        const long ga = (long)line * nzny + 2 * rp, gb = ga + 1;
-        cx<T> a = (kk < kst) ? spt[ga * ldc + kk] : cx<T>{T(0), T(0)};
-        cx<T> b = (kk < kst) ? spt[gb * ldc + kk] : cx<T>{T(0), T(0)};
-        if (kk == 0 || 2 * kk == N) {
+    Stage<T, H, 0, +1, NL, NTH, false> i0;
+    i0.load([&](int line, int k) {
+        const cx<T>* X = spt + ((long)line * nrows + row) * ldc;
+        const int j = H - k;
+        cx<T> a = (k < kst) ? X[k] : cx<T>{T(0), T(0)};
+        cx<T> b = (j < kst) ? X[j] : cx<T>{T(0), T(0)};
+        if (k == 0) {  // X[0] and X[H] are real
             a.y = T(0);
             b.y = T(0);
         }
-        if (2 * k > N) {
-            a.y = -a.y;
-            b.y = -b.y;
-        }
-        return cx<T>{a.x - b.y, a.y + b.x};
+        const cx<T> e = tw_g[k];  // conj(E_k)
+        const T sx = a.x + b.x, sy = a.y - b.y;  // X[k] + conj X[H-k]
+        const T dx = a.x - b.x, dy = a.y + b.y;  // X[k] - conj X[H-k]
+        const T px = dx * e.x + dy * e.y, py = dy * e.x - dx * e.y;  // E_k (X[k] - conj X[H-k])
+        return cx<T>{sx - py, sy + px};
     });
     i0.compute(tw);
     i0.template write<LS>(buf);
     __syncthreads();
-    middle_stages<T, N, +1, NL, NTH, false, LS>(buf, tw);
-    Stage<T, N, SI - 1, +1, NL, NTH, false> il;
+    Stage<T, H, 1, +1, NL, NTH, false> il;
     il.template read<LS>(buf);
     il.compute(tw);
     __syncthreads();  // every line read: the buffer now takes the real rows
-    T* rows = reinterpret_cast<T*>(buf);  // [field][row of the pair][x]
-    il.store([&](int line, int p, cx<T> z) {
-        rows[(line * 2) * N + p] = z.x;
-        rows[(line * 2 + 1) * N + p] = z.y;
+    T* rows = reinterpret_cast<T*>(buf);  // [field][x]
+    il.store([&](int line, int m, cx<T> z) {
+        rows[line * N + 2 * m] = z.x;
+        rows[line * N + 2 * m + 1] = z.y;
     });
     __syncthreads();
-    const int ra = 2 * (int)rp;
-    const int k = 1 + ra / ny;
-    const long row0 = (long)k * plane + (long)(ra - (k - 1) * ny) * N;  // field index of the pair's first row
-    for (int q = threadIdx.x; q < 2 * N; q += NTH) {
-        const int r = q / N, x = q - r * N;
-        const long i = row0 + (long)r * N + x;
-        auto v = [&](int j) { return (double)rows[(j * 2 + r) * N + x]; };
+    const int k = 1 + row / ny;
+    const long row0 = (long)k * plane + (long)(row - (k - 1) * ny) * N;  // field index of the row's first point
+    for (int x = threadIdx.x; x < N; x += NTH) {
+        const long i = row0 + x;
+        auto v = [&](int f) { return (double)rows[f * N + x]; };
         double a, b;
         if constexpr (SCALE == 2) {
             lag_scale(v, cst, 4.0, a, b);
@@ -1037,7 +1063,7 @@ void launch_fused(hipStream_t s, const IN& a, int comp0, T* o0, T* o1, T* o2, lo
                   int nz, const Twiddle& tin, const Twiddle& tout) {
     constexpr int NL = rg_lines<T, NIN, NOUT, false>();
     constexpr int NTH = rg_threads<T, NIN, NOUT, false>();
-    const long nb = xcd_blocks<NC>((nrp + NL - 1) / NL);
+    const long nb = xcd_blocks<NC>(FusedWork<NL>(nrp, ny_in).groups());
     hipLaunchKernelGGL((kr_fused_x<T, NIN, NOUT, OP, NC, IN>), dim3((unsigned)nb), dim3(NTH), 0, s, a, comp0, o0, o1,
                        o2, nrp, ny_in, plane_in, nz, twp<T>(tin), twp<T>(tout));
 }
@@ -1087,20 +1113,20 @@ bool vort_pad_x(hipStream_t s, const VortIn<T>& g, T* o0, T* o1, T* o2, int nz, 
 template <typename T>
 bool lag_contract_x(hipStream_t s, int scale, const cx<T>* spt, int ldc, int kst, int nz, int ny, double* lm,
                     double* mm, const LagUpdate& u, double cst, const Twiddle& tw) {
-    const long nrp = (long)nz * ny / 2;
-    if (ny % 2 != 0 || (tw.n != 64 && tw.n != 128 && tw.n != 256)) return false;
+    const int nrows = nz * ny;
+    if (tw.n != 64 && tw.n != 128 && tw.n != 256) return false;
     // the 21 filtered spectra read; L M, M M (2 delta) or the F_* update and
     // Cs_opt2 (4 delta) written, FP64
     FamScope fs(s, FAM_LAG_CONTRACT_X,
                 21.0 * nz * ny * ldc * 2.0 * sizeof(T) + (scale == 2 ? 2.0 : 7.0) * nz * ny * (double)tw.n * 8.0);
 #define LCCASE(NN_)                                                                                         \
     if (tw.n == NN_) {                                                                                      \
-        const dim3 g((unsigned)nrp), b(lag_threads<T, NN_>());                                              \
+        const dim3 g((unsigned)nrows), b(lag_threads<T, NN_>());                                            \
         if (scale == 2)                                                                                     \
-            hipLaunchKernelGGL((kr_lag_contract<T, NN_, 2>), g, b, 0, s, spt, ldc, kst, nrp, ny,            \
+            hipLaunchKernelGGL((kr_lag_contract<T, NN_, 2>), g, b, 0, s, spt, ldc, kst, nrows, ny,          \
                                (long)NN_ * ny, lm, mm, u, cst, twp<T>(tw));                                 \
         else                                                                                                \
-            hipLaunchKernelGGL((kr_lag_contract<T, NN_, 4>), g, b, 0, s, spt, ldc, kst, nrp, ny,            \
+            hipLaunchKernelGGL((kr_lag_contract<T, NN_, 4>), g, b, 0, s, spt, ldc, kst, nrows, ny,          \
                                (long)NN_ * ny, lm, mm, u, cst, twp<T>(tw));                                 \
         return true;                                                                                        \
     }

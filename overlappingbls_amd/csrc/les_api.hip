@@ -671,18 +671,26 @@ void avg_slices(les_ctx* c, int jt) {
     }
 }
 
+// obukhov's ordered plane sums on the auxiliary stream, after everything
+// enqueued so far on the context's stream; ev_obk marks their end
+void obukhov_sums_fork(les_ctx* c, int jt) {
+    obukhov_theta_copy(c);
+    fork(c);
+    obukhov_sums_dev(c, jt, c->aux);
+    les_note(c, hipEventRecord(c->ev_obk, c->aux));
+}
+
 template <typename W>
 void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     // newstep: RHS_f = RHS (103_flosol.f90:311-313) as a pointer exchange
     std::swap(c->f3[LES_F_RHSX], c->f3[LES_F_RHSX_F]);
     std::swap(c->f3[LES_F_RHSY], c->f3[LES_F_RHSY_F]);
     std::swap(c->f3[LES_F_RHSZ], c->f3[LES_F_RHSZ_F]);
-    // obukhov's ordered plane sums overlap the horizontal derivatives and
-    // the theta gradients
-    obukhov_theta_copy(c);
-    fork(c);
-    obukhov_sums_dev(c, jt, c->aux);
-    les_note(c, hipEventRecord(c->ev_obk, c->aux));
+    // obukhov's ordered plane sums overlap the CFL pass (when les_run
+    // launched them before it), the horizontal derivatives and the theta
+    // gradients
+    if (!c->obk_early) obukhov_sums_fork(c, jt);
+    c->obk_early = 0;
     const bool st = stats && stats_step(c, jt);
     const bool ovl = c->overlap && !tape && !st;  // taped and statistics steps stay in order
     if (ovl) theta_pre_overlap<W>(c, jt);        // theta's filter, gradients and pads on aux
@@ -1284,6 +1292,10 @@ int les_run(les_ctx* c, int jt0, int nsteps, int* failed_jt) {
             // check_cfl of newstep (901_debug.f90:85-99), with sgs_stag's u_lag
             // accumulation in the same pass (u, v, w do not change in between)
             const bool lag = c->fuse && !c->sgs_frozen;
+            // the sums read u, v, psi_m, sgs_t3 and theta at the wall level,
+            // which the CFL pass leaves alone
+            obukhov_sums_fork(c, jt0 + done + s);
+            c->obk_early = 1;
             max3_dev(c, c->cfl_dev + s, lag);
             c->lag_done = lag;
             LES_DISPATCH(step_body, c, jt0 + done + s);

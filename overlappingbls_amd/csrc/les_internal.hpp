@@ -131,6 +131,7 @@ struct les_ctx {
     int fuse;                  // fused kernels (LES_FUSE=0 at creation: the separate ones)
     int overlap;               // theta's RHS and update on aux (LES_OVERLAP=0: in order)
     int lag_done;              // this step's u_lag accumulation already done (k_max3_lag)
+    int obk_early;             // this step's obukhov sums already launched on aux (les_run)
     double* tscr[5];           // theta's private scratch while it runs on aux:
                                // work3[0..2], work3[6], big3[3] stand-ins
     les::LinCtx* lin;          // linearised-step buffers (les_lin.hip), or null
