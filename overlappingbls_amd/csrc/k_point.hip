@@ -100,16 +100,27 @@ void convert(hipStream_t s, TO* out, const TF* in, long n) {
 template void convert<float, double>(hipStream_t, float*, const double*, long);
 template void convert<double, float>(hipStream_t, double*, const float*, long);
 
-// one block per level: out[l] = sum(plane l) / nn, deterministic tree order
-__global__ __launch_bounds__(256) void k_level_means(const double* __restrict__ f, long plane, double nn,
-                                                      double* __restrict__ out) {
-    __shared__ double sh[256];
+// one block per level: out[l] = sum(plane l) / nn, in a fixed order (four
+// running sums per thread over its strided values, combined in order, then
+// a tree over the block), so every run gives the same bits.  Four
+// independent sums keep four loads of a thread in flight.
+constexpr int LM_THREADS = 1024;
+__global__ __launch_bounds__(LM_THREADS) void k_level_means(const double* __restrict__ f, long plane, double nn,
+                                                             double* __restrict__ out) {
+    __shared__ double sh[LM_THREADS];
     const double* p = f + (long)blockIdx.x * plane;
-    double s = 0.0;
-    for (long i = threadIdx.x; i < plane; i += 256) s += p[i];
-    sh[threadIdx.x] = s;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    long i = threadIdx.x;
+    for (; i + 3 * LM_THREADS < plane; i += 4 * LM_THREADS) {
+        s0 += p[i];
+        s1 += p[i + LM_THREADS];
+        s2 += p[i + 2 * LM_THREADS];
+        s3 += p[i + 3 * LM_THREADS];
+    }
+    for (; i < plane; i += LM_THREADS) s0 += p[i];
+    sh[threadIdx.x] = (s0 + s1) + (s2 + s3);
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = LM_THREADS / 2; w > 0; w >>= 1) {
         if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
@@ -117,7 +128,8 @@ __global__ __launch_bounds__(256) void k_level_means(const double* __restrict__ 
 }
 
 void level_means(hipStream_t s, const double* f, size_t plane, int lev0, int nlev, double nn, double* out) {
-    hipLaunchKernelGGL(k_level_means, dim3(nlev), dim3(256), 0, s, f + (size_t)lev0 * plane, (long)plane, nn, out);
+    hipLaunchKernelGGL(k_level_means, dim3(nlev), dim3(LM_THREADS), 0, s, f + (size_t)lev0 * plane, (long)plane, nn,
+                       out);
 }
 
 // ---- sums in the reference's order ---------------------------------------

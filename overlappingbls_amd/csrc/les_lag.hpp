@@ -60,7 +60,9 @@ __device__ __forceinline__ void lag_update(const LagUpdate& u, long i, int k, do
     if (u.init_qn) { fnn = NN; fqn = 0.03 * NN; }
     lag_relax(fqn, fnn, QN, NN, u.opftdelta, u.lagran_dt);
     const double cs4d = fmax(LAG_CLIP, fqn / fnn);
-    double beta = pow(cs4d / cs2d, u.bexp);
+    // bexp = log 2 / (log 4 - log 2) is 1 in double arithmetic, where
+    // pow(x, 1) = x exactly
+    double beta = (u.bexp == 1.0) ? cs4d / cs2d : pow(cs4d / cs2d, u.bexp);
     if (k == u.nz) beta = 1.0;
     const double betaclip = fmax(beta, 1.0 / (2.0 * 4.0));
     u.FLM[i] = flm; u.FMM[i] = fmm; u.FQN[i] = fqn; u.FNN[i] = fnn;
