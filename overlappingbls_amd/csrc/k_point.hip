@@ -687,12 +687,13 @@ __global__ void k_interpolag(const double* __restrict__ ul, const double* __rest
                              const double* __restrict__ o1, const double* __restrict__ o2,
                              const double* __restrict__ o3, double* f0, double* f1, double* f2, double* f3, KP p,
                              double csc) {
-    const long n = p.plane * p.nz;
-    GRID_LOOP(t, n) {
-        const long i = p.plane + t;
-        const int k = (int)(i / p.plane);
-        const long q = i - (long)k * p.plane;
-        const int jy = (int)(q / p.nx), jx = (int)(q - (long)jy * p.nx);
+    // one level per blockIdx.y: no 64-bit index division
+    const int qi = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (qi < (int)p.plane) {
+        const int k = 1 + (int)blockIdx.y;
+        const long q = qi;
+        const long i = (long)k * p.plane + q;
+        const int jy = qi / p.nx, jx = qi - jy * p.nx;
         const double xu = ulag_at(ul, ust, i, k, p.plane, p.nz, csc);
         const double yv = ulag_at(vl, vst, i, k, p.plane, p.nz, csc);
         const double xp = -xu * p.dt * csc / p.dx;
@@ -711,7 +712,9 @@ __global__ void k_interpolag(const double* __restrict__ ul, const double* __rest
         const int ax = (copysign(1.0, xp) > 0.0) ? 1 : -1;
         const int ay = (copysign(1.0, yp) > 0.0) ? 1 : -1;
         const int az = (copysign(1.0, zp) > 0.0) ? 1 : -1;
-        const int x2 = (jx + ax + p.nx) % p.nx, y2 = (jy + ay + p.ny) % p.ny;
+        int x2 = jx + ax, y2 = jy + ay;  // periodic neighbours
+        x2 = (x2 < 0) ? x2 + p.nx : ((x2 >= p.nx) ? x2 - p.nx : x2);
+        y2 = (y2 < 0) ? y2 + p.ny : ((y2 >= p.ny) ? y2 - p.ny : y2);
         int k2 = k + az;
         k2 = k2 < 1 ? 1 : (k2 > p.nz ? p.nz : k2);
         const double cx = fabs(xp), cy = fabs(yp), cz = fabs(zp);
@@ -742,7 +745,7 @@ void interpolag_dev(les_ctx* c) {
         les_note(c, hipMemcpyAsync(c->f3[ids[m]], c->fold[m], sizeof(double) * c->plane, hipMemcpyDeviceToDevice,
                                    c->stream));
     }
-    hipLaunchKernelGGL(k_interpolag, grid_for(p.plane * p.nz), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_interpolag, dim3((unsigned)((p.plane + 255) / 256), (unsigned)p.nz), dim3(256), 0, c->stream,
                        (const double*)c->f3[LES_F_U_LAG], (const double*)c->f3[LES_F_V_LAG],
                        (const double*)c->f3[LES_F_W_LAG], (const double*)c->ust, (const double*)c->vst,
                        (const double*)c->fold[0], (const double*)c->fold[1], (const double*)c->fold[2],
