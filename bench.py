@@ -17,8 +17,10 @@ roofline: after the timed steps, one plain step and one dynamic-SGS update
 step run with every launch of the library's kernel families bracketed by HIP
 events on the stream it runs on (les_profile_step); each family's device
 time per step is (cs_count - 1) plain + 1 update over cs_count.  The family
-with the most device time per step is the roofline kernel: `achieved` = its
-algorithmic bytes per launch / its average launch time.  `step` adds the
+with the most device time per step is the roofline kernel.  The timed steps
+then run again from the same initial fields with only that family's
+launches bracketed (les_profile_run): `achieved` = its algorithmic bytes
+per launch / its average launch time over those steps.  `step` adds the
 whole step's fraction: the algorithmic bytes of the stage list below (frozen
 here) over the timed ms per step.  `traffic` is the HBM bytes per launch of
 that family from the committed rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE runs
@@ -89,6 +91,10 @@ def parse():
                     help="the workload of the counter summary (run under rocprofv3 --pmc)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="per-family HBM traffic from two rocprofv3 --pmc runs of --pmc-run")
+    ap.add_argument("--trace-summary", nargs=2, metavar=("TRACE_DIR", "BENCH_OUT"),
+                    help="per-family kernel durations from a rocprofv3 --kernel-trace run of this bench "
+                         "(--precision fp64 or mixed, --no-cpu --no-adjoint) against the live averages "
+                         "that the same run printed (its output file)")
     ap.add_argument("--mixed-drift", type=int, default=0, metavar="N",
                     help="only: run FP64 and mixed N steps side by side and print their difference every N/10 steps")
     return ap.parse_args()
@@ -302,8 +308,9 @@ _POINT_FAMILIES = [
     (("kr_yspec", "k_yline_spec"), "yline.spec"),
     (("k_vort<",), "vort_pad_x"),
 ]
-# the first kernel of every step: the CFL pass (les_run)
-_STEP_START = ("k_max3_lag", "k_max3_part")
+# the first kernel of every step: obukhov's plane sums, which les_run launches
+# before the CFL pass, or the CFL pass where a step has no such sums
+_STEP_START = (("k_obukhov_sums",), ("k_max3_lag", "k_max3_part"))
 # kernels that follow another one inside the same bracket of les_profile_step,
 # so that a family's launches count brackets, as the library's profiler does
 _SECONDARY = ("k_project_bc", "k_max3_final", "k_scalar_top", "k_hz_fold")
@@ -371,8 +378,11 @@ def pmc_summary(fetch_dir: str, write_dir: str, workload: str):
         if not copies:
             raise SystemExit("no calibration copy in " + d)
         factor = field_bytes / (copies[0] * 1024.0)
-        starts = [i for i, (_, n, _) in enumerate(rows) if any(k in n for k in _STEP_START)]
-        if len(starts) != 5:
+        for marks in _STEP_START:
+            starts = [i for i, (_, n, _) in enumerate(rows) if any(k in n for k in marks)]
+            if len(starts) == 5:
+                break
+        else:
             raise SystemExit(f"expected 5 steps in {d}, found {len(starts)}")
         steps = {4: rows[starts[3]:starts[4]], 5: rows[starts[4]:]}
         out[counter] = {"factor": factor, "steps": steps}
@@ -416,6 +426,79 @@ def pmc_summary(fetch_dir: str, write_dir: str, workload: str):
               + ("all match" if not bad else f"mismatches {bad}"))
     with open(os.path.join(HERE, "profiles", "traffic.json"), "w") as fh:
         json.dump(data, fh, indent=1)
+
+
+def trace_summary(trace_dir: str, bench_out: str, cs_count: int = 5):
+    """Per-family device time of a traced bench run (rocprofv3
+    --kernel-trace of `bench.py --precision fp64|mixed --no-cpu --no-adjoint`)
+    beside the live HIP-event averages the same run printed.  The trace holds,
+    in order: W warm-up steps, the K timed steps, the two profiled steps
+    (plain, then dynamic-SGS update; les_profile_step, every family
+    bracketed), the rerun of the W + K steps with only the roofline family
+    bracketed (les_profile_run), then the kernel table.  Steps are cut at
+    their first kernel (_STEP_START)."""
+    import csv
+    import glob
+    files = sorted(glob.glob(os.path.join(trace_dir, "**", "*kernel_trace.csv"), recursive=True))
+    if not files:
+        raise SystemExit(f"no kernel_trace.csv under {trace_dir}")
+    with open(files[0]) as f:
+        ks = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"])
+                    for r in csv.DictReader(f))
+    with open(bench_out) as f:
+        line = [x for x in f.read().splitlines() if x.startswith('{"metric"')][-1]
+    run = json.loads(line)
+    W, K = run["warmup"], run["steps"]
+    nst = 2 * (W + K) + 2
+    for marks in _STEP_START:
+        starts = [i for i, k in enumerate(ks) if any(m in k[2] for m in marks)]
+        if len(starts) == nst:
+            break
+    else:
+        raise SystemExit(f"expected {nst} steps in the trace, found {len(starts)}")
+    # the kernel table after the rerun starts with back-to-back TDMA launches
+    end = next((i for i in range(starts[-1], len(ks) - 1)
+                if "k_tdma" in ks[i][2] and "k_tdma" in ks[i + 1][2]), len(ks))
+
+    def fam_time(lo, hi):
+        out = {}
+        for t0, t1, name in ks[lo:hi]:
+            o = out.setdefault(kernel_family(name), [0.0, 0])
+            o[0] += (t1 - t0) / 1e3
+            if not any(x in name for x in _SECONDARY):  # timed with the kernel it follows
+                o[1] += 1
+        return out
+    timed = fam_time(starts[W], starts[W + K])
+    plain = fam_time(starts[W + K], starts[W + K + 1])
+    update = fam_time(starts[W + K + 1], starts[W + K + 2])
+    rerun = fam_time(starts[2 * W + K + 2], end)
+    w = (cs_count - 1) / cs_count
+    live = run.get("families", {})
+    roof = run["roofline"]
+    # wall time of each timed step, first kernel to the next step's first kernel
+    walls = {"plain": [], "update": []}
+    for j in range(W, W + K):
+        jt = j + 1
+        walls["update" if jt % cs_count == 0 else "plain"].append((ks[starts[j + 1]][0] - ks[starts[j]][0]) / 1e6)
+    med = {k: sorted(v)[len(v) // 2] for k, v in walls.items() if v}
+    print("timed steps under the trace, median wall ms: " + ", ".join(f"{k} {v:.3f}" for k, v in med.items()))
+    print(f"run: {run['config']['precision']}, {run['ms_per_step']:.3f} ms per timed step; roofline family "
+          f"{roof['kernel']}: live {roof['ms_per_launch'] * 1e3:.1f} us per launch over the rerun steps, "
+          f"traced {rerun[roof['kernel']][0] / rerun[roof['kernel']][1]:.1f} us "
+          f"({rerun[roof['kernel']][1]} launches)")
+    print("| family | launches/step | traced us/launch, timed steps | traced us/launch, rerun steps | "
+          "traced us/launch, profiled steps | live us/launch, profiled steps | live ms/step, profiled steps |")
+    print("|---|---|---|---|---|---|---|")
+    fams = sorted(live, key=lambda f: -live[f]["ms_per_step"])
+    for fam in fams:
+        p, u = plain.get(fam, [0.0, 0]), update.get(fam, [0.0, 0])
+        n = w * p[1] + (1 - w) * u[1]
+        prof_us = (w * p[0] + (1 - w) * u[0]) / n if n else 0.0
+        t, r = timed.get(fam, [0.0, 0]), rerun.get(fam, [0.0, 0])
+        lv = live[fam]
+        print(f"| {fam} | {n:.1f} | {t[0] / t[1] if t[1] else 0:.1f} | {r[0] / r[1] if r[1] else 0:.1f} | "
+              f"{prof_us:.1f} | {lv['ms_per_step'] / lv['launches_per_step'] * 1e3:.1f} | "
+              f"{lv['ms_per_step']:.3f} |")
 
 
 def traffic_from_profiles(family: str):
@@ -525,6 +608,9 @@ def main():
     if args.pmc_summary:
         pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.workload)
         return
+    if args.trace_summary:
+        trace_summary(args.trace_summary[0], args.trace_summary[1], config.workload(args.workload).cs_count)
+        return
     if args.adjoint_only:
         print(json.dumps(adjoint_leg(timed_only=True)))
         return
@@ -543,7 +629,7 @@ def main():
     n = p.nxt * p.nyt * (p.nzt - 1)
     fields = config.initial_fields(nl, seed=2024)
     fams = {"tdma": K_TDMA, "xline_deriv": K_XDERIV, "yline_deriv": K_YDERIV, "pad32": K_PAD, "copy": K_COPY}
-    runs, kts, states, profs = {}, {}, {}, {}
+    runs, kts, states, profs, roof_runs = {}, {}, {}, {}, {}
     for prec in (("fp64", "mixed") if args.precision == "both" else (args.precision,)):
         # One live context at a time: a context's main and auxiliary streams
         # overlap only when they get hardware queues of their own, and a
@@ -560,6 +646,19 @@ def main():
                             for name in ("U", "V", "W", "THETA")}
             # a plain step and an update step with the kernel families timed
             profs[prec] = family_profile(ctx, args.warmup + args.steps + 1, nl.cs_count)
+            if prec == ("fp64" if args.precision in ("both", "fp64") else "mixed"):
+                # the timed steps again, in a fresh context from the same
+                # initial fields, with only the family of most device time
+                # bracketed by events
+                ctx.close()
+                dom = max(profs[prec], key=lambda f: profs[prec][f]["ms_per_step"])
+                ctx = lib.create(p)
+                if prec == "mixed":
+                    ctx.set_precision(LES_MIXED)
+                config.load_state(ctx, fields)
+                ctx.run(1, args.warmup)
+                r = ctx.profile_run(args.warmup + 1, args.steps, dom)[dom]
+                roof_runs[prec] = {"family": dom, "ms": r[0], "bytes": r[1], "launches": r[2]}
             kts[prec] = kernel_table(ctx, fams)
         ctx.close()
     if args.profile_only:
@@ -569,17 +668,20 @@ def main():
     kt = kts[head]
     value, ms_step = runs[head]["value"], runs[head]["ms_per_step"]
     prof = profs[head]
-    dom = max(prof, key=lambda f: prof[f]["ms_per_step"])     # most device time per step
-    d = prof[dom]
-    achieved = d["bytes_per_step"] / (d["ms_per_step"] * 1e-3) / 1e9
+    rr = roof_runs[head]
+    dom = rr["family"]                      # most device time per step (family_profile)
+    achieved = rr["bytes"] / (rr["ms"] * 1e-3) / 1e9
     sb = step_bytes(n, nl.cs_count)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(dom),
                 "traffic_source": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, profiles/traffic.json",
                 "kernel": dom,
-                "bytes_per_launch": d["bytes_per_step"] / d["launches_per_step"],
-                "ms_per_launch": d["ms_per_step"] / d["launches_per_step"],
-                "share_of_step": d["ms_per_step"] / ms_step,
+                "measured_over": f"steps {args.warmup + 1}..{args.warmup + args.steps} rerun from the same "
+                                 f"initial fields with only this family bracketed by HIP events",
+                "bytes_per_launch": rr["bytes"] / rr["launches"],
+                "ms_per_launch": rr["ms"] / rr["launches"],
+                "launches": rr["launches"],
+                "share_of_step": rr["ms"] / args.steps / ms_step,
                 "step": {"bytes": sb, "GBs": sb / (ms_step * 1e-3) / 1e9,
                          "frac": sb / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
                          "stages_Ns": sum(STAGES_NS.values()), "update_Ns_per_cs_count": LAGRANGE_NS}}

@@ -418,6 +418,11 @@ int les_time_kernel(les_ctx* ctx, int kernel, int reps, double* ms, double* byte
 int les_nfamilies(void);
 const char* les_family_name(int fam);
 int les_profile_step(les_ctx* ctx, int jt, double* ms, double* bytes, int* launches);
+/* The same over nsteps steps from jt0 (as les_run(ctx, jt0, nsteps, ...)),
+ * with only family `family` bracketed (-1: every family), so that one
+ * family's launches are timed in steps that otherwise run undisturbed;
+ * ms/bytes/launches are summed over the steps.  No reference counterpart. */
+int les_profile_run(les_ctx* ctx, int jt0, int nsteps, int family, double* ms, double* bytes, int* launches);
 
 /* HIP stream the context enqueues on (hipStream_t as void*). */
 void* les_stream(les_ctx* ctx);

@@ -171,6 +171,7 @@ def bind(lib: C.CDLL, prefix: str):
         "lin_nstages": [],
         "lin_stage_times": [P, i, _PD, _PD],
         "profile_step": [P, i, _PD, _PD, C.POINTER(i)],
+        "profile_run": [P, i, i, i, _PD, _PD, C.POINTER(i)],
         "nfamilies": [],
     }
     fns = {}
@@ -440,6 +441,24 @@ class LesContext:
         ms, nb = np.zeros(n), np.zeros(n)
         cnt = (C.c_int * n)()
         self._call("profile_step", self._h, int(jt), _pd(ms), _pd(nb), cnt)
+        return {self._f["family_name"](k).decode(): (float(ms[k]), float(nb[k]), int(cnt[k])) for k in range(n)}
+
+    def family_index(self, name):
+        n = self._f["nfamilies"]()
+        for k in range(n):
+            if self._f["family_name"](k).decode() == name:
+                return k
+        raise KeyError(name)
+
+    def profile_run(self, jt0, nsteps, family=None):
+        """Run nsteps steps from jt0 with one kernel family (by name; None:
+        every family) timed (les_profile_run): {family: (device ms,
+        algorithmic bytes, launches)} summed over the steps."""
+        n = self._f["nfamilies"]()
+        ms, nb = np.zeros(n), np.zeros(n)
+        cnt = (C.c_int * n)()
+        fam = -1 if family is None else self.family_index(family)
+        self._call("profile_run", self._h, int(jt0), int(nsteps), fam, _pd(ms), _pd(nb), cnt)
         return {self._f["family_name"](k).decode(): (float(ms[k]), float(nb[k]), int(cnt[k])) for k in range(n)}
 
     # ---- tangent-linear model and adjoint (FP64, frozen SGS) -------------

@@ -26,11 +26,12 @@ struct FamProf {
     std::vector<Rec> recs;
     size_t used = 0;
     bool overflow = false;
+    int only = -1;  // the one family timed, or -1 for every family
 };
 FamProf* g_prof = nullptr;
 
 FamScope::FamScope(hipStream_t st, int fam, double bytes) {
-    if (!g_prof) return;
+    if (!g_prof || (g_prof->only >= 0 && g_prof->only != fam)) return;
     if (g_prof->used >= g_prof->recs.size()) {
         g_prof->overflow = true;
         return;
@@ -57,9 +58,16 @@ int les_nfamilies(void) { return FAM_N; }
 const char* les_family_name(int fam) { return fam_name(fam); }
 
 int les_profile_step(les_ctx* c, int jt, double* ms, double* bytes, int* launches) {
-    if (!c || !ms || !bytes || !launches || g_prof) return LES_E_BADARG;
+    return les_profile_run(c, jt, 1, -1, ms, bytes, launches);
+}
+
+int les_profile_run(les_ctx* c, int jt0, int nsteps, int family, double* ms, double* bytes, int* launches) {
+    if (!c || !ms || !bytes || !launches || g_prof || nsteps < 1 || family < -1 || family >= FAM_N)
+        return LES_E_BADARG;
     FamProf prof;
-    prof.recs.resize(4096);
+    prof.only = family;
+    // a step launches fewer than 512 instrumented kernels
+    prof.recs.resize((size_t)512 * (size_t)(family < 0 ? nsteps : (nsteps + 7) / 8));
     size_t made = 0;
     hipError_t e = hipSuccess;
     for (; made < prof.recs.size() && e == hipSuccess; ++made) {
@@ -70,7 +78,7 @@ int les_profile_step(les_ctx* c, int jt, double* ms, double* bytes, int* launche
     if (e == hipSuccess) {
         int failed = 0;
         g_prof = &prof;
-        st = les_run(c, jt, 1, &failed);
+        st = les_run(c, jt0, nsteps, &failed);
         g_prof = nullptr;
     }
     for (int f = 0; f < FAM_N; ++f) {

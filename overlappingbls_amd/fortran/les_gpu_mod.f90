@@ -72,6 +72,7 @@ public :: les_create, les_destroy, les_set_field, les_get_field, les_step, les_r
           les_write_restart, les_read_restart, les_write_avgslice, &
           les_lagrange_sdep, les_interpolag_sdep, les_set_sgs_init_flags, les_get_sgs_init_flags, &
           les_set_sponge, les_set_wind_stress, les_get_profiles, les_sync, &
+          les_nfamilies, les_profile_step, les_profile_run, &
           les_check
 
 interface
@@ -133,6 +134,25 @@ interface
     integer(c_int) function les_sync(ctx) bind(C, name="les_sync")
         import :: c_int, c_ptr
         type(c_ptr), value :: ctx
+    end function
+    ! per-family device time (les_nfamilies() entries per array)
+    integer(c_int) function les_nfamilies() bind(C, name="les_nfamilies")
+        import :: c_int
+    end function
+    integer(c_int) function les_profile_step(ctx, jt, ms, bytes, launches) bind(C, name="les_profile_step")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: jt
+        real(c_double), intent(out) :: ms(*), bytes(*)
+        integer(c_int), intent(out) :: launches(*)
+    end function
+    integer(c_int) function les_profile_run(ctx, jt0, nsteps, family, ms, bytes, launches) &
+            bind(C, name="les_profile_run")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: jt0, nsteps, family
+        real(c_double), intent(out) :: ms(*), bytes(*)
+        integer(c_int), intent(out) :: launches(*)
     end function
     integer(c_int) function les_set_field(ctx, f, h, ld, lev0, nlev) bind(C, name="les_set_field")
         import :: c_int, c_ptr, c_double
