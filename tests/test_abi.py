@@ -110,3 +110,22 @@ def test_fortran_binding_compiles_and_links(tmp_path):
     subprocess.run([fc, str(tmp_path / "dr.o"), str(tmp_path / "di.o"), str(tmp_path / "mm.o"), str(tmp_path / "m.o"),
                     "-o", str(tmp_path / "drive_routines"), lib.LIB, f"-Wl,-rpath,{libdir}"], check=True, cwd=tmp_path)
     assert os.path.exists(tmp_path / "drive_routines")
+
+
+def test_counter_summary_maps_every_family():
+    """bench.py's counter and trace summaries assign kernels to the
+    profiler's families by name: every family the library reports
+    (les_family_name, callable without a GPU) is the image of some kernel the
+    library contains (its host-side launch symbols)."""
+    import bench
+    from overlappingbls_amd import lib
+    if not os.path.exists(lib.LIB):
+        pytest.skip("library not built")
+    L = C.CDLL(lib.LIB)
+    L.les_family_name.restype = C.c_char_p
+    fams = {L.les_family_name(k).decode() for k in range(L.les_nfamilies())}
+    out = subprocess.run(["nm", "-C", "--defined-only", lib.LIB], capture_output=True, text=True, check=True).stdout
+    kernels = [ln.split(" ", 2)[2] for ln in out.splitlines()
+               if re.search(r"\s[DVdv]\s.*les::(\(anonymous namespace\)::)?kr?_", ln)]
+    mapped = {bench.kernel_family(k) for k in kernels}
+    assert fams <= mapped, sorted(fams - mapped)
