@@ -1298,7 +1298,7 @@ void dpdz_dev(les_ctx* c) {
 // ---- RHS assembly and AB2, 103_flosol.f90:115-162, 319-389 -----------------
 template <typename W>
 struct RhsArgs {
-    double *rx, *ry, *rz, *rxf, *ryf, *rzf;
+    W *rx, *ry, *rz, *rxf, *ryf, *rzf;  // the RHS history, in the work precision
     const W *cx, *cy, *cz, *dx, *dy, *dz, *beta;
     const double* sponge;
     double *du, *dv, *dw;  // dudt diagnostics (103_flosol.f90:119-156), or null
@@ -1356,7 +1356,7 @@ __global__ void k_rhs_ab2(RhsArgs<W> a, double* u, double* v, double* w, KP p, R
             ry = ry - sk * (v[i] - o.mn[o.nlev + k - 1]);
             rz = rz - a.sponge[k] * (w[i] - o.mn[2 * o.nlev + k - 1]);
         }
-        a.rx[i] = rx; a.ry[i] = ry; a.rz[i] = rz;
+        a.rx[i] = (W)rx; a.ry[i] = (W)ry; a.rz[i] = (W)rz;
         if (a.du) {
             double du = -(double)a.dx[i], dv = -(double)a.dy[i];
             if (o.ocean) {
@@ -1374,10 +1374,10 @@ __global__ void k_rhs_ab2(RhsArgs<W> a, double* u, double* v, double* w, KP p, R
         }
         double rxf, ryf, rzf;
         if (o.euler) {
-            a.rxf[i] = rx; a.ryf[i] = ry; a.rzf[i] = rz;
+            a.rxf[i] = (W)rx; a.ryf[i] = (W)ry; a.rzf[i] = (W)rz;
             rxf = rx; ryf = ry; rzf = rz;
         } else {
-            rxf = a.rxf[i]; ryf = a.ryf[i]; rzf = a.rzf[i];
+            rxf = (double)a.rxf[i]; ryf = (double)a.ryf[i]; rzf = (double)a.rzf[i];
         }
         u[i] = u[i] + p.dt * (tadv1 * rx + tadv2 * rxf + 0.0 + o.force);
         v[i] = v[i] + p.dt * (tadv1 * ry + tadv2 * ryf + 0.0);
@@ -1387,12 +1387,13 @@ __global__ void k_rhs_ab2(RhsArgs<W> a, double* u, double* v, double* w, KP p, R
 
 template <typename W>
 void rhs_ab2_dev(les_ctx* c, int jt) {
-    FamScope fs(c->stream, FAM_RHS_AB2, (7.0 * sizeof(W) + 96.0) * (double)c->plane * c->nz);
+    // the seven terms and the RHS history (work precision), u, v, w read and written
+    FamScope fs(c->stream, FAM_RHS_AB2, (13.0 * sizeof(W) + 48.0) * (double)c->plane * c->nz);
     KP p = kp(c);
     const les_params& P = c->prm;
     RhsArgs<W> a;
-    a.rx = c->f3[LES_F_RHSX]; a.ry = c->f3[LES_F_RHSY]; a.rz = c->f3[LES_F_RHSZ];
-    a.rxf = c->f3[LES_F_RHSX_F]; a.ryf = c->f3[LES_F_RHSY_F]; a.rzf = c->f3[LES_F_RHSZ_F];
+    a.rx = WF<W>(c, LES_F_RHSX); a.ry = WF<W>(c, LES_F_RHSY); a.rz = WF<W>(c, LES_F_RHSZ);
+    a.rxf = WF<W>(c, LES_F_RHSX_F); a.ryf = WF<W>(c, LES_F_RHSY_F); a.rzf = WF<W>(c, LES_F_RHSZ_F);
     a.cx = WF<W>(c, LES_F_CX); a.cy = WF<W>(c, LES_F_CY); a.cz = WF<W>(c, LES_F_CZ);
     a.dx = WF<W>(c, LES_F_DIVTX); a.dy = WF<W>(c, LES_F_DIVTY); a.dz = WF<W>(c, LES_F_DIVTZ);
     a.beta = WF<W>(c, LES_F_BETA_SCAL);
@@ -1427,9 +1428,10 @@ void rhs_ab2_dev(les_ctx* c, int jt) {
                        c->f3[LES_F_V], c->f3[LES_F_W], p, o);
 }
 
-__global__ void k_ab2(double* u, double* v, double* w, const double* __restrict__ rx,
-                      const double* __restrict__ ry, const double* __restrict__ rz, const double* __restrict__ rxf,
-                      const double* __restrict__ ryf, const double* __restrict__ rzf, KP p, double force) {
+template <typename W>
+__global__ void k_ab2(double* u, double* v, double* w, const W* __restrict__ rx, const W* __restrict__ ry,
+                      const W* __restrict__ rz, const W* __restrict__ rxf, const W* __restrict__ ryf,
+                      const W* __restrict__ rzf, KP p, double force) {
     const long n = p.plane * p.nz;  // levels 0..nz-1
     const double tadv1 = 1.5, tadv2 = 1.0 - tadv1;
     GRID_LOOP(i, n) {
@@ -1437,19 +1439,20 @@ __global__ void k_ab2(double* u, double* v, double* w, const double* __restrict_
             u[i] = BOGUS; v[i] = BOGUS; w[i] = BOGUS;
             continue;
         }
-        u[i] = u[i] + p.dt * (tadv1 * rx[i] + tadv2 * rxf[i] + 0.0 + force);
-        v[i] = v[i] + p.dt * (tadv1 * ry[i] + tadv2 * ryf[i] + 0.0);
-        w[i] = w[i] + p.dt * (tadv1 * rz[i] + tadv2 * rzf[i] + 0.0);
+        u[i] = u[i] + p.dt * (tadv1 * (double)rx[i] + tadv2 * (double)rxf[i] + 0.0 + force);
+        v[i] = v[i] + p.dt * (tadv1 * (double)ry[i] + tadv2 * (double)ryf[i] + 0.0);
+        w[i] = w[i] + p.dt * (tadv1 * (double)rz[i] + tadv2 * (double)rzf[i] + 0.0);
     }
 }
 
+template <typename W>
 void ab2_dev(les_ctx* c) {
     KP p = kp(c);
     const les_params& P = c->prm;
-    hipLaunchKernelGGL(k_ab2, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, c->f3[LES_F_U], c->f3[LES_F_V],
-                       c->f3[LES_F_W], (const double*)c->f3[LES_F_RHSX], (const double*)c->f3[LES_F_RHSY],
-                       (const double*)c->f3[LES_F_RHSZ], (const double*)c->f3[LES_F_RHSX_F],
-                       (const double*)c->f3[LES_F_RHSY_F], (const double*)c->f3[LES_F_RHSZ_F], p,
+    hipLaunchKernelGGL(k_ab2<W>, grid_for(p.plane * p.nz), dim3(256), 0, c->stream, c->f3[LES_F_U], c->f3[LES_F_V],
+                       c->f3[LES_F_W], (const W*)WF<W>(c, LES_F_RHSX), (const W*)WF<W>(c, LES_F_RHSY),
+                       (const W*)WF<W>(c, LES_F_RHSZ), (const W*)WF<W>(c, LES_F_RHSX_F),
+                       (const W*)WF<W>(c, LES_F_RHSY_F), (const W*)WF<W>(c, LES_F_RHSZ_F), p,
                        P.use_mean_p_force ? P.mean_p_force : 0.0);
 }
 
@@ -1457,7 +1460,7 @@ void ab2_dev(les_ctx* c) {
 // :183-185, when du != null) fused with project (110_project.f90:16-40):
 // u, v on levels 1..nz-1, w from level 2
 template <typename W>
-__global__ void k_gradp_project(double* rx, double* ry, double* rz, double* u, double* v, double* w,
+__global__ void k_gradp_project(W* rx, W* ry, W* rz, double* u, double* v, double* w,
                                 const W* __restrict__ px, const W* __restrict__ py, const W* __restrict__ pz, KP p,
                                 double* du, double* dv, double* dw) {
     const long n = p.plane * (p.nz - 1);
@@ -1466,9 +1469,9 @@ __global__ void k_gradp_project(double* rx, double* ry, double* rz, double* u, d
         const long i = p.plane + t;
         const int k = (int)(i / p.plane);
         const double gx = px[i], gy = py[i], gz = pz[i];
-        rx[i] = rx[i] - gx;
-        ry[i] = ry[i] - gy;
-        rz[i] = rz[i] - gz;
+        rx[i] = (W)((double)rx[i] - gx);
+        ry[i] = (W)((double)ry[i] - gy);
+        rz[i] = (W)((double)rz[i] - gz);
         if (du) {
             du[i] = du[i] - gx;
             dv[i] = dv[i] - gy;
@@ -1533,12 +1536,13 @@ void theta_rows_dev(les_ctx* c) {
 // theta_rows = false: theta's rows are left to theta_rows_dev
 template <typename W>
 void gradp_project_dev(les_ctx* c, int jt, bool theta_rows) {
-    FamScope fs(c->stream, FAM_GRADP_PROJECT, (96.0 + 3.0 * sizeof(W)) * (double)c->plane * (c->nz - 1));
+    // RHS (work precision) read and written, u, v, w read and written, grad p read
+    FamScope fs(c->stream, FAM_GRADP_PROJECT, (48.0 + 9.0 * sizeof(W)) * (double)c->plane * (c->nz - 1));
     KP p = kp(c);
     const les_params& P = c->prm;
     const bool st = stats_step(c, jt);
     hipLaunchKernelGGL(k_gradp_project<W>, grid_for(p.plane * (p.nz - 1)), dim3(256), 0, c->stream,
-                       c->f3[LES_F_RHSX], c->f3[LES_F_RHSY], c->f3[LES_F_RHSZ], c->f3[LES_F_U], c->f3[LES_F_V],
+                       WF<W>(c, LES_F_RHSX), WF<W>(c, LES_F_RHSY), WF<W>(c, LES_F_RHSZ), c->f3[LES_F_U], c->f3[LES_F_V],
                        c->f3[LES_F_W], (const W*)WF<W>(c, LES_F_DPDX), (const W*)WF<W>(c, LES_F_DPDY),
                        (const W*)WF<W>(c, LES_F_DPDZ), p, st ? c->st_diag[0] : nullptr,
                        st ? c->st_diag[1] : nullptr, st ? c->st_diag[2] : nullptr);
@@ -1682,6 +1686,7 @@ void max3_dev(les_ctx* c, double* out, bool lag) {
     template void press_hz_sums_dev<W>(les_ctx*, double*, double);                                          \
     template void dpdz_dev<W>(les_ctx*);                                                                    \
     template void rhs_ab2_dev<W>(les_ctx*, int);                                                            \
+    template void ab2_dev<W>(les_ctx*);                                                                     \
     template void gradp_project_dev<W>(les_ctx*, int, bool);                                                \
     template void project_dev<W>(les_ctx*);
 LES_INST_POINT(double)

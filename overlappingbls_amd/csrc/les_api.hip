@@ -278,11 +278,24 @@ bool stokes_profile(const les_ctx* c, std::vector<double>& ust, std::vector<doub
     return true;
 }
 
+// The momentum RHS history: state stored in the precision of the mode
+// (FP32 in LES_MIXED, converted when the mode changes).
+bool is_work_state(int id) {
+    return id == LES_F_RHSX || id == LES_F_RHSY || id == LES_F_RHSZ || id == LES_F_RHSX_F || id == LES_F_RHSY_F ||
+           id == LES_F_RHSZ_F;
+}
 // Work fields: recomputed inside every step, stored in the precision of the
-// mode (FP32 in LES_MIXED).  Every other 3-D field is FP64 state.
+// mode (FP32 in LES_MIXED), and the work-precision state above.  Every
+// other 3-D field is FP64 state.
 bool is_work_field(int id) {
     return (id >= LES_F_DUDX && id <= LES_F_DPDZ) || id == LES_F_NU_T || (id >= LES_F_S11 && id <= LES_F_BETA_SCAL) ||
-           (id >= LES_F_WORK0 && id <= LES_F_WORK2);
+           (id >= LES_F_WORK0 && id <= LES_F_WORK2) || is_work_state(id);
+}
+// exchange two fields' storage in both precisions
+void swap_fields(les_ctx* c, int a, int b) {
+    std::swap(c->f3[a], c->f3[b]);
+    std::swap(c->f32buf[a], c->f32buf[b]);
+    std::swap(c->wk[a], c->wk[b]);
 }
 
 // storage of a 3-D field: double or float pointer
@@ -683,9 +696,7 @@ void obukhov_sums_fork(les_ctx* c, int jt) {
 template <typename W>
 void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     // newstep: RHS_f = RHS (103_flosol.f90:311-313) as a pointer exchange
-    std::swap(c->f3[LES_F_RHSX], c->f3[LES_F_RHSX_F]);
-    std::swap(c->f3[LES_F_RHSY], c->f3[LES_F_RHSY_F]);
-    std::swap(c->f3[LES_F_RHSZ], c->f3[LES_F_RHSZ_F]);
+    for (int m = 0; m < 3; ++m) swap_fields(c, LES_F_RHSX + m, LES_F_RHSX_F + m);
     // obukhov's ordered plane sums overlap the CFL pass (when les_run
     // launched them before it), the horizontal derivatives and the theta
     // gradients
@@ -993,6 +1004,15 @@ int les_set_precision(les_ctx* c, int precision) {
             if (c->f32buf[f]) les_note(c, hipMemsetAsync(c->f32buf[f], 0, F3 * 4, c->stream));
         c->phf = ph;
     }
+    // the work-precision state follows the mode
+    if (precision != c->precision) {
+        const long n = (long)(c->plane * c->nlev);
+        for (int f = 0; f < LES_N_FIELDS3D; ++f) {
+            if (!is_work_state(f)) continue;
+            if (precision == LES_MIXED) convert<float, double>(c->stream, c->f32buf[f], c->f3[f], n);
+            else convert<double, float>(c->stream, c->f3[f], c->f32buf[f], n);
+        }
+    }
     c->precision = precision;
     for (int f = 0; f < LES_N_FIELDS3D; ++f) {
         if (!is_work_field(f)) c->wk[f] = nullptr;
@@ -1249,12 +1269,12 @@ int les_press_stag_array(les_ctx* c, int jt) {
 int les_ab2(les_ctx* c, int jt) {
     if (!c) return LES_E_BADARG;
     if (jt == 1 && !c->prm.restart_vel) {
-        const size_t n = c->plane * c->nlev * 8;
+        const size_t n = c->plane * c->nlev * (mixed(c) ? 4 : 8);
         for (int m = 0; m < 3; ++m)
-            les_note(c, hipMemcpyAsync(c->f3[LES_F_RHSX_F + m], c->f3[LES_F_RHSX + m], n, hipMemcpyDeviceToDevice,
+            les_note(c, hipMemcpyAsync(c->wk[LES_F_RHSX_F + m], c->wk[LES_F_RHSX + m], n, hipMemcpyDeviceToDevice,
                                        c->stream));
     }
-    ab2_dev(c);
+    LES_DISPATCH(ab2_dev, c);
     return finish(c);
 }
 // The pressure TDMA alone (109_press_stag_array.f90:201-319): host half

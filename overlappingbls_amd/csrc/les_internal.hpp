@@ -319,7 +319,7 @@ template <typename W> void press_hz_sums_dev(les_ctx* c, double* out, double mod
 template <typename W> void dpdz_dev(les_ctx* c);
 // RHS assembly + Admas_Bashforth, and RHS -= grad p + project, each one pass
 template <typename W> void rhs_ab2_dev(les_ctx* c, int jt);
-void ab2_dev(les_ctx* c);
+template <typename W> void ab2_dev(les_ctx* c);
 template <typename W> void gradp_project_dev(les_ctx* c, int jt, bool theta_rows = true);
 void theta_rows_dev(les_ctx* c);
 template <typename W> void project_dev(les_ctx* c);

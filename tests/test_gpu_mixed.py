@@ -194,6 +194,27 @@ def test_mixed_then_fp64(oracle_mod, gpu_lib):
     assert not bad, bad
 
 
+def test_rhs_history_follows_the_mode(gpu_lib):
+    """The momentum RHS history is state kept in the work precision: FP32
+    values once LES_MIXED is selected, converted back (exactly) by LES_FP64;
+    the FP64-only RHS_T is untouched."""
+    nl = config.workload("small")
+    g = gpu_lib.create(nl.params())
+    shape = g.get("RHSX").shape
+    vals = {n: 1.0 + 1e-3 * config.uniform(40 + k, 1, shape)
+            for k, n in enumerate(("RHSX", "RHSY", "RHSZ", "RHSX_F", "RHSY_F", "RHSZ_F", "RHST"))}
+    for n, a in vals.items():
+        g.set(n, a)
+    g.set_precision(LES_MIXED)
+    for n, a in vals.items():
+        want = a if n == "RHST" else a.astype(np.float32).astype(np.float64)
+        assert np.array_equal(g.get(n), want), n
+    g.set_precision(LES_FP64)
+    for n, a in vals.items():
+        want = a if n == "RHST" else a.astype(np.float32).astype(np.float64)
+        assert np.array_equal(g.get(n), want), n
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_state_has_no_nyquist_modes_after_a_step(gpu_lib, mixed):
     """filter_data of u, v, w ends the step (103_flosol.f90:242-244): their
