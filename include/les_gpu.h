@@ -148,10 +148,11 @@ typedef struct les_ctx les_ctx;
 int les_create(const les_params* prm, les_ctx** out);
 int les_destroy(les_ctx* ctx);
 /* LES_MIXED: the work fields (derivatives, strain, stresses, divergences,
- * convective terms, theta gradients, beta, p and grad p) and the momentum
- * RHS history (RHSx/y/z and their previous-step copies) are FP32; u, v, w,
- * theta, RHS_T, the SGS state, the TDMA, the mean mode and the statistics
- * stay FP64.  The RHS history is converted when the mode changes. */
+ * convective terms, theta gradients, beta, p and grad p), the momentum RHS
+ * history (RHSx/y/z and their previous-step copies) and the Lagrangian
+ * velocity sums (u_lag, v_lag, w_lag) are FP32; u, v, w, theta, RHS_T,
+ * Cs_opt2, F_LM .. F_NN, the TDMA, the mean mode and the statistics stay
+ * FP64.  The FP32 state is converted when the mode changes. */
 int les_set_precision(les_ctx* ctx, int precision);
 /* Stokes-drift profiles ust(0:nzt), vst(0:nzt) (014_stokes_drift.f90:29-115). */
 int les_set_stokes(les_ctx* ctx, const double* ust, const double* vst);

@@ -504,21 +504,24 @@ void strain_dev(les_ctx* c) {
                        p);
 }
 
-__global__ void k_lag_acc(double* ul, double* vl, double* wl, const double* __restrict__ u,
-                          const double* __restrict__ v, const double* __restrict__ w, long n) {
+// u_lag, v_lag, w_lag are kept in the work precision
+template <typename W>
+__global__ void k_lag_acc(W* ul, W* vl, W* wl, const double* __restrict__ u, const double* __restrict__ v,
+                          const double* __restrict__ w, long n) {
     GRID_LOOP(i, n) {
-        ul[i] = ul[i] + u[i];
-        vl[i] = vl[i] + v[i];
-        wl[i] = wl[i] + w[i];
+        ul[i] = (W)((double)ul[i] + u[i]);
+        vl[i] = (W)((double)vl[i] + v[i]);
+        wl[i] = (W)((double)wl[i] + w[i]);
     }
 }
 
+template <typename W>
 void lag_accumulate_dev(les_ctx* c) {
-    FamScope fs(c->stream, FAM_LAGRANGE_POINT, 9.0 * 8.0 * (double)c->plane * (double)c->nlev);
+    FamScope fs(c->stream, FAM_LAGRANGE_POINT, (24.0 + 6.0 * sizeof(W)) * (double)c->plane * (double)c->nlev);
     const long n = (long)(c->plane * c->nlev);
-    hipLaunchKernelGGL(k_lag_acc, grid_for(n), dim3(256), 0, c->stream, c->f3[LES_F_U_LAG], c->f3[LES_F_V_LAG],
-                       c->f3[LES_F_W_LAG], (const double*)c->f3[LES_F_U], (const double*)c->f3[LES_F_V],
-                       (const double*)c->f3[LES_F_W], n);
+    hipLaunchKernelGGL(k_lag_acc<W>, grid_for(n), dim3(256), 0, c->stream, WF<W>(c, LES_F_U_LAG),
+                       WF<W>(c, LES_F_V_LAG), WF<W>(c, LES_F_W_LAG), (const double*)c->f3[LES_F_U],
+                       (const double*)c->f3[LES_F_V], (const double*)c->f3[LES_F_W], n);
 }
 
 void cs_init_dev(les_ctx* c) { fill_levels(c->stream, c->f3[LES_F_CS_OPT2], c->plane, 1, c->nz, 0.03); }
@@ -666,7 +669,8 @@ void nu_t_stress_dev(les_ctx* c) {
 }
 
 // ---- interpolag_Sdep, 106_06_interpolag_Sdep.f90:4-336 -------------------
-__device__ __forceinline__ double ulag_at(const double* __restrict__ ul, const double* __restrict__ st, long i,
+template <typename W>
+__device__ __forceinline__ double ulag_at(const W* __restrict__ ul, const double* __restrict__ st, long i,
                                           int k, long plane, int nz, double csc) {
     if (k == 1) return ul[i] / csc + st[1];                                 // :181-186
     if (k == nz) return ul[i - plane] + st[nz - 1];                         // :195-199 (raw sum)
@@ -674,15 +678,17 @@ __device__ __forceinline__ double ulag_at(const double* __restrict__ ul, const d
     return a + 0.5 * (st[k] + st[k - 1]);
 }
 
-__device__ __forceinline__ double wlag_at(const double* __restrict__ wl, long i, int k, long plane, int nz,
+template <typename W>
+__device__ __forceinline__ double wlag_at(const W* __restrict__ wl, long i, int k, long plane, int nz,
                                           double csc) {
     if (k == 1) return 0.25 * (wl[i + plane] / csc);                        // :221
     if (k == nz) return 0.25 * (wl[i - plane] / csc);                       // :225
     return wl[i] / csc;
 }
 
-__global__ void k_interpolag(const double* __restrict__ ul, const double* __restrict__ vl,
-                             const double* __restrict__ wl, const double* __restrict__ ust,
+template <typename W>
+__global__ void k_interpolag(const W* __restrict__ ul, const W* __restrict__ vl,
+                             const W* __restrict__ wl, const double* __restrict__ ust,
                              const double* __restrict__ vst, const double* __restrict__ o0,
                              const double* __restrict__ o1, const double* __restrict__ o2,
                              const double* __restrict__ o3, double* f0, double* f1, double* f2, double* f3, KP p,
@@ -734,8 +740,10 @@ __global__ void k_interpolag(const double* __restrict__ ul, const double* __rest
     }
 }
 
+template <typename W>
 void interpolag_dev(les_ctx* c) {
-    FamScope fs(c->stream, FAM_LAGRANGE_POINT, 14.0 * 8.0 * (double)c->plane * (double)c->nlev);
+    // F_* old read and new written (FP64), u_lag, v_lag, w_lag read and cleared
+    FamScope fs(c->stream, FAM_LAGRANGE_POINT, (64.0 + 6.0 * sizeof(W)) * (double)c->plane * (double)c->nlev);
     KP p = kp(c);
     const int ids[4] = {LES_F_F_LM, LES_F_F_MM, LES_F_F_QN, LES_F_F_NN};
     // the old F_* become the interpolation source by a buffer exchange; the
@@ -745,14 +753,14 @@ void interpolag_dev(les_ctx* c) {
         les_note(c, hipMemcpyAsync(c->f3[ids[m]], c->fold[m], sizeof(double) * c->plane, hipMemcpyDeviceToDevice,
                                    c->stream));
     }
-    hipLaunchKernelGGL(k_interpolag, dim3((unsigned)((p.plane + 255) / 256), (unsigned)p.nz), dim3(256), 0, c->stream,
-                       (const double*)c->f3[LES_F_U_LAG], (const double*)c->f3[LES_F_V_LAG],
-                       (const double*)c->f3[LES_F_W_LAG], (const double*)c->ust, (const double*)c->vst,
+    hipLaunchKernelGGL(k_interpolag<W>, dim3((unsigned)((p.plane + 255) / 256), (unsigned)p.nz), dim3(256), 0,
+                       c->stream, (const W*)WF<W>(c, LES_F_U_LAG), (const W*)WF<W>(c, LES_F_V_LAG),
+                       (const W*)WF<W>(c, LES_F_W_LAG), (const double*)c->ust, (const double*)c->vst,
                        (const double*)c->fold[0], (const double*)c->fold[1], (const double*)c->fold[2],
                        (const double*)c->fold[3], c->f3[ids[0]], c->f3[ids[1]], c->f3[ids[2]], c->f3[ids[3]], p,
                        (double)c->prm.cs_count);
     const long n = (long)(c->plane * c->nlev);                              // :330-332
-    fill_runs<double>(c->stream, n, 0.0, {c->f3[LES_F_U_LAG], c->f3[LES_F_V_LAG], c->f3[LES_F_W_LAG]});
+    fill_runs<W>(c->stream, n, 0.0, {WF<W>(c, LES_F_U_LAG), WF<W>(c, LES_F_V_LAG), WF<W>(c, LES_F_W_LAG)});
 }
 
 // ---- lagrange_Sdep, 106_03_lagrange_Sdep.f90:70-331 -------------------------
@@ -1594,16 +1602,17 @@ __global__ __launch_bounds__(256) void k_max3_part(const double* __restrict__ u,
 
 // k_max3_part over levels lo..hi fused with k_lag_acc over all n values:
 // u, v, w are read once
+template <typename W>
 __global__ __launch_bounds__(256) void k_max3_lag(const double* __restrict__ u, const double* __restrict__ v,
-                                                  const double* __restrict__ w, double* ul, double* vl, double* wl,
+                                                  const double* __restrict__ w, W* ul, W* vl, W* wl,
                                                   long n, long lo, long hi, double* __restrict__ part) {
     __shared__ double sh[3][256];
     double m[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
         const double a = u[i], b = v[i], d = w[i];
-        ul[i] = ul[i] + a;
-        vl[i] = vl[i] + b;
-        wl[i] = wl[i] + d;
+        ul[i] = (W)((double)ul[i] + a);
+        vl[i] = (W)((double)vl[i] + b);
+        wl[i] = (W)((double)wl[i] + d);
         if (i >= lo && i < hi) {
             m[0] = fmax(m[0], a);
             m[1] = fmax(m[1], b);
@@ -1644,17 +1653,25 @@ __global__ __launch_bounds__(256) void k_max3_final(const double* __restrict__ p
 }
 
 void max3_dev(les_ctx* c, double* out, bool lag) {
-    FamScope fs(c->stream, FAM_CFL_MAX, (lag ? 9.0 : 3.0) * 8.0 * (double)c->plane * (double)c->nlev);
+    const bool m32 = c->precision == LES_MIXED;  // u_lag in the work precision
+    FamScope fs(c->stream, FAM_CFL_MAX,
+                (24.0 + (lag ? 6.0 * (m32 ? 4.0 : 8.0) : 0.0)) * (double)c->plane * (double)c->nlev);
     const long lo = (long)c->plane, hi = (long)c->plane * c->nz;
     const long n = lag ? (long)c->plane * (long)c->nlev : hi - lo;
     long nb = (n + 255) / 256;
     if (nb > MAXB) nb = MAXB;
     const double dt = c->prm.dt;
     double* part = c->red + 64;  // red holds >= 64 + 3*MAXB doubles
-    if (lag)
-        hipLaunchKernelGGL(k_max3_lag, dim3((unsigned)nb), dim3(256), 0, c->stream, (const double*)c->f3[LES_F_U],
-                           (const double*)c->f3[LES_F_V], (const double*)c->f3[LES_F_W], c->f3[LES_F_U_LAG],
-                           c->f3[LES_F_V_LAG], c->f3[LES_F_W_LAG], n, lo, hi, part);
+    if (lag && m32)
+        hipLaunchKernelGGL(k_max3_lag<float>, dim3((unsigned)nb), dim3(256), 0, c->stream,
+                           (const double*)c->f3[LES_F_U], (const double*)c->f3[LES_F_V],
+                           (const double*)c->f3[LES_F_W], WF<float>(c, LES_F_U_LAG), WF<float>(c, LES_F_V_LAG),
+                           WF<float>(c, LES_F_W_LAG), n, lo, hi, part);
+    else if (lag)
+        hipLaunchKernelGGL(k_max3_lag<double>, dim3((unsigned)nb), dim3(256), 0, c->stream,
+                           (const double*)c->f3[LES_F_U], (const double*)c->f3[LES_F_V],
+                           (const double*)c->f3[LES_F_W], WF<double>(c, LES_F_U_LAG), WF<double>(c, LES_F_V_LAG),
+                           WF<double>(c, LES_F_W_LAG), n, lo, hi, part);
     else
         hipLaunchKernelGGL(k_max3_part, dim3((unsigned)nb), dim3(256), 0, c->stream, (const double*)c->f3[LES_F_U],
                            (const double*)c->f3[LES_F_V], (const double*)c->f3[LES_F_W], lo, hi, part);
@@ -1686,6 +1703,8 @@ void max3_dev(les_ctx* c, double* out, bool lag) {
     template void press_hz_sums_dev<W>(les_ctx*, double*, double);                                          \
     template void dpdz_dev<W>(les_ctx*);                                                                    \
     template void rhs_ab2_dev<W>(les_ctx*, int);                                                            \
+    template void lag_accumulate_dev<W>(les_ctx*);                                                          \
+    template void interpolag_dev<W>(les_ctx*);                                                              \
     template void ab2_dev<W>(les_ctx*);                                                                     \
     template void gradp_project_dev<W>(les_ctx*, int, bool);                                                \
     template void project_dev<W>(les_ctx*);

@@ -278,11 +278,12 @@ bool stokes_profile(const les_ctx* c, std::vector<double>& ust, std::vector<doub
     return true;
 }
 
-// The momentum RHS history: state stored in the precision of the mode
-// (FP32 in LES_MIXED, converted when the mode changes).
+// The momentum RHS history and the Lagrangian velocity sums u_lag, v_lag,
+// w_lag: state stored in the precision of the mode (FP32 in LES_MIXED,
+// converted when the mode changes).
 bool is_work_state(int id) {
     return id == LES_F_RHSX || id == LES_F_RHSY || id == LES_F_RHSZ || id == LES_F_RHSX_F || id == LES_F_RHSY_F ||
-           id == LES_F_RHSZ_F;
+           id == LES_F_RHSZ_F || id == LES_F_U_LAG || id == LES_F_V_LAG || id == LES_F_W_LAG;
 }
 // Work fields: recomputed inside every step, stored in the precision of the
 // mode (FP32 in LES_MIXED), and the work-precision state above.  Every
@@ -348,7 +349,7 @@ void lagrange(les_ctx* c, int jt) {
     const les_params& P = c->prm;
     const int nz = c->nz;
     const long np21 = 21L * nz;  // planes
-    interpolag_dev(c);
+    interpolag_dev<W>(c);
     W* raw = scr<W>(c->lag_buf);
     W* b2 = raw + (size_t)np21 * c->plane;
     W* b4 = b2 + (size_t)np21 * c->plane;
@@ -398,7 +399,7 @@ void sgs_stag(les_ctx* c, int jt, bool keep_s = true) {
     const les_params& P = c->prm;
     const bool init = (jt == 1 && P.inilag);
     const bool update = !c->sgs_frozen && (jt >= P.dyn_init || P.restart_vel) && (jt % P.cs_count) == 0;
-    if (!c->sgs_frozen && !c->lag_done) lag_accumulate_dev(c);
+    if (!c->sgs_frozen && !c->lag_done) lag_accumulate_dev<W>(c);
     if (c->fuse && !keep_s && !init && !update) {
         strain_stress_dev<W>(c);
         return;
@@ -1244,7 +1245,7 @@ int les_lagrange_sdep(les_ctx* c, int jt) {
 }
 int les_interpolag_sdep(les_ctx* c) {
     if (!c) return LES_E_BADARG;
-    interpolag_dev(c);
+    LES_DISPATCH(interpolag_dev, c);
     return finish(c);
 }
 int les_divstress_uv(les_ctx* c, int divt, int tx, int ty, int tz) { return divstress_entry(c, divt, tx, ty, tz, 0); }

@@ -290,12 +290,12 @@ inline void join(les_ctx* c) {
 }
 template <typename W> void wallstress_dev(les_ctx* c, int top, const double* u1, const double* v1);
 template <typename W> void strain_dev(les_ctx* c);
-void lag_accumulate_dev(les_ctx* c);
+template <typename W> void lag_accumulate_dev(les_ctx* c);
 template <typename W> void nu_t_stress_dev(les_ctx* c);
 template <typename W> void strain_stress_dev(les_ctx* c);
 template <typename W> void ddz_w_accum_dev(les_ctx* c, const W* f);
 void cs_init_dev(les_ctx* c);
-void interpolag_dev(les_ctx* c);
+template <typename W> void interpolag_dev(les_ctx* c);
 template <typename W> void lagrange_prep_dev(les_ctx* c, W* buf21);
 template <typename W> void lagrange_contract_dev(les_ctx* c, const W* b2, const W* b4, int init_lm, int init_qn);
 template <typename W, typename TI> void ddz_uv_dev(les_ctx* c, W* d, const TI* f);

@@ -194,25 +194,26 @@ def test_mixed_then_fp64(oracle_mod, gpu_lib):
     assert not bad, bad
 
 
-def test_rhs_history_follows_the_mode(gpu_lib):
-    """The momentum RHS history is state kept in the work precision: FP32
-    values once LES_MIXED is selected, converted back (exactly) by LES_FP64;
-    the FP64-only RHS_T is untouched."""
+WORK_STATE = ("RHSX", "RHSY", "RHSZ", "RHSX_F", "RHSY_F", "RHSZ_F", "U_LAG", "V_LAG", "W_LAG")
+
+
+def test_work_precision_state_follows_the_mode(gpu_lib):
+    """The momentum RHS history and u_lag, v_lag, w_lag are state kept in
+    the work precision: FP32 values once LES_MIXED is selected, converted
+    back (exactly) by LES_FP64; FP64 state such as RHS_T and F_LM is
+    untouched."""
     nl = config.workload("small")
     g = gpu_lib.create(nl.params())
     shape = g.get("RHSX").shape
     vals = {n: 1.0 + 1e-3 * config.uniform(40 + k, 1, shape)
-            for k, n in enumerate(("RHSX", "RHSY", "RHSZ", "RHSX_F", "RHSY_F", "RHSZ_F", "RHST"))}
+            for k, n in enumerate(WORK_STATE + ("RHST", "F_LM"))}
     for n, a in vals.items():
         g.set(n, a)
-    g.set_precision(LES_MIXED)
-    for n, a in vals.items():
-        want = a if n == "RHST" else a.astype(np.float32).astype(np.float64)
-        assert np.array_equal(g.get(n), want), n
-    g.set_precision(LES_FP64)
-    for n, a in vals.items():
-        want = a if n == "RHST" else a.astype(np.float32).astype(np.float64)
-        assert np.array_equal(g.get(n), want), n
+    for prec in (LES_MIXED, LES_FP64):
+        g.set_precision(prec)
+        for n, a in vals.items():
+            want = a.astype(np.float32).astype(np.float64) if n in WORK_STATE else a
+            assert np.array_equal(g.get(n), want), (prec, n)
 
 
 @pytest.mark.parametrize("mixed", [False, True])
