@@ -67,7 +67,7 @@ int les_profile_run(les_ctx* c, int jt0, int nsteps, int family, double* ms, dou
     FamProf prof;
     prof.only = family;
     // a step launches fewer than 512 instrumented kernels
-    prof.recs.resize((size_t)512 * (size_t)(family < 0 ? nsteps : (nsteps + 7) / 8));
+    prof.recs.resize((size_t)512 * (size_t)nsteps);
     size_t made = 0;
     hipError_t e = hipSuccess;
     for (; made < prof.recs.size() && e == hipSuccess; ++made) {
