@@ -420,7 +420,7 @@ void divstress(les_ctx* c, W* out, const W* tx, const W* ty, const W* tz, int wv
 }
 
 template <typename W>
-void convec(les_ctx* c) {
+void convec(les_ctx* c, bool uvw_on_aux = false) {
     const int nz = c->nz;
     const size_t np = c->plane, np2 = c->plane2;
     W* ub = scr<W>(c->u_big);
@@ -428,9 +428,11 @@ void convec(les_ctx* c) {
     W* wb = scr<W>(c->w_big);
     W* big[6];
     for (int i = 0; i < 6; ++i) big[i] = scr<W>(c->big3[i]);
-    pad_f<W, double>(c, F(c, LES_F_U), ub, (int)c->nlev);
-    pad_f<W, double>(c, F(c, LES_F_V), vb, (int)c->nlev);
-    pad_f<W, double>(c, F(c, LES_F_W), wb, (int)c->nlev);
+    if (!uvw_on_aux) {
+        pad_f<W, double>(c, F(c, LES_F_U), ub, (int)c->nlev);
+        pad_f<W, double>(c, F(c, LES_F_V), vb, (int)c->nlev);
+        pad_f<W, double>(c, F(c, LES_F_W), wb, (int)c->nlev);
+    }
     // vorticity and the x pass of its pads in one kernel (x-padded rows in
     // big[3..5]), then the y passes into big[0..2]
     const VortIn<W> vg{WF<W>(c, LES_F_DUDY), WF<W>(c, LES_F_DUDZ), WF<W>(c, LES_F_DVDX),
@@ -453,6 +455,7 @@ void convec(les_ctx* c) {
     // cross product and the x pass of its unpads in one kernel (x-unpadded
     // rows in big[3..5]), then the y passes
     const ConvIn<W> ci{ub, vb, wb, big[0], big[1], big[2], c->ust, c->vst};
+    if (uvw_on_aux) les_note(c, hipStreamWaitEvent(c->stream, c->ev_uvw, 0));
     if (fz && cross_unpad_x<W>(c->stream, ci, 0, big[3], big[4], big[5], nz, c->ny2, c->tw_x2, c->tw_x)) {
         W* dst[3] = {cxf, cyf, czf};
         for (int m = 0; m < 3; ++m)
@@ -506,6 +509,23 @@ void theta_pre(les_ctx* c, int jt) {
     ddz_uv_dev<W, double>(c, dtdz, th);
     theta_prep_dev<W>(c);
     theta_pads<W>(c, dtdx, dtdy, dtdz);
+}
+
+// convec's zero_padding of u, v, w on the auxiliary stream, right after
+// obukhov's plane sums there: they read only u, v, w, which nothing changes
+// before the step's update, so they overlap the derivatives, the wall model
+// and the SGS stresses; convec waits for ev_uvw before the cross product.
+// The x passes take the auxiliary scratch (tscr[3]).
+template <typename W>
+void uvw_pads_overlap(les_ctx* c) {
+    std::swap(c->stream, c->aux);
+    std::swap(c->work3[6], c->tscr[3]);
+    pad_f<W, double>(c, F(c, LES_F_U), scr<W>(c->u_big), (int)c->nlev);
+    pad_f<W, double>(c, F(c, LES_F_V), scr<W>(c->v_big), (int)c->nlev);
+    pad_f<W, double>(c, F(c, LES_F_W), scr<W>(c->w_big), (int)c->nlev);
+    les_note(c, hipEventRecord(c->ev_uvw, c->stream));
+    std::swap(c->work3[6], c->tscr[3]);
+    std::swap(c->stream, c->aux);
 }
 
 // theta_pre on the auxiliary stream, after obukhov's plane sums there.  The
@@ -705,6 +725,7 @@ void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     c->obk_early = 0;
     const bool st = stats && stats_step(c, jt);
     const bool ovl = c->overlap && !tape && !st;  // taped and statistics steps stay in order
+    if (ovl) uvw_pads_overlap<W>(c);             // convec's padded u, v, w on aux
     if (ovl) theta_pre_overlap<W>(c, jt);        // theta's filter, gradients and pads on aux
     const int nl = (int)c->nlev;
     const double *u = F(c, LES_F_U), *v = F(c, LES_F_V), *w = F(c, LES_F_W);
@@ -730,7 +751,7 @@ void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     divstress<W>(c, WF<W>(c, LES_F_DIVTX), txx, txy, txz, 0);
     divstress<W>(c, WF<W>(c, LES_F_DIVTY), txy, tyy, tyz, 0);
     divstress<W>(c, WF<W>(c, LES_F_DIVTZ), txz, tyz, tzz, 1);
-    convec<W>(c);
+    convec<W>(c, ovl);
     if (ovl) theta_post_overlap<W>(c, jt);
     else theta_post<W>(c, jt, tape);
     rhs_ab2_dev<W>(c, jt);
@@ -892,7 +913,8 @@ int les_create(const les_params* prm, les_ctx** out) {
     if (hipStreamCreate(&c->aux) != hipSuccess || hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_obk, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_th, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->ev_th, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_uvw, hipEventDisableTiming) != hipSuccess) {
         les_destroy(c);
         return LES_E_HIP;
     }
@@ -979,6 +1001,7 @@ int les_destroy(les_ctx* c) {
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev_obk) (void)hipEventDestroy(c->ev_obk);
     if (c->ev_th) (void)hipEventDestroy(c->ev_th);
+    if (c->ev_uvw) (void)hipEventDestroy(c->ev_uvw);
     if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

@@ -78,6 +78,7 @@ struct les_ctx {
     hipStream_t aux;                // second stream (default priority) for the ordered plane sums
     hipEvent_t ev_fork, ev_join;
     hipEvent_t ev_obk, ev_th;       // obukhov sums / plane means of theta done on aux
+    hipEvent_t ev_uvw;              // convec's padded u, v, w done on aux
     // device fields (FP64), nlev planes each
     double* f3[LES_N_FIELDS3D];
     // work fields in the precision of the current mode (double: f3[id];
