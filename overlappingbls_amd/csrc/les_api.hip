@@ -511,10 +511,11 @@ void theta_pre(les_ctx* c, int jt) {
     theta_pads<W>(c, dtdx, dtdy, dtdz);
 }
 
-// convec's zero_padding of u, v, w on the auxiliary stream, right after
-// obukhov's plane sums there: they read only u, v, w, which nothing changes
-// before the step's update, so they overlap the derivatives, the wall model
-// and the SGS stresses; convec waits for ev_uvw before the cross product.
+// convec's zero_padding of u, v, w on the auxiliary stream, first in the
+// step there: they read only u, v, w, which nothing changes before the
+// step's update, so they overlap the CFL pass, the derivatives, the wall
+// model and the SGS stresses; convec waits for ev_uvw before the cross
+// product.
 // The x passes take the auxiliary scratch (tscr[3]).
 template <typename W>
 void uvw_pads_overlap(les_ctx* c) {
@@ -528,7 +529,7 @@ void uvw_pads_overlap(les_ctx* c) {
     std::swap(c->stream, c->aux);
 }
 
-// theta_pre on the auxiliary stream, after obukhov's plane sums there.  The
+// theta_pre on the auxiliary stream, after the padded u, v, w there.  The
 // momentum path reads none of theta's fields until calc_beta, which waits for
 // ev_th (recorded after the plane means); the x passes of the pads take the
 // auxiliary scratch, so the momentum pads on the context's stream run
@@ -705,13 +706,16 @@ void avg_slices(les_ctx* c, int jt) {
     }
 }
 
-// obukhov's ordered plane sums on the auxiliary stream, after everything
-// enqueued so far on the context's stream; ev_obk marks their end
+// obukhov's ordered plane sums on their own stream, after everything
+// enqueued so far on the context's stream (which the auxiliary stream also
+// waits for); ev_obk marks their end.  A sum is one wavefront's dependent
+// chain, so on a stream of its own it does not hold up the auxiliary work.
 void obukhov_sums_fork(les_ctx* c, int jt) {
     obukhov_theta_copy(c);
     fork(c);
-    obukhov_sums_dev(c, jt, c->aux);
-    les_note(c, hipEventRecord(c->ev_obk, c->aux));
+    les_note(c, hipStreamWaitEvent(c->sums, c->ev_fork, 0));
+    obukhov_sums_dev(c, jt, c->sums);
+    les_note(c, hipEventRecord(c->ev_obk, c->sums));
 }
 
 template <typename W>
@@ -830,6 +834,7 @@ int divstress_entry(les_ctx* c, int divt, int tx, int ty, int tz, int wver) {
 void les::step_fp64(les_ctx* c, int jt, Tape* tape, bool stats) { step_body<double>(c, jt, tape, stats); }
 
 int les::finish(les_ctx* c) {
+    les_note(c, hipStreamSynchronize(c->sums));
     les_note(c, hipStreamSynchronize(c->aux));
     hipError_t e = hipStreamSynchronize(c->stream);
     les_note(c, e);
@@ -910,7 +915,8 @@ int les_create(const les_params* prm, les_ctx** out) {
     if (noalloc_scratch(c->dev, &c->noalloc) != hipSuccess) { delete c; return LES_E_HIP; }
     c->err = hipSuccess;
     if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return LES_E_HIP; }
-    if (hipStreamCreate(&c->aux) != hipSuccess || hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+    if (hipStreamCreate(&c->aux) != hipSuccess || hipStreamCreate(&c->sums) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_obk, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_th, hipEventDisableTiming) != hipSuccess ||
@@ -994,6 +1000,7 @@ int les_destroy(les_ctx* c) {
     if (!c) return LES_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->aux) (void)hipStreamSynchronize(c->aux);
+    if (c->sums) (void)hipStreamSynchronize(c->sums);
     lin_destroy(c);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->cfl_host) (void)hipHostFree(c->cfl_host);
@@ -1003,6 +1010,7 @@ int les_destroy(les_ctx* c) {
     if (c->ev_th) (void)hipEventDestroy(c->ev_th);
     if (c->ev_uvw) (void)hipEventDestroy(c->ev_uvw);
     if (c->aux) (void)hipStreamDestroy(c->aux);
+    if (c->sums) (void)hipStreamDestroy(c->sums);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return LES_OK;

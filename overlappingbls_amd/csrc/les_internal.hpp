@@ -75,9 +75,10 @@ struct les_ctx {
     int precision;
     int dev;
     hipStream_t stream;
-    hipStream_t aux;                // second stream (default priority) for the ordered plane sums
+    hipStream_t aux;                // second stream (default priority): theta's chain, the padded u, v, w
+    hipStream_t sums;               // third stream (default priority): obukhov's ordered plane sums
     hipEvent_t ev_fork, ev_join;
-    hipEvent_t ev_obk, ev_th;       // obukhov sums / plane means of theta done on aux
+    hipEvent_t ev_obk, ev_th;       // obukhov sums done (sums) / plane means of theta done (aux)
     hipEvent_t ev_uvw;              // convec's padded u, v, w done on aux
     // device fields (FP64), nlev planes each
     double* f3[LES_N_FIELDS3D];
