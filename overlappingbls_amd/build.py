@@ -42,9 +42,9 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
         if len(procs) >= jobs:
-            _wait(procs.pop(0))
-    for p in procs:
-        _wait(p)
+            _wait(procs.pop(0), procs)
+    while procs:
+        _wait(procs.pop(0), procs)
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
@@ -54,11 +54,16 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     return LIB
 
 
-def _wait(item):
+def _wait(item, others=()):
+    """Wait for one compile; on failure stop the other compiles still
+    running (so none is left writing its object) and raise."""
     src, p = item
     out, _ = p.communicate()
     text = out.decode(errors="replace")
     if p.returncode != 0:
+        for _, q in others:
+            q.kill()
+            q.communicate()
         raise RuntimeError(f"hipcc failed on {src}:\n{text}")
     msgs = [l for l in text.splitlines() if "warning" in l and "packed-fp32-ops" not in l]
     if msgs:
