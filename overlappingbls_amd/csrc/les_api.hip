@@ -589,7 +589,7 @@ void theta_post(les_ctx* c, int jt, Tape* tape = nullptr) {
 // come last as in the serial order).  The caller joins the streams before the
 // step ends.
 template <typename W>
-void theta_post_overlap(les_ctx* c, int jt) {
+void theta_post_overlap(les_ctx* c, int jt, Tape* tape = nullptr) {
     const les_params& P = c->prm;
     const size_t np = c->plane;
     if (!theta_active(c, jt)) {
@@ -598,6 +598,8 @@ void theta_post_overlap(les_ctx* c, int jt) {
     }
     les_note(c, hipStreamWaitEvent(c->stream, c->ev_th, 0));
     calc_beta_dev<W>(c, c->th_mean);
+    // the filtered theta and its plane means, before the update on aux
+    if (tape) tape_save(c, tape, TAPE_THETA);
     fork(c);
     double** scratch[5] = {&c->work3[0], &c->work3[1], &c->work3[2], &c->work3[6], &c->big3[3]};
     std::swap(c->stream, c->aux);
@@ -728,7 +730,7 @@ void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     if (!c->obk_early) obukhov_sums_fork(c, jt);
     c->obk_early = 0;
     const bool st = stats && stats_step(c, jt);
-    const bool ovl = c->overlap && !tape && !st;  // taped and statistics steps stay in order
+    const bool ovl = c->overlap && !st;  // statistics steps stay in order
     if (ovl) uvw_pads_overlap<W>(c);             // convec's padded u, v, w on aux
     if (ovl) theta_pre_overlap<W>(c, jt);        // theta's filter, gradients and pads on aux
     const int nl = (int)c->nlev;
@@ -756,7 +758,7 @@ void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     divstress<W>(c, WF<W>(c, LES_F_DIVTY), txy, tyy, tyz, 0);
     divstress<W>(c, WF<W>(c, LES_F_DIVTZ), txz, tyz, tzz, 1);
     convec<W>(c, ovl);
-    if (ovl) theta_post_overlap<W>(c, jt);
+    if (ovl) theta_post_overlap<W>(c, jt, tape);
     else theta_post<W>(c, jt, tape);
     rhs_ab2_dev<W>(c, jt);
     press_stag_array<W>(c, jt);
