@@ -534,8 +534,8 @@ void strain_tl(les_ctx* c, LinBufs& b) {
 }
 void strain_adj(les_ctx* c, LinBufs& b) {
     const LP p = lin_params(c);
+    // lambda_S needs no clearing: k_stress_adj, its only source, writes all of levels 1..nz
     hipLaunchKernelGGL(k_strain_adj, lgrid(p.plane * p.nz), dim3(256), 0, c->stream, lin_grad(b), lin_strain(b), p);
-    for (int m = 0; m < 6; ++m) clear_levels(c, b, b.v[L_S11 + m], 1, p.nz);
 }
 
 // ---- Nu_t and tau_ij (106_sgs_stag.f90:206-315), Cs_opt2 frozen ------------------
@@ -602,10 +602,10 @@ __global__ void k_stress_tl(TapeS ts, LinStrain dS, double* dnu, LinTau dt, LP p
     }
 }
 
-// level k (1..nz): lambda_S(k) and the total lambda_Nu(k) from the tau rows
-// that read level k (rows k and k-1) and from the scalar fluxes (lnu on
-// entry); clears the tau and Nu adjoints of level k afterwards is done by the
-// host once every level has been gathered
+// level k (1..nz): lambda_S(k), written (this is its only source), and the
+// total lambda_Nu(k) from the tau rows that read level k (rows k and k-1) and
+// from the scalar fluxes (lnu on entry); the host clears lambda_Nu once every
+// level has been gathered
 __global__ void k_stress_adj(TapeS ts, LinStrain lS, const double* __restrict__ lnu, LinTau lt, LP p) {
     const long n = p.plane * p.nz;
     const int nz = p.nz;
@@ -674,8 +674,8 @@ __global__ void k_stress_adj(TapeS ts, LinStrain lS, const double* __restrict__ 
             ls[0] += 2.0 * s[0] * a; ls[3] += 2.0 * s[3] * a; ls[5] += 2.0 * s[5] * a;
             ls[1] += 4.0 * s[1] * a; ls[2] += 4.0 * s[2] * a; ls[4] += 4.0 * s[4] * a;
         }
-        lS.s11[i] += ls[0]; lS.s12[i] += ls[1]; lS.s13[i] += ls[2];
-        lS.s22[i] += ls[3]; lS.s23[i] += ls[4]; lS.s33[i] += ls[5];
+        lS.s11[i] = ls[0]; lS.s12[i] = ls[1]; lS.s13[i] = ls[2];
+        lS.s22[i] = ls[3]; lS.s23[i] = ls[4]; lS.s33[i] = ls[5];
     }
 }
 
@@ -688,11 +688,8 @@ void stress_adj(les_ctx* c, const Tape& t, LinBufs& b) {
     const LP p = lin_params(c);
     hipLaunchKernelGGL(k_stress_adj, lgrid(p.plane * p.nz), dim3(256), 0, c->stream, tape_s(t), lin_strain(b),
                        (const double*)b.v[L_NU], lin_tau(b), p);
+    // the tau adjoints need no clearing: k_div_adj writes levels 1..nz-1 of all six
     clear_levels(c, b, b.v[L_NU], 1, p.nz);
-    const int ids[4] = {L_TXX, L_TXY, L_TYY, L_TZZ};
-    for (int m = 0; m < 4; ++m) clear_levels(c, b, b.v[ids[m]], 1, p.nz - 1);
-    clear_levels(c, b, b.v[L_TXZ], 2, p.nz - 1);
-    clear_levels(c, b, b.v[L_TYZ], 2, p.nz - 1);
 }
 
 // ---- divstress (107_divstress.f90:36-102) ------------------------------------------
@@ -727,9 +724,9 @@ __global__ void k_div_adj(LinTau lt, const double* __restrict__ t0, const double
     LGRID(t, n) {
         const long i = p.plane + t;
         const int k = (int)(i / p.plane);
-        lt.xx[i] += -t0[i];
-        lt.xy[i] += -t1[i] - t2[i];
-        lt.yy[i] += -t3[i];
+        lt.xx[i] = -t0[i];
+        lt.xy[i] = -t1[i] - t2[i];
+        lt.yy[i] = -t3[i];
         // txz(k): DIVX rows k-1 (+1/dz) and k (-1/dz), DIVZ row k via ddx
         double ax = -t4[i] - ldx[i] / p.dz;
         double ay = -t5[i] - ldy[i] / p.dz;
@@ -737,13 +734,13 @@ __global__ void k_div_adj(LinTau lt, const double* __restrict__ t0, const double
             ax += ldx[i - p.plane] / p.dz;
             ay += ldy[i - p.plane] / p.dz;
         }
-        lt.xz[i] += ax;
-        lt.yz[i] += ay;
+        lt.xz[i] = ax;
+        lt.yz[i] = ay;
         // tzz(k): DIVZ rows k (+1/dz, k in 2..nz-1) and k+1 (-1/dz)
         double az = 0.0;
         if (k >= 2) az += ldz[i] / p.dz;
         if (k + 1 <= nz - 1) az -= ldz[i + p.plane] / p.dz;
-        lt.zz[i] += az;
+        lt.zz[i] = az;
     }
 }
 void div_adj(les_ctx* c, LinBufs& b, double* const* t) {
@@ -751,8 +748,8 @@ void div_adj(les_ctx* c, LinBufs& b, double* const* t) {
     hipLaunchKernelGGL(k_div_adj, lgrid(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, lin_tau(b),
                        (const double*)t[0], (const double*)t[1], (const double*)t[2], (const double*)t[3],
                        (const double*)t[4], (const double*)t[5], b.v[L_DVX], b.v[L_DVY], b.v[L_DVZ], p);
-    clear_levels(c, b, b.v[L_DVX], 1, p.nz - 1);
-    clear_levels(c, b, b.v[L_DVY], 1, p.nz - 1);
+    // lambda_DIVX, DIVY: k_rhs_adj writes levels 1..nz-1 (no clearing); the
+    // pressure adjoint adds into lambda_DIVZ first, so it is cleared
     clear_levels(c, b, b.v[L_DVZ], 1, p.nz);
 }
 
@@ -851,11 +848,13 @@ __global__ void k_cross_adj(TapeBig tb, const double* __restrict__ ust, const do
         const double lzu = (j + 1 >= 2 && j + 1 <= nz - 1) ? l.c3[up] : 0.0;
         const double lxd = (j >= 2) ? l.c1[dn] : 0.0, lyd = (j >= 2) ? l.c2[dn] : 0.0;
         const double lxo = (j >= 2 && j <= nz - 1) ? lx : 0.0, lyo = (j >= 2 && j <= nz - 1) ? ly : 0.0;
+        // the padded vorticity is read by the cross product only: its
+        // adjoint on levels 1..nz is written (o3 is zero on level nz)
         if (j <= nz - 1) {
             l.ub[i] += ly * b3[i] - 0.5 * b2[i] * lz - 0.5 * b2[up] * lzu;
             l.vb[i] += -lx * b3[i] + 0.5 * b1[i] * lz + 0.5 * b1[up] * lzu;
-            l.o3[i] += -(vb[i] + vst[j]) * lx + (ub[i] + ust[j]) * ly;
         }
+        l.o3[i] = (j <= nz - 1) ? -(vb[i] + vst[j]) * lx + (ub[i] + ust[j]) * ly : 0.0;
         if (j >= 2 && j <= nz - 1) l.wb[i] += 0.5 * b2[i] * (lxd + lxo) - 0.5 * b1[i] * (lyd + lyo);
         double a2 = 0.5 * wb[i] * (lxd + lxo);
         double a1 = -0.5 * wb[i] * (lyd + lyo);
@@ -863,8 +862,8 @@ __global__ void k_cross_adj(TapeBig tb, const double* __restrict__ ust, const do
             a2 += -0.5 * (ub[i] + ub[dn] + ust[j] + ust[j - 1]) * lz;
             a1 += 0.5 * (vb[i] + vb[dn] + vst[j] + vst[j - 1]) * lz;
         }
-        l.o2[i] += a2;
-        l.o1[i] += a1;
+        l.o2[i] = a2;
+        l.o1[i] = a1;
     }
 }
 void cross_tl(les_ctx* c, const Tape& t, LinBufs& b) {
@@ -903,16 +902,22 @@ __global__ void k_sadv_adj(TapeBig tb, const double* __restrict__ ust, const dou
     LGRID(t, n) {
         const long i = P2 + t;
         const int j = (int)(i / P2);
+        // the scalar advection is the only reader of the padded theta
+        // gradients and the first (in reverse order) of the padded velocities:
+        // their adjoints on levels 1..nz-1 are written, not accumulated
         const double r = l.rm[i];
-        l.ub[i] += r * tb.t1[i];
-        l.vb[i] += r * tb.t2[i];
-        l.t1[i] += (tb.ub[i] + ust[j]) * r;
-        l.t2[i] += (tb.vb[i] + vst[j]) * r;
+        l.ub[i] = r * tb.t1[i];
+        l.vb[i] = r * tb.t2[i];
+        l.t1[i] = (tb.ub[i] + ust[j]) * r;
+        l.t2[i] = (tb.vb[i] + vst[j]) * r;
+        double aw = 0.0, at = 0.0;
         if (j >= 2) {
             const double s = l.rm[i - P2] + ((j <= nz - 1) ? r : 0.0);
-            l.wb[i] += 0.5 * tb.t3[i] * s;
-            l.t3[i] += 0.5 * tb.wb[i] * s;
+            aw = 0.5 * tb.t3[i] * s;
+            at = 0.5 * tb.wb[i] * s;
         }
+        l.wb[i] = aw;
+        l.t3[i] = at;
     }
 }
 void sadv_tl(les_ctx* c, const Tape& t, LinBufs& b) {
@@ -1145,7 +1150,6 @@ void beta_adj(les_ctx* c, const Tape& t, LinBufs& b, double g) {
     hipLaunchKernelGGL(k_beta_adj_pt, lgrid(p.plane * (p.nz - 1)), dim3(256), 0, c->stream, b.v[L_TH],
                        b.v[L_BETA], (const double*)t.tmean, (const double*)b.small, p, g,
                        (double)(c->nx * c->ny));
-    clear_levels(c, b, b.v[L_BETA], 2, p.nz - 1);
 }
 
 // ---- AB2 updates, RHS assembly, projection ----------------------------------------------
@@ -1195,10 +1199,12 @@ __global__ void k_rhs_adj(LinRhs a, LP p, RhsFlags f, const double* __restrict__
         const long i = p.plane + t;
         const int k = (int)(i / p.plane);
         const double lx = a.rx[i], ly = a.ry[i], lz = a.rz[i];
-        a.cx[i] += -lx; a.dx[i] += -lx;
-        a.cy[i] += -ly; a.dy[i] += -ly;
-        a.cz[i] += -lz; a.dz[i] += -lz;
-        if (f.buoy) a.beta[i] += lz;
+        // the RHS is the only reader of the convective term, DIVX, DIVY and
+        // beta: their adjoints are written here, not accumulated
+        a.cx[i] = -lx; a.dx[i] = -lx;
+        a.cy[i] = -ly; a.dy[i] = -ly;
+        a.cz[i] = -lz; a.dz[i] += -lz;
+        a.beta[i] = f.buoy ? lz : 0.0;
         if (f.cori) {
             a.v[i] += f.coriol * lx;
             a.u[i] += -f.coriol * ly;

@@ -312,9 +312,7 @@ void s_thpad_adj(Run& r) {
     add_unpad(r, V(r, L_TX), V(r, L_TB1), 1, nz - 1, 9.0 / 4.0);
     add_unpad(r, V(r, L_TY), V(r, L_TB2), 1, nz - 1, 9.0 / 4.0);
     add_unpad(r, V(r, L_TZ), V(r, L_TB3), 2, nz - 1, 9.0 / 4.0);
-    clear_all(r, L_TB1);
-    clear_all(r, L_TB2);
-    clear_all(r, L_TB3);
+    // no clearing: the scalar advection's adjoint writes levels 1..nz-1 of all three
 }
 
 // 6: wall stress at the bottom (104_wallstress.f90:98-182)
@@ -382,9 +380,7 @@ void s_cpad_adj(Run& r) {
     add_unpad(r, V(r, L_U), V(r, L_UB), 1, nz - 1, 9.0 / 4.0);
     add_unpad(r, V(r, L_V), V(r, L_VB), 1, nz - 1, 9.0 / 4.0);
     add_unpad(r, V(r, L_W), V(r, L_WB), 1, nz - 1, 9.0 / 4.0);
-    clear_all(r, L_UB);
-    clear_all(r, L_VB);
-    clear_all(r, L_WB);
+    // no clearing: the scalar advection's adjoint writes levels 1..nz-1 first
 }
 void s_vort_tl(Run& r) { vort_tl(r.c, r.b); }
 void s_vort_adj(Run& r) { vort_adj(r.c, r.b); }
@@ -396,8 +392,7 @@ void s_vpad_tl(Run& r) {
 void s_vpad_adj(Run& r) {
     const int nz = NZ(r);
     for (int m = 0; m < 3; ++m) {
-        add_unpad(r, V(r, L_O1 + m), V(r, L_OB1 + m), 1, nz, 9.0 / 4.0);
-        clear_all(r, L_OB1 + m);
+        add_unpad(r, V(r, L_O1 + m), V(r, L_OB1 + m), 1, nz, 9.0 / 4.0);  // the cross product writes these
     }
 }
 void s_cross_tl(Run& r) { cross_tl(r.c, r.t, r.b); }
@@ -410,8 +405,7 @@ void s_cunpad_tl(Run& r) {
 void s_cunpad_adj(Run& r) {
     const int nz = NZ(r);
     for (int m = 0; m < 3; ++m) {
-        add_pad(r, V(r, L_CB1 + m), V(r, L_CX + m), 1, nz - 1, 4.0 / 9.0);
-        clear_all(r, L_CX + m);
+        add_pad(r, V(r, L_CB1 + m), V(r, L_CX + m), 1, nz - 1, 4.0 / 9.0);  // rhs writes lambda_C
     }
 }
 
