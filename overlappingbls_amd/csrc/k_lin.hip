@@ -520,7 +520,7 @@ __global__ void k_strain_adj(LinGrad g, LinStrain S, LP p) {
             wx += 0.25 * S.s13[r];
             wy += 0.25 * S.s23[r];
         }
-        g.ux[i] += a11; g.vy[i] += a22; g.wz[i] += a33;
+        g.ux[i] = a11; g.vy[i] = a22; g.wz[i] = a33;  // read by the strain only
         g.uy[i] += a12; g.vx[i] += a12;
         g.uz[i] += 0.5 * l13; g.vz[i] += 0.5 * l23;
         g.wx[i] += wx; g.wy[i] += wy;
@@ -688,8 +688,8 @@ void stress_adj(les_ctx* c, const Tape& t, LinBufs& b) {
     const LP p = lin_params(c);
     hipLaunchKernelGGL(k_stress_adj, lgrid(p.plane * p.nz), dim3(256), 0, c->stream, tape_s(t), lin_strain(b),
                        (const double*)b.v[L_NU], lin_tau(b), p);
-    // the tau adjoints need no clearing: k_div_adj writes levels 1..nz-1 of all six
-    clear_levels(c, b, b.v[L_NU], 1, p.nz);
+    // no clearing: k_div_adj writes levels 1..nz-1 of the six tau adjoints and
+    // k_sflux_adj levels 1..nz of lambda_Nu
 }
 
 // ---- divstress (107_divstress.f90:36-102) ------------------------------------------
@@ -785,12 +785,13 @@ __global__ void k_vort_adj(double* l1, double* l2, double* l3, LinGrad g, LP p) 
         if (j >= 2 && j <= nz - 1) { wy += l1[i]; wx += l2[i]; }
         if (j == nz) { wy += 0.5 * l1[i]; wx += 0.5 * l2[i]; }
         if (j == nz - 1) { wy += 0.5 * l1[up]; wx += 0.5 * l2[up]; }
-        g.wy[i] += wy;
-        g.wx[i] += -wx;
-        g.vz[i] += -l1[i];
-        g.uz[i] += l2[i];
-        g.vx[i] += l3[i];
-        g.uy[i] += -l3[i];
+        // first (in reverse order) of these six gradient adjoints: written
+        g.wy[i] = wy;
+        g.wx[i] = -wx;
+        g.vz[i] = -l1[i];
+        g.uz[i] = l2[i];
+        g.vx[i] = l3[i];
+        g.uy[i] = -l3[i];
     }
 }
 void vort_tl(les_ctx* c, LinBufs& b) {
@@ -972,13 +973,19 @@ __global__ void k_sflux_adj(const double* __restrict__ nu, const double* __restr
         if (j - 1 >= 2 && j - 1 <= nz - 2) a += 0.5 * (tx[dn] * lfx[dn] + ty[dn] * lfy[dn]);
         if (j == nz) a += tx[dn] * lfx[dn] + ty[dn] * lfy[dn];
         if (j >= 2) a += tz[i] * lfz[i];
-        lnu[i] += iPr * a;
+        // first (in reverse order) of the adjoints of Nu and of the theta
+        // gradients: written on levels 1..nz, not accumulated
+        lnu[i] = iPr * a;
+        double ax = 0.0, ay = 0.0, az = 0.0;
         if (j <= nz - 1) {
             const double na = nu_avg(nu, i, j, p.plane, nz);
-            ltx[i] += iPr * na * lfx[i];
-            lty[i] += iPr * na * lfy[i];
-            if (j >= 2) ltz[i] += iPr * nu[i] * lfz[i];
+            ax = iPr * na * lfx[i];
+            ay = iPr * na * lfy[i];
+            if (j >= 2) az = iPr * nu[i] * lfz[i];
         }
+        ltx[i] = ax;
+        lty[i] = ay;
+        ltz[i] = az;
     }
 }
 void sflux_tl(les_ctx* c, const Tape& t, LinBufs& b) {
@@ -994,9 +1001,7 @@ void sflux_adj(les_ctx* c, const Tape& t, LinBufs& b) {
                        (const double*)t.dT[0], (const double*)t.dT[1], (const double*)t.dT[2], b.v[L_NU], b.v[L_TX],
                        b.v[L_TY], b.v[L_TZ], (const double*)b.v[L_FX], (const double*)b.v[L_FY],
                        (const double*)b.v[L_FZ], p, 1.0 / c->prm.Pr);
-    clear_levels(c, b, b.v[L_FX], 1, p.nz - 1);
-    clear_levels(c, b, b.v[L_FY], 1, p.nz - 1);
-    clear_levels(c, b, b.v[L_FZ], 2, p.nz);
+    // the flux adjoints need no clearing: k_sdiv_adj writes them
 }
 
 // ---- bulk surface flux of lbc = 0 (010_scalars.f90:359-394) -------------------------
@@ -1070,12 +1075,13 @@ __global__ void k_sdiv_adj(double* lrt, double* ladv, double* lfx, double* lfy, 
     LGRID(t, n) {
         const long i = p.plane + t;
         const int j = (int)(i / p.plane);
-        if (j <= nz - 1) {
-            ladv[i] += -lrt[i];
-            lfx[i] += -t1[i];
-            lfy[i] += -t2[i];
-        }
-        lfz[i] += (((j >= 2) ? lrt[i - p.plane] : 0.0) - ((j <= nz - 1) ? lrt[i] : 0.0)) / p.dz;
+        // the scalar RHS is the only reader of the advection term and the
+        // fluxes: their adjoints are written on levels 1..nz
+        const bool in = j <= nz - 1;
+        ladv[i] = in ? -lrt[i] : 0.0;
+        lfx[i] = in ? -t1[i] : 0.0;
+        lfy[i] = in ? -t2[i] : 0.0;
+        lfz[i] = (((j >= 2) ? lrt[i - p.plane] : 0.0) - (in ? lrt[i] : 0.0)) / p.dz;
     }
 }
 void sdiv_tl(les_ctx* c, LinBufs& b, const double* t1, const double* t2) {

@@ -263,15 +263,15 @@ void s_deriv_tl(Run& r) {
     ddz_uv_tl(c, V(r, L_VZ), V(r, L_V));
     ddz_w_tl(c, V(r, L_WZ), V(r, L_W));
 }
+// lambda_f += the adjoint of its three gradients; the gradient adjoints are
+// not cleared: their first source in the adjoint step writes them (the
+// vorticity and strain for u, v, w; the scalar SGS flux for theta)
 void grad_adj_field(Run& r, int f, int fx, int fy, int fz, int wform) {
     les_ctx* c = r.c;
     const int nl = NL(r);
     ddx_f<double, double>(c, W3(r, 0), V(r, fx), 0, nl);
     ddy_f<double, double>(c, W3(r, 1), V(r, fy), 0, nl);
     grad_adj(c, V(r, f), W3(r, 0), W3(r, 1), V(r, fz), wform);
-    clear_all(r, fx);
-    clear_all(r, fy);
-    clear_all(r, fz);
 }
 void s_deriv_adj(Run& r) {
     grad_adj_field(r, L_U, L_UX, L_UY, L_UZ, 0);
@@ -414,8 +414,7 @@ void s_sadv_tl(Run& r) { sadv_tl(r.c, r.t, r.b); }
 void s_sadv_adj(Run& r) { sadv_adj(r.c, r.t, r.b); }
 void s_sunpad_tl(Run& r) { unpad_f<double>(r.c, V(r, L_RM) + NP2(r), V(r, L_ADV) + NP(r), NZ(r) - 1); }
 void s_sunpad_adj(Run& r) {
-    add_pad(r, V(r, L_RM), V(r, L_ADV), 1, NZ(r) - 1, 4.0 / 9.0);
-    clear_all(r, L_ADV);
+    add_pad(r, V(r, L_RM), V(r, L_ADV), 1, NZ(r) - 1, 4.0 / 9.0);  // k_sdiv_adj writes lambda_ADV
 }
 void s_sflux_tl(Run& r) { sflux_tl(r.c, r.t, r.b); }
 void s_sflux_adj(Run& r) { sflux_adj(r.c, r.t, r.b); }
