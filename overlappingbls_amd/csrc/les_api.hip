@@ -732,7 +732,6 @@ void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     const bool st = stats && stats_step(c, jt);
     const bool ovl = c->overlap && !st;  // statistics steps stay in order
     if (ovl) uvw_pads_overlap<W>(c);             // convec's padded u, v, w on aux
-    if (ovl) theta_pre_overlap<W>(c, jt);        // theta's filter, gradients and pads on aux
     const int nl = (int)c->nlev;
     const double *u = F(c, LES_F_U), *v = F(c, LES_F_V), *w = F(c, LES_F_W);
     ddx_f<W, double>(c, WF<W>(c, LES_F_DUDX), u, 0, nl);
@@ -744,7 +743,11 @@ void step_body(les_ctx* c, int jt, Tape* tape = nullptr, bool stats = true) {
     ddz_uv_dev<W, double>(c, WF<W>(c, LES_F_DUDZ), u);
     ddz_uv_dev<W, double>(c, WF<W>(c, LES_F_DVDZ), v);
     ddz_w_dev<W, double>(c, WF<W>(c, LES_F_DWDZ), w);
-    if (!ovl) theta_pre<W>(c, jt);
+    // theta's filter, gradients and pads: on aux (submitted after the
+    // derivatives, so that the host, which has a bounded number of launches
+    // outstanding, hands the momentum stream its work first), or in order
+    if (ovl) theta_pre_overlap<W>(c, jt);
+    else theta_pre<W>(c, jt);
     les_note(c, hipStreamWaitEvent(c->stream, c->ev_obk, 0));
     obukhov_finish_dev(c, jt);
     if (tape) tape_save(c, tape, TAPE_OBUKHOV);
