@@ -501,15 +501,29 @@ def trace_summary(trace_dir: str, bench_out: str, cs_count: int = 5):
               f"{lv['ms_per_step']:.3f} |")
 
 
-def traffic_from_profiles(family: str):
+def traffic_from_profiles(family: str, workload: str = "sbl256"):
     """HBM bytes per launch of a kernel family measured with rocprofv3 --pmc
-    (committed summary profiles/traffic.json, keyed by family)."""
+    (committed summary profiles/traffic.json, keyed by family); None for a
+    workload the summary was not taken on."""
     path = os.path.join(HERE, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         data = json.load(f)
+    if not str(data.get("_workload", "")).startswith(workload + ","):
+        return None
     return data.get(family)
+
+
+def workload_text(name, nl, p):
+    if p.ocean_flag:
+        return (f"{name}: {p.nxt}x{p.nyt}x{p.nzt - 1} ocean bottom boundary layer (geostrophic current "
+                f"{nl.ug_dim:g} m/s, rough wall z0={p.zo:g} z_i at the computational top, stress-free "
+                f"surface, no wind stress), one scalar, Lagrangian scale-dependent SGS "
+                f"(cs_count={nl.cs_count}), dt={p.dt:g}")
+    return (f"{name}: {p.nxt}x{p.nyt}x{p.nzt - 1} surface BL, stable (wt_s={nl.wt_s / nl.T_scale:+g} u*T), "
+            f"Lagrangian scale-dependent SGS (cs_count={nl.cs_count}), rough log-law wall, stress-free top, "
+            f"dt={p.dt:g}")
 
 
 def family_profile(ctx, jt, cs_count):
@@ -673,7 +687,7 @@ def main():
     achieved = rr["bytes"] / (rr["ms"] * 1e-3) / 1e9
     sb = step_bytes(n, nl.cs_count)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(dom),
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(dom, args.workload),
                 "traffic_source": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, profiles/traffic.json",
                 "kernel": dom,
                 "measured_over": f"steps {args.warmup + 1}..{args.warmup + args.steps} rerun from the same "
@@ -713,10 +727,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if head == "fp64" else "f32+f64",
         "data": "synthetic seeded fields (counter-based RNG), log-law profile + 10% noise below z=0.5",
-        "config": {"workload": f"{args.workload}: {p.nxt}x{p.nyt}x{p.nzt - 1} surface BL, stable "
-                               f"(wt_s={nl.wt_s / nl.T_scale:+g} u*T), Lagrangian scale-dependent SGS "
-                               f"(cs_count={nl.cs_count}), rough log-law wall, stress-free top, "
-                               f"dt={p.dt:g}", "precision": head, "parallelism": "1 GPU"},
+        "config": {"workload": workload_text(args.workload, nl, p), "precision": head, "parallelism": "1 GPU"},
         "roofline": roofline,
         "families": {f: {k: (round(v, 5) if isinstance(v, float) else v) for k, v in x.items()}
                      for f, x in sorted(prof.items(), key=lambda kv: -kv[1]["ms_per_step"])},

@@ -101,6 +101,9 @@ class Namelist:
     flag_relaxForce: bool = False
     spatial_flux_flag: bool = False
     flag_dynWT: bool = False
+    # roughness length / z_i; None: the reference's patch constant ZO1 / z_i
+    # (008_bottombc.f90:18, :119), a module constant there, not a namelist entry
+    zo: float | None = None
 
     def params(self) -> LesParams:
         """les_params as init_nondimensional leaves them (102_initial.f90:176-256);
@@ -133,6 +136,8 @@ class Namelist:
         p.z_i, p.u_scale, p.T_scale = z_i, us, self.T_scale
         p.nu_molec = self.nu_molec
         p.zo, p.d0 = ZO1 / z_i, DO1 / z_i          # patches, 008_bottombc.f90:119-120
+        if self.zo is not None:
+            p.zo = self.zo
         p.Pr, p.wt_s, p.dTdz_top = self.Pr, self.wt_s, self.dTdz_top
         p.Co = self.Co
         if self.coriolis_forcing or self.ocean_flag:   # :208-212, :235-238
@@ -290,6 +295,18 @@ def workload(name: str) -> Namelist:
                         ubc_mom="wall", u_star=1.0, agl_stress=20.0, freq_coriolis=1e-4, ug_dim=0.0,
                         vg_dim=0.0, alpha_w=2.0e-4, stokes_flag=True, stokes_action="mono", amp_w=1.13,
                         lambda_w=60.0, agl_Ust=10.0, wt_s=0.01 * 300.0, **base)
+    # the ocean bottom boundary layer at 512 x 512 x 256: the reference's BBL
+    # physics (BBL/readin/param.nml: the ocean branch, no wind stress, a
+    # geostrophic current of 0.25 m/s, a rough wall at the computational top,
+    # the same 282.7 x 282.7 x 45 m domain, dt = 0.15 s, u_scale = 0.0122 m/s,
+    # no surface heat flux) on a grid twice as fine horizontally and with 256
+    # levels, rough wall z0 = 1e-3 z_i, the dynamic SGS from the start as in
+    # the surface-layer workloads; one scalar (theta)
+    if name == "bbl512":
+        return Namelist(nxt=512, nyt=512, nzt=257, lx_tot=282.743334, ly_tot=282.743334, lz_tot=45.0,
+                        z_i=45.0, dt=0.15, u_scale=0.0122, ocean_flag=True, lbc_mom="stress free",
+                        ubc_mom="wall", u_star=0.0, freq_coriolis=1e-4, ug_dim=0.25, vg_dim=0.0,
+                        inv_strength=0.1, dTdz_top=0.0, wt_s=0.0, zo=1e-3)
     # deep ocean: stress-free floor, deep-water Stokes drift, damping method 3,
     # cross flow
     if name == "ocean_deep":

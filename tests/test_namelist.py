@@ -114,6 +114,19 @@ def test_mean_pressure_force_default_and_zero():
     assert math.isclose(p.mean_p_force, 2.0e-4 / 0.25 * 100.0)
 
 
+def test_roughness_override_and_the_ocean_bbl_workload():
+    """zo defaults to the patch constant ZO1 / z_i (008_bottombc.f90:18,
+    :119); the 512 x 512 x 256 bottom-boundary-layer workload sets 1e-3 and
+    takes the reference BBL's ocean branch and scales."""
+    p = config.workload("sbl256").params()
+    assert math.isclose(p.zo, config.ZO1 / 100.0)
+    p = config.workload("bbl512").params()
+    assert (p.nxt, p.nyt, p.nzt) == (512, 512, 257)
+    assert p.zo == 1e-3 and p.ocean_flag == 1 and p.u_star == 0.0
+    assert math.isclose(p.dt, 0.15 * 0.0122 / 45.0)
+    assert math.isclose(p.lx_tot, 282.743334 / 45.0)
+
+
 def test_reference_param_files():
     """The three param.nml files the reference ships (its own configurations)
     read into parameter blocks: the ocean branch with a wind-stressed
