@@ -129,3 +129,16 @@ def test_counter_summary_maps_every_family():
                if re.search(r"\s[DVdv]\s.*les::(\(anonymous namespace\)::)?kr?_", ln)]
     mapped = {bench.kernel_family(k) for k in kernels}
     assert fams <= mapped, sorted(fams - mapped)
+
+
+def test_bench_line_describes_its_workload():
+    """The bench line names the workload it ran, and the counter-based traffic
+    (taken on the 256^3 surface layer) is attached to that workload only."""
+    import bench
+    from overlappingbls_amd import config
+    for name, word in (("sbl256", "surface BL"), ("bbl512", "ocean bottom boundary layer")):
+        nl = config.workload(name)
+        text = bench.workload_text(name, nl, nl.params())
+        assert text.startswith(name + ":") and word in text, text
+    assert bench.traffic_from_profiles("yline.pad", "sbl256") is not None
+    assert bench.traffic_from_profiles("yline.pad", "bbl512") is None
